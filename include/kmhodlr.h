@@ -1,0 +1,165 @@
+/*
+ * kmhodlr -- C ABI of the MI355X (gfx950) HODLR Gaussian-process engine.
+ *
+ * The reference (KernelMatrices.jl 0.9.0) has no FFI: its boundary is the
+ * Julia method surface of the HODLR path.  Each entry point below names the
+ * reference method it replaces (file:line under the reference tree).  A Julia
+ * shim binds these with ccall; the Python binding uses ctypes.
+ *
+ * Conventions
+ *   - Every call returns an int status (KMH_OK = 0).  On failure
+ *     kmh_last_error() gives a thread-local message that bindings rethrow
+ *     with the reference's exception text.
+ *   - Points are n x d doubles, row-major (point i at pts[i*d .. i*d+d-1]);
+ *     d must be 2.
+ *   - Vectors / right-hand sides are n x nrhs doubles, column-major.
+ *   - Functions without a suffix take HOST pointers and are synchronous.
+ *     Functions with the suffix _dev take DEVICE pointers (on the context's
+ *     device) and are ordered on the context's stream; call
+ *     kmh_ctx_synchronize() before reading results on the host.
+ *   - The library owns all device memory of a handle.  A handle is not
+ *     thread-safe; distinct handles on one context share its stream.
+ */
+#ifndef KMHODLR_H
+#define KMHODLR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define KMH_OK               0
+#define KMH_E_BAD_ARG        1  /* invalid argument (sizes, ranks, kernel)      */
+#define KMH_E_NOT_PD         2  /* Cholesky met a non-positive pivot (PosDefException) */
+#define KMH_E_NOT_FACTORIZED 3  /* solve/logdet/gradient before factorization   */
+#define KMH_E_OOM            4  /* device allocation failed                      */
+#define KMH_E_BUDGET         5  /* request exceeds the context's HBM budget      */
+#define KMH_E_DEVICE         6  /* HIP runtime error                             */
+#define KMH_E_ACA            7  /* "Incompatible maxrank or matrix dimensions."  */
+#define KMH_E_STATE          8  /* operation invalid in the handle's state       */
+
+/* ---- kernel families (the reference's kernel closures) ---- */
+#define KMH_RATIONAL2  1  /* p1^2/(1+(r/p2)^2)^2          README.md:42-48              */
+#define KMH_CAUCHY     2  /* p1^2/(1+(p2 r)^2)            example/basicexample.jl:62-68 */
+#define KMH_MATERN12   3  /* t0*exp(-r/rho)               src/covariancefunctions.jl:258-266, nu=1/2 */
+#define KMH_MATERN32   4  /* t0*(1+sqrt3 r/rho)exp(-sqrt3 r/rho), nu=3/2, same lines */
+
+/* ---- derivative codes (src/covariancefunctions.jl:268-330, 424-472) ---- */
+#define KMH_D0   0   /* the kernel itself (nugget applied)   */
+#define KMH_D1   1   /* d/dp1                                */
+#define KMH_D2   2   /* d/dp2                                */
+#define KMH_D11 11   /* d2/dp1^2                             */
+#define KMH_D12 12   /* d2/dp1 dp2                           */
+#define KMH_D22 22   /* d2/dp2^2                             */
+
+typedef struct kmh_kernel {
+    int32_t family;     /* KMH_RATIONAL2 ... KMH_MATERN32                     */
+    int32_t deriv;      /* KMH_D0 ... ; derivative kernels carry no nugget    */
+    double  params[4];  /* (p1, p2) = (t0 = sigma^2, rho) for Matern          */
+    double  nugget;     /* added where the two points are equal (x == y)      */
+} kmh_kernel;
+
+/* ---- build options (constructor.jl:6-8 keyword/positional arguments) ---- */
+#define KMH_ACA        0
+#define KMH_NYSTROM    1
+#define KMH_FIXED_LEVEL 0   /* structstypes.jl:30-32 */
+#define KMH_LOG_LEVEL   1   /* structstypes.jl:35-37 */
+#define KMH_FP64       0
+
+typedef struct kmh_build_opts {
+    int32_t method;      /* KMH_ACA or KMH_NYSTROM (kwarg nystrom)            */
+    int32_t level_kind;  /* KMH_FIXED_LEVEL or KMH_LOG_LEVEL                  */
+    int32_t level;       /* lv of FixedLevel(lv) / LogLevel(lv)               */
+    int32_t maxrank;     /* maxrank (0 = no limit for ACA; Nystrom rank)      */
+    double  tol;         /* ep: ACA relative tolerance                        */
+    int32_t precision;   /* KMH_FP64                                          */
+    int32_t reserved[3];
+} kmh_build_opts;
+
+typedef struct kmh_info {
+    int64_t n;           /* matrix size                                       */
+    int32_t levels;      /* number of off-diagonal levels                     */
+    int32_t nleaves;     /* 2^levels                                          */
+    int32_t factorized;  /* 1 after kmh_hodlr_factorize                       */
+    int32_t method;      /* KMH_ACA / KMH_NYSTROM                             */
+    int64_t device_bytes;/* device memory held by the handle                  */
+} kmh_info;
+
+/* factor-apply operations on W (K = W W^T), HODLR/src/baseoverloads.jl:95-150 */
+#define KMH_W      0   /* y = W x      */
+#define KMH_WT     1   /* y = W^T x    */
+#define KMH_WINV   2   /* y = W \ x    */
+#define KMH_WTINV  3   /* y = W^T \ x  */
+
+typedef struct kmh_ctx kmh_ctx;
+typedef struct kmh_hodlr kmh_hodlr;
+
+/* ---- errors and devices ---- */
+const char* kmh_last_error(void);
+int kmh_device_count(int* count);
+
+/* ---- context: device, stream, HBM budget for handles (0 = unlimited) ---- */
+int kmh_ctx_create(int device, uint64_t hbm_budget_bytes, kmh_ctx** out);
+int kmh_ctx_destroy(kmh_ctx* ctx);
+int kmh_ctx_synchronize(kmh_ctx* ctx);
+int kmh_ctx_bytes_in_use(kmh_ctx* ctx, uint64_t* out);
+/* Measurement hooks (no counterpart in the reference): between begin and end
+ * every grouped FP64 GEMM launch on the context's stream is bracketed by HIP
+ * events; end returns the launches' algorithmic flops (sum of 2mnk), their
+ * summed duration in ms and the launch count.                               */
+int kmh_ctx_profile_begin(kmh_ctx* ctx);
+int kmh_ctx_profile_end(kmh_ctx* ctx, double* gemm_flops, double* gemm_ms, int64_t* gemm_launches);
+
+/* ---- HODLR.KernelHODLR(K, ep, maxrank, lvl; nystrom)  constructor.jl:6-58 ----
+ * K = KernelMatrix(pts, pts, params, kernel)  src/structstypes.jl:17-23      */
+int kmh_hodlr_build(kmh_ctx* ctx, const double* pts, int64_t n, int32_t d,
+                    const kmh_kernel* kernel, const kmh_build_opts* opts,
+                    kmh_hodlr** out);
+int kmh_hodlr_build_dev(kmh_ctx* ctx, const double* pts_dev, int64_t n, int32_t d,
+                        const kmh_kernel* kernel, const kmh_build_opts* opts,
+                        kmh_hodlr** out);
+int kmh_hodlr_destroy(kmh_hodlr* h);
+int kmh_hodlr_info(kmh_hodlr* h, kmh_info* out);
+
+/* leaf boundaries: nleaves+1 offsets (indices.jl:29-37)                      */
+int kmh_hodlr_leaf_offsets(kmh_hodlr* h, int64_t* offsets);
+/* per-block ranks, level-major: level 1 (finest) first, 2^(L-l) blocks each;
+ * total 2^L - 1 entries (length of each U[l][k], V[l][k] column set)         */
+int kmh_hodlr_ranks(kmh_hodlr* h, int32_t* ranks);
+/* dense full(HK) before factorization, n x n column-major
+ * (HODLR/src/baseoverloads.jl:313-326; for tests)                            */
+int kmh_hodlr_full(kmh_hodlr* h, double* out);
+/* U and V of block k at level l (1-based level): U is rows x rank, V is
+ * cols x rank, column-major; only before factorization                      */
+int kmh_hodlr_block_uv(kmh_hodlr* h, int32_t level, int32_t block, double* U, double* V);
+
+/* ---- HK * x  (HODLR/src/baseoverloads.jl:152-179) ---- */
+int kmh_hodlr_matvec(kmh_hodlr* h, const double* x, double* y, int64_t nrhs);
+int kmh_hodlr_matvec_dev(kmh_hodlr* h, const double* x, double* y, int64_t nrhs);
+
+/* ---- HODLR.symmetricfactorize!(HK)  HODLR/src/symfact.jl:6-50 ---- */
+int kmh_hodlr_factorize(kmh_hodlr* h);
+
+/* ---- mul!/ldiv!(t, HK.W or adjoint(HK.W), v)  HODLR/src/baseoverloads.jl:95-150 */
+int kmh_hodlr_factor_apply(kmh_hodlr* h, int32_t op, const double* x, double* y, int64_t nrhs);
+int kmh_hodlr_factor_apply_dev(kmh_hodlr* h, int32_t op, const double* x, double* y, int64_t nrhs);
+
+/* ---- HK \ b  (HODLR/src/baseoverloads.jl:181-192, 252-263) ---- */
+int kmh_hodlr_solve(kmh_hodlr* h, const double* b, double* x, int64_t nrhs);
+int kmh_hodlr_solve_dev(kmh_hodlr* h, const double* b, double* x, int64_t nrhs);
+
+/* ---- logdet(HK)  HODLR/src/baseoverloads.jl:194-209 ---- */
+int kmh_hodlr_logdet(kmh_hodlr* h, double* out);
+
+/* ---- HODLR.stoch_gradient(K, HK, y, dfuns, vecs)  stochasticgradient.jl:30-45
+ * dks: p derivative kernels; y: n; saa: n x nv column-major; grad: p        */
+int kmh_stoch_gradient(kmh_hodlr* h, const kmh_kernel* dks, int32_t p,
+                       const double* y, const double* saa, int64_t nv, double* grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KMHODLR_H */

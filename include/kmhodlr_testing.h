@@ -1,0 +1,33 @@
+/*
+ * Test hooks of libkmhodlr: direct access to the batched dense primitives
+ * (GEMM, Cholesky, triangular solve, panel QR) on one matrix, host buffers
+ * in and out.  They have no counterpart in the reference; they exist so the
+ * GPU test-suite can check each primitive against numpy before the HODLR
+ * algorithms that are built from them.
+ */
+#ifndef KMHODLR_TESTING_H
+#define KMHODLR_TESTING_H
+
+#include "kmhodlr.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* C = alpha op(A) op(B) + beta C, column-major; lower != 0 updates only the
+ * 64x64 tiles on or below the diagonal */
+int kmh_test_gemm(kmh_ctx* ctx, int32_t ta, int32_t tb, int32_t m, int32_t n, int32_t k,
+                  const double* A, int32_t lda, const double* B, int32_t ldb,
+                  double* Cm, int32_t ldc, double alpha, double beta, int32_t lower);
+/* in-place lower Cholesky; returns KMH_E_NOT_PD on a non-positive pivot */
+int kmh_test_potrf(kmh_ctx* ctx, int32_t n, double* A, int32_t lda);
+/* B <- L^{-1} B (trans = 0) or L^{-T} B (trans = 1) */
+int kmh_test_trsm(kmh_ctx* ctx, int32_t trans, int32_t n, const double* Lm, int32_t ldl,
+                  int32_t ncols, double* B, int32_t ldb);
+/* thin QR of an m x r panel: A <- Q, R (r x r) */
+int kmh_test_qr(kmh_ctx* ctx, int32_t m, int32_t r, double* A, int32_t lda, double* R, int32_t ldr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KMHODLR_TESTING_H */

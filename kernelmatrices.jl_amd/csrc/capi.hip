@@ -1,0 +1,314 @@
+// C ABI entry points of libkmhodlr (declared in include/kmhodlr.h).
+#include <memory>
+
+#include "hodlr.h"
+#include "la.h"
+
+struct kmh_hodlr {
+    std::unique_ptr<kmh::Hodlr> h;
+};
+
+using kmh::Error;
+
+#define KMH_TRY(...)                                                        \
+    try {                                                                   \
+        __VA_ARGS__;                                                        \
+        return KMH_OK;                                                      \
+    } catch (const Error& e) {                                              \
+        kmh::set_last_error(e.what());                                      \
+        return e.code;                                                      \
+    } catch (const std::bad_alloc&) {                                       \
+        kmh::set_last_error("host allocation failed");                      \
+        return KMH_E_OOM;                                                   \
+    } catch (const std::exception& e) {                                     \
+        kmh::set_last_error(e.what());                                      \
+        return KMH_E_DEVICE;                                                \
+    }
+
+static kmh::Hodlr& H_of(kmh_hodlr* h) {
+    KMH_CHECK(h && h->h, KMH_E_BAD_ARG, "null HODLR handle");
+    h->h->ctx->set_device();
+    return *h->h;
+}
+
+// Host wrappers: stage x into device memory, run, copy back.
+template <class F>
+static void with_device_rhs(kmh::Hodlr& H, const double* x, double* y, int64_t nrhs, bool two, F f) {
+    kmh::Ctx& c = *H.ctx;
+    KMH_CHECK(x && y, KMH_E_BAD_ARG, "null vector");
+    KMH_CHECK(nrhs >= 1 && nrhs < (int64_t(1) << 20), KMH_E_BAD_ARG, "invalid number of right-hand sides");
+    const size_t bytes = (size_t)H.n * nrhs * sizeof(double);
+    double* dx = static_cast<double*>(c.scratch(bytes));
+    double* dy = two ? static_cast<double*>(c.scratch(bytes)) : dx;
+    KMH_HIP(hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, c.stream));
+    f(dx, dy);
+    KMH_HIP(hipMemcpyAsync(y, dy, bytes, hipMemcpyDeviceToHost, c.stream));
+    if (two) c.scratch_free(dy);
+    c.scratch_free(dx);
+    c.sync();
+}
+
+extern "C" {
+
+const char* kmh_last_error(void) { return kmh::last_error_cstr(); }
+
+int kmh_device_count(int* count) {
+    KMH_TRY({
+        KMH_CHECK(count, KMH_E_BAD_ARG, "null output");
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            n = 0;
+        }
+        *count = n;
+    })
+}
+
+int kmh_ctx_create(int device, uint64_t hbm_budget_bytes, kmh_ctx** out) {
+    KMH_TRY({
+        KMH_CHECK(out, KMH_E_BAD_ARG, "null output");
+        int n = 0;
+        KMH_HIP(hipGetDeviceCount(&n));
+        KMH_CHECK(device >= 0 && device < n, KMH_E_BAD_ARG, "device index out of range");
+        std::unique_ptr<kmh_ctx> c(new kmh_ctx());
+        c->c.device = device;
+        c->c.budget = hbm_budget_bytes;
+        KMH_HIP(hipSetDevice(device));
+        KMH_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+        c->c.arena_size = size_t(16) << 20;
+        KMH_HIP(hipMalloc(&c->c.arena, c->c.arena_size));
+        KMH_HIP(hipHostMalloc(&c->c.staging, c->c.arena_size, hipHostMallocDefault));
+        KMH_HIP(hipMalloc(&c->c.d_info, 256));
+        KMH_HIP(hipMemset(c->c.d_info, 0, 256));
+        // keep freed stream-ordered scratch cached in the device pool
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+            uint64_t thr = ~uint64_t(0);
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        }
+        (void)hipGetLastError();
+        *out = c.release();
+    })
+}
+
+int kmh_ctx_destroy(kmh_ctx* ctx) {
+    KMH_TRY({
+        if (!ctx) return KMH_OK;
+        kmh::Ctx& c = ctx->c;
+        c.set_device();
+        (void)hipStreamSynchronize(c.stream);
+        for (auto e : c.ev_pool) (void)hipEventDestroy(e);
+        if (c.arena) (void)hipFree(c.arena);
+        if (c.staging) (void)hipHostFree(c.staging);
+        if (c.d_info) (void)hipFree(c.d_info);
+        (void)hipStreamDestroy(c.stream);
+        delete ctx;
+    })
+}
+
+int kmh_ctx_synchronize(kmh_ctx* ctx) {
+    KMH_TRY({
+        KMH_CHECK(ctx, KMH_E_BAD_ARG, "null context");
+        ctx->c.set_device();
+        ctx->c.sync();
+    })
+}
+
+int kmh_ctx_bytes_in_use(kmh_ctx* ctx, uint64_t* out) {
+    KMH_TRY({
+        KMH_CHECK(ctx && out, KMH_E_BAD_ARG, "null argument");
+        *out = ctx->c.in_use;
+    })
+}
+
+int kmh_ctx_profile_begin(kmh_ctx* ctx) {
+    KMH_TRY({
+        KMH_CHECK(ctx, KMH_E_BAD_ARG, "null context");
+        kmh::Ctx& c = ctx->c;
+        c.set_device();
+        c.sync();
+        c.prof = true;
+        c.ev_used = 0;
+        c.ev_flops.clear();
+    })
+}
+
+int kmh_ctx_profile_end(kmh_ctx* ctx, double* gemm_flops, double* gemm_ms, int64_t* gemm_launches) {
+    KMH_TRY({
+        KMH_CHECK(ctx && gemm_flops && gemm_ms && gemm_launches, KMH_E_BAD_ARG, "null argument");
+        kmh::Ctx& c = ctx->c;
+        c.set_device();
+        c.sync();
+        double fl = 0.0, ms = 0.0;
+        for (size_t i = 0; i < c.ev_flops.size(); ++i) {
+            float t = 0.f;
+            KMH_HIP(hipEventElapsedTime(&t, c.ev_pool[2 * i], c.ev_pool[2 * i + 1]));
+            fl += c.ev_flops[i];
+            ms += t;
+        }
+        *gemm_flops = fl;
+        *gemm_ms = ms;
+        *gemm_launches = (int64_t)c.ev_flops.size();
+        c.prof = false;
+        c.ev_used = 0;
+        c.ev_flops.clear();
+    })
+}
+
+static int build_common(kmh_ctx* ctx, const double* pts, bool dev, int64_t n, int32_t d, const kmh_kernel* kernel,
+                        const kmh_build_opts* opts, kmh_hodlr** out) {
+    KMH_TRY({
+        KMH_CHECK(ctx && kernel && opts && out, KMH_E_BAD_ARG, "null argument");
+        std::unique_ptr<kmh_hodlr> h(new kmh_hodlr());
+        h->h.reset(kmh::hodlr_build(ctx->c, pts, dev, n, d, *kernel, *opts));
+        *out = h.release();
+    })
+}
+
+int kmh_hodlr_build(kmh_ctx* ctx, const double* pts, int64_t n, int32_t d, const kmh_kernel* kernel,
+                    const kmh_build_opts* opts, kmh_hodlr** out) {
+    return build_common(ctx, pts, false, n, d, kernel, opts, out);
+}
+
+int kmh_hodlr_build_dev(kmh_ctx* ctx, const double* pts_dev, int64_t n, int32_t d, const kmh_kernel* kernel,
+                        const kmh_build_opts* opts, kmh_hodlr** out) {
+    return build_common(ctx, pts_dev, true, n, d, kernel, opts, out);
+}
+
+int kmh_hodlr_destroy(kmh_hodlr* h) {
+    KMH_TRY({
+        if (!h) return KMH_OK;
+        if (h->h) h->h->ctx->set_device();
+        delete h;
+    })
+}
+
+int kmh_hodlr_info(kmh_hodlr* h, kmh_info* out) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        KMH_CHECK(out, KMH_E_BAD_ARG, "null output");
+        out->n = H.n;
+        out->levels = H.L;
+        out->nleaves = H.nleaves();
+        out->factorized = H.factorized ? 1 : 0;
+        out->method = H.opts.method;
+        out->device_bytes = (int64_t)H.device_bytes();
+    })
+}
+
+int kmh_hodlr_leaf_offsets(kmh_hodlr* h, int64_t* offsets) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        KMH_CHECK(offsets, KMH_E_BAD_ARG, "null output");
+        for (size_t i = 0; i < H.leaf_off.size(); ++i) offsets[i] = H.leaf_off[i];
+    })
+}
+
+int kmh_hodlr_ranks(kmh_hodlr* h, int32_t* ranks) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        KMH_CHECK(ranks, KMH_E_BAD_ARG, "null output");
+        size_t q = 0;
+        for (int l = 0; l < H.L; ++l)
+            for (int r : H.ranks[l]) ranks[q++] = r;
+    })
+}
+
+int kmh_hodlr_full(kmh_hodlr* h, double* out) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        KMH_CHECK(out, KMH_E_BAD_ARG, "null output");
+        kmh::hodlr_full_host(H, out);
+    })
+}
+
+int kmh_hodlr_block_uv(kmh_hodlr* h, int32_t level, int32_t block, double* U, double* V) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        KMH_CHECK(U && V, KMH_E_BAD_ARG, "null output");
+        kmh::hodlr_block_uv_host(H, level, block, U, V);
+    })
+}
+
+int kmh_hodlr_matvec(kmh_hodlr* h, const double* x, double* y, int64_t nrhs) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        with_device_rhs(H, x, y, nrhs, true, [&](double* dx, double* dy) { kmh::hodlr_matvec(H, dx, dy, (int)nrhs); });
+    })
+}
+
+int kmh_hodlr_matvec_dev(kmh_hodlr* h, const double* x, double* y, int64_t nrhs) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        KMH_CHECK(x && y && nrhs >= 1, KMH_E_BAD_ARG, "invalid vectors");
+        kmh::hodlr_matvec(H, x, y, (int)nrhs);
+    })
+}
+
+int kmh_hodlr_factorize(kmh_hodlr* h) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        kmh::hodlr_factorize(H);
+    })
+}
+
+int kmh_hodlr_factor_apply(kmh_hodlr* h, int32_t op, const double* x, double* y, int64_t nrhs) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        KMH_CHECK(H.factorized, KMH_E_NOT_FACTORIZED, "No solves without factorization.");
+        with_device_rhs(H, x, y, nrhs, false, [&](double* dx, double*) { kmh::hodlr_factor_apply(H, op, dx, (int)nrhs); });
+    })
+}
+
+int kmh_hodlr_factor_apply_dev(kmh_hodlr* h, int32_t op, const double* x, double* y, int64_t nrhs) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        KMH_CHECK(H.factorized, KMH_E_NOT_FACTORIZED, "No solves without factorization.");
+        KMH_CHECK(x && y && nrhs >= 1, KMH_E_BAD_ARG, "invalid vectors");
+        kmh::Ctx& c = *H.ctx;
+        if (x != y) kmh::copy2d(c, {{x, (int)H.n, y, (int)H.n, (int)H.n, (int)nrhs, 1.0}});
+        kmh::hodlr_factor_apply(H, op, y, (int)nrhs);
+    })
+}
+
+int kmh_hodlr_solve(kmh_hodlr* h, const double* b, double* x, int64_t nrhs) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        KMH_CHECK(H.factorized, KMH_E_NOT_FACTORIZED, "No solves without factorization.");
+        with_device_rhs(H, b, x, nrhs, false, [&](double* dx, double*) { kmh::hodlr_solve(H, dx, (int)nrhs); });
+    })
+}
+
+int kmh_hodlr_solve_dev(kmh_hodlr* h, const double* b, double* x, int64_t nrhs) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        KMH_CHECK(H.factorized, KMH_E_NOT_FACTORIZED, "No solves without factorization.");
+        KMH_CHECK(b && x && nrhs >= 1, KMH_E_BAD_ARG, "invalid vectors");
+        kmh::Ctx& c = *H.ctx;
+        if (b != x) kmh::copy2d(c, {{b, (int)H.n, x, (int)H.n, (int)H.n, (int)nrhs, 1.0}});
+        kmh::hodlr_solve(H, x, (int)nrhs);
+    })
+}
+
+int kmh_hodlr_logdet(kmh_hodlr* h, double* out) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        KMH_CHECK(out, KMH_E_BAD_ARG, "null output");
+        KMH_CHECK(H.factorized, KMH_E_NOT_FACTORIZED, "No logdet without factorization.");
+        *out = H.logdet;
+    })
+}
+
+int kmh_stoch_gradient(kmh_hodlr* h, const kmh_kernel* dks, int32_t p, const double* y, const double* saa,
+                       int64_t nv, double* grad) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        KMH_CHECK(H.factorized, KMH_E_NOT_FACTORIZED, "The matrix needs to be factorized for this to work.");
+        KMH_CHECK(dks && y && grad && p >= 1, KMH_E_BAD_ARG, "null argument");
+        KMH_CHECK(nv >= 1 && saa, KMH_E_BAD_ARG, "need at least one SAA vector");
+        kmh::hodlr_stoch_gradient(H, dks, p, y, saa, nv, grad);
+    })
+}
+
+}  // extern "C"

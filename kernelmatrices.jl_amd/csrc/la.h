@@ -1,0 +1,83 @@
+// Batched dense linear algebra on gfx950 (FP64): the primitives the HODLR
+// build, factorization, solve and gradient are composed of.
+//
+// All routines take lists of problems of uneven sizes and run each step as
+// one grouped launch on the context's stream.  Matrices are column-major.
+#pragma once
+#include "common.h"
+
+namespace kmh {
+
+// C = alpha * op(A) * op(B) + beta * C, op(A) is m x k, op(B) is k x n.
+// beta == 0 never reads C.  lower != 0 skips 64x64 output tiles strictly
+// above the diagonal (symmetric rank-k updates of a lower triangle).
+struct GemmProb {
+    int m, n, k;
+    const double* A; int lda;
+    const double* B; int ldb;
+    double* C; int ldc;
+    double alpha, beta;
+    int lower;
+};
+void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs);
+
+// A square matrix (or its leading n x n part) with leading dimension ld.
+struct SqMat {
+    double* A;
+    int n;
+    int ld;
+};
+// In-place lower Cholesky of each matrix; the strict upper triangle is
+// zeroed.  A non-positive or NaN pivot sets ctx.d_info[0] = 1.
+void potrf(Ctx& c, const std::vector<SqMat>& mats);
+
+// B <- L^{-1} B (trans = false) or L^{-T} B (trans = true), L lower n x n.
+struct TrsmProb {
+    const double* L; int n; int ldl;
+    double* B; int ncols; int ldb;
+};
+void trsm_left(Ctx& c, bool trans, const std::vector<TrsmProb>& probs);
+
+// Thin QR of each m x r panel (m >= r): A <- Q (orthonormal columns),
+// R (r x r upper, leading dimension ldr) written in full (lower part zero).
+// Block classical Gram-Schmidt with reorthogonalisation across 64-column
+// blocks, TSQR (Householder tiles in LDS + binary reduction tree) inside.
+struct QrPanel {
+    double* A; int m; int r; int lda;
+    double* R; int ldr;
+};
+void qr_panels(Ctx& c, const std::vector<QrPanel>& panels);
+
+// Batched 2-D copy / transpose / elementwise helpers.
+struct Copy2D {
+    const double* src; int lds;
+    double* dst; int ldd;
+    int m, n;
+    double alpha;   // dst = alpha * src (transposed: dst(j,i) = alpha * src(i,j))
+};
+void copy2d(Ctx& c, const std::vector<Copy2D>& probs, bool transpose = false);
+// dst(i,j) = a * dst(i,j) + b * src(i,j) for i < m, j < n
+struct Axpby2D {
+    const double* src; int lds;
+    double* dst; int ldd;
+    int m, n;
+    double a, b;
+};
+void axpby2d(Ctx& c, const std::vector<Axpby2D>& probs);
+// dst = identity (n x n) ; or dst(i,i) += v
+struct DiagOp {
+    double* A; int n; int ld;
+    double v;
+};
+void set_identity(Ctx& c, const std::vector<DiagOp>& mats);   // A = v * I
+void add_diag(Ctx& c, const std::vector<DiagOp>& mats);       // A += v * I
+
+// out[j] = sum_i log(A_j(i,i)) for each matrix j (deterministic order).
+void log_diag_sums(Ctx& c, const std::vector<SqMat>& mats, double* d_out);
+// out = sum of d_in[0..count) in a fixed order (one workgroup).
+void sum_vector(Ctx& c, const double* d_in, int count, double* d_out);
+// out[j] = dot(X[:,j], Y[:,j]) for j < ncols (deterministic order).
+void column_dots(Ctx& c, const double* X, int ldx, const double* Y, int ldy,
+                 int m, int ncols, double* d_out);
+
+}  // namespace kmh

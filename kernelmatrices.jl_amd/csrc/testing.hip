@@ -1,0 +1,96 @@
+// Test hooks (include/kmhodlr_testing.h): one-matrix wrappers around the
+// batched primitives, host buffers in and out.
+#include "kmhodlr_testing.h"
+#include "la.h"
+
+
+namespace {
+
+struct Dev {
+    kmh::Ctx& c;
+    double* p = nullptr;
+    size_t bytes = 0;
+    Dev(kmh::Ctx& cc, const double* host, size_t count) : c(cc) {
+        bytes = std::max<size_t>(8, count * sizeof(double));
+        p = static_cast<double*>(c.scratch(bytes));
+        if (host && count) KMH_HIP(hipMemcpyAsync(p, host, count * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    }
+    void to_host(double* host, size_t count) {
+        KMH_HIP(hipMemcpyAsync(host, p, count * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    }
+    ~Dev() { c.scratch_free(p); }
+};
+
+}  // namespace
+
+#define TRY_(...)                                                   \
+    try {                                                           \
+        __VA_ARGS__;                                                \
+        return KMH_OK;                                              \
+    } catch (const kmh::Error& e) {                                 \
+        kmh::set_last_error(e.what());                              \
+        return e.code;                                              \
+    } catch (const std::exception& e) {                             \
+        kmh::set_last_error(e.what());                              \
+        return KMH_E_DEVICE;                                        \
+    }
+
+extern "C" {
+
+int kmh_test_gemm(kmh_ctx* ctx, int32_t ta, int32_t tb, int32_t m, int32_t n, int32_t k, const double* A, int32_t lda,
+                  const double* B, int32_t ldb, double* Cm, int32_t ldc, double alpha, double beta, int32_t lower) {
+    TRY_({
+        kmh::Ctx& c = ctx->c;
+        c.set_device();
+        const size_t na = (size_t)lda * (ta ? m : k), nb = (size_t)ldb * (tb ? k : n), nc = (size_t)ldc * n;
+        Dev a(c, A, na), b(c, B, nb), cc(c, Cm, nc);
+        kmh::gemm(c, ta != 0, tb != 0, {{m, n, k, a.p, lda, b.p, ldb, cc.p, ldc, alpha, beta, lower}});
+        cc.to_host(Cm, nc);
+        c.sync();
+    })
+}
+
+int kmh_test_potrf(kmh_ctx* ctx, int32_t n, double* A, int32_t lda) {
+    TRY_({
+        kmh::Ctx& c = ctx->c;
+        c.set_device();
+        KMH_HIP(hipMemsetAsync(c.d_info, 0, sizeof(int), c.stream));
+        Dev a(c, A, (size_t)lda * n);
+        kmh::potrf(c, {{a.p, n, lda}});
+        a.to_host(A, (size_t)lda * n);
+        int info = 0;
+        KMH_HIP(hipMemcpyAsync(&info, c.d_info, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        if (info) {
+            KMH_HIP(hipMemsetAsync(c.d_info, 0, sizeof(int), c.stream));
+            c.sync();
+            throw kmh::Error(KMH_E_NOT_PD, "PosDefException: matrix is not positive definite");
+        }
+    })
+}
+
+int kmh_test_trsm(kmh_ctx* ctx, int32_t trans, int32_t n, const double* Lm, int32_t ldl, int32_t ncols, double* B,
+                  int32_t ldb) {
+    TRY_({
+        kmh::Ctx& c = ctx->c;
+        c.set_device();
+        Dev l(c, Lm, (size_t)ldl * n), b(c, B, (size_t)ldb * ncols);
+        kmh::trsm_left(c, trans != 0, {{l.p, n, ldl, b.p, ncols, ldb}});
+        b.to_host(B, (size_t)ldb * ncols);
+        c.sync();
+    })
+}
+
+int kmh_test_qr(kmh_ctx* ctx, int32_t m, int32_t r, double* A, int32_t lda, double* R, int32_t ldr) {
+    TRY_({
+        kmh::Ctx& c = ctx->c;
+        c.set_device();
+        Dev a(c, A, (size_t)lda * r), rr(c, nullptr, (size_t)ldr * r);
+        kmh::qr_panels(c, {{a.p, m, r, lda, rr.p, ldr}});
+        a.to_host(A, (size_t)lda * r);
+        rr.to_host(R, (size_t)ldr * r);
+        c.sync();
+    })
+}
+
+}  // extern "C"

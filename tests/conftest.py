@@ -3,6 +3,15 @@ import sys
 
 import pytest
 
+# numpy and scipy each bundle an OpenBLAS; their thread pools contend when
+# calls alternate, so the oracle runs with one BLAS thread (the reference's
+# own BLAS.set_num_threads(1), test/runtests.jl:9).
+try:
+    from threadpoolctl import threadpool_limits
+    threadpool_limits(1)
+except Exception:  # pragma: no cover
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "kernelmatrices.jl_amd")
 for p in (ROOT, PKG):

@@ -1,0 +1,178 @@
+"""GPU parity of the HODLR path against the CPU oracle (same seeded inputs),
+called through the C ABI (via the kmhodlr binding).
+
+Bars: per-block ranks equal; full(HK) elementwise; logdet within 1e-10
+relative; solve residual <= 1e-9 against full(HK); factor applies as
+test/runtests.jl:28-45."""
+import numpy as np
+import pytest
+
+from oracle import hodlr as H
+from oracle import kernels as kk
+from oracle.kdsort import kd_sort
+
+pytestmark = pytest.mark.gpu
+
+FAM = {kk.RATIONAL2: "rational_kernel", kk.CAUCHY: "cauchy_kernel", kk.MATERN12: "matern12",
+       kk.MATERN32: "matern32"}
+
+
+def gpu_kernel(km, fam, nugget):
+    return getattr(km, FAM[fam])(nugget)
+
+
+def build_pair(km, x, fam, params, nugget, ep, maxrank, lvl_kind, lv, nystrom):
+    ko = kk.Kernel(fam, params, nugget=nugget)
+    Ko = H.KernelMatrix(x, x, ko)
+    lo = H.FixedLevel(lv) if lvl_kind == "fixed" else H.LogLevel(lv)
+    HKo = H.KernelHODLR(Ko, ep, maxrank, lo, nystrom=nystrom)
+    Kg = km.KernelMatrix(x, x, params, gpu_kernel(km, fam, nugget))
+    lg = km.FixedLevel(lv) if lvl_kind == "fixed" else km.LogLevel(lv)
+    HKg = km.KernelHODLR(Kg, ep, maxrank, lg, nystrom=nystrom)
+    return Ko, HKo, Kg, HKg
+
+
+CASES = {
+    # config-1 case: n = 1024 N(0,I) points, README kernel, ACA 1e-12, LogLevel(8)
+    "cfg1_aca": dict(n=1024, pts="normal", fam=kk.RATIONAL2, params=(1.0, 1.0), nugget=1.0,
+                     ep=1e-12, maxrank=0, lvl=("log", 8), nys=False),
+    "matern32_aca_4096": dict(n=4096, pts="unif", fam=kk.MATERN32, params=(1.0, 0.1), nugget=1e-6,
+                              ep=1e-10, maxrank=0, lvl=("fixed", 4), nys=False),
+    "matern32_nys72_4096": dict(n=4096, pts="unif", fam=kk.MATERN32, params=(1.0, 0.1), nugget=1e-6,
+                                ep=1e-10, maxrank=72, lvl=("fixed", 4), nys=True),
+    "matern12_aca_uneven": dict(n=1000, pts="unif", fam=kk.MATERN12, params=(1.0, 0.2), nugget=1e-6,
+                                ep=1e-8, maxrank=0, lvl=("fixed", 3), nys=False),
+    "matern12_nys_uneven": dict(n=1000, pts="unif", fam=kk.MATERN12, params=(1.0, 0.2), nugget=1e-6,
+                                ep=0.0, maxrank=30, lvl=("fixed", 3), nys=True),
+    "cauchy_runtests": dict(n=1024, pts="box10", fam=kk.CAUCHY, params=(3.0, 2.0), nugget=1e-12,
+                            ep=2.2e-16, maxrank=72, lvl=("log", 8), nys=True),
+    "aca_maxrank": dict(n=2048, pts="unif", fam=kk.MATERN32, params=(1.0, 0.1), nugget=1e-6,
+                        ep=1e-12, maxrank=20, lvl=("fixed", 3), nys=False),
+}
+
+
+def points(kind, n, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "normal":
+        return rng.standard_normal((n, 2))
+    if kind == "box10":
+        return kd_sort(rng.random((n, 2)) * 10.0)
+    return kd_sort(rng.random((n, 2)))
+
+
+@pytest.fixture(scope="module")
+def km(gpu_lib):
+    import kmhodlr
+    return kmhodlr
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_build_factor_solve_logdet_parity(km, name):
+    c = CASES[name]
+    print("case", name, flush=True)
+    x = points(c["pts"], c["n"], seed=len(name))
+    Ko, HKo, Kg, HKg = build_pair(km, x, c["fam"], c["params"], c["nugget"], c["ep"], c["maxrank"],
+                                  c["lvl"][0], c["lvl"][1], c["nys"])
+    n = c["n"]
+    # ranks (constructor.jl:42-53)
+    assert HKg.ranks() == HKo.ranks
+    # full(HK) before factorization
+    Fo = HKo.full()
+    Fg = HKg.full()
+    scale = np.abs(Fo).max()
+    assert np.abs(Fg - Fo).max() <= 1e-12 * scale
+    rng = np.random.default_rng(99)
+    y = rng.standard_normal(n)
+    # unfactorized matvec (baseoverloads.jl:152-169)
+    assert np.allclose(HKg.matvec(y), Fo @ y, rtol=0, atol=1e-11 * np.abs(Fo @ y).max())
+    Y3 = rng.standard_normal((n, 3))
+    assert np.allclose(HKg.matvec(Y3), Fo @ Y3, rtol=0, atol=1e-11 * np.abs(Fo @ Y3).max())
+    # factorize
+    H.symmetricfactorize(HKo)
+    km.symmetricfactorize(HKg)
+    assert HKg.is_factorized and HKg.U is None
+    ldo = HKo.logdet()
+    ldg = HKg.logdet()
+    assert abs(ldg - ldo) <= 1e-10 * abs(ldo), (ldg, ldo)
+    ldd = np.linalg.slogdet(Fo)[1]
+    assert abs(ldg - ldd) <= 1e-9 * abs(ldd)
+    xg = HKg.solve(y)
+    res = np.linalg.norm(Fo @ xg - y) / np.linalg.norm(y)
+    assert res <= 1e-9, res
+    xo = HKo.solve(y)
+    assert np.linalg.norm(xg - xo) <= 1e-7 * np.linalg.norm(xo)
+    # multiple right-hand sides
+    Xg = HKg.solve(Y3)
+    assert np.linalg.norm(Fo @ Xg - Y3) <= 1e-9 * np.linalg.norm(Y3)
+    # factor applies (test/runtests.jl:28-45)
+    W = HKg.W
+    for a, b in ((W.ldiv(y), HKo.W.ldiv(y)), (W.adjoint().ldiv(y), HKo.W.ldiv_t(y)),
+                 (W.mul(y), HKo.W.mul(y)), (W.adjoint().mul(y), HKo.W.mul_t(y))):
+        assert np.linalg.norm(a - b) <= 1e-9 * np.linalg.norm(b)
+    # factorized matvec is W W^T x
+    assert np.allclose(HKg.matvec(y), Fo @ y, rtol=0, atol=1e-9 * np.abs(Fo @ y).max())
+    # negloglik
+    nll = km.negloglik(HKg, y)
+    assert abs(nll - H.negloglik(HKo, y)) <= 1e-10 * abs(H.negloglik(HKo, y))
+
+
+def test_dense_cholesky_oracle_8192(km):
+    """Dense FP64 Cholesky of full(HK) at n = 8192 (Nystrom r = 72)."""
+    n = 8192
+    x = points("unif", n, 5)
+    Kg = km.KernelMatrix(x, x, (1.0, 0.1), km.matern32(1e-6))
+    HKg = km.KernelHODLR(Kg, 0.0, 72, km.FixedLevel(5), nystrom=True)
+    F = HKg.full()
+    km.symmetricfactorize(HKg)
+    Lc = np.linalg.cholesky(F)
+    ld = 2.0 * np.sum(np.log(np.diag(Lc)))
+    assert abs(HKg.logdet() - ld) <= 1e-10 * abs(ld)
+    y = np.random.default_rng(1).standard_normal(n)
+    xg = HKg.solve(y)
+    assert np.linalg.norm(F @ xg - y) <= 1e-9 * np.linalg.norm(y)
+    # approximation error against the exact kernel matrix, stated
+    Ko = H.KernelMatrix(x, x, kk.Kernel(kk.MATERN32, (1.0, 0.1), nugget=1e-6)).full()
+    rel = np.linalg.norm(F - Ko) / np.linalg.norm(Ko)
+    print("nystrom-72 relative Frobenius error vs exact K at n=8192: %.3e" % rel)
+    assert rel < 1e-2
+
+
+def test_errors_match_reference(km):
+    n = 600
+    x = points("unif", n, 3)
+    Kg = km.KernelMatrix(x, x, (1.0, 0.2), km.matern32(1e-6))
+    HKg = km.KernelHODLR(Kg, 1e-8, 0, km.FixedLevel(2))
+    with pytest.raises(km.ErrorException, match="No solves without factorization."):
+        HKg.solve(np.ones(n))
+    with pytest.raises(km.ErrorException, match="No logdet without factorization."):
+        HKg.logdet()
+    with pytest.raises(km.ErrorException, match="nystrom rank is too big"):
+        km.KernelHODLR(Kg, 0.0, 149, km.FixedLevel(2), nystrom=True)
+    with pytest.raises(km.ErrorException, match=">=2"):
+        km.KernelHODLR(Kg, 0.0, 1, km.FixedLevel(2), nystrom=True)
+    with pytest.raises(km.ErrorException, match="Incompatible maxrank"):
+        km.KernelHODLR(Kg, 1e-8, 1, km.FixedLevel(2))
+    bad = km.KernelMatrix(x, x, (1.0, 0.2), km.matern32(-2.0))
+    HKb = km.KernelHODLR(bad, 1e-8, 0, km.FixedLevel(2))
+    with pytest.raises(km.PosDefException):
+        km.symmetricfactorize(HKb)
+    km.symmetricfactorize(HKg)
+    with pytest.raises(km.ErrorException, match="already factorized"):
+        km.symmetricfactorize(HKg)
+
+
+@pytest.mark.parametrize("fam", [kk.MATERN12, kk.MATERN32])
+def test_stoch_gradient_parity(km, fam):
+    n, nv = 2048, 16
+    x = points("unif", n, 11)
+    params = (1.3, 0.12)
+    Ko, HKo, Kg, HKg = build_pair(km, x, fam, params, 1e-6, 0.0, 40, "fixed", 3, True)
+    H.symmetricfactorize(HKo)
+    km.symmetricfactorize(HKg)
+    rng = np.random.default_rng(5)
+    y = rng.standard_normal(n)
+    vecs = [rng.choice([-1.0, 1.0], size=n) for _ in range(nv)]
+    go = H.stoch_gradient(Ko, HKo, y, [Ko.kernel.derivative(kk.D1), Ko.kernel.derivative(kk.D2)], vecs)
+    k = Kg.kernel
+    gg = km.stoch_gradient(Kg, HKg, y, [k.d1, k.d2], vecs)
+    assert np.allclose(gg, go, rtol=1e-9, atol=0), (gg, go)

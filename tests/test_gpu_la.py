@@ -1,0 +1,127 @@
+"""GPU checks of the batched FP64 primitives (GEMM on MFMA, Cholesky,
+triangular solve, panel QR) against numpy, through the test hooks."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx(gpu_lib):
+    import kmhodlr
+    return kmhodlr.Context(0)
+
+
+def _gemm(ctx, ta, tb, A, B, Cm, alpha=1.0, beta=0.0, lower=0):
+    lib = ctx.lib
+    A = np.asfortranarray(A)
+    B = np.asfortranarray(B)
+    Cm = np.asfortranarray(Cm.copy())
+    m, n = Cm.shape
+    k = A.shape[0] if ta else A.shape[1]
+    rc = lib.kmh_test_gemm(ctx.ptr, ta, tb, m, n, k, A.ctypes.data, A.shape[0], B.ctypes.data, B.shape[0],
+                           Cm.ctypes.data, m, alpha, beta, lower)
+    assert rc == 0, lib.last_error()
+    return Cm
+
+
+@pytest.mark.parametrize("ta", [0, 1])
+@pytest.mark.parametrize("tb", [0, 1])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 5, 3), (64, 64, 16), (130, 70, 33), (72, 65, 5000)])
+def test_gemm_matches_numpy(ctx, ta, tb, m, n, k):
+    print("gemm", ta, tb, m, n, k, flush=True)
+    rng = np.random.default_rng(m * 1000 + n * 10 + k)
+    A = rng.standard_normal((k, m) if ta else (m, k))
+    B = rng.standard_normal((n, k) if tb else (k, n))
+    C0 = rng.standard_normal((m, n))
+    got = _gemm(ctx, ta, tb, A, B, C0, alpha=-0.75, beta=0.5)
+    ref = -0.75 * ((A.T if ta else A) @ (B.T if tb else B)) + 0.5 * C0
+    assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()) * max(1, k) ** 0.5
+
+
+def test_gemm_beta_zero_ignores_nan(ctx):
+    A = np.ones((8, 4))
+    B = np.ones((4, 8))
+    C0 = np.full((8, 8), np.nan)
+    got = _gemm(ctx, 0, 0, A, B, C0, 1.0, 0.0)
+    assert np.all(got == 4.0)
+
+
+def test_gemm_lower_skips_upper_tiles(ctx):
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((200, 7))
+    C0 = np.zeros((200, 200))
+    got = _gemm(ctx, 0, 1, A, A, C0, 1.0, 0.0, lower=1)
+    ref = A @ A.T
+    assert np.allclose(np.tril(got), np.tril(ref), atol=1e-13)
+    assert np.all(got[:64, 128:] == 0.0)
+
+
+@pytest.mark.parametrize("n", [1, 5, 64, 65, 200, 257])
+def test_potrf(ctx, n):
+    print("potrf", n, flush=True)
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n, n))
+    A = np.asfortranarray(X @ X.T + n * np.eye(n))
+    B = A.copy(order="F")
+    rc = ctx.lib.kmh_test_potrf(ctx.ptr, n, B.ctypes.data, n)
+    assert rc == 0, ctx.lib.last_error()
+    Lr = np.linalg.cholesky(A)
+    assert np.allclose(B, Lr, rtol=0, atol=1e-12 * np.abs(Lr).max())
+    assert np.all(np.triu(B, 1) == 0.0)
+
+
+def test_potrf_not_pd(ctx):
+    A = np.asfortranarray(np.diag([1.0, -1.0, 2.0]))
+    rc = ctx.lib.kmh_test_potrf(ctx.ptr, 3, A.ctypes.data, 3)
+    assert rc == 2 and "PosDef" in ctx.lib.last_error()
+    # the flag is cleared: a good matrix factors again
+    B = np.asfortranarray(np.eye(3))
+    assert ctx.lib.kmh_test_potrf(ctx.ptr, 3, B.ctypes.data, 3) == 0
+
+
+@pytest.mark.parametrize("trans", [0, 1])
+@pytest.mark.parametrize("n,ncols", [(3, 2), (64, 300), (257, 70)])
+def test_trsm(ctx, trans, n, ncols):
+    print("trsm", trans, n, ncols, flush=True)
+    rng = np.random.default_rng(n + ncols)
+    Lm = np.asfortranarray(np.tril(rng.standard_normal((n, n))) + 4 * np.eye(n))
+    B = np.asfortranarray(rng.standard_normal((n, ncols)))
+    X = B.copy(order="F")
+    rc = ctx.lib.kmh_test_trsm(ctx.ptr, trans, n, Lm.ctypes.data, n, ncols, X.ctypes.data, n)
+    assert rc == 0, ctx.lib.last_error()
+    ref = np.linalg.solve(Lm.T if trans else Lm, B)
+    assert np.allclose(X, ref, rtol=1e-11, atol=1e-11)
+
+
+@pytest.mark.parametrize("m,r", [(5, 3), (64, 64), (256, 72), (300, 64), (1000, 130), (5000, 72), (33000, 70)])
+def test_qr(ctx, m, r):
+    print("qr", m, r, flush=True)
+    rng = np.random.default_rng(m + r)
+    A0 = rng.standard_normal((m, r)) * np.logspace(0, -8, r)    # graded columns
+    A = np.asfortranarray(A0.copy())
+    R = np.zeros((r, r), order="F")
+    rc = ctx.lib.kmh_test_qr(ctx.ptr, m, r, A.ctypes.data, m, R.ctypes.data, r)
+    assert rc == 0, ctx.lib.last_error()
+    assert np.abs(A.T @ A - np.eye(r)).max() < 1e-12
+    assert np.abs(A @ R - A0).max() <= 1e-13 * np.abs(A0).max()
+    assert np.all(np.tril(R, -1) == 0.0)
+
+
+def test_qr_rank_deficient(ctx):
+    """Zero and repeated columns inside a 64-column block (Householder
+    tiles) and numerically dependent columns in a later block still give
+    orthonormal Q."""
+    rng = np.random.default_rng(7)
+    m, r = 700, 80
+    A0 = rng.standard_normal((m, r))
+    A0[:, 10] = 0.0
+    A0[:, 50] = A0[:, 3]
+    A0[:, 70:] = A0[:, :10] @ rng.standard_normal((10, 10)) + 1e-15 * rng.standard_normal((m, 10))
+    A = np.asfortranarray(A0.copy())
+    R = np.zeros((r, r), order="F")
+    assert ctx.lib.kmh_test_qr(ctx.ptr, m, r, A.ctypes.data, m, R.ctypes.data, r) == 0
+    assert np.abs(A.T @ A - np.eye(r)).max() < 1e-10
+    assert np.abs(A @ R - A0).max() <= 1e-13 * np.abs(A0).max()
