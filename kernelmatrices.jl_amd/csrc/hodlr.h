@@ -36,6 +36,7 @@ struct Hodlr {
     DevBuf lm;                                   // landmark points, x[0..r), y[0..r)
     DevBuf kpp;                                  // chol(K(P,P)) lower, r x r
     DevBuf kxp;                                  // K(x, P), n x r, ld n
+    DevBuf ynys;                                 // K(x, P) K(P,P)^{-1}, n x r, ld n
 
     // factor
     bool factorized = false;
@@ -44,10 +45,23 @@ struct Hodlr {
     DevBuf G;                                    // I - chol(C)^{-1} per block
     double logdet = 0.0;
 
+    bool nystrom() const { return opts.method == KMH_NYSTROM; }
+    // U (rows a0..b1) and V (rows c0..d1) of block k at level l before
+    // factorization, leading dimension n.  Nystrom blocks all share the two
+    // panels K(x,P) and K(x,P)K(P,P)^{-1} (src/factorizations.jl:86-94);
+    // ACA blocks live in P.
+    const double* Uptr(int l, int k) const {
+        const Block& b = blocks[l][k];
+        return nystrom() ? kxp.as<double>() + b.a0 : P.as<double>() + (size_t)coff[l] * n + b.a0;
+    }
+    const double* Vptr(int l, int k) const {
+        const Block& b = blocks[l][k];
+        return nystrom() ? ynys.as<double>() + b.c0 : P.as<double>() + (size_t)coff[l] * n + b.c0;
+    }
     int64_t leaf_size(int i) const { return leaf_off[i + 1] - leaf_off[i]; }
     int nleaves() const { return (int)leaf_off.size() - 1; }
     size_t device_bytes() const {
-        return pts.bytes + leaves.bytes + P.bytes + lm.bytes + kpp.bytes + kxp.bytes + X.bytes + G.bytes;
+        return pts.bytes + leaves.bytes + P.bytes + lm.bytes + kpp.bytes + kxp.bytes + ynys.bytes + X.bytes + G.bytes;
     }
 };
 

@@ -115,25 +115,14 @@ static void build_nystrom(Hodlr& H) {
     copy2d(c, {{H.kxp.as<double>(), (int)n, yt, r, (int)n, r, 1.0}}, true);
     trsm_left(c, false, {{H.kpp.as<double>(), r, r, yt, (int)n, r}});
     trsm_left(c, true, {{H.kpp.as<double>(), r, r, yt, (int)n, r}});
+    H.ynys.alloc(&c, (size_t)n * r * sizeof(double));
+    copy2d(c, {{yt, r, H.ynys.as<double>(), (int)n, r, (int)n, 1.0}}, true);
     H.rpad.assign(H.L, r);
     H.coff.assign(H.L + 1, 0);
     for (int l = 0; l < H.L; ++l) H.coff[l + 1] = H.coff[l] + r;
     H.Rtot = H.coff[H.L];
-    H.P.alloc(&c, (size_t)n * H.Rtot * sizeof(double));
-    double* P = H.P.as<double>();
-    std::vector<Copy2D> uc, vc;
     H.ranks.assign(H.L, {});
-    for (int l = 0; l < H.L; ++l) {
-        double* Pl = P + (size_t)H.coff[l] * n;
-        for (const auto& b : H.blocks[l]) {
-            // U = K(rows, P) on the second child, V = (F \ K(P, cols))^T on the first
-            uc.push_back({H.kxp.as<double>() + b.a0, (int)n, Pl + b.a0, (int)n, (int)(b.b1 - b.a0), r, 1.0});
-            vc.push_back({yt + (size_t)b.c0 * r, r, Pl + b.c0, (int)n, r, (int)(b.d1 - b.c0), 1.0});
-            H.ranks[l].push_back(r);
-        }
-    }
-    copy2d(c, uc);
-    copy2d(c, vc, true);
+    for (int l = 0; l < H.L; ++l) H.ranks[l].assign(H.blocks[l].size(), r);
     c.scratch_free(yt);
 }
 
@@ -371,8 +360,23 @@ void hodlr_factorize(Hodlr& H) {
     for (int l = 0; l < L; ++l) nblk += (int)H.blocks[l].size();
     double* parts = static_cast<double*>(c.scratch((size_t)(nl + nblk + 1) * sizeof(double)));
     log_diag_sums(c, lm, parts);
-    // leaf inverse on every level's U and V (symfact.jl:21)
-    if (L > 0) leaf_trsm(H, false, H.P.as<double>(), n, (int)H.Rtot);
+    // Nystrom: every block's U and V are row ranges of A = K(x,P) and
+    // B = K(x,P)K(P,P)^{-1}, and every block factor W_l is block-diagonal over
+    // whole nodes, so the transformed panels of all higher levels stay row
+    // ranges of W_l^{-1}...W_1^{-1} L^{-1} [A B]: the cascade updates these
+    // 2r columns instead of every higher level's panel (same W).
+    double* AB = nullptr;
+    const int r = H.nys_r;
+    if (H.nystrom() && L > 0) {
+        H.P.alloc(&c, (size_t)n * H.Rtot * sizeof(double));
+        AB = static_cast<double*>(c.scratch((size_t)n * 2 * r * sizeof(double)));
+        copy2d(c, {{H.kxp.as<double>(), (int)n, AB, (int)n, (int)n, r, 1.0},
+                   {H.ynys.as<double>(), (int)n, AB + (size_t)n * r, (int)n, (int)n, r, 1.0}});
+        leaf_trsm(H, false, AB, n, 2 * r);
+    } else if (L > 0) {
+        // leaf inverse on every level's U and V (symfact.jl:21)
+        leaf_trsm(H, false, H.P.as<double>(), n, (int)H.Rtot);
+    }
     // per-level factors
     H.coff_blk.assign(L, {});
     size_t xtot = 0;
@@ -390,6 +394,15 @@ void hodlr_factorize(Hodlr& H) {
     for (int lv = 0; lv < L; ++lv) {
         const auto& bl = H.blocks[lv];
         double* P = H.P.as<double>() + (size_t)H.coff[lv] * n;
+        if (AB) {
+            // this level's panels: V = B rows (first child), U = A rows (second child)
+            std::vector<Copy2D> cp;
+            for (const auto& b : bl) {
+                cp.push_back({AB + (size_t)n * r + b.c0, (int)n, P + b.c0, (int)n, (int)(b.d1 - b.c0), r, 1.0});
+                cp.push_back({AB + b.a0, (int)n, P + b.a0, (int)n, (int)(b.b1 - b.a0), r, 1.0});
+            }
+            copy2d(c, cp);
+        }
         size_t rsz = 0;
         for (size_t k = 0; k < bl.size(); ++k) rsz += (size_t)3 * H.ranks[lv][k] * H.ranks[lv][k];
         double* Rb = static_cast<double*>(c.scratch(rsz * sizeof(double)));
@@ -437,10 +450,13 @@ void hodlr_factorize(Hodlr& H) {
         add_diag(c, minus1);                         // X = chol(C) - I
         c.scratch_free(Rb);
         // Woodbury update of every higher level (symfact.jl:34-37)
-        if (lv + 1 < L)
-            woodbury(H, lv, W_INV, H.P.as<double>() + (size_t)H.coff[lv + 1] * n, n,
-                     (int)(H.Rtot - H.coff[lv + 1]));
+        if (lv + 1 < L) {
+            if (AB) woodbury(H, lv, W_INV, AB, n, 2 * r);
+            else woodbury(H, lv, W_INV, H.P.as<double>() + (size_t)H.coff[lv + 1] * n, n,
+                          (int)(H.Rtot - H.coff[lv + 1]));
+        }
     }
+    if (AB) c.scratch_free(AB);
     double* dsum = parts + nl + nblk;
     sum_vector(c, parts, nl + nblk, dsum);
     double hs = 0.0;
@@ -507,13 +523,12 @@ void hodlr_matvec(Hodlr& H, const double* x, double* y, int nrhs) {
     std::vector<std::vector<GemmProb>> up(H.L);
     size_t off = 0;
     for (int l = 0; l < H.L; ++l) {
-        const double* P = H.P.as<double>() + (size_t)H.coff[l] * n;
         for (size_t k = 0; k < H.blocks[l].size(); ++k) {
             const auto& b = H.blocks[l][k];
             const int r = H.ranks[l][k];
             const int h1 = (int)(b.d1 - b.c0), h2 = (int)(b.b1 - b.a0);
-            const double* U = P + b.a0;
-            const double* V = P + b.c0;
+            const double* U = H.Uptr(l, (int)k);
+            const double* V = H.Vptr(l, (int)k);
             double* s1 = S + off;
             double* s2 = s1 + (size_t)r * nrhs;
             off += (size_t)2 * r * nrhs;
@@ -529,14 +544,19 @@ void hodlr_matvec(Hodlr& H, const double* x, double* y, int nrhs) {
     c.scratch_free(S);
 }
 
+// copy an h x r column block (ld n on the device) to a dense host array
+static void fetch_cols(Ctx& c, const double* src, int64_t ld, int64_t h, int r, double* dst) {
+    if (h <= 0 || r <= 0) return;
+    KMH_HIP(hipMemcpy2DAsync(dst, h * sizeof(double), src, ld * sizeof(double), h * sizeof(double), r,
+                             hipMemcpyDeviceToHost, c.stream));
+}
+
 void hodlr_full_host(Hodlr& H, double* out) {
     KMH_CHECK(!H.factorized, KMH_E_STATE, "full(HK) needs the unfactorized U and V.");
     Ctx& c = *H.ctx;
     const int64_t n = H.n;
     std::vector<double> lv(H.leaves.bytes / sizeof(double));
-    std::vector<double> P((size_t)n * H.Rtot);
     KMH_HIP(hipMemcpyAsync(lv.data(), H.leaves.p, H.leaves.bytes, hipMemcpyDeviceToHost, c.stream));
-    if (H.Rtot) KMH_HIP(hipMemcpyAsync(P.data(), H.P.p, P.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     c.sync();
     for (int i = 0; i < H.nleaves(); ++i) {
         const int64_t s = H.leaf_off[i], m = H.leaf_size(i);
@@ -545,16 +565,20 @@ void hodlr_full_host(Hodlr& H, double* out) {
             for (int64_t q = 0; q < m; ++q) out[(s + q) + (s + j) * n] = Lm[q + j * m];
     }
     for (int l = 0; l < H.L; ++l) {
-        const double* Pl = P.data() + (size_t)H.coff[l] * n;
         for (size_t k = 0; k < H.blocks[l].size(); ++k) {
             const auto& b = H.blocks[l][k];
             const int r = H.ranks[l][k];
-            for (int64_t j = b.c0; j < b.d1; ++j)
-                for (int64_t i = b.a0; i < b.b1; ++i) {
+            const int64_t h2 = b.b1 - b.a0, h1 = b.d1 - b.c0;
+            std::vector<double> U((size_t)h2 * r), V((size_t)h1 * r);
+            fetch_cols(c, H.Uptr(l, (int)k), n, h2, r, U.data());
+            fetch_cols(c, H.Vptr(l, (int)k), n, h1, r, V.data());
+            c.sync();
+            for (int64_t j = 0; j < h1; ++j)
+                for (int64_t i = 0; i < h2; ++i) {
                     double s = 0.0;
-                    for (int t = 0; t < r; ++t) s += Pl[i + (size_t)t * n] * Pl[j + (size_t)t * n];
-                    out[i + j * n] = s;
-                    out[j + i * n] = s;
+                    for (int t = 0; t < r; ++t) s += U[i + (size_t)t * h2] * V[j + (size_t)t * h1];
+                    out[(b.a0 + i) + (b.c0 + j) * n] = s;
+                    out[(b.c0 + j) + (b.a0 + i) * n] = s;
                 }
         }
     }
@@ -567,12 +591,8 @@ void hodlr_block_uv_host(Hodlr& H, int level, int block, double* U, double* V) {
     Ctx& c = *H.ctx;
     const auto& b = H.blocks[level - 1][block];
     const int r = H.ranks[level - 1][block];
-    const int64_t n = H.n;
-    const double* Pl = H.P.as<double>() + (size_t)H.coff[level - 1] * n;
-    KMH_HIP(hipMemcpy2DAsync(U, (b.b1 - b.a0) * sizeof(double), Pl + b.a0, n * sizeof(double),
-                             (b.b1 - b.a0) * sizeof(double), r, hipMemcpyDeviceToHost, c.stream));
-    KMH_HIP(hipMemcpy2DAsync(V, (b.d1 - b.c0) * sizeof(double), Pl + b.c0, n * sizeof(double),
-                             (b.d1 - b.c0) * sizeof(double), r, hipMemcpyDeviceToHost, c.stream));
+    fetch_cols(c, H.Uptr(level - 1, block), H.n, b.b1 - b.a0, r, U);
+    fetch_cols(c, H.Vptr(level - 1, block), H.n, b.d1 - b.c0, r, V);
     c.sync();
 }
 

@@ -708,6 +708,30 @@ def sdblock_mul_t(Bj, Bk, Bjk, src, Sp, Spj, Spk, Spjk):
     return out
 
 
+def sdblock_full_exact(Bj, Bk, Bjk, Sp, Spj, Spk, Spjk):
+    """The exact second parameter derivative of a Nystrom block
+    K1 S^-1 K2 (K1 = K(rows,P), S = K(P,P), K2 = K(P,cols)):
+    the eleven terms of d_k d_j (K1 S^-1 K2).  The reference's
+    SDBlock_mul (HODLR/src/utils.jl:110-132) flips the signs of the terms
+    from d_k(-K1 S^-1 S_j S^-1 K2) and uses K2 instead of K2_j in two
+    terms; this version is pinned by finite differences in the tests."""
+    c = lambda b: _cs(Sp, b)
+    K1, K2 = Bk.K1p, Bj.Kp2
+    K1j, K1k, K2j, K2k = Bj.K1pd, Bk.K1pd, Bj.Kp2d, Bk.Kp2d
+    out = Bjk.K1pjk @ c(K2)
+    out = out - K1j @ c(Spk @ c(K2))
+    out = out + K1j @ c(K2k)
+    out = out - K1k @ c(Spj @ c(K2))
+    out = out + K1 @ c(Spk @ c(Spj @ c(K2)))
+    out = out - K1 @ c(Spjk @ c(K2))
+    out = out + K1 @ c(Spj @ c(Spk @ c(K2)))
+    out = out - K1 @ c(Spj @ c(K2k))
+    out = out + K1k @ c(K2j)
+    out = out - K1 @ c(Spk @ c(K2j))
+    out = out + K1 @ c(Bjk.Kp2jk)
+    return out
+
+
 class SecondDerivative:
     """Leaves + blocks + S_jk of an exact second derivative
     (constructor.jl:100-121, stochastichessian.jl:38-41)."""
@@ -722,8 +746,9 @@ class SecondDerivative:
         self.Sjk = kernel_matrix(djk, lndmk, lndmk)
 
 
-def deriv2mul(DKj, DKk, D2, src):
-    """HODLR/src/utils.jl:157-177."""
+def deriv2mul(DKj, DKk, D2, src, exact=False):
+    """HODLR/src/utils.jl:157-177 (exact=False, the reference's block terms);
+    exact=True uses sdblock_full_exact for the off-diagonal blocks."""
     src = np.asarray(src, dtype=np.float64)
     tgt = np.zeros_like(src)
     for j, (a0, b1, c0, d1) in enumerate(DKj.leafindices):
@@ -731,8 +756,13 @@ def deriv2mul(DKj, DKk, D2, src):
     for j in range(len(DKj.B)):
         for k, (a0, b1, c0, d1) in enumerate(DKj.nonleafindices[j]):
             args = (DKj.B[j][k], DKk.B[j][k], D2.B[j][k])
-            tgt[c0:d1] += sdblock_mul_t(*args, src[a0:b1], DKj.S, DKj.Sj, DKk.Sj, D2.Sjk)
-            tgt[a0:b1] += sdblock_mul(*args, src[c0:d1], DKj.S, DKj.Sj, DKk.Sj, D2.Sjk)
+            if exact:
+                blk = sdblock_full_exact(*args, DKj.S, DKj.Sj, DKk.Sj, D2.Sjk)
+                tgt[c0:d1] += blk.T @ src[a0:b1]
+                tgt[a0:b1] += blk @ src[c0:d1]
+            else:
+                tgt[c0:d1] += sdblock_mul_t(*args, src[a0:b1], DKj.S, DKj.Sj, DKk.Sj, D2.Sjk)
+                tgt[a0:b1] += sdblock_mul(*args, src[c0:d1], DKj.S, DKj.Sj, DKk.Sj, D2.Sjk)
     return tgt
 
 
