@@ -154,6 +154,65 @@ int kmh_hodlr_solve_dev(kmh_hodlr* h, const double* b, double* x, int64_t nrhs);
 /* ---- logdet(HK)  HODLR/src/baseoverloads.jl:194-209 ---- */
 int kmh_hodlr_logdet(kmh_hodlr* h, double* out);
 
+/* ---- batched form: independent problems (different n, kernels, ranks)
+ * advance together, one grouped launch per step across all of them.  No
+ * reference counterpart: the reference loops problems one by one.  Each
+ * request gets its own status; a failing request does not stop the others. */
+typedef struct kmh_build_req {
+    const double* pts;      /* n x d row-major; host or device (on_device)    */
+    int64_t n;
+    int32_t d;
+    int32_t on_device;
+    kmh_kernel kernel;
+    kmh_build_opts opts;
+} kmh_build_req;
+int kmh_hodlr_build_batch(kmh_ctx* ctx, int32_t count, const kmh_build_req* reqs,
+                          kmh_hodlr** out, int32_t* status);
+int kmh_hodlr_factorize_batch(kmh_ctx* ctx, int32_t count, kmh_hodlr* const* hs, int32_t* status);
+/* x_i = K_i^{-1} b_i for device blocks (n_i x nrhs_i); b_i may equal x_i    */
+int kmh_hodlr_solve_batch_dev(kmh_ctx* ctx, int32_t count, kmh_hodlr* const* hs,
+                              const double* const* b, double* const* x, const int64_t* nrhs);
+
+/* ---- serving: continuous batching of GP likelihood requests ----
+ * One request = nll_objective (maxlikfunctions.jl:29-50): build, factorize,
+ * negloglik and, when derivative kernels are given, stoch_gradient.  No
+ * reference counterpart (the reference evaluates one objective at a time).
+ * kmh_server_step admits waiting requests whose HBM footprint fits the budget
+ * (0 = unlimited) while fewer than max_inflight run, then advances every
+ * in-flight request by one stage (build, leaf stage, one tree level, finish)
+ * with the stages of all requests in shared grouped launches.  Inputs are
+ * copied at submit; results are kept until read with kmh_server_result.    */
+#define KMH_SERVE_MAX_PARAMS 4
+typedef struct kmh_server kmh_server;
+typedef struct kmh_serve_req {
+    const double* pts;        /* n x 2 row-major, host                      */
+    int64_t n;
+    int32_t d;
+    int32_t ndk;              /* number of derivative kernels (0: no gradient) */
+    kmh_kernel kernel;
+    kmh_build_opts opts;
+    const double* y;          /* data vector, n                             */
+    const double* saa;        /* n x nv SAA vectors (column-major) if ndk > 0 */
+    int64_t nv;
+    kmh_kernel dks[KMH_SERVE_MAX_PARAMS];
+} kmh_serve_req;
+typedef struct kmh_serve_result {
+    int32_t status;           /* KMH_OK or the request's error              */
+    int32_t ngrad;
+    double  nll;              /* 0.5 logdet + 0.5 y' K^-1 y                 */
+    double  logdet;
+    double  grad[KMH_SERVE_MAX_PARAMS];
+    double  queue_us;         /* submit -> admission                        */
+    double  latency_us;       /* submit -> completion                       */
+} kmh_serve_result;
+int kmh_server_create(kmh_ctx* ctx, int32_t max_inflight, uint64_t hbm_budget_bytes, kmh_server** out);
+int kmh_server_destroy(kmh_server* srv);
+int kmh_server_submit(kmh_server* srv, const kmh_serve_req* req, int64_t* id);
+int kmh_server_step(kmh_server* srv, int32_t* completed);
+int kmh_server_pending(kmh_server* srv, int32_t* waiting, int32_t* inflight, uint64_t* reserved_bytes);
+/* KMH_E_STATE while the request is still queued or running                 */
+int kmh_server_result(kmh_server* srv, int64_t id, kmh_serve_result* out);
+
 /* ---- HODLR.stoch_gradient(K, HK, y, dfuns, vecs)  stochasticgradient.jl:30-45
  * dks: p derivative kernels; y: n; saa: n x nv column-major; grad: p        */
 int kmh_stoch_gradient(kmh_hodlr* h, const kmh_kernel* dks, int32_t p,

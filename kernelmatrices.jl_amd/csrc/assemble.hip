@@ -11,11 +11,10 @@
 #include <algorithm>
 
 #include "assemble.h"
-#include "kernelfn.h"
 
 namespace kmh {
 
-__global__ __launch_bounds__(256) void k_eval(const EvalDesc* __restrict__ ds, int nd, KDev k) {
+__global__ __launch_bounds__(256) void k_eval(const EvalDesc* __restrict__ ds, int nd) {
     __shared__ double cx[64], cy[64];
     int lo = 0, hi = nd - 1;
     const int t = blockIdx.x;
@@ -25,6 +24,7 @@ __global__ __launch_bounds__(256) void k_eval(const EvalDesc* __restrict__ ds, i
         else hi = mid - 1;
     }
     const EvalDesc d = ds[lo];
+    const KDev k = d.k;
     const int lt = t - d.tile0;
     const int tn = (d.n + 63) / 64;
     const int row0 = (lt / tn) * 64, col0 = (lt % tn) * 64;
@@ -46,6 +46,12 @@ __global__ __launch_bounds__(256) void k_eval(const EvalDesc* __restrict__ ds, i
 }
 
 void eval_blocks(Ctx& c, const kmh_kernel& kern, std::vector<EvalDesc> ds) {
+    const KDev k = to_kdev(kern);
+    for (auto& d : ds) d.k = k;
+    eval_blocks(c, std::move(ds));
+}
+
+void eval_blocks(Ctx& c, std::vector<EvalDesc> ds) {
     int tiles = 0;
     std::vector<EvalDesc> act;
     for (auto d : ds) {
@@ -56,7 +62,7 @@ void eval_blocks(Ctx& c, const kmh_kernel& kern, std::vector<EvalDesc> ds) {
     }
     if (act.empty()) return;
     EvalDesc* dd = c.upload_vec(act);
-    hipLaunchKernelGGL(k_eval, dim3(tiles), dim3(256), 0, c.stream, dd, (int)act.size(), to_kdev(kern));
+    hipLaunchKernelGGL(k_eval, dim3(tiles), dim3(256), 0, c.stream, dd, (int)act.size());
     KMH_HIP(hipGetLastError());
 }
 
@@ -152,13 +158,15 @@ __device__ void restricted_maxabs(const double* x, int len, const unsigned* mask
     else { *val = 0.0; *idx = ff; }
 }
 
-__global__ __launch_bounds__(AT) void k_aca(const double* __restrict__ px, const double* __restrict__ py,
-                                            KDev k, double rtol, AcaDesc* ds) {
+__global__ __launch_bounds__(AT) void k_aca(const AcaDesc* __restrict__ ds) {
     extern __shared__ char smem[];
     __shared__ AcaShared sh;
-    __shared__ int s_ctl[4];
     __shared__ double s_z;
     const AcaDesc d = ds[blockIdx.x];
+    const KDev k = d.k;
+    const double rtol = d.rtol;
+    const double* px = d.px;
+    const double* py = d.py;
     const int tid = threadIdx.x;
     const int wr = (d.nro + 31) / 32, wc = (d.ncl + 31) / 32;
     unsigned* urow = reinterpret_cast<unsigned*>(smem);
@@ -301,7 +309,6 @@ __global__ __launch_bounds__(AT) void k_aca(const double* __restrict__ px, const
         ++frnk;
     }
     if (tid == 0) { *d.status = status; *d.rank = frnk; }
-    (void)s_ctl;
 }
 
 size_t aca_shared_bytes(int nro, int ncl, int cap) {
@@ -309,8 +316,7 @@ size_t aca_shared_bytes(int nro, int ncl, int cap) {
     return ((words * 4 + 15) & ~size_t(15)) + (size_t)3 * cap * sizeof(double);
 }
 
-void aca_launch(Ctx& c, const kmh_kernel& kern, const double* px, const double* py, double rtol,
-                const std::vector<AcaDesc>& ds) {
+void aca_launch(Ctx& c, const std::vector<AcaDesc>& ds) {
     if (ds.empty()) return;
     size_t shm = 0;
     for (const auto& d : ds) shm = std::max(shm, aca_shared_bytes(d.nro, d.ncl, d.cap));
@@ -318,8 +324,7 @@ void aca_launch(Ctx& c, const kmh_kernel& kern, const double* px, const double* 
     AcaDesc* dd = c.upload_vec(ds);
     KMH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_aca),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    hipLaunchKernelGGL(k_aca, dim3((unsigned)ds.size()), dim3(AT), shm, c.stream, px, py, to_kdev(kern), rtol,
-                       dd);
+    hipLaunchKernelGGL(k_aca, dim3((unsigned)ds.size()), dim3(AT), shm, c.stream, dd);
     KMH_HIP(hipGetLastError());
 }
 

@@ -300,6 +300,78 @@ int kmh_hodlr_logdet(kmh_hodlr* h, double* out) {
     })
 }
 
+int kmh_hodlr_build_batch(kmh_ctx* ctx, int32_t count, const kmh_build_req* reqs, kmh_hodlr** out,
+                          int32_t* status) {
+    KMH_TRY({
+        KMH_CHECK(ctx && reqs && out && status && count >= 0, KMH_E_BAD_ARG, "null argument");
+        std::vector<kmh::BuildReq> rq;
+        for (int i = 0; i < count; ++i)
+            rq.push_back({reqs[i].pts, reqs[i].on_device != 0, reqs[i].n, reqs[i].d, reqs[i].kernel, reqs[i].opts});
+        std::vector<kmh::Status> st;
+        auto hs = kmh::hodlr_build_batch(ctx->c, rq, st);
+        int first = KMH_OK;
+        for (int i = 0; i < count; ++i) {
+            status[i] = st[i].code;
+            out[i] = nullptr;
+            if (hs[i]) {
+                out[i] = new kmh_hodlr();
+                out[i]->h.reset(hs[i]);
+            } else if (first == KMH_OK) {
+                first = st[i].code;
+                kmh::set_last_error(st[i].msg);
+            }
+        }
+        if (first != KMH_OK) return first;
+    })
+}
+
+int kmh_hodlr_factorize_batch(kmh_ctx* ctx, int32_t count, kmh_hodlr* const* hs, int32_t* status) {
+    KMH_TRY({
+        KMH_CHECK(ctx && hs && status && count >= 0, KMH_E_BAD_ARG, "null argument");
+        std::vector<kmh::Hodlr*> v;
+        for (int i = 0; i < count; ++i) {
+            KMH_CHECK(hs[i] && hs[i]->h && hs[i]->h->ctx == &ctx->c, KMH_E_BAD_ARG, "handle from another context");
+            v.push_back(hs[i]->h.get());
+        }
+        std::vector<kmh::Status> st;
+        kmh::hodlr_factorize_batch(ctx->c, v, st);
+        int first = KMH_OK;
+        for (int i = 0; i < count; ++i) {
+            status[i] = st[i].code;
+            if (st[i].code != KMH_OK && first == KMH_OK) {
+                first = st[i].code;
+                kmh::set_last_error(st[i].msg);
+            }
+        }
+        if (first != KMH_OK) return first;
+    })
+}
+
+int kmh_hodlr_solve_batch_dev(kmh_ctx* ctx, int32_t count, kmh_hodlr* const* hs, const double* const* b,
+                              double* const* x, const int64_t* nrhs) {
+    KMH_TRY({
+        KMH_CHECK(ctx && hs && b && x && nrhs && count >= 0, KMH_E_BAD_ARG, "null argument");
+        kmh::Ctx& c = ctx->c;
+        c.set_device();
+        std::vector<kmh::Hodlr*> v;
+        std::vector<double*> T;
+        std::vector<int> k;
+        std::vector<kmh::Copy2D> cp;
+        for (int i = 0; i < count; ++i) {
+            KMH_CHECK(hs[i] && hs[i]->h && hs[i]->h->ctx == &c, KMH_E_BAD_ARG, "handle from another context");
+            kmh::Hodlr& H = *hs[i]->h;
+            KMH_CHECK(H.factorized, KMH_E_NOT_FACTORIZED, "No solves without factorization.");
+            KMH_CHECK(nrhs[i] >= 1 && b[i] && x[i], KMH_E_BAD_ARG, "invalid right-hand side");
+            if (b[i] != x[i]) cp.push_back({b[i], (int)H.n, x[i], (int)H.n, (int)H.n, (int)nrhs[i], 1.0});
+            v.push_back(&H);
+            T.push_back(x[i]);
+            k.push_back((int)nrhs[i]);
+        }
+        kmh::copy2d(c, cp);
+        kmh::hodlr_solve_batch(c, v, T, k);
+    })
+}
+
 int kmh_stoch_gradient(kmh_hodlr* h, const kmh_kernel* dks, int32_t p, const double* y, const double* saa,
                        int64_t nv, double* grad) {
     KMH_TRY({

@@ -1,5 +1,8 @@
-// HODLR matrix held on one device: tree, leaves, off-diagonal panels, factor.
+// HODLR matrices held on one device: tree, leaves, off-diagonal panels,
+// factor.  Every stage runs over a list of handles, so several independent
+// problems (of different sizes, kernels and ranks) share each grouped launch.
 #pragma once
+#include <string>
 #include <vector>
 
 #include "common.h"
@@ -26,9 +29,10 @@ struct Hodlr {
     kmh_build_opts opts{};
 
     DevBuf pts;                                  // x[0..n), y[0..n)
+    DevBuf info;                                 // device failure flag (Cholesky)
     DevBuf leaves;                               // leaf i at leaf_mat_off[i], ld = size
     std::vector<int64_t> leaf_mat_off;
-    DevBuf P;                                    // n x Rtot, ld n: U/V (then Q) per level
+    DevBuf P;                                    // n x Rtot, ld n: ACA U/V, then Q per level
 
     // Nystrom state (kept for the derivative HODLR)
     int nys_r = 0;
@@ -45,6 +49,11 @@ struct Hodlr {
     DevBuf G;                                    // I - chol(C)^{-1} per block
     double logdet = 0.0;
 
+    // transient state of a factorization in progress
+    double* AB = nullptr;                        // Nystrom cascade panels (n x 2r)
+    double* parts = nullptr;                     // log-diagonal partial sums
+    int nparts = 0;
+
     bool nystrom() const { return opts.method == KMH_NYSTROM; }
     // U (rows a0..b1) and V (rows c0..d1) of block k at level l before
     // factorization, leading dimension n.  Nystrom blocks all share the two
@@ -60,10 +69,46 @@ struct Hodlr {
     }
     int64_t leaf_size(int i) const { return leaf_off[i + 1] - leaf_off[i]; }
     int nleaves() const { return (int)leaf_off.size() - 1; }
+    int nblocks() const {
+        int s = 0;
+        for (const auto& b : blocks) s += (int)b.size();
+        return s;
+    }
+    double* leaf_ptr(int i) const { return leaves.as<double>() + leaf_mat_off[i]; }
     size_t device_bytes() const {
-        return pts.bytes + leaves.bytes + P.bytes + lm.bytes + kpp.bytes + kxp.bytes + ynys.bytes + X.bytes + G.bytes;
+        return pts.bytes + info.bytes + leaves.bytes + P.bytes + lm.bytes + kpp.bytes + kxp.bytes + ynys.bytes +
+               X.bytes + G.bytes;
     }
 };
+
+struct BuildReq {
+    const double* pts;
+    bool on_device;
+    int64_t n;
+    int d;
+    kmh_kernel kern;
+    kmh_build_opts opts;
+};
+
+// Per-request outcome of a batched stage.
+struct Status {
+    int code = KMH_OK;
+    std::string msg;
+};
+
+// Build every request (one grouped launch per step across requests).
+// Requests that fail get a status and a null handle; the others proceed.
+std::vector<Hodlr*> hodlr_build_batch(Ctx& c, const std::vector<BuildReq>& reqs, std::vector<Status>& st);
+// Resumable factorization stages (continuous batching): begin (leaf stage),
+// one level per handle (levels may differ), end (logdet, flags).
+void factor_begin_batch(Ctx& c, const std::vector<Hodlr*>& hs);
+void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& hs, const std::vector<int>& lvs);
+void factor_end_batch(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<Status>& st);
+// symmetricfactorize! of every handle, level by level across handles.
+void hodlr_factorize_batch(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<Status>& st);
+// K^{-1} T in place for each handle's device block T (n x nrhs, ld n)
+void hodlr_solve_batch(Ctx& c, const std::vector<Hodlr*>& hs, const std::vector<double*>& T,
+                       const std::vector<int>& nrhs);
 
 Hodlr* hodlr_build(Ctx& c, const double* pts, bool pts_on_device, int64_t n, int d, const kmh_kernel& k,
                    const kmh_build_opts& o);
@@ -77,5 +122,6 @@ void hodlr_full_host(Hodlr& H, double* out);
 void hodlr_block_uv_host(Hodlr& H, int level, int block, double* U, double* V);
 void hodlr_stoch_gradient(Hodlr& H, const kmh_kernel* dks, int p, const double* y_host, const double* saa_host,
                           int64_t nv, double* grad);
+void check_kernel(const kmh_kernel& k, bool allow_deriv);
 
 }  // namespace kmh

@@ -1,14 +1,16 @@
 // HODLR build / symmetric factorization / solve / logdet / matvec on one
 // device, composed from the batched primitives of la.h and assemble.h.
+// Every stage takes a list of handles: independent problems of different
+// sizes share each grouped launch (their levels are aligned from the leaves).
 //
 // Data layout in HBM (column-major):
 //   pts    x[0..n) then y[0..n)
 //   leaves leaf i (m_i x m_i, ld m_i) at leaf_mat_off[i]; Cholesky in place
 //   P      n x Rtot, ld n.  Level l owns columns coff[l] .. coff[l]+rpad[l];
 //          block (a0,b1,c0,d1) keeps U in rows a0..b1 and V in rows c0..d1
-//          of those columns (ranks below rpad are zero-padded).  Every row
-//          belongs to exactly one child per level, so each level's panel is
-//          one n x rpad matrix.  The factorization overwrites U and V by the
+//          of those columns (ACA ranks below rpad are zero-padded).  Every
+//          row belongs to exactly one child per level, so each level's panel
+//          is one n x rpad matrix.  The factorization leaves there the
 //          orthonormal Q factors, so P then holds M = blkdiag(Q_V, Q_U) of
 //          every block (HODLR/src/utils.jl:29-35).
 //   X, G   per block 2r x 2r: X = chol([I R1R2'; R2R1' I]) - I and
@@ -54,7 +56,7 @@ static void tree_indices(Hodlr& H) {
     }
 }
 
-static void check_kernel(const kmh_kernel& k, bool allow_deriv) {
+void check_kernel(const kmh_kernel& k, bool allow_deriv) {
     KMH_CHECK(k.family >= KMH_RATIONAL2 && k.family <= KMH_MATERN32, KMH_E_BAD_ARG, "unknown kernel family");
     const int dv = k.deriv;
     KMH_CHECK(dv == KMH_D0 || dv == KMH_D1 || dv == KMH_D2 || dv == KMH_D11 || dv == KMH_D12 || dv == KMH_D22,
@@ -66,214 +68,298 @@ static void check_kernel(const kmh_kernel& k, bool allow_deriv) {
     if (k.family != KMH_CAUCHY) KMH_CHECK(k.params[1] != 0.0, KMH_E_BAD_ARG, "kernel length scale must be nonzero");
 }
 
-static void check_info(Ctx& c, const char* what) {
-    int info = 0;
-    KMH_HIP(hipMemcpyAsync(&info, c.d_info, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+// Read the per-handle Cholesky flags (one sync) and mark failures.
+static void collect_info(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<Status>& st, const char* what) {
+    std::vector<int> flags(hs.size(), 0);
+    for (size_t i = 0; i < hs.size(); ++i)
+        if (hs[i]) KMH_HIP(hipMemcpyAsync(&flags[i], hs[i]->info.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     c.sync();
-    if (info) {
-        KMH_HIP(hipMemsetAsync(c.d_info, 0, sizeof(int), c.stream));
-        throw Error(KMH_E_NOT_PD, std::string("PosDefException: matrix is not positive definite; ") + what);
-    }
+    for (size_t i = 0; i < hs.size(); ++i)
+        if (hs[i] && flags[i] && st[i].code == KMH_OK) {
+            st[i].code = KMH_E_NOT_PD;
+            st[i].msg = std::string("PosDefException: matrix is not positive definite; ") + what;
+        }
 }
 
-static void build_nystrom(Hodlr& H) {
-    Ctx& c = *H.ctx;
+static void reset_info(Ctx& c, const std::vector<Hodlr*>& hs) {
+    for (auto* h : hs)
+        if (h) KMH_HIP(hipMemsetAsync(h->info.p, 0, sizeof(int), c.stream));
+}
+
+// ---------------------------------------------------------------------------
+// build
+// ---------------------------------------------------------------------------
+
+static void validate(const BuildReq& r) {
+    KMH_CHECK(r.d == 2, KMH_E_BAD_ARG, "points must be 2-dimensional");
+    KMH_CHECK(r.n >= 1 && r.n < (int64_t(1) << 31), KMH_E_BAD_ARG, "invalid number of points");
+    KMH_CHECK(r.pts != nullptr, KMH_E_BAD_ARG, "null points");
+    check_kernel(r.kern, false);
+    const auto& o = r.opts;
+    KMH_CHECK(o.method == KMH_ACA || o.method == KMH_NYSTROM, KMH_E_BAD_ARG, "unknown compression method");
+    KMH_CHECK(o.level_kind == KMH_FIXED_LEVEL || o.level_kind == KMH_LOG_LEVEL, KMH_E_BAD_ARG, "unknown level kind");
+    KMH_CHECK(o.precision == KMH_FP64, KMH_E_BAD_ARG, "unsupported precision mode");
+    KMH_CHECK(o.maxrank >= 0, KMH_E_BAD_ARG, "maxrank must be >= 0");
+}
+
+static void nystrom_setup(Hodlr& H) {
     const int r = H.opts.maxrank;
     KMH_CHECK(r >= 2, KMH_E_BAD_ARG, "Please supply a fixed off-diagonal rank that is >=2.");
     int64_t minleaf = H.n;
     for (int i = 0; i < H.nleaves(); ++i) minleaf = std::min(minleaf, H.leaf_size(i) - 1);
     KMH_CHECK(r < minleaf, KMH_E_BAD_ARG, "Your nystrom rank is too big. Reduce the HODLR level or nystrom rank.");
-    const int64_t n = H.n;
     // landmarks: Int64.(round.(LinRange(1, n, r)))  (constructor.jl:31)
     H.nyind.resize(r);
     for (int j = 0; j < r; ++j) {
         const double t = (double)j / (double)(r - 1);
-        H.nyind[j] = (int64_t)std::nearbyint((1.0 - t) * 1.0 + t * (double)n) - 1;
+        H.nyind[j] = (int64_t)std::nearbyint((1.0 - t) * 1.0 + t * (double)H.n) - 1;
     }
     H.nys_r = r;
-    const double* px = H.pts.as<double>();
-    const double* py = px + n;
+    Ctx& c = *H.ctx;
     H.lm.alloc(&c, 2 * (size_t)r * sizeof(double));
-    double* lx = H.lm.as<double>();
-    double* ly = lx + r;
-    {
-        std::vector<Copy2D> g;
-        for (int j = 0; j < r; ++j) {
-            g.push_back({px + H.nyind[j], 1, lx + j, 1, 1, 1, 1.0});
-            g.push_back({py + H.nyind[j], 1, ly + j, 1, 1, 1, 1.0});
-        }
-        copy2d(c, g);
-    }
     H.kpp.alloc(&c, (size_t)r * r * sizeof(double));
-    H.kxp.alloc(&c, (size_t)n * r * sizeof(double));
-    eval_blocks(c, H.kern, {{lx, ly, lx, ly, r, r, H.kpp.as<double>(), r, 0, 0},
-                            {px, py, lx, ly, (int)n, r, H.kxp.as<double>(), (int)n, 0, 0}});
-    potrf(c, {{H.kpp.as<double>(), r, r}});
-    check_info(c, "Cholesky of the Nystrom landmark matrix K(P,P) failed.");
-    // Y^T = K(P,P)^{-1} K(P, x)  (src/factorizations.jl:92)
-    double* yt = static_cast<double*>(c.scratch((size_t)r * n * sizeof(double)));
-    copy2d(c, {{H.kxp.as<double>(), (int)n, yt, r, (int)n, r, 1.0}}, true);
-    trsm_left(c, false, {{H.kpp.as<double>(), r, r, yt, (int)n, r}});
-    trsm_left(c, true, {{H.kpp.as<double>(), r, r, yt, (int)n, r}});
-    H.ynys.alloc(&c, (size_t)n * r * sizeof(double));
-    copy2d(c, {{yt, r, H.ynys.as<double>(), (int)n, r, (int)n, 1.0}}, true);
+    H.kxp.alloc(&c, (size_t)H.n * r * sizeof(double));
+    H.ynys.alloc(&c, (size_t)H.n * r * sizeof(double));
     H.rpad.assign(H.L, r);
     H.coff.assign(H.L + 1, 0);
     for (int l = 0; l < H.L; ++l) H.coff[l + 1] = H.coff[l] + r;
     H.Rtot = H.coff[H.L];
     H.ranks.assign(H.L, {});
     for (int l = 0; l < H.L; ++l) H.ranks[l].assign(H.blocks[l].size(), r);
-    c.scratch_free(yt);
 }
 
-static void build_aca(Hodlr& H) {
-    Ctx& c = *H.ctx;
-    const int64_t n = H.n;
-    const double* px = H.pts.as<double>();
-    const double* py = px + n;
-    const int L = H.L;
-    std::vector<int> cap(L), maxsz_lv(L, 1);
-    for (int l = 0; l < L; ++l) {
-        for (const auto& b : H.blocks[l]) {
-            const int h = (int)std::min(b.b1 - b.a0, b.d1 - b.c0);
-            const int ms = H.opts.maxrank == 0 ? h : std::min(H.opts.maxrank, h);
-            maxsz_lv[l] = std::max(maxsz_lv[l], ms);
-        }
-        cap[l] = std::min(maxsz_lv[l], 256);
-    }
-    std::vector<DevBuf> tmp(L);
+// ACA of every block of every level of every ACA handle, one launch per
+// attempt; a level whose rank reached its column capacity reruns wider.
+static void build_aca_batch(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<Status>& st,
+                            const std::vector<int>& idx) {
+    struct Lv { int h; int l; int cap; int maxsz; DevBuf tmp; bool redo; };
+    std::vector<Lv> lvs;
     int nblk = 0;
-    for (int l = 0; l < L; ++l) nblk += (int)H.blocks[l].size();
+    for (int i : idx) {
+        Hodlr& H = *hs[i];
+        for (int l = 0; l < H.L; ++l) {
+            int ms_max = 1;
+            for (const auto& b : H.blocks[l]) {
+                const int h = (int)std::min(b.b1 - b.a0, b.d1 - b.c0);
+                ms_max = std::max(ms_max, H.opts.maxrank == 0 ? h : std::min(H.opts.maxrank, h));
+            }
+            lvs.push_back({i, l, std::min(ms_max, 256), ms_max, DevBuf(), true});
+            nblk += (int)H.blocks[l].size();
+        }
+    }
+    if (lvs.empty()) return;
     DevBuf out;
-    out.alloc(&c, 2 * sizeof(int) * std::max(1, nblk));
+    out.alloc(&c, 2 * sizeof(int) * nblk);
     int* d_rank = out.as<int>();
     int* d_stat = d_rank + nblk;
-    std::vector<int> h_out(2 * std::max(1, nblk));
-    std::vector<char> redo(L, 1);
-    H.ranks.assign(L, {});
-    for (int attempt = 0;; ++attempt) {
+    std::vector<int> h_out(2 * nblk);
+    for (;;) {
         std::vector<AcaDesc> ds;
         int bi = 0;
-        for (int l = 0; l < L; ++l) {
-            if (!redo[l]) { bi += (int)H.blocks[l].size(); continue; }
-            tmp[l].alloc(&c, (size_t)n * cap[l] * sizeof(double));
-            double* T = tmp[l].as<double>();
-            for (const auto& b : H.blocks[l]) {
+        for (auto& v : lvs) {
+            Hodlr& H = *hs[v.h];
+            if (!v.redo) { bi += (int)H.blocks[v.l].size(); continue; }
+            v.tmp.alloc(&c, (size_t)H.n * v.cap * sizeof(double));
+            double* T = v.tmp.as<double>();
+            const double* px = H.pts.as<double>();
+            for (const auto& b : H.blocks[v.l]) {
                 const int nro = (int)(b.b1 - b.a0), ncl = (int)(b.d1 - b.c0);
                 const int h = std::min(nro, ncl);
                 const int ms = H.opts.maxrank == 0 ? h : std::min(H.opts.maxrank, h);
-                ds.push_back({(int)b.a0, nro, (int)b.c0, ncl, T + b.a0, T + b.c0, (int)n, ms, cap[l], 0,
-                              d_rank + bi, d_stat + bi});
+                ds.push_back({px, px + H.n, (int)b.a0, nro, (int)b.c0, ncl, T + b.a0, T + b.c0, (int)H.n, ms,
+                              v.cap, 0, d_rank + bi, d_stat + bi, H.opts.tol, to_kdev(H.kern)});
                 ++bi;
             }
         }
-        aca_launch(c, H.kern, px, py, H.opts.tol, ds);
+        aca_launch(c, ds);
         KMH_HIP(hipMemcpyAsync(h_out.data(), d_rank, 2 * sizeof(int) * nblk, hipMemcpyDeviceToHost, c.stream));
         c.sync();
         bool again = false;
         bi = 0;
-        for (int l = 0; l < L; ++l) {
-            bool lv_redo = false;
-            for (size_t k = 0; k < H.blocks[l].size(); ++k, ++bi) {
-                if (!redo[l]) continue;
-                const int st = h_out[nblk + bi];
-                KMH_CHECK(st != 1, KMH_E_ACA, "Incompatible maxrank or matrix dimensions.");
-                if (st == 2) lv_redo = true;
+        for (auto& v : lvs) {
+            Hodlr& H = *hs[v.h];
+            bool wider = false;
+            for (size_t k = 0; k < H.blocks[v.l].size(); ++k, ++bi) {
+                if (!v.redo) continue;
+                const int s = h_out[nblk + bi];
+                if (s == 1 && st[v.h].code == KMH_OK) {
+                    st[v.h].code = KMH_E_ACA;
+                    st[v.h].msg = "Incompatible maxrank or matrix dimensions.";
+                }
+                if (s == 2) wider = true;
             }
-            if (redo[l] && lv_redo) {
-                KMH_CHECK(cap[l] < maxsz_lv[l], KMH_E_DEVICE, "ACA capacity logic error");
-                cap[l] = std::min(maxsz_lv[l], cap[l] * 2);
-                KMH_CHECK((size_t)3 * cap[l] * 8 < 96 * 1024, KMH_E_BAD_ARG,
-                          "ACA rank exceeds the supported maximum of 4096");
-                again = true;
+            if (v.redo && wider && st[v.h].code == KMH_OK) {
+                KMH_CHECK(v.cap < v.maxsz, KMH_E_DEVICE, "ACA capacity logic error");
+                v.cap = std::min(v.maxsz, v.cap * 2);
+                if ((size_t)3 * v.cap * 8 >= 96 * 1024) {
+                    st[v.h].code = KMH_E_BAD_ARG;
+                    st[v.h].msg = "ACA rank exceeds the supported maximum of 4096";
+                    wider = false;
+                }
             }
-            redo[l] = lv_redo ? 1 : 0;
+            v.redo = wider && st[v.h].code == KMH_OK;
+            again = again || v.redo;
         }
         if (!again) break;
     }
-    // compact into P
-    H.rpad.assign(L, 1);
-    int bi = 0;
-    for (int l = 0; l < L; ++l)
-        for (size_t k = 0; k < H.blocks[l].size(); ++k, ++bi) {
-            H.ranks[l].push_back(h_out[bi]);
-            H.rpad[l] = std::max(H.rpad[l], h_out[bi]);
-        }
-    H.coff.assign(L + 1, 0);
-    for (int l = 0; l < L; ++l) H.coff[l + 1] = H.coff[l] + H.rpad[l];
-    H.Rtot = H.coff[L];
-    H.P.alloc(&c, (size_t)n * H.Rtot * sizeof(double));
-    KMH_HIP(hipMemsetAsync(H.P.p, 0, H.P.bytes, c.stream));
+    // ranks, panel widths, compaction into P
     std::vector<Copy2D> cp;
-    double* P = H.P.as<double>();
-    for (int l = 0; l < L; ++l) {
-        double* Pl = P + (size_t)H.coff[l] * n;
-        const double* T = tmp[l].as<double>();
-        for (size_t k = 0; k < H.blocks[l].size(); ++k) {
-            const auto& b = H.blocks[l][k];
-            const int r = H.ranks[l][k];
-            cp.push_back({T + b.a0, (int)n, Pl + b.a0, (int)n, (int)(b.b1 - b.a0), r, 1.0});
-            cp.push_back({T + b.c0, (int)n, Pl + b.c0, (int)n, (int)(b.d1 - b.c0), r, 1.0});
+    int bi = 0;
+    size_t q = 0;
+    for (int i : idx) {
+        Hodlr& H = *hs[i];
+        H.ranks.assign(H.L, {});
+        H.rpad.assign(H.L, 1);
+        size_t q0 = q;
+        for (int l = 0; l < H.L; ++l, ++q)
+            for (size_t k = 0; k < H.blocks[l].size(); ++k, ++bi) {
+                H.ranks[l].push_back(h_out[bi]);
+                H.rpad[l] = std::max(H.rpad[l], h_out[bi]);
+            }
+        if (st[i].code != KMH_OK) continue;
+        H.coff.assign(H.L + 1, 0);
+        for (int l = 0; l < H.L; ++l) H.coff[l + 1] = H.coff[l] + H.rpad[l];
+        H.Rtot = H.coff[H.L];
+        H.P.alloc(&c, (size_t)H.n * H.Rtot * sizeof(double));
+        KMH_HIP(hipMemsetAsync(H.P.p, 0, H.P.bytes, c.stream));
+        double* P = H.P.as<double>();
+        for (int l = 0; l < H.L; ++l) {
+            double* Pl = P + (size_t)H.coff[l] * H.n;
+            const double* T = lvs[q0 + l].tmp.as<double>();
+            for (size_t k = 0; k < H.blocks[l].size(); ++k) {
+                const auto& b = H.blocks[l][k];
+                const int r = H.ranks[l][k];
+                cp.push_back({T + b.a0, (int)H.n, Pl + b.a0, (int)H.n, (int)(b.b1 - b.a0), r, 1.0});
+                cp.push_back({T + b.c0, (int)H.n, Pl + b.c0, (int)H.n, (int)(b.d1 - b.c0), r, 1.0});
+            }
         }
     }
     copy2d(c, cp);
     c.sync();
 }
 
+std::vector<Hodlr*> hodlr_build_batch(Ctx& c, const std::vector<BuildReq>& reqs, std::vector<Status>& st) {
+    c.set_device();
+    const size_t R = reqs.size();
+    st.assign(R, Status());
+    std::vector<std::unique_ptr<Hodlr>> own(R);
+    std::vector<Hodlr*> hs(R, nullptr);
+    std::vector<std::vector<double>> soa(R);
+    std::vector<Copy2D> dev_pts;
+    for (size_t i = 0; i < R; ++i) {
+        try {
+            const BuildReq& q = reqs[i];
+            validate(q);
+            own[i].reset(new Hodlr());
+            Hodlr& H = *own[i];
+            H.ctx = &c;
+            H.n = q.n;
+            H.kern = q.kern;
+            H.opts = q.opts;
+            tree_indices(H);
+            H.info.alloc(&c, 256);
+            KMH_HIP(hipMemsetAsync(H.info.p, 0, sizeof(int), c.stream));
+            H.pts.alloc(&c, 2 * (size_t)q.n * sizeof(double));
+            double* px = H.pts.as<double>();
+            if (!q.on_device) {
+                soa[i].resize(2 * (size_t)q.n);
+                for (int64_t j = 0; j < q.n; ++j) {
+                    soa[i][j] = q.pts[2 * j];
+                    soa[i][q.n + j] = q.pts[2 * j + 1];
+                }
+                KMH_HIP(hipMemcpyAsync(px, soa[i].data(), soa[i].size() * sizeof(double), hipMemcpyHostToDevice,
+                                       c.stream));
+            } else {
+                dev_pts.push_back({q.pts, 2, px, (int)q.n, 2, (int)q.n, 1.0});
+            }
+            const int nl = H.nleaves();
+            H.leaf_mat_off.assign(nl + 1, 0);
+            for (int l = 0; l < nl; ++l) H.leaf_mat_off[l + 1] = H.leaf_mat_off[l] + H.leaf_size(l) * H.leaf_size(l);
+            H.leaves.alloc(&c, (size_t)H.leaf_mat_off[nl] * sizeof(double));
+            if (H.L > 0 && H.nystrom()) nystrom_setup(H);
+            if (H.L == 0) {
+                H.coff.assign(1, 0);
+                H.Rtot = 0;
+            }
+            hs[i] = own[i].get();
+        } catch (const Error& e) {
+            st[i].code = e.code;
+            st[i].msg = e.what();
+            own[i].reset();
+        }
+    }
+    copy2d(c, dev_pts, true);
+    // landmark gathers (Nystrom)
+    std::vector<Copy2D> g;
+    std::vector<EvalDesc> ev;
+    std::vector<SqMat> kpp;
+    std::vector<int> nys, aca;
+    for (size_t i = 0; i < R; ++i) {
+        if (!hs[i]) continue;
+        Hodlr& H = *hs[i];
+        const double* px = H.pts.as<double>();
+        const double* py = px + H.n;
+        const KDev kd = to_kdev(H.kern);
+        // leaves (constructor.jl:39)
+        for (int l = 0; l < H.nleaves(); ++l) {
+            const int64_t s = H.leaf_off[l];
+            const int m = (int)H.leaf_size(l);
+            ev.push_back({px + s, py + s, px + s, py + s, m, m, H.leaf_ptr(l), m, 0, 0, kd});
+        }
+        if (H.L > 0 && H.nystrom()) {
+            nys.push_back((int)i);
+            const int r = H.nys_r;
+            double* lx = H.lm.as<double>();
+            double* ly = lx + r;
+            for (int j = 0; j < r; ++j) {
+                g.push_back({px + H.nyind[j], 1, lx + j, 1, 1, 1, 1.0});
+                g.push_back({py + H.nyind[j], 1, ly + j, 1, 1, 1, 1.0});
+            }
+            ev.push_back({lx, ly, lx, ly, r, r, H.kpp.as<double>(), r, 0, 0, kd});
+            ev.push_back({px, py, lx, ly, (int)H.n, r, H.kxp.as<double>(), (int)H.n, 0, 0, kd});
+            kpp.push_back({H.kpp.as<double>(), r, r, H.info.as<int>()});
+        } else if (H.L > 0) {
+            aca.push_back((int)i);
+        }
+    }
+    copy2d(c, g);
+    eval_blocks(c, ev);
+    if (!nys.empty()) {
+        potrf(c, kpp);
+        collect_info(c, hs, st, "Cholesky of the Nystrom landmark matrix K(P,P) failed.");
+        // Y = K(x,P) K(P,P)^{-1}: transpose, two triangular solves, transpose back
+        std::vector<Copy2D> t1, t2;
+        std::vector<TrsmProb> tr;
+        std::vector<double*> yts;
+        for (int i : nys) {
+            Hodlr& H = *hs[i];
+            const int r = H.nys_r;
+            double* yt = static_cast<double*>(c.scratch((size_t)r * H.n * sizeof(double)));
+            yts.push_back(yt);
+            t1.push_back({H.kxp.as<double>(), (int)H.n, yt, r, (int)H.n, r, 1.0});
+            tr.push_back({H.kpp.as<double>(), r, r, yt, (int)H.n, r});
+            t2.push_back({yt, r, H.ynys.as<double>(), (int)H.n, r, (int)H.n, 1.0});
+        }
+        copy2d(c, t1, true);
+        trsm_left(c, false, tr);
+        trsm_left(c, true, tr);
+        copy2d(c, t2, true);
+        for (double* p : yts) c.scratch_free(p);
+    }
+    if (!aca.empty()) build_aca_batch(c, hs, st, aca);
+    c.sync();
+    std::vector<Hodlr*> out(R, nullptr);
+    for (size_t i = 0; i < R; ++i)
+        if (hs[i] && st[i].code == KMH_OK) out[i] = own[i].release();
+    return out;
+}
+
 Hodlr* hodlr_build(Ctx& c, const double* pts, bool dev, int64_t n, int d, const kmh_kernel& k,
                    const kmh_build_opts& o) {
-    KMH_CHECK(d == 2, KMH_E_BAD_ARG, "points must be 2-dimensional");
-    KMH_CHECK(n >= 1 && n < (int64_t(1) << 31), KMH_E_BAD_ARG, "invalid number of points");
-    KMH_CHECK(pts != nullptr, KMH_E_BAD_ARG, "null points");
-    check_kernel(k, false);
-    KMH_CHECK(o.method == KMH_ACA || o.method == KMH_NYSTROM, KMH_E_BAD_ARG, "unknown compression method");
-    KMH_CHECK(o.level_kind == KMH_FIXED_LEVEL || o.level_kind == KMH_LOG_LEVEL, KMH_E_BAD_ARG, "unknown level kind");
-    KMH_CHECK(o.precision == KMH_FP64, KMH_E_BAD_ARG, "unsupported precision mode");
-    KMH_CHECK(o.maxrank >= 0, KMH_E_BAD_ARG, "maxrank must be >= 0");
-    c.set_device();
-    std::unique_ptr<Hodlr> H(new Hodlr());
-    H->ctx = &c;
-    H->n = n;
-    H->kern = k;
-    H->opts = o;
-    tree_indices(*H);
-    H->pts.alloc(&c, 2 * (size_t)n * sizeof(double));
-    double* px = H->pts.as<double>();
-    if (!dev) {
-        std::vector<double> soa(2 * (size_t)n);
-        for (int64_t i = 0; i < n; ++i) {
-            soa[i] = pts[2 * i];
-            soa[n + i] = pts[2 * i + 1];
-        }
-        KMH_HIP(hipMemcpyAsync(px, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
-        c.sync();
-    } else {
-        copy2d(c, {{pts, 2, px, (int)n, 2, (int)n, 1.0}}, true);
-    }
-    const double* py = px + n;
-    // leaves (constructor.jl:39)
-    const int nl = H->nleaves();
-    H->leaf_mat_off.assign(nl + 1, 0);
-    for (int i = 0; i < nl; ++i) H->leaf_mat_off[i + 1] = H->leaf_mat_off[i] + H->leaf_size(i) * H->leaf_size(i);
-    H->leaves.alloc(&c, (size_t)H->leaf_mat_off[nl] * sizeof(double));
-    {
-        std::vector<EvalDesc> ev;
-        for (int i = 0; i < nl; ++i) {
-            const int64_t s = H->leaf_off[i];
-            const int m = (int)H->leaf_size(i);
-            ev.push_back({px + s, py + s, px + s, py + s, m, m, H->leaves.as<double>() + H->leaf_mat_off[i], m, 0, 0});
-        }
-        eval_blocks(c, k, ev);
-    }
-    if (H->L > 0) {
-        if (o.method == KMH_NYSTROM) build_nystrom(*H);
-        else build_aca(*H);
-    } else {
-        H->coff.assign(1, 0);
-        H->Rtot = 0;
-    }
-    c.sync();
-    return H.release();
+    std::vector<Status> st;
+    auto hs = hodlr_build_batch(c, {{pts, dev, n, d, k, o}}, st);
+    if (st[0].code != KMH_OK) throw Error(st[0].code, st[0].msg);
+    return hs[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -281,15 +367,24 @@ Hodlr* hodlr_build(Ctx& c, const double* pts, bool dev, int64_t n, int d, const 
 // ---------------------------------------------------------------------------
 enum { W_INV = 0, W_INVT = 1, W_MUL = 2, W_MULT = 3 };
 
-static double* leaf_ptr(Hodlr& H, int i) { return H.leaves.as<double>() + H.leaf_mat_off[i]; }
+struct Rhs {
+    Hodlr* H;
+    double* T;
+    int64_t ldT;
+    int ncols;
+    int lv = 0;   // level whose factors apply (woodbury_batch)
+};
 
-static void leaf_trsm(Hodlr& H, bool trans, double* T, int64_t ldT, int ncols) {
+static void leaf_trsm_batch(Ctx& c, bool trans, const std::vector<Rhs>& rs) {
     std::vector<TrsmProb> pr;
-    for (int i = 0; i < H.nleaves(); ++i) {
-        const int m = (int)H.leaf_size(i);
-        pr.push_back({leaf_ptr(H, i), m, m, T + H.leaf_off[i], ncols, (int)ldT});
+    for (const auto& x : rs) {
+        Hodlr& H = *x.H;
+        for (int i = 0; i < H.nleaves(); ++i) {
+            const int m = (int)H.leaf_size(i);
+            pr.push_back({H.leaf_ptr(i), m, m, x.T + H.leaf_off[i], x.ncols, (int)x.ldT});
+        }
     }
-    trsm_left(*H.ctx, trans, pr);
+    trsm_left(c, trans, pr);
 }
 
 // T <- op(L_leaf) T, blockwise (L lower with zero upper triangle)
@@ -302,42 +397,48 @@ static void leaf_mul(Hodlr& H, bool trans, double* T, int ncols) {
     for (int i = 0; i < H.nleaves(); ++i) {
         const int m = (int)H.leaf_size(i);
         const int64_t s = H.leaf_off[i];
-        g.push_back({m, ncols, m, leaf_ptr(H, i), m, tmp + s, (int)n, T + s, (int)n, 1.0, 0.0, 0});
+        g.push_back({m, ncols, m, H.leaf_ptr(i), m, tmp + s, (int)n, T + s, (int)n, 1.0, 0.0, 0});
     }
     gemm(c, trans, false, g);
     c.scratch_free(tmp);
 }
 
-// Apply the level-lv block factors to T (n x ncols, ld ldT) in place.
-static void woodbury(Hodlr& H, int lv, int mode, double* T, int64_t ldT, int ncols) {
-    Ctx& c = *H.ctx;
-    const int64_t n = H.n;
-    const auto& bl = H.blocks[lv];
+// Apply the level-lv block factors of each handle to its T in place:
+// S = M^T T, S2 = op(G or X) S, T -/+= M S2.  One grouped launch per step.
+static void woodbury_batch(Ctx& c, int mode, const std::vector<Rhs>& rs) {
     size_t ssz = 0;
-    for (size_t k = 0; k < bl.size(); ++k) ssz += (size_t)2 * H.ranks[lv][k] * ncols;
+    for (const auto& x : rs)
+        for (int r : x.H->ranks[x.lv]) ssz += (size_t)2 * r * x.ncols;
+    if (ssz == 0) return;
     double* S = static_cast<double*>(c.scratch(2 * ssz * sizeof(double)));
     double* S2 = S + ssz;
-    const double* P = H.P.as<double>() + (size_t)H.coff[lv] * n;
     const bool inv = (mode == W_INV || mode == W_INVT);
     const bool tr = (mode == W_INVT || mode == W_MULT);
-    const double* Mb = inv ? H.G.as<double>() : H.X.as<double>();
+    const double sg = inv ? -1.0 : 1.0;
     std::vector<GemmProb> tn, op, up;
     size_t off = 0;
-    for (size_t k = 0; k < bl.size(); ++k) {
-        const auto& b = bl[k];
-        const int r = H.ranks[lv][k];
-        const int h1 = (int)(b.d1 - b.c0), h2 = (int)(b.b1 - b.a0);
-        const double* Q1 = P + b.c0;
-        const double* Q2 = P + b.a0;
-        double* s = S + off;
-        double* s2 = S2 + off;
-        off += (size_t)2 * r * ncols;
-        tn.push_back({r, ncols, h1, Q1, (int)n, T + b.c0, (int)ldT, s, 2 * r, 1.0, 0.0, 0});
-        tn.push_back({r, ncols, h2, Q2, (int)n, T + b.a0, (int)ldT, s + r, 2 * r, 1.0, 0.0, 0});
-        op.push_back({2 * r, ncols, 2 * r, Mb + H.coff_blk[lv][k], 2 * r, s, 2 * r, s2, 2 * r, 1.0, 0.0, 0});
-        const double sg = inv ? -1.0 : 1.0;
-        up.push_back({h1, ncols, r, Q1, (int)n, s2, 2 * r, T + b.c0, (int)ldT, sg, 1.0, 0});
-        up.push_back({h2, ncols, r, Q2, (int)n, s2 + r, 2 * r, T + b.a0, (int)ldT, sg, 1.0, 0});
+    for (const auto& x : rs) {
+        Hodlr& H = *x.H;
+        const int lv = x.lv;
+        const int64_t n = H.n;
+        const double* P = H.P.as<double>() + (size_t)H.coff[lv] * n;
+        const double* Mb = inv ? H.G.as<double>() : H.X.as<double>();
+        const int nc = x.ncols;
+        for (size_t k = 0; k < H.blocks[lv].size(); ++k) {
+            const auto& b = H.blocks[lv][k];
+            const int r = H.ranks[lv][k];
+            const int h1 = (int)(b.d1 - b.c0), h2 = (int)(b.b1 - b.a0);
+            const double* Q1 = P + b.c0;
+            const double* Q2 = P + b.a0;
+            double* s = S + off;
+            double* s2 = S2 + off;
+            off += (size_t)2 * r * nc;
+            tn.push_back({r, nc, h1, Q1, (int)n, x.T + b.c0, (int)x.ldT, s, 2 * r, 1.0, 0.0, 0});
+            tn.push_back({r, nc, h2, Q2, (int)n, x.T + b.a0, (int)x.ldT, s + r, 2 * r, 1.0, 0.0, 0});
+            op.push_back({2 * r, nc, 2 * r, Mb + H.coff_blk[lv][k], 2 * r, s, 2 * r, s2, 2 * r, 1.0, 0.0, 0});
+            up.push_back({h1, nc, r, Q1, (int)n, s2, 2 * r, x.T + b.c0, (int)x.ldT, sg, 1.0, 0});
+            up.push_back({h2, nc, r, Q2, (int)n, s2 + r, 2 * r, x.T + b.a0, (int)x.ldT, sg, 1.0, 0});
+        }
     }
     gemm(c, true, false, tn);
     gemm(c, tr, false, op);
@@ -345,83 +446,57 @@ static void woodbury(Hodlr& H, int lv, int mode, double* T, int64_t ldT, int nco
     c.scratch_free(S);
 }
 
-void hodlr_factorize(Hodlr& H) {
-    KMH_CHECK(!H.factorized, KMH_E_STATE, "The matrix is already factorized.");
-    Ctx& c = *H.ctx;
-    c.set_device();
-    const int64_t n = H.n;
-    const int L = H.L, nl = H.nleaves();
-    KMH_HIP(hipMemsetAsync(c.d_info, 0, sizeof(int), c.stream));
-    // leaf Cholesky (symfact.jl:17)
-    std::vector<SqMat> lm;
-    for (int i = 0; i < nl; ++i) lm.push_back({leaf_ptr(H, i), (int)H.leaf_size(i), (int)H.leaf_size(i)});
-    potrf(c, lm);
-    int nblk = 0;
-    for (int l = 0; l < L; ++l) nblk += (int)H.blocks[l].size();
-    double* parts = static_cast<double*>(c.scratch((size_t)(nl + nblk + 1) * sizeof(double)));
-    log_diag_sums(c, lm, parts);
-    // Nystrom: every block's U and V are row ranges of A = K(x,P) and
-    // B = K(x,P)K(P,P)^{-1}, and every block factor W_l is block-diagonal over
-    // whole nodes, so the transformed panels of all higher levels stay row
-    // ranges of W_l^{-1}...W_1^{-1} L^{-1} [A B]: the cascade updates these
-    // 2r columns instead of every higher level's panel (same W).
-    double* AB = nullptr;
-    const int r = H.nys_r;
-    if (H.nystrom() && L > 0) {
-        H.P.alloc(&c, (size_t)n * H.Rtot * sizeof(double));
-        AB = static_cast<double*>(c.scratch((size_t)n * 2 * r * sizeof(double)));
-        copy2d(c, {{H.kxp.as<double>(), (int)n, AB, (int)n, (int)n, r, 1.0},
-                   {H.ynys.as<double>(), (int)n, AB + (size_t)n * r, (int)n, (int)n, r, 1.0}});
-        leaf_trsm(H, false, AB, n, 2 * r);
-    } else if (L > 0) {
-        // leaf inverse on every level's U and V (symfact.jl:21)
-        leaf_trsm(H, false, H.P.as<double>(), n, (int)H.Rtot);
-    }
-    // per-level factors
-    H.coff_blk.assign(L, {});
-    size_t xtot = 0;
-    for (int l = 0; l < L; ++l)
-        for (size_t k = 0; k < H.blocks[l].size(); ++k) {
-            H.coff_blk[l].push_back((int64_t)xtot);
-            const size_t r2 = 2 * (size_t)H.ranks[l][k];
-            xtot += r2 * r2;
-        }
-    H.X.alloc(&c, std::max<size_t>(1, xtot) * sizeof(double));
-    H.G.alloc(&c, std::max<size_t>(1, xtot) * sizeof(double));
-    double* X = H.X.as<double>();
-    double* G = H.G.as<double>();
-    int pbase = nl;
-    for (int lv = 0; lv < L; ++lv) {
-        const auto& bl = H.blocks[lv];
+// ---------------------------------------------------------------------------
+// symmetricfactorize!  (symfact.jl:6-50), across handles
+// ---------------------------------------------------------------------------
+
+// One level per handle (levels may differ across handles).
+void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vector<int>& lvs) {
+    // Nystrom: this level's panels are row ranges of the cascade panels
+    std::vector<Copy2D> cp;
+    for (size_t ai = 0; ai < act.size(); ++ai) {
+        Hodlr& H = *act[ai];
+        const int lv = lvs[ai];
+        if (!H.AB) continue;
+        const int64_t n = H.n;
+        const int r = H.nys_r;
         double* P = H.P.as<double>() + (size_t)H.coff[lv] * n;
-        if (AB) {
-            // this level's panels: V = B rows (first child), U = A rows (second child)
-            std::vector<Copy2D> cp;
-            for (const auto& b : bl) {
-                cp.push_back({AB + (size_t)n * r + b.c0, (int)n, P + b.c0, (int)n, (int)(b.d1 - b.c0), r, 1.0});
-                cp.push_back({AB + b.a0, (int)n, P + b.a0, (int)n, (int)(b.b1 - b.a0), r, 1.0});
-            }
-            copy2d(c, cp);
+        for (const auto& b : H.blocks[lv]) {
+            cp.push_back({H.AB + (size_t)n * r + b.c0, (int)n, P + b.c0, (int)n, (int)(b.d1 - b.c0), r, 1.0});
+            cp.push_back({H.AB + b.a0, (int)n, P + b.a0, (int)n, (int)(b.b1 - b.a0), r, 1.0});
         }
-        size_t rsz = 0;
-        for (size_t k = 0; k < bl.size(); ++k) rsz += (size_t)3 * H.ranks[lv][k] * H.ranks[lv][k];
-        double* Rb = static_cast<double*>(c.scratch(rsz * sizeof(double)));
-        std::vector<QrPanel> qp;
-        std::vector<GemmProb> bm;
-        std::vector<DiagOp> ident_x, ident_g, minus1;
-        std::vector<Copy2D> lowleft;
-        std::vector<SqMat> cm;
-        std::vector<TrsmProb> ginv;
-        std::vector<Axpby2D> neg;
-        size_t off = 0;
-        for (size_t k = 0; k < bl.size(); ++k) {
-            const auto& b = bl[k];
+    }
+    copy2d(c, cp);
+    size_t rsz = 0;
+    for (size_t ai = 0; ai < act.size(); ++ai)
+        for (int r : act[ai]->ranks[lvs[ai]]) rsz += (size_t)3 * r * r;
+    double* Rb = static_cast<double*>(c.scratch(rsz * sizeof(double)));
+    std::vector<QrPanel> qp;
+    std::vector<GemmProb> bm;
+    std::vector<DiagOp> ident_x, ident_g, plus1, minus1;
+    std::vector<Copy2D> lowleft;
+    std::vector<SqMat> cm;
+    std::vector<double*> cm_out;
+    std::vector<TrsmProb> ginv;
+    std::vector<Axpby2D> neg;
+    size_t off = 0;
+    for (size_t ai = 0; ai < act.size(); ++ai) {
+        Hodlr& H = *act[ai];
+        const int lv = lvs[ai];
+        const int64_t n = H.n;
+        double* P = H.P.as<double>() + (size_t)H.coff[lv] * n;
+        double* X = H.X.as<double>();
+        double* G = H.G.as<double>();
+        int pb = H.nleaves();
+        for (int l = 0; l < lv; ++l) pb += (int)H.blocks[l].size();
+        for (size_t k = 0; k < H.blocks[lv].size(); ++k) {
+            const auto& b = H.blocks[lv][k];
             const int r = H.ranks[lv][k];
             double* R1 = Rb + off;
             double* R2 = R1 + (size_t)r * r;
             double* Bm = R2 + (size_t)r * r;
             off += (size_t)3 * r * r;
-            // lrsymfact(V, U): V (first child) first (symfact.jl:31)
+            // lrsymfact(V, U): V (first child) first (symfact.jl:31, HODLR/src/utils.jl:29-35)
             qp.push_back({P + b.c0, (int)(b.d1 - b.c0), r, (int)n, R1, r});
             qp.push_back({P + b.a0, (int)(b.b1 - b.a0), r, (int)n, R2, r});
             bm.push_back({r, r, r, R1, r, R2, r, Bm, r, 1.0, 0.0, 0});
@@ -429,74 +504,210 @@ void hodlr_factorize(Hodlr& H) {
             double* Gb = G + H.coff_blk[lv][k];
             ident_x.push_back({Xb, 2 * r, 2 * r, 1.0});
             ident_g.push_back({Gb, 2 * r, 2 * r, 1.0});
+            plus1.push_back({Gb, 2 * r, 2 * r, 1.0});
             minus1.push_back({Xb, 2 * r, 2 * r, -1.0});
             lowleft.push_back({Bm, r, Xb + r, 2 * r, r, r, 1.0});
-            cm.push_back({Xb, 2 * r, 2 * r});
+            cm.push_back({Xb, 2 * r, 2 * r, H.info.as<int>()});
+            cm_out.push_back(H.parts + pb + (int)k);
             ginv.push_back({Xb, 2 * r, 2 * r, Gb, 2 * r, 2 * r});
             neg.push_back({Gb, 2 * r, Gb, 2 * r, 2 * r, 2 * r, 0.0, -1.0});
         }
-        qr_panels(c, qp);
-        gemm(c, false, true, bm);                    // R1 R2^T
-        set_identity(c, ident_x);
-        copy2d(c, lowleft, true);                    // C = [I, *; (R1R2^T)^T, I] (lower part)
-        potrf(c, cm);                                // chol(C)
-        log_diag_sums(c, cm, parts + pbase);
-        pbase += (int)bl.size();
-        set_identity(c, ident_g);
-        trsm_left(c, false, ginv);                   // G = chol(C)^{-1}
-        axpby2d(c, neg);
-        for (auto& d : ident_g) d.v = 1.0;
-        add_diag(c, ident_g);                        // G = I - chol(C)^{-1}
-        add_diag(c, minus1);                         // X = chol(C) - I
-        c.scratch_free(Rb);
-        // Woodbury update of every higher level (symfact.jl:34-37)
-        if (lv + 1 < L) {
-            if (AB) woodbury(H, lv, W_INV, AB, n, 2 * r);
-            else woodbury(H, lv, W_INV, H.P.as<double>() + (size_t)H.coff[lv + 1] * n, n,
-                          (int)(H.Rtot - H.coff[lv + 1]));
+    }
+    qr_panels(c, qp);
+    gemm(c, false, true, bm);                    // R1 R2^T
+    set_identity(c, ident_x);
+    copy2d(c, lowleft, true);                    // C = [I, *; (R1R2^T)^T, I] (lower part)
+    potrf(c, cm);                                // chol(C)
+    log_diag_sums(c, cm, cm_out);
+    set_identity(c, ident_g);
+    trsm_left(c, false, ginv);                   // G = chol(C)^{-1}
+    axpby2d(c, neg);
+    add_diag(c, plus1);                          // G = I - chol(C)^{-1}
+    add_diag(c, minus1);                         // X = chol(C) - I
+    c.scratch_free(Rb);
+    // Woodbury update of every higher level (symfact.jl:34-37)
+    std::vector<Rhs> wr;
+    for (size_t ai = 0; ai < act.size(); ++ai) {
+        Hodlr* h = act[ai];
+        Hodlr& H = *h;
+        const int lv = lvs[ai];
+        if (lv + 1 >= H.L) continue;
+        if (H.AB) wr.push_back({h, H.AB, H.n, 2 * H.nys_r, lv});
+        else wr.push_back({h, H.P.as<double>() + (size_t)H.coff[lv + 1] * H.n, H.n, (int)(H.Rtot - H.coff[lv + 1]), lv});
+    }
+    woodbury_batch(c, W_INV, wr);
+}
+
+// Leaf stage: storage, leaf Cholesky, leaf solves on the panels.
+void factor_begin_batch(Ctx& c, const std::vector<Hodlr*>& hs) {
+    c.set_device();
+    reset_info(c, hs);
+    std::vector<SqMat> lm;
+    std::vector<double*> lm_out;
+    std::vector<Copy2D> abinit;
+    std::vector<Rhs> lt;
+    for (Hodlr* h : hs) {
+        Hodlr& H = *h;
+        const int64_t n = H.n;
+        const int L = H.L, nl = H.nleaves();
+        H.nparts = nl + H.nblocks();
+        H.parts = static_cast<double*>(c.scratch((size_t)(H.nparts + 1) * sizeof(double)));
+        for (int i = 0; i < nl; ++i) {
+            lm.push_back({H.leaf_ptr(i), (int)H.leaf_size(i), (int)H.leaf_size(i), H.info.as<int>()});
+            lm_out.push_back(H.parts + i);
+        }
+        // per-block factor storage
+        H.coff_blk.assign(L, {});
+        size_t xtot = 0;
+        for (int l = 0; l < L; ++l)
+            for (size_t k = 0; k < H.blocks[l].size(); ++k) {
+                H.coff_blk[l].push_back((int64_t)xtot);
+                const size_t r2 = 2 * (size_t)H.ranks[l][k];
+                xtot += r2 * r2;
+            }
+        H.X.alloc(&c, std::max<size_t>(1, xtot) * sizeof(double));
+        H.G.alloc(&c, std::max<size_t>(1, xtot) * sizeof(double));
+        if (L == 0) continue;
+        if (H.nystrom()) {
+            // Every block's U and V are row ranges of A = K(x,P) and
+            // B = K(x,P)K(P,P)^{-1}, and every W_l is block-diagonal over whole
+            // nodes, so the transformed panels of all higher levels stay row
+            // ranges of W_l^{-1}...W_1^{-1} L^{-1} [A B]: the cascade updates
+            // these 2r columns instead of every higher level's panel (same W).
+            const int r = H.nys_r;
+            H.P.alloc(&c, (size_t)n * H.Rtot * sizeof(double));
+            H.AB = static_cast<double*>(c.scratch((size_t)n * 2 * r * sizeof(double)));
+            abinit.push_back({H.kxp.as<double>(), (int)n, H.AB, (int)n, (int)n, r, 1.0});
+            abinit.push_back({H.ynys.as<double>(), (int)n, H.AB + (size_t)n * r, (int)n, (int)n, r, 1.0});
+            lt.push_back({h, H.AB, n, 2 * r});
+        } else {
+            // leaf inverse on every level's U and V (symfact.jl:21)
+            lt.push_back({h, H.P.as<double>(), n, (int)H.Rtot});
         }
     }
-    if (AB) c.scratch_free(AB);
-    double* dsum = parts + nl + nblk;
-    sum_vector(c, parts, nl + nblk, dsum);
-    double hs = 0.0;
-    KMH_HIP(hipMemcpyAsync(&hs, dsum, sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    c.scratch_free(parts);
-    check_info(c, "Cholesky factorization failed.");
-    H.logdet = 2.0 * hs;
-    H.factorized = true;
+    // leaf Cholesky (symfact.jl:17)
+    potrf(c, lm);
+    log_diag_sums(c, lm, lm_out);
+    copy2d(c, abinit);
+    leaf_trsm_batch(c, false, lt);
+}
+
+// logdet accumulation, failure flags, release of the transient state.
+void factor_end_batch(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<Status>& st) {
+    std::vector<SumJob> sj;
+    for (Hodlr* h : hs) sj.push_back({h->parts, h->nparts, h->parts + h->nparts});
+    sum_vectors(c, sj);
+    std::vector<double> sums(hs.size());
+    for (size_t i = 0; i < hs.size(); ++i)
+        KMH_HIP(hipMemcpyAsync(&sums[i], hs[i]->parts + hs[i]->nparts, sizeof(double), hipMemcpyDeviceToHost,
+                               c.stream));
+    st.assign(hs.size(), Status());
+    collect_info(c, hs, st, "Cholesky factorization failed.");
+    for (size_t j = 0; j < hs.size(); ++j) {
+        Hodlr& H = *hs[j];
+        if (H.AB) c.scratch_free(H.AB);
+        c.scratch_free(H.parts);
+        H.AB = nullptr;
+        H.parts = nullptr;
+        if (st[j].code != KMH_OK) {
+            KMH_HIP(hipMemsetAsync(H.info.p, 0, sizeof(int), c.stream));
+        } else {
+            H.logdet = 2.0 * sums[j];
+            H.factorized = true;
+        }
+    }
+}
+
+void hodlr_factorize_batch(Ctx& c, const std::vector<Hodlr*>& hs_in, std::vector<Status>& st) {
+    c.set_device();
+    st.assign(hs_in.size(), Status());
+    std::vector<Hodlr*> hs;
+    std::vector<size_t> pos;
+    for (size_t i = 0; i < hs_in.size(); ++i) {
+        Hodlr* h = hs_in[i];
+        if (!h) { st[i].code = KMH_E_BAD_ARG; st[i].msg = "null HODLR handle"; continue; }
+        if (h->factorized) { st[i].code = KMH_E_STATE; st[i].msg = "The matrix is already factorized."; continue; }
+        hs.push_back(h);
+        pos.push_back(i);
+    }
+    factor_begin_batch(c, hs);
+    int maxL = 0;
+    for (Hodlr* h : hs) maxL = std::max(maxL, h->L);
+    for (int lv = 0; lv < maxL; ++lv) {
+        std::vector<Hodlr*> act;
+        for (Hodlr* h : hs)
+            if (h->L > lv) act.push_back(h);
+        factor_level_batch(c, act, std::vector<int>(act.size(), lv));
+    }
+    std::vector<Status> st2;
+    factor_end_batch(c, hs, st2);
+    for (size_t j = 0; j < hs.size(); ++j) st[pos[j]] = st2[j];
+}
+
+void hodlr_factorize(Hodlr& H) {
+    std::vector<Status> st;
+    hodlr_factorize_batch(*H.ctx, {&H}, st);
+    if (st[0].code != KMH_OK) throw Error(st[0].code, st[0].msg);
+}
+
+// ---------------------------------------------------------------------------
+// solve and factor applies
+// ---------------------------------------------------------------------------
+
+void hodlr_solve_batch(Ctx& c, const std::vector<Hodlr*>& hs, const std::vector<double*>& T,
+                       const std::vector<int>& nrhs) {
+    std::vector<Rhs> rs;
+    int maxL = 0;
+    for (size_t i = 0; i < hs.size(); ++i) {
+        KMH_CHECK(hs[i]->factorized, KMH_E_NOT_FACTORIZED, "No solves without factorization.");
+        rs.push_back({hs[i], T[i], hs[i]->n, nrhs[i]});
+        maxL = std::max(maxL, hs[i]->L);
+    }
+    // W^{-1}: leaves, then levels 1..L; W^{-T}: levels L..1, then leaves
+    leaf_trsm_batch(c, false, rs);
+    for (int l = 0; l < maxL; ++l) {
+        std::vector<Rhs> a;
+        for (auto x : rs)
+            if (x.H->L > l) { x.lv = l; a.push_back(x); }
+        woodbury_batch(c, W_INV, a);
+    }
+    for (int l = maxL - 1; l >= 0; --l) {
+        std::vector<Rhs> a;
+        for (auto x : rs)
+            if (x.H->L > l) { x.lv = l; a.push_back(x); }
+        woodbury_batch(c, W_INVT, a);
+    }
+    leaf_trsm_batch(c, true, rs);
 }
 
 void hodlr_factor_apply(Hodlr& H, int op, double* T, int nrhs) {
     KMH_CHECK(H.factorized, KMH_E_NOT_FACTORIZED, "No solves without factorization.");
-    const int64_t n = H.n;
+    Ctx& c = *H.ctx;
+    const std::vector<Rhs> rs{{&H, T, H.n, nrhs, 0}};
+    auto at = [&](int l) { return std::vector<Rhs>{{&H, T, H.n, nrhs, l}}; };
     switch (op) {
         case KMH_WINV:
-            leaf_trsm(H, false, T, n, nrhs);
-            for (int l = 0; l < H.L; ++l) woodbury(H, l, W_INV, T, n, nrhs);
+            leaf_trsm_batch(c, false, rs);
+            for (int l = 0; l < H.L; ++l) woodbury_batch(c, W_INV, at(l));
             break;
         case KMH_WTINV:
-            for (int l = H.L - 1; l >= 0; --l) woodbury(H, l, W_INVT, T, n, nrhs);
-            leaf_trsm(H, true, T, n, nrhs);
+            for (int l = H.L - 1; l >= 0; --l) woodbury_batch(c, W_INVT, at(l));
+            leaf_trsm_batch(c, true, rs);
             break;
         case KMH_W:
-            for (int l = H.L - 1; l >= 0; --l) woodbury(H, l, W_MUL, T, n, nrhs);
+            for (int l = H.L - 1; l >= 0; --l) woodbury_batch(c, W_MUL, at(l));
             leaf_mul(H, false, T, nrhs);
             break;
         case KMH_WT:
             leaf_mul(H, true, T, nrhs);
-            for (int l = 0; l < H.L; ++l) woodbury(H, l, W_MULT, T, n, nrhs);
+            for (int l = 0; l < H.L; ++l) woodbury_batch(c, W_MULT, at(l));
             break;
         default:
             throw Error(KMH_E_BAD_ARG, "unknown factor operation");
     }
 }
 
-void hodlr_solve(Hodlr& H, double* T, int nrhs) {
-    KMH_CHECK(H.factorized, KMH_E_NOT_FACTORIZED, "No solves without factorization.");
-    hodlr_factor_apply(H, KMH_WINV, T, nrhs);
-    hodlr_factor_apply(H, KMH_WTINV, T, nrhs);
-}
+void hodlr_solve(Hodlr& H, double* T, int nrhs) { hodlr_solve_batch(*H.ctx, {&H}, {T}, {nrhs}); }
 
 void hodlr_matvec(Hodlr& H, const double* x, double* y, int nrhs) {
     Ctx& c = *H.ctx;
@@ -511,7 +722,7 @@ void hodlr_matvec(Hodlr& H, const double* x, double* y, int nrhs) {
     for (int i = 0; i < H.nleaves(); ++i) {
         const int m = (int)H.leaf_size(i);
         const int64_t s = H.leaf_off[i];
-        g.push_back({m, nrhs, m, leaf_ptr(H, i), m, x + s, (int)n, y + s, (int)n, 1.0, 0.0, 0});
+        g.push_back({m, nrhs, m, H.leaf_ptr(i), m, x + s, (int)n, y + s, (int)n, 1.0, 0.0, 0});
     }
     gemm(c, false, false, g);
     if (H.L == 0) return;

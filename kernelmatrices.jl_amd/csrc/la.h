@@ -26,9 +26,10 @@ struct SqMat {
     double* A;
     int n;
     int ld;
+    int* info = nullptr;   // failure flag for potrf (null: the context's flag)
 };
 // In-place lower Cholesky of each matrix; the strict upper triangle is
-// zeroed.  A non-positive or NaN pivot sets ctx.d_info[0] = 1.
+// zeroed.  A non-positive or NaN pivot sets *info (or ctx.d_info[0]) to 1.
 void potrf(Ctx& c, const std::vector<SqMat>& mats);
 
 // B <- L^{-1} B (trans = false) or L^{-T} B (trans = true), L lower n x n.
@@ -74,8 +75,16 @@ void add_diag(Ctx& c, const std::vector<DiagOp>& mats);       // A += v * I
 
 // out[j] = sum_i log(A_j(i,i)) for each matrix j (deterministic order).
 void log_diag_sums(Ctx& c, const std::vector<SqMat>& mats, double* d_out);
+// *outs[j] = sum_i log(A_j(i,i)), one output address per matrix.
+void log_diag_sums(Ctx& c, const std::vector<SqMat>& mats, const std::vector<double*>& outs);
 // out = sum of d_in[0..count) in a fixed order (one workgroup).
 void sum_vector(Ctx& c, const double* d_in, int count, double* d_out);
+struct SumJob {
+    const double* in;
+    int count;
+    double* out;
+};
+void sum_vectors(Ctx& c, const std::vector<SumJob>& jobs);
 // out[j] = dot(X[:,j], Y[:,j]) for j < ncols (deterministic order).
 void column_dots(Ctx& c, const double* X, int ldx, const double* Y, int ldy,
                  int m, int ncols, double* d_out);

@@ -36,16 +36,11 @@ struct GemmDesc {
 constexpr int GT = 64;
 constexpr int GK = 16;
 
-__device__ __forceinline__ int find_desc_by_start(const int* starts, int stride_ints, int nd, int t) {
-    // starts are non-decreasing; returns the last desc whose start <= t
-    int lo = 0, hi = nd - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (starts[(size_t)mid * stride_ints] <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
+// Lane layout of V_MFMA_F64_16X16X4F64 on gfx950 (measured with one-hot
+// operands): A(i,k) is held by lane i + 16k, B(k,j) by lane j + 16k, and
+// accumulator register r of lane j + 16g holds D(4r + g, j).  (The FP32
+// 16x16x4 form instead holds D(4g + r, j).)
+__device__ __forceinline__ int mfma_d_row(int g, int r) { return 4 * r + g; }
 
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ ds, int nd) {
@@ -114,7 +109,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ ds, i
         for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int row = row0 + wr + mb * 16 + 4 * (lane >> 4) + j;
+                const int row = row0 + wr + mb * 16 + mfma_d_row(lane >> 4, j);
                 const int col = col0 + wc + nb * 16 + (lane & 15);
                 if (row < d.m && col < d.n) {
                     double* cp = d.C + row + (size_t)col * d.ldc;
@@ -160,14 +155,16 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
     for (const auto& p : probs)
         if (p.m > 0 && p.n > 0) total_tiles += ceil_div(p.m, GT) * ceil_div(p.n, GT);
     if (total_tiles == 0) return;
-    const long long target = 2048;
+    // Split-k decision from the problem's own shape only, so a problem gives
+    // bitwise the same result whatever else shares the launch.
     std::vector<int> split(probs.size(), 1), kchunk(probs.size(), 0);
     size_t wsz = 0;
     for (size_t i = 0; i < probs.size(); ++i) {
         const auto& p = probs[i];
         if (p.m <= 0 || p.n <= 0 || p.lower) continue;
-        if (p.k >= 1024 && total_tiles < target) {
-            long long s = std::min<long long>(ceil_div(p.k, 512), std::max<long long>(1, target / total_tiles));
+        const long long own = ceil_div(p.m, GT) * ceil_div(p.n, GT);
+        if (p.k >= 1024 && own < 64) {
+            long long s = std::min<long long>(ceil_div(p.k, 512), std::max<long long>(1, 256 / own));
             if (s > 1) {
                 int kc = (int)(ceil_div(ceil_div(p.k, s), GK) * GK);
                 s = ceil_div(p.k, kc);
@@ -237,9 +234,10 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
 // ---------------------------------------------------------------------------
 constexpr int PB = 64;
 
-__global__ __launch_bounds__(256) void k_potrf_diag(const SqMat* __restrict__ ms, int* info) {
+__global__ __launch_bounds__(256) void k_potrf_diag(const SqMat* __restrict__ ms, int* info0) {
     __shared__ double S[PB][PB + 1];   // S[col][row]
     const SqMat m = ms[blockIdx.x];
+    int* info = m.info ? m.info : info0;
     const int n = m.n, tid = threadIdx.x;
     for (int e = tid; e < PB * PB; e += 256) {
         const int i = e % PB, j = e / PB;
@@ -390,7 +388,7 @@ void potrf(Ctx& c, const std::vector<SqMat>& mats) {
             if (m.n <= k0) continue;
             const int nb = std::min(PB, m.n - k0);
             double* Akk = m.A + k0 + (size_t)k0 * m.ld;
-            diag.push_back({Akk, nb, m.ld});
+            diag.push_back({Akk, nb, m.ld, m.info});
             const int rest = m.n - k0 - nb;
             if (rest > 0) {
                 pan.push_back({Akk, nb, m.ld, Akk + nb, rest, m.ld, chunks, 0});
@@ -785,34 +783,53 @@ static void diag_launch(Ctx& c, const std::vector<DiagOp>& mats, bool set) {
 void set_identity(Ctx& c, const std::vector<DiagOp>& mats) { diag_launch(c, mats, true); }
 void add_diag(Ctx& c, const std::vector<DiagOp>& mats) { diag_launch(c, mats, false); }
 
-__global__ __launch_bounds__(256) void k_log_diag(const SqMat* __restrict__ ms, double* out) {
+struct LogDiagDesc {
+    const double* A;
+    int n, ld;
+    double* out;
+};
+
+__global__ __launch_bounds__(256) void k_log_diag(const LogDiagDesc* __restrict__ ms) {
     __shared__ double sh[4];
-    const SqMat m = ms[blockIdx.x];
+    const LogDiagDesc m = ms[blockIdx.x];
     double s = 0.0;
     for (int i = threadIdx.x; i < m.n; i += 256) s += log(m.A[i + (size_t)i * m.ld]);
     const double t = block_sum(s, sh);
-    if (threadIdx.x == 0) out[blockIdx.x] = t;
+    if (threadIdx.x == 0) *m.out = t;
+}
+
+void log_diag_sums(Ctx& c, const std::vector<SqMat>& mats, const std::vector<double*>& outs) {
+    if (mats.empty()) return;
+    std::vector<LogDiagDesc> ds;
+    for (size_t j = 0; j < mats.size(); ++j) ds.push_back({mats[j].A, mats[j].n, mats[j].ld, outs[j]});
+    LogDiagDesc* d = c.upload_vec(ds);
+    hipLaunchKernelGGL(k_log_diag, dim3((unsigned)ds.size()), dim3(256), 0, c.stream, d);
+    KMH_HIP(hipGetLastError());
 }
 
 void log_diag_sums(Ctx& c, const std::vector<SqMat>& mats, double* d_out) {
-    if (mats.empty()) return;
-    SqMat* d = c.upload_vec(mats);
-    hipLaunchKernelGGL(k_log_diag, dim3((unsigned)mats.size()), dim3(256), 0, c.stream, d, d_out);
-    KMH_HIP(hipGetLastError());
+    std::vector<double*> outs;
+    for (size_t j = 0; j < mats.size(); ++j) outs.push_back(d_out + j);
+    log_diag_sums(c, mats, outs);
 }
 
-__global__ __launch_bounds__(256) void k_sum_vec(const double* in, int count, double* out) {
+__global__ __launch_bounds__(256) void k_sum_vec(const SumJob* __restrict__ jobs) {
     __shared__ double sh[4];
+    const SumJob j = jobs[blockIdx.x];
     double s = 0.0;
-    for (int i = threadIdx.x; i < count; i += 256) s += in[i];
+    for (int i = threadIdx.x; i < j.count; i += 256) s += j.in[i];
     const double t = block_sum(s, sh);
-    if (threadIdx.x == 0) *out = t;
+    if (threadIdx.x == 0) *j.out = t;
 }
 
-void sum_vector(Ctx& c, const double* d_in, int count, double* d_out) {
-    hipLaunchKernelGGL(k_sum_vec, dim3(1), dim3(256), 0, c.stream, d_in, count, d_out);
+void sum_vectors(Ctx& c, const std::vector<SumJob>& jobs) {
+    if (jobs.empty()) return;
+    SumJob* d = c.upload_vec(jobs);
+    hipLaunchKernelGGL(k_sum_vec, dim3((unsigned)jobs.size()), dim3(256), 0, c.stream, d);
     KMH_HIP(hipGetLastError());
 }
+
+void sum_vector(Ctx& c, const double* d_in, int count, double* d_out) { sum_vectors(c, {{d_in, count, d_out}}); }
 
 __global__ __launch_bounds__(256) void k_coldots(const double* X, int ldx, const double* Y, int ldy, int m,
                                                  double* out) {

@@ -1,0 +1,337 @@
+// Serving loop for GP likelihood requests with continuous batching.
+//
+// A request is one nll_objective evaluation (maxlikfunctions.jl:29-50):
+// build the Nystrom (or ACA) HODLR, symmetricfactorize!, negloglik and,
+// when derivative kernels are given, stoch_gradient.  Requests wait in a
+// queue and are admitted while their HBM footprint fits the budget and a
+// slot is free.  Each step advances every in-flight request by one stage
+// (build, leaf stage, one tree level, finish); the stages of all in-flight
+// requests run together in the grouped launches of the batched primitives,
+// so requests that arrive while others are on the GPU join at the next step
+// and their work overlaps.  A request's result is the same alone and in the
+// mix: every reduction has a fixed order independent of the other requests.
+#include <algorithm>
+#include <chrono>
+#include <deque>
+#include <map>
+#include <memory>
+
+#include "hodlr.h"
+#include "la.h"
+#include "serve.h"
+
+namespace kmh {
+
+static double now_us() {
+    using namespace std::chrono;
+    return (double)duration_cast<nanoseconds>(steady_clock::now().time_since_epoch()).count() * 1e-3;
+}
+
+enum Stage { ST_WAIT = 0, ST_BUILD, ST_LEAF, ST_LEVEL, ST_FINISH, ST_DONE };
+
+struct Req {
+    int64_t id = 0;
+    int stage = ST_WAIT;
+    int lv = 0;
+    std::vector<double> pts, y, saa;
+    int64_t n = 0;
+    int nv = 0;
+    kmh_kernel kern{};
+    kmh_build_opts opts{};
+    std::vector<kmh_kernel> dks;
+    std::unique_ptr<Hodlr> H;
+    uint64_t footprint = 0;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    Status st;
+    double nll = 0, logdet = 0;
+    std::vector<double> grad;
+};
+
+struct Server {
+    Ctx* ctx = nullptr;
+    int max_inflight = 1;
+    uint64_t budget = 0;
+    uint64_t reserved = 0;
+    int64_t next_id = 1;
+    std::deque<Req*> waiting;
+    std::vector<Req*> inflight;
+    std::map<int64_t, std::unique_ptr<Req>> all;
+};
+
+// Device bytes a request holds while in flight (FP64): points, leaves,
+// Nystrom panels, level slabs, per-block factors and the transient cascade
+// panels; ACA panels are bounded by maxrank (or the leaf size).
+uint64_t request_footprint(int64_t n, const kmh_build_opts& o, int nv) {
+    const int fl = [&] { int l = -1; int64_t m = n; while (m) { m >>= 1; ++l; } return l; }();
+    int L = o.level_kind == KMH_FIXED_LEVEL ? std::min(std::max(0, o.level), fl)
+                                            : std::min(std::max(0, fl - o.level), fl);
+    const int64_t nleaf = int64_t(1) << L;
+    const int64_t m = (n + nleaf - 1) / nleaf;
+    int r = o.maxrank;
+    if (o.method == KMH_ACA && r == 0) r = (int)std::min<int64_t>(m, 512);
+    uint64_t b = 16 * (uint64_t)n + 8 * (uint64_t)n * m;
+    b += 8 * (uint64_t)n * r * (uint64_t)L;                       // level slabs
+    b += 8 * (uint64_t)(nleaf - 1) * 2 * (uint64_t)(4 * r * r);   // X and G
+    if (o.method == KMH_NYSTROM) b += 8 * (uint64_t)n * r * 4 + 16 * (uint64_t)r * r;
+    b += 8 * (uint64_t)n * (2 + nv) * 2;                          // right-hand sides
+    return b;
+}
+
+static void finish_batch(Server& S, std::vector<Req*>& rs) {
+    Ctx& c = *S.ctx;
+    std::vector<Hodlr*> hs;
+    for (Req* q : rs) hs.push_back(q->H.get());
+    std::vector<Status> st;
+    factor_end_batch(c, hs, st);
+    std::vector<Hodlr*> ok;
+    std::vector<Req*> okr;
+    for (size_t i = 0; i < rs.size(); ++i) {
+        if (st[i].code != KMH_OK) rs[i]->st = st[i];
+        else { ok.push_back(hs[i]); okr.push_back(rs[i]); }
+    }
+    // negloglik = 0.5 logdet + 0.5 y' K^{-1} y  (maxlikfunctions.jl:2-9)
+    if (!ok.empty()) {
+        size_t tot = 0;
+        for (Hodlr* h : ok) tot += 2 * (size_t)h->n;
+        double* buf = static_cast<double*>(c.scratch((tot + ok.size()) * sizeof(double)));
+        double* dots = buf + tot;
+        std::vector<double*> T;
+        std::vector<int> k;
+        size_t off = 0;
+        for (size_t i = 0; i < ok.size(); ++i) {
+            const int64_t n = ok[i]->n;
+            KMH_HIP(hipMemcpyAsync(buf + off, okr[i]->y.data(), n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+            KMH_HIP(hipMemcpyAsync(buf + off + n, buf + off, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+            T.push_back(buf + off + n);
+            k.push_back(1);
+            off += 2 * n;
+        }
+        hodlr_solve_batch(c, ok, T, k);
+        off = 0;
+        for (size_t i = 0; i < ok.size(); ++i) {
+            const int64_t n = ok[i]->n;
+            column_dots(c, buf + off, (int)n, buf + off + n, (int)n, (int)n, 1, dots + i);
+            off += 2 * n;
+        }
+        std::vector<double> hd(ok.size());
+        KMH_HIP(hipMemcpyAsync(hd.data(), dots, ok.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        c.scratch_free(buf);
+        for (size_t i = 0; i < ok.size(); ++i) {
+            okr[i]->logdet = ok[i]->logdet;
+            okr[i]->nll = 0.5 * ok[i]->logdet + 0.5 * hd[i];
+        }
+        // stochastic gradient (stochasticgradient.jl:30-45)
+        for (Req* q : okr) {
+            if (q->dks.empty()) continue;
+            q->grad.assign(q->dks.size(), 0.0);
+            try {
+                hodlr_stoch_gradient(*q->H, q->dks.data(), (int)q->dks.size(), q->y.data(), q->saa.data(), q->nv,
+                                     q->grad.data());
+            } catch (const Error& e) {
+                q->st.code = e.code;
+                q->st.msg = e.what();
+            }
+        }
+    }
+    for (Req* q : rs) q->stage = ST_DONE;
+}
+
+int server_step(Server& S) {
+    Ctx& c = *S.ctx;
+    c.set_device();
+    // admission by footprint and slots
+    std::vector<Req*> fresh;
+    while (!S.waiting.empty() && (int)S.inflight.size() < S.max_inflight) {
+        Req* q = S.waiting.front();
+        if (S.budget && S.reserved + q->footprint > S.budget) {
+            if (S.inflight.empty() && fresh.empty()) {
+                // can never fit: fail it instead of blocking the queue
+                S.waiting.pop_front();
+                q->st.code = KMH_E_BUDGET;
+                q->st.msg = "request footprint exceeds the HBM budget";
+                q->stage = ST_DONE;
+                q->t_done = now_us();
+                continue;
+            }
+            break;
+        }
+        S.waiting.pop_front();
+        S.reserved += q->footprint;
+        q->t_admit = now_us();
+        q->stage = ST_BUILD;
+        S.inflight.push_back(q);
+        fresh.push_back(q);
+    }
+    // group in-flight requests by stage (each advances one stage this step)
+    std::vector<Req*> build, leaf, level, fin;
+    for (Req* q : S.inflight) {
+        switch (q->stage) {
+            case ST_BUILD: build.push_back(q); break;
+            case ST_LEAF: leaf.push_back(q); break;
+            case ST_LEVEL: level.push_back(q); break;
+            case ST_FINISH: fin.push_back(q); break;
+            default: break;
+        }
+    }
+    if (!level.empty()) {
+        std::vector<Hodlr*> hs;
+        std::vector<int> lvs;
+        for (Req* q : level) { hs.push_back(q->H.get()); lvs.push_back(q->lv); }
+        factor_level_batch(c, hs, lvs);
+        for (Req* q : level) {
+            if (++q->lv >= q->H->L) q->stage = ST_FINISH;
+        }
+    }
+    if (!leaf.empty()) {
+        std::vector<Hodlr*> hs;
+        for (Req* q : leaf) hs.push_back(q->H.get());
+        factor_begin_batch(c, hs);
+        for (Req* q : leaf) {
+            q->lv = 0;
+            q->stage = q->H->L > 0 ? ST_LEVEL : ST_FINISH;
+        }
+    }
+    if (!build.empty()) {
+        std::vector<BuildReq> br;
+        for (Req* q : build) br.push_back({q->pts.data(), false, q->n, 2, q->kern, q->opts});
+        std::vector<Status> st;
+        auto hs = hodlr_build_batch(c, br, st);
+        for (size_t i = 0; i < build.size(); ++i) {
+            if (hs[i]) {
+                build[i]->H.reset(hs[i]);
+                build[i]->stage = ST_LEAF;
+            } else {
+                build[i]->st = st[i];
+                build[i]->stage = ST_DONE;
+            }
+        }
+    }
+    if (!fin.empty()) finish_batch(S, fin);
+    // retire finished requests
+    int done = 0;
+    std::vector<Req*> keep;
+    for (Req* q : S.inflight) {
+        if (q->stage == ST_DONE) {
+            q->t_done = now_us();
+            q->H.reset();
+            S.reserved -= q->footprint;
+            ++done;
+        } else {
+            keep.push_back(q);
+        }
+    }
+    S.inflight.swap(keep);
+    return done;
+}
+
+}  // namespace kmh
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+struct kmh_server {
+    kmh::Server s;
+};
+
+#define SRV_TRY(...)                                                \
+    try {                                                           \
+        __VA_ARGS__;                                                \
+        return KMH_OK;                                              \
+    } catch (const kmh::Error& e) {                                 \
+        kmh::set_last_error(e.what());                              \
+        return e.code;                                              \
+    } catch (const std::exception& e) {                             \
+        kmh::set_last_error(e.what());                              \
+        return KMH_E_DEVICE;                                        \
+    }
+
+extern "C" {
+
+int kmh_server_create(kmh_ctx* ctx, int32_t max_inflight, uint64_t hbm_budget_bytes, kmh_server** out) {
+    SRV_TRY({
+        KMH_CHECK(ctx && out && max_inflight >= 1, KMH_E_BAD_ARG, "invalid server arguments");
+        auto* s = new kmh_server();
+        s->s.ctx = &ctx->c;
+        s->s.max_inflight = max_inflight;
+        s->s.budget = hbm_budget_bytes;
+        *out = s;
+    })
+}
+
+int kmh_server_destroy(kmh_server* srv) {
+    SRV_TRY({
+        if (srv) {
+            srv->s.ctx->sync();
+            delete srv;
+        }
+    })
+}
+
+int kmh_server_submit(kmh_server* srv, const kmh_serve_req* rq, int64_t* id) {
+    SRV_TRY({
+        KMH_CHECK(srv && rq && id && rq->pts && rq->y, KMH_E_BAD_ARG, "null argument");
+        KMH_CHECK(rq->n >= 2 && rq->d == 2, KMH_E_BAD_ARG, "invalid request size");
+        KMH_CHECK(rq->ndk >= 0 && rq->ndk <= KMH_SERVE_MAX_PARAMS, KMH_E_BAD_ARG, "too many derivative kernels");
+        KMH_CHECK(rq->ndk == 0 || (rq->saa && rq->nv >= 1), KMH_E_BAD_ARG, "gradient requests need SAA vectors");
+        kmh::check_kernel(rq->kernel, false);
+        std::unique_ptr<kmh::Req> q(new kmh::Req());
+        q->id = srv->s.next_id++;
+        q->n = rq->n;
+        q->pts.assign(rq->pts, rq->pts + 2 * rq->n);
+        q->y.assign(rq->y, rq->y + rq->n);
+        q->kern = rq->kernel;
+        q->opts = rq->opts;
+        q->nv = rq->ndk ? (int)rq->nv : 0;
+        if (rq->ndk) {
+            q->saa.assign(rq->saa, rq->saa + (size_t)rq->n * rq->nv);
+            for (int j = 0; j < rq->ndk; ++j) {
+                kmh::check_kernel(rq->dks[j], true);
+                q->dks.push_back(rq->dks[j]);
+            }
+        }
+        q->footprint = kmh::request_footprint(rq->n, rq->opts, q->nv);
+        q->t_submit = kmh::now_us();
+        *id = q->id;
+        srv->s.waiting.push_back(q.get());
+        srv->s.all[q->id] = std::move(q);
+    })
+}
+
+int kmh_server_step(kmh_server* srv, int32_t* completed) {
+    SRV_TRY({
+        KMH_CHECK(srv, KMH_E_BAD_ARG, "null server");
+        const int d = kmh::server_step(srv->s);
+        if (completed) *completed = d;
+    })
+}
+
+int kmh_server_pending(kmh_server* srv, int32_t* waiting, int32_t* inflight, uint64_t* reserved_bytes) {
+    SRV_TRY({
+        KMH_CHECK(srv, KMH_E_BAD_ARG, "null server");
+        if (waiting) *waiting = (int32_t)srv->s.waiting.size();
+        if (inflight) *inflight = (int32_t)srv->s.inflight.size();
+        if (reserved_bytes) *reserved_bytes = srv->s.reserved;
+    })
+}
+
+int kmh_server_result(kmh_server* srv, int64_t id, kmh_serve_result* out) {
+    SRV_TRY({
+        KMH_CHECK(srv && out, KMH_E_BAD_ARG, "null argument");
+        auto it = srv->s.all.find(id);
+        KMH_CHECK(it != srv->s.all.end(), KMH_E_BAD_ARG, "unknown request id");
+        kmh::Req& q = *it->second;
+        KMH_CHECK(q.stage == kmh::ST_DONE, KMH_E_STATE, "request not finished");
+        out->status = q.st.code;
+        out->nll = q.nll;
+        out->logdet = q.logdet;
+        out->ngrad = (int32_t)q.grad.size();
+        for (int j = 0; j < KMH_SERVE_MAX_PARAMS; ++j) out->grad[j] = j < (int)q.grad.size() ? q.grad[j] : 0.0;
+        out->queue_us = q.t_admit > 0 ? q.t_admit - q.t_submit : 0.0;
+        out->latency_us = q.t_done - q.t_submit;
+        if (q.st.code != KMH_OK) kmh::set_last_error(q.st.msg);
+        srv->s.all.erase(it);
+    })
+}
+
+}  // extern "C"

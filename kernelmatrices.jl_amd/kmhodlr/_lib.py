@@ -3,7 +3,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libkmhodlr.so")
+LIB_PATH = os.environ.get("KMHODLR_LIB", os.path.join(HERE, "libkmhodlr.so"))
 
 KMH_OK = 0
 KMH_E_BAD_ARG = 1
@@ -32,6 +32,26 @@ class kmh_build_opts(C.Structure):
     _fields_ = [("method", C.c_int32), ("level_kind", C.c_int32), ("level", C.c_int32),
                 ("maxrank", C.c_int32), ("tol", C.c_double), ("precision", C.c_int32),
                 ("reserved", C.c_int32 * 3)]
+
+
+class kmh_build_req(C.Structure):
+    _fields_ = [("pts", C.c_void_p), ("n", C.c_int64), ("d", C.c_int32), ("on_device", C.c_int32),
+                ("kernel", kmh_kernel), ("opts", kmh_build_opts)]
+
+
+KMH_SERVE_MAX_PARAMS = 4
+
+
+class kmh_serve_req(C.Structure):
+    _fields_ = [("pts", C.c_void_p), ("n", C.c_int64), ("d", C.c_int32), ("ndk", C.c_int32),
+                ("kernel", kmh_kernel), ("opts", kmh_build_opts), ("y", C.c_void_p), ("saa", C.c_void_p),
+                ("nv", C.c_int64), ("dks", kmh_kernel * KMH_SERVE_MAX_PARAMS)]
+
+
+class kmh_serve_result(C.Structure):
+    _fields_ = [("status", C.c_int32), ("ngrad", C.c_int32), ("nll", C.c_double), ("logdet", C.c_double),
+                ("grad", C.c_double * KMH_SERVE_MAX_PARAMS), ("queue_us", C.c_double),
+                ("latency_us", C.c_double)]
 
 
 class kmh_info(C.Structure):
@@ -70,6 +90,15 @@ SIGNATURES = {
     "kmh_hodlr_solve_dev": [_P, _P, _P, _I64],
     "kmh_hodlr_logdet": [_P, _D],
     "kmh_stoch_gradient": [_P, C.POINTER(kmh_kernel), C.c_int32, _P, _P, _I64, _P],
+    "kmh_hodlr_build_batch": [_P, C.c_int32, C.POINTER(kmh_build_req), C.POINTER(_P), C.POINTER(C.c_int32)],
+    "kmh_hodlr_factorize_batch": [_P, C.c_int32, C.POINTER(_P), C.POINTER(C.c_int32)],
+    "kmh_hodlr_solve_batch_dev": [_P, C.c_int32, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int64)],
+    "kmh_server_create": [_P, C.c_int32, C.c_uint64, C.POINTER(_P)],
+    "kmh_server_destroy": [_P],
+    "kmh_server_submit": [_P, C.POINTER(kmh_serve_req), C.POINTER(C.c_int64)],
+    "kmh_server_step": [_P, C.POINTER(C.c_int32)],
+    "kmh_server_pending": [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_uint64)],
+    "kmh_server_result": [_P, C.c_int64, C.POINTER(kmh_serve_result)],
 }
 
 # include/kmhodlr_testing.h

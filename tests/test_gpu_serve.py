@@ -1,0 +1,133 @@
+"""Serving loop: continuous batching of likelihood(+gradient) requests.
+Each result must be the same alone and in the mix, admission respects the
+HBM budget, and results match the oracle's nll_objective pieces."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle.kdsort import kd_sort
+
+pytestmark = pytest.mark.gpu
+
+
+def _mkreq(L, x, y, saa, params, level, rank, grad=True):
+    r = L.kmh_serve_req()
+    r.pts = x.ctypes.data
+    r.n = x.shape[0]
+    r.d = 2
+    r.kernel.family = L.KMH_MATERN32
+    r.kernel.params[0], r.kernel.params[1] = params
+    r.kernel.nugget = 1e-6
+    r.opts.method = L.KMH_NYSTROM
+    r.opts.level_kind = L.KMH_FIXED_LEVEL
+    r.opts.level = level
+    r.opts.maxrank = rank
+    r.opts.precision = L.KMH_FP64
+    r.y = y.ctypes.data
+    if grad:
+        r.ndk = 2
+        r.saa = saa.ctypes.data
+        r.nv = saa.shape[1]
+        for j, code in enumerate((L.KMH_D1, L.KMH_D2)):
+            r.dks[j].family = L.KMH_MATERN32
+            r.dks[j].deriv = code
+            r.dks[j].params[0], r.dks[j].params[1] = params
+    return r
+
+
+def _problems(n_list, seed=0, nv=8):
+    rng = np.random.default_rng(seed)
+    out = []
+    for n in n_list:
+        x = np.ascontiguousarray(kd_sort(rng.random((n, 2))))
+        y = rng.standard_normal(n)
+        saa = np.asfortranarray(rng.choice([-1.0, 1.0], size=(n, nv)))
+        params = (rng.uniform(0.5, 2.0), rng.uniform(0.05, 0.3))
+        level = max(1, int(np.log2(n / 256)))
+        out.append((x, y, saa, params, level))
+    return out
+
+
+def _serve(lib, L, ctx, probs, max_inflight, budget=0, order=None):
+    srv = C.c_void_p()
+    assert lib.kmh_server_create(ctx.ptr, max_inflight, budget, C.byref(srv)) == 0
+    ids = {}
+    keep = []
+    order = order or list(range(len(probs)))
+    for i in order:
+        x, y, saa, params, level = probs[i]
+        r = _mkreq(L, x, y, saa, params, level, 40)
+        keep.append(r)
+        rid = C.c_int64()
+        assert lib.kmh_server_submit(srv, C.byref(r), C.byref(rid)) == 0, lib.last_error()
+        ids[i] = rid.value
+    peak = 0
+    for _ in range(1000):
+        done = C.c_int32()
+        assert lib.kmh_server_step(srv, C.byref(done)) == 0, lib.last_error()
+        w, f, rb = C.c_int32(), C.c_int32(), C.c_uint64()
+        lib.kmh_server_pending(srv, C.byref(w), C.byref(f), C.byref(rb))
+        peak = max(peak, f.value)
+        if w.value == 0 and f.value == 0:
+            break
+    res = {}
+    for i, rid in ids.items():
+        out = L.kmh_serve_result()
+        rc = lib.kmh_server_result(srv, rid, C.byref(out))
+        res[i] = (rc, out.status, out.nll, out.logdet, list(out.grad[:out.ngrad]), out.latency_us)
+    lib.kmh_server_destroy(srv)
+    return res, peak
+
+
+def test_serve_alone_equals_mixed(gpu_lib):
+    import kmhodlr as km
+    from kmhodlr import _lib as L
+    lib = km.load()
+    ctx = km.Context(0)
+    probs = _problems([1200, 4096, 2500, 700, 3000])
+    alone, peak1 = _serve(lib, L, ctx, probs, 1)
+    mixed, peakm = _serve(lib, L, ctx, probs, 8, order=[3, 1, 4, 0, 2])
+    assert peak1 == 1 and peakm > 1
+    for i in range(len(probs)):
+        assert alone[i][0] == 0 and alone[i][1] == 0
+        assert alone[i][2] == mixed[i][2] and alone[i][3] == mixed[i][3]
+        assert alone[i][4] == mixed[i][4]
+        print("request", i, "nll", alone[i][2], "grad", alone[i][4])
+
+
+def test_serve_matches_direct_path(gpu_lib):
+    import kmhodlr as km
+    from kmhodlr import _lib as L
+    lib = km.load()
+    ctx = km.Context(0)
+    probs = _problems([2000, 1500], seed=3)
+    res, _ = _serve(lib, L, ctx, probs, 4)
+    for i, (x, y, saa, params, level) in enumerate(probs):
+        K = km.KernelMatrix(x, x, params, km.matern32(1e-6))
+        HK = km.KernelHODLR(K, 0.0, 40, km.FixedLevel(level), nystrom=True)
+        km.symmetricfactorize(HK)
+        nll = km.negloglik(HK, y)
+        g = km.stoch_gradient(K, HK, y, [K.kernel.d1, K.kernel.d2], saa)
+        assert abs(res[i][2] - nll) <= 1e-12 * abs(nll)
+        assert np.allclose(res[i][4], g, rtol=1e-12, atol=0)
+
+
+def test_serve_admission_budget(gpu_lib):
+    import kmhodlr as km
+    from kmhodlr import _lib as L
+    from kmhodlr import _lib
+    lib = km.load()
+    ctx = km.Context(0)
+    probs = _problems([2048, 2048, 2048, 2048], seed=5)
+    # budget for two of these requests at a time; per-request footprint as
+    # the server counts it (n=2048, 8 leaves of 256, r=40, 3 levels, nv=8)
+    n, m, r, L3, nv = 2048, 256, 40, 3, 8
+    fp = (16 * n + 8 * n * m + 8 * n * r * L3 + 8 * 7 * 2 * 4 * r * r + 8 * n * r * 4 + 16 * r * r
+          + 8 * n * (2 + nv) * 2)
+    res, peak = _serve(lib, L, ctx, probs, 8, budget=2 * fp + 1)
+    assert peak <= 2
+    assert all(r[1] == 0 for r in res.values())
+    # a request larger than the whole budget fails alone, without blocking
+    res2, _ = _serve(lib, L, ctx, probs[:2], 8, budget=1000)
+    assert all(r[1] == L.KMH_E_BUDGET for r in res2.values())
