@@ -1,15 +1,20 @@
-"""Benchmark: GP log-likelihood evaluations per second on one MI355X.
+"""Benchmark: GP log-likelihood evaluations per second on one MI355X,
+served (continuous batching) vs plain (one request at a time), n = 65536.
 
-Workload (n = 65536): points uniform in [0,1]^2 (numpy PCG64 seed 0),
-KD-sorted; Matern nu=3/2 kernel sigma^2 = 1, rho = 0.1, 1e-6 nugget;
-FixedLevel(8) (256 leaves of 256 points); Nystrom rank 72 (default) or ACA
-tol 1e-10.  One step = one log-likelihood evaluation, negloglik
-(maxlikfunctions.jl:2-9): HODLR build + symmetricfactorize! + logdet + one
-solve against y ~ N(0,1), all FP64 on the GPU, inputs resident in HBM.
+Workload: points uniform in [0,1]^2 (numpy PCG64 seed 0), KD-sorted,
+y ~ N(0,1); Matern nu=3/2 with a 1e-6 nugget; FixedLevel(8) (256 leaves of
+256 points); Nystrom rank 72 (the served objective of the reference,
+nll_objective, always uses Nystrom).  A request is one log-likelihood
+evaluation, negloglik (maxlikfunctions.jl:2-9): HODLR build +
+symmetricfactorize! + logdet + one solve, FP64 on the GPU, at its own kernel
+parameters (sigma ~ U[0.5, 2], rho ~ U[0.05, 0.2]), inputs resident in HBM.
 
-Prints one JSON line with throughput, p50/p99 latency, the roofline of the
-grouped FP64 MFMA GEMM (the dominant kernel, timed with HIP events on the
-library's stream) and a CPU baseline (the numpy oracle, one BLAS thread).
+Legs: `plain` runs the requests one after another through the per-request
+C ABI; `served` keeps `--inflight` requests in the server, which advances
+all of them stage by stage in shared grouped launches.  Both legs see the
+same requests; their logdets are compared.  The JSON line carries the
+roofline of the grouped FP64 MFMA GEMM (the dominant kernel; HIP events on
+the library's stream) and a CPU baseline (the numpy oracle, one BLAS thread).
 """
 import argparse
 import ctypes as C
@@ -36,6 +41,14 @@ def make_inputs(n, seed=0):
     return x, y
 
 
+def request_params(count, seed=1):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sig = rng.uniform(0.5, 2.0, count)
+    rho = rng.uniform(0.05, 0.2, count)
+    return [(float(s * s), float(r)) for s, r in zip(sig, rho)]
+
+
 def factor_flops(n, m, ranks, nblocks):
     """Algorithmic FP64 flops of symmetricfactorize! with structured M
     (leaf potrf + leaf trsm + panel QR + capacitance + Woodbury cascade)."""
@@ -50,36 +63,38 @@ def factor_flops(n, m, ranks, nblocks):
     return f
 
 
-def cpu_baseline(n, lvl, method, budget_s=25.0):
+def cpu_baseline(n, lvl, params):
     """The oracle (numpy/scipy restatement of the reference) on the host,
-    one BLAS thread, timed on the same workload."""
-    import numpy as np
+    one BLAS thread, one evaluation."""
     from threadpoolctl import threadpool_limits
     from oracle import hodlr as H
     from oracle import kernels as kk
     x, y = make_inputs(n)
     with threadpool_limits(1):
         t0 = time.perf_counter()
-        K = H.KernelMatrix(x, x, kk.Kernel(kk.MATERN32, (1.0, 0.1), nugget=1e-6))
-        if method == "nystrom":
-            HK = H.KernelHODLR(K, 0.0, 72, H.FixedLevel(lvl), nystrom=True)
-        else:
-            HK = H.KernelHODLR(K, 1e-10, 0, H.FixedLevel(lvl))
+        K = H.KernelMatrix(x, x, kk.Kernel(kk.MATERN32, params, nugget=1e-6))
+        HK = H.KernelHODLR(K, 0.0, 72, H.FixedLevel(lvl), nystrom=True)
         H.symmetricfactorize(HK)
         nll = H.negloglik(HK, y)
         dt = time.perf_counter() - t0
-    return dt, nll, HK.logdet()
+    return dt, nll
+
+
+def pct(v, q):
+    s = sorted(v)
+    return s[min(len(s) - 1, max(0, int(math.ceil(q * len(s))) - 1))]
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=32, help="timed requests per leg")
+    ap.add_argument("--warmup", type=int, default=4, help="untimed requests per leg")
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--level", type=int, default=8)
-    ap.add_argument("--method", choices=["nystrom", "aca"], default="nystrom")
+    ap.add_argument("--inflight", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--legs", default="plain,served")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -102,28 +117,35 @@ def main():
     sol_d = torch.empty_like(y_d)
     torch.cuda.synchronize()
     ctx = km.Context(0)
-
-    kern = L.kmh_kernel()
-    kern.family = L.KMH_MATERN32
-    kern.deriv = 0
-    kern.params[0], kern.params[1] = 1.0, 0.1
-    kern.nugget = 1e-6
-    opts = L.kmh_build_opts()
-    opts.method = L.KMH_NYSTROM if args.method == "nystrom" else L.KMH_ACA
-    opts.level_kind = L.KMH_FIXED_LEVEL
-    opts.level = args.level
-    opts.maxrank = 72 if args.method == "nystrom" else 0
-    opts.tol = 1e-10
-    opts.precision = L.KMH_FP64
+    total = args.warmup + args.steps
+    params = request_params(total)
 
     def check(rc):
         if rc != 0:
             raise RuntimeError(lib.last_error())
 
-    info = {}
+    def kernel_of(p):
+        k = L.kmh_kernel()
+        k.family = L.KMH_MATERN32
+        k.deriv = 0
+        k.params[0], k.params[1] = p
+        k.nugget = 1e-6
+        return k
 
-    def one_eval():
+    opts = L.kmh_build_opts()
+    opts.method = L.KMH_NYSTROM
+    opts.level_kind = L.KMH_FIXED_LEVEL
+    opts.level = args.level
+    opts.maxrank = 72
+    opts.tol = 0.0
+    opts.precision = L.KMH_FP64
+    info = {}
+    out = {}
+
+    # ---- plain: one request at a time ----------------------------------
+    def plain_eval(p):
         h = C.c_void_p()
+        kern = kernel_of(p)
         check(lib.kmh_hodlr_build_dev(ctx.ptr, pts_d.data_ptr(), n, 2, C.byref(kern), C.byref(opts), C.byref(h)))
         check(lib.kmh_hodlr_factorize(h))
         ld = C.c_double()
@@ -134,88 +156,146 @@ def main():
         if not info:
             ii = L.kmh_info()
             check(lib.kmh_hodlr_info(h, C.byref(ii)))
-            nb = 2 ** ii.levels - 1
-            rk = (C.c_int32 * max(1, nb))()
-            check(lib.kmh_hodlr_ranks(h, rk))
             info["levels"] = ii.levels
-            info["ranks"] = list(rk)[:nb]
             info["device_bytes"] = ii.device_bytes
         check(lib.kmh_hodlr_destroy(h))
         return 0.5 * ld.value + 0.5 * quad, ld.value
 
-    for _ in range(args.warmup):
-        one_eval()
-    torch.cuda.synchronize()
-    check(lib.kmh_ctx_profile_begin(ctx.ptr))
-    lat = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s = time.perf_counter()
-        nll, ld = one_eval()
-        lat.append(time.perf_counter() - s)
-    torch.cuda.synchronize()
-    total = time.perf_counter() - t0
-    gfl, gms, gl = C.c_double(), C.c_double(), C.c_int64()
-    check(lib.kmh_ctx_profile_end(ctx.ptr, C.byref(gfl), C.byref(gms), C.byref(gl)))
+    legs = args.legs.split(",")
+    plain_ld = {}
+    if "plain" in legs:
+        for i in range(args.warmup):
+            plain_eval(params[i])
+        torch.cuda.synchronize()
+        check(lib.kmh_ctx_profile_begin(ctx.ptr))
+        lat = []
+        t0 = time.perf_counter()
+        for i in range(args.warmup, total):
+            s = time.perf_counter()
+            nll, ld = plain_eval(params[i])
+            lat.append(1e3 * (time.perf_counter() - s))
+            plain_ld[i] = ld
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        gfl, gms, gl = C.c_double(), C.c_double(), C.c_int64()
+        check(lib.kmh_ctx_profile_end(ctx.ptr, C.byref(gfl), C.byref(gms), C.byref(gl)))
+        out["plain"] = {"evals_per_s": args.steps / el, "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99),
+                        "gemm_tflops": gfl.value / (gms.value * 1e-3) / 1e12 if gms.value else 0.0,
+                        "gemm_flops_per_eval": gfl.value / args.steps, "gemm_ms_per_eval": gms.value / args.steps,
+                        "gemm_launches_per_eval": gl.value / args.steps}
 
-    lat_ms = sorted(1e3 * v for v in lat)
-    p50 = lat_ms[len(lat_ms) // 2]
-    p99 = lat_ms[min(len(lat_ms) - 1, int(math.ceil(0.99 * len(lat_ms))) - 1)]
-    evals_per_s = args.steps / total
-    gemm_tflops = gfl.value / (gms.value * 1e-3) / 1e12 if gms.value > 0 else 0.0
-    L_ = info["levels"]
+    # ---- served: continuous batching -----------------------------------
+    if "served" in legs:
+        srv = C.c_void_p()
+        check(lib.kmh_server_create(ctx.ptr, args.inflight, 0, C.byref(srv)))
+        reqs = []
+
+        def submit(i):
+            r = L.kmh_serve_req()
+            r.pts = pts_d.data_ptr()
+            r.n = n
+            r.d = 2
+            r.ndk = 0
+            r.on_device = 1
+            r.kernel = kernel_of(params[i])
+            r.opts = opts
+            r.y = y_d.data_ptr()
+            reqs.append(r)
+            rid = C.c_int64()
+            check(lib.kmh_server_submit(srv, C.byref(r), C.byref(rid)))
+            return rid.value
+
+        def run(idx, profile):
+            """closed loop: keep `inflight` requests outstanding"""
+            pending = list(idx)
+            live = {}
+            res = {}
+            if profile:
+                check(lib.kmh_ctx_profile_begin(ctx.ptr))
+            t0 = time.perf_counter()
+            while pending or live:
+                while pending and len(live) < args.inflight:
+                    i = pending.pop(0)
+                    live[submit(i)] = i
+                done = C.c_int32()
+                check(lib.kmh_server_step(srv, C.byref(done)))
+                if done.value:
+                    for rid in list(live):
+                        o = L.kmh_serve_result()
+                        rc = lib.kmh_server_result(srv, rid, C.byref(o))
+                        if rc == L.KMH_E_STATE:
+                            continue
+                        check(rc)
+                        if o.status != 0:
+                            raise RuntimeError("request failed: %s" % lib.last_error())
+                        res[live.pop(rid)] = (o.nll, o.logdet, o.latency_us * 1e-3)
+            el = time.perf_counter() - t0
+            prof = None
+            if profile:
+                gfl, gms, gl = C.c_double(), C.c_double(), C.c_int64()
+                check(lib.kmh_ctx_profile_end(ctx.ptr, C.byref(gfl), C.byref(gms), C.byref(gl)))
+                prof = (gfl.value, gms.value, gl.value)
+            return res, el, prof
+
+        run(range(args.warmup), False)
+        torch.cuda.synchronize()
+        res, el, prof = run(range(args.warmup, total), True)
+        lat = [v[2] for v in res.values()]
+        out["served"] = {"evals_per_s": args.steps / el, "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99),
+                         "gemm_tflops": prof[0] / (prof[1] * 1e-3) / 1e12 if prof[1] else 0.0,
+                         "gemm_flops_per_eval": prof[0] / args.steps, "gemm_ms_per_eval": prof[1] / args.steps,
+                         "gemm_launches_total": prof[2], "inflight": args.inflight}
+        if plain_ld:
+            errs = [abs(res[i][1] - plain_ld[i]) / abs(plain_ld[i]) for i in plain_ld]
+            out["served"]["max_rel_logdet_err_vs_plain"] = max(errs)
+        lib.kmh_server_destroy(srv)
+
+    head = out.get("served") or out.get("plain")
+    L_ = info.get("levels", args.level)
     nblocks = [2 ** (L_ - l - 1) for l in range(L_)]
-    ranks_lv, q = [], 0
-    for l in range(L_):
-        ranks_lv.append(max(info["ranks"][q:q + nblocks[l]]))
-        q += nblocks[l]
     m = n // (2 ** L_)
-    ff = factor_flops(n, m, ranks_lv, nblocks)
-    out = {
+    ff = factor_flops(n, m, [72] * L_, nblocks)
+    line = {
         "metric": "GP loglik evals/s + p99 latency, n=2^16, served vs plain GPU at 1e-8 rel err",
-        "value": round(evals_per_s, 4),
+        "value": round(head["evals_per_s"], 4),
         "unit": "evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * total / args.steps, 3),
-        "p50_ms": round(p50, 3),
-        "p99_ms": round(p99, 3),
+        "ms_per_step": round(1e3 / head["evals_per_s"], 3),
+        "p50_ms": round(head["p50_ms"], 3),
+        "p99_ms": round(head["p99_ms"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "plain FP64 GPU form, %s, n=%d, matern32 rho=0.1 nugget=1e-6, FixedLevel(%d)"
-                               % ("nystrom r=72" if args.method == "nystrom" else "ACA tol=1e-10", n, args.level),
-                   "n": n, "method": args.method, "leaf": m, "ranks_per_level_max": ranks_lv},
-        "roofline": {"bound": "mfma", "kernel": "k_gemm (grouped FP64 GEMM, v_mfma_f64_16x16x4)",
-                     "achieved": round(gemm_tflops, 3), "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(gemm_tflops / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": None,
-                     "gemm_flops_per_eval": gfl.value / args.steps,
-                     "gemm_ms_per_eval": gms.value / args.steps,
-                     "gemm_launches_per_eval": gl.value / args.steps,
-                     "factor_flops_structured": ff,
-                     "eval_tflops_structured": round(ff / (1e-3 * p50) / 1e12, 3)},
-        "nll": nll,
-        "logdet": ld,
+        "config": {"workload": "n=65536 Matern-3/2 (sigma~U[0.5,2], rho~U[0.05,0.2], nugget 1e-6) "
+                               "Nystrom r=72 FixedLevel(8) negloglik requests, %s form"
+                               % ("served+batched FP64" if "served" in out else "plain FP64"),
+                   "n": n, "leaf": m, "nystrom_rank": 72, "levels": L_, "inflight": args.inflight},
+        "legs": {k: {kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in d.items()}
+                 for k, d in out.items()},
+        "roofline": {"bound": "mfma", "kernel": "k_gemm (grouped FP64 GEMM, v_mfma_f64_16x16x4f64)",
+                     "achieved": round(head["gemm_tflops"], 3), "peak": FP64_MFMA_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(head["gemm_tflops"] / FP64_MFMA_PEAK_TFLOPS, 4),
+                     "traffic": None, "factor_flops_structured_per_eval": ff},
     }
+    if "served" in out and "plain" in out:
+        line["speedup_served_vs_plain"] = round(out["served"]["evals_per_s"] / out["plain"]["evals_per_s"], 3)
     if not args.no_cpu:
-        # bounded CPU sample: the same eval at n=16384 (FixedLevel(6), same
-        # 256-point leaves), one BLAS thread, scaled to n=65536 by the ratio
-        # of structured factorization flops
+        # bounded CPU sample: one eval of the same workload shape at n=16384
+        # (FixedLevel(6), same 256-point leaves), scaled to n=65536 by the
+        # ratio of structured factorization flops
         ns, ls = 16384, args.level - 2
-        dt, nll_c, ld_c = cpu_baseline(ns, ls, args.method)
-        Ls = ls
-        nb_s = [2 ** (Ls - l - 1) for l in range(Ls)]
-        rk_s = ranks_lv[:Ls] if args.method == "aca" else [72] * Ls
-        ratio = ff / factor_flops(ns, ns // 2 ** Ls, rk_s, nb_s)
-        out["cpu_baseline"] = {"value": round(1.0 / (dt * ratio), 6), "unit": "evals/s", "cores": 1,
-                               "kind": "port",
-                               "sample": "oracle (numpy/scipy restatement) one eval at n=%d FixedLevel(%d) took "
-                                         "%.2f s on 1 thread; scaled by structured-flop ratio %.2f to n=%d"
-                                         % (ns, ls, dt, ratio, n)}
-    print(json.dumps(out))
+        dt, _ = cpu_baseline(ns, ls, params[0])
+        ratio = ff / factor_flops(ns, ns // 2 ** ls, [72] * ls, [2 ** (ls - l - 1) for l in range(ls)])
+        line["cpu_baseline"] = {"value": round(1.0 / (dt * ratio), 6), "unit": "evals/s", "cores": 1,
+                                "kind": "port",
+                                "sample": "oracle (numpy/scipy restatement, 1 BLAS thread) one eval at n=%d "
+                                          "FixedLevel(%d) took %.2f s; scaled by structured-flop ratio %.2f to n=%d"
+                                          % (ns, ls, dt, ratio, n)}
+    print(json.dumps(line))
 
 
 if __name__ == "__main__":

@@ -185,10 +185,14 @@ int kmh_hodlr_solve_batch_dev(kmh_ctx* ctx, int32_t count, kmh_hodlr* const* hs,
 #define KMH_SERVE_MAX_PARAMS 4
 typedef struct kmh_server kmh_server;
 typedef struct kmh_serve_req {
-    const double* pts;        /* n x 2 row-major, host                      */
+    const double* pts;        /* n x 2 row-major (host, or device if on_device) */
     int64_t n;
     int32_t d;
     int32_t ndk;              /* number of derivative kernels (0: no gradient) */
+    int32_t on_device;        /* pts and y are device pointers the caller keeps
+                                 alive until the request completes; saa stays
+                                 on the host                                   */
+    int32_t reserved;
     kmh_kernel kernel;
     kmh_build_opts opts;
     const double* y;          /* data vector, n                             */
@@ -217,6 +221,20 @@ int kmh_server_result(kmh_server* srv, int64_t id, kmh_serve_result* out);
  * dks: p derivative kernels; y: n; saa: n x nv column-major; grad: p        */
 int kmh_stoch_gradient(kmh_hodlr* h, const kmh_kernel* dks, int32_t p,
                        const double* y, const double* saa, int64_t nv, double* grad);
+
+/* ---- HODLR.stoch_fisher(K, HK, DKs, vecs)  stochastichessian.jl:91-115
+ * F: p x p column-major (symmetric)                                         */
+int kmh_stoch_fisher(kmh_hodlr* h, const kmh_kernel* dks, int32_t p, const double* saa, int64_t nv, double* F);
+
+/* ---- HODLR.stoch_hessian(K, HK, y, d1funs, d2funs, vecs)  stochastichessian.jl:117-147
+ * d2: p(p+1)/2 kernels in the order (1,1),(1,2)..(1,p),(2,2)..: family 0
+ * stands for the reference's ZeroFunction.  exact_second = 0 reproduces the
+ * reference's second-derivative block terms (HODLR/src/utils.jl:110-155);
+ * exact_second = 1 uses the exact second derivative of the Nystrom blocks
+ * (the reference's terms differ from it off the diagonal blocks).
+ * Hout: p x p column-major (symmetric)                                      */
+int kmh_stoch_hessian(kmh_hodlr* h, const kmh_kernel* d1, int32_t p, const kmh_kernel* d2,
+                      const double* y, const double* saa, int64_t nv, int32_t exact_second, double* Hout);
 
 #ifdef __cplusplus
 }

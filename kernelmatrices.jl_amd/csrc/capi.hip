@@ -1,6 +1,7 @@
 // C ABI entry points of libkmhodlr (declared in include/kmhodlr.h).
 #include <memory>
 
+#include "deriv.h"
 #include "hodlr.h"
 #include "la.h"
 
@@ -380,6 +381,25 @@ int kmh_stoch_gradient(kmh_hodlr* h, const kmh_kernel* dks, int32_t p, const dou
         KMH_CHECK(dks && y && grad && p >= 1, KMH_E_BAD_ARG, "null argument");
         KMH_CHECK(nv >= 1 && saa, KMH_E_BAD_ARG, "need at least one SAA vector");
         kmh::hodlr_stoch_gradient(H, dks, p, y, saa, nv, grad);
+    })
+}
+
+int kmh_stoch_fisher(kmh_hodlr* h, const kmh_kernel* dks, int32_t p, const double* saa, int64_t nv, double* F) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        KMH_CHECK(H.factorized, KMH_E_NOT_FACTORIZED, "The matrix needs to be factorized.");
+        KMH_CHECK(dks && saa && F && p >= 1 && nv >= 1, KMH_E_BAD_ARG, "null argument");
+        kmh::hodlr_stoch_fisher(H, dks, p, saa, nv, F);
+    })
+}
+
+int kmh_stoch_hessian(kmh_hodlr* h, const kmh_kernel* d1, int32_t p, const kmh_kernel* d2, const double* y,
+                      const double* saa, int64_t nv, int32_t exact_second, double* Hout) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        KMH_CHECK(H.factorized, KMH_E_NOT_FACTORIZED, "The matrix needs to be factorized.");
+        KMH_CHECK(d1 && d2 && y && saa && Hout && p >= 1 && nv >= 1, KMH_E_BAD_ARG, "null argument");
+        kmh::hodlr_stoch_hessian(H, d1, p, d2, y, saa, nv, exact_second != 0, Hout);
     })
 }
 

@@ -630,6 +630,7 @@ void hodlr_factorize_batch(Ctx& c, const std::vector<Hodlr*>& hs_in, std::vector
         hs.push_back(h);
         pos.push_back(i);
     }
+    if (hs.empty()) return;
     factor_begin_batch(c, hs);
     int maxL = 0;
     for (Hodlr* h : hs) maxL = std::max(maxL, h->L);

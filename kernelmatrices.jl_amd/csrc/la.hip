@@ -377,6 +377,7 @@ __global__ __launch_bounds__(256) void k_zero_upper(const SqMat* __restrict__ ms
 }
 
 void potrf(Ctx& c, const std::vector<SqMat>& mats) {
+    if (mats.empty()) return;
     int nmax = 0;
     for (const auto& m : mats) nmax = std::max(nmax, m.n);
     for (int k0 = 0; k0 < nmax; k0 += PB) {

@@ -34,6 +34,8 @@ struct Req {
     int stage = ST_WAIT;
     int lv = 0;
     std::vector<double> pts, y, saa;
+    const double* dpts = nullptr;   // device-resident inputs (not owned)
+    const double* dy = nullptr;
     int64_t n = 0;
     int nv = 0;
     kmh_kernel kern{};
@@ -100,7 +102,11 @@ static void finish_batch(Server& S, std::vector<Req*>& rs) {
         size_t off = 0;
         for (size_t i = 0; i < ok.size(); ++i) {
             const int64_t n = ok[i]->n;
-            KMH_HIP(hipMemcpyAsync(buf + off, okr[i]->y.data(), n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+            if (okr[i]->dy)
+                KMH_HIP(hipMemcpyAsync(buf + off, okr[i]->dy, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+            else
+                KMH_HIP(hipMemcpyAsync(buf + off, okr[i]->y.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                                       c.stream));
             KMH_HIP(hipMemcpyAsync(buf + off + n, buf + off, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
             T.push_back(buf + off + n);
             k.push_back(1);
@@ -124,6 +130,10 @@ static void finish_batch(Server& S, std::vector<Req*>& rs) {
         // stochastic gradient (stochasticgradient.jl:30-45)
         for (Req* q : okr) {
             if (q->dks.empty()) continue;
+            if (q->y.empty()) {
+                q->y.resize(q->n);
+                KMH_HIP(hipMemcpy(q->y.data(), q->dy, q->n * sizeof(double), hipMemcpyDeviceToHost));
+            }
             q->grad.assign(q->dks.size(), 0.0);
             try {
                 hodlr_stoch_gradient(*q->H, q->dks.data(), (int)q->dks.size(), q->y.data(), q->saa.data(), q->nv,
@@ -194,7 +204,8 @@ int server_step(Server& S) {
     }
     if (!build.empty()) {
         std::vector<BuildReq> br;
-        for (Req* q : build) br.push_back({q->pts.data(), false, q->n, 2, q->kern, q->opts});
+        for (Req* q : build)
+            br.push_back({q->dpts ? q->dpts : q->pts.data(), q->dpts != nullptr, q->n, 2, q->kern, q->opts});
         std::vector<Status> st;
         auto hs = hodlr_build_batch(c, br, st);
         for (size_t i = 0; i < build.size(); ++i) {
@@ -278,8 +289,13 @@ int kmh_server_submit(kmh_server* srv, const kmh_serve_req* rq, int64_t* id) {
         std::unique_ptr<kmh::Req> q(new kmh::Req());
         q->id = srv->s.next_id++;
         q->n = rq->n;
-        q->pts.assign(rq->pts, rq->pts + 2 * rq->n);
-        q->y.assign(rq->y, rq->y + rq->n);
+        if (rq->on_device) {
+            q->dpts = rq->pts;
+            q->dy = rq->y;
+        } else {
+            q->pts.assign(rq->pts, rq->pts + 2 * rq->n);
+            q->y.assign(rq->y, rq->y + rq->n);
+        }
         q->kern = rq->kernel;
         q->opts = rq->opts;
         q->nv = rq->ndk ? (int)rq->nv : 0;

@@ -7,7 +7,7 @@ from . import _lib
 from ._lib import load, LibraryMissing
 from .api import (Context, ErrorException, FactorHODLR, FixedLevel, HBMBudgetExceeded, Kernel,
                   KernelHODLR, KernelMatrix, LogLevel, PosDefException, cauchy_kernel, default_context,
-                  full, ldiv, logdet, matern12, matern32, negloglik, rational_kernel, stoch_gradient,
+                  full, ldiv, logdet, matern12, matern32, negloglik, rational_kernel, stoch_fisher, stoch_gradient, stoch_hessian,
                   symmetricfactorize)
 
 
@@ -18,6 +18,8 @@ class HODLR:
     LogLevel = LogLevel
     symmetricfactorize = staticmethod(symmetricfactorize)
     stoch_gradient = staticmethod(stoch_gradient)
+    stoch_fisher = staticmethod(stoch_fisher)
+    stoch_hessian = staticmethod(stoch_hessian)
     full = staticmethod(full)
     logdet = staticmethod(logdet)
     negloglik = staticmethod(negloglik)

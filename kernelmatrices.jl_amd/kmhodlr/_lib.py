@@ -44,6 +44,7 @@ KMH_SERVE_MAX_PARAMS = 4
 
 class kmh_serve_req(C.Structure):
     _fields_ = [("pts", C.c_void_p), ("n", C.c_int64), ("d", C.c_int32), ("ndk", C.c_int32),
+                ("on_device", C.c_int32), ("reserved", C.c_int32),
                 ("kernel", kmh_kernel), ("opts", kmh_build_opts), ("y", C.c_void_p), ("saa", C.c_void_p),
                 ("nv", C.c_int64), ("dks", kmh_kernel * KMH_SERVE_MAX_PARAMS)]
 
@@ -93,6 +94,9 @@ SIGNATURES = {
     "kmh_hodlr_build_batch": [_P, C.c_int32, C.POINTER(kmh_build_req), C.POINTER(_P), C.POINTER(C.c_int32)],
     "kmh_hodlr_factorize_batch": [_P, C.c_int32, C.POINTER(_P), C.POINTER(C.c_int32)],
     "kmh_hodlr_solve_batch_dev": [_P, C.c_int32, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int64)],
+    "kmh_stoch_fisher": [_P, C.POINTER(kmh_kernel), C.c_int32, _P, _I64, _P],
+    "kmh_stoch_hessian": [_P, C.POINTER(kmh_kernel), C.c_int32, C.POINTER(kmh_kernel), _P, _P, _I64, C.c_int32,
+                          _P],
     "kmh_server_create": [_P, C.c_int32, C.c_uint64, C.POINTER(_P)],
     "kmh_server_destroy": [_P],
     "kmh_server_submit": [_P, C.POINTER(kmh_serve_req), C.POINTER(C.c_int64)],

@@ -68,6 +68,14 @@ class Kernel:
     def d2(self):
         return self.derivative(L.KMH_D2)
 
+    @property
+    def d12(self):
+        return self.derivative(L.KMH_D12)
+
+    @property
+    def d22(self):
+        return self.derivative(L.KMH_D22)
+
     def desc(self, parms):
         k = L.kmh_kernel()
         k.family = self.family
@@ -410,3 +418,59 @@ def stoch_gradient(K, HK, dat, dfuns, vecs, plel=False, verbose=False, shuffle=F
     _check(HK.lib, HK.lib.kmh_stoch_gradient(HK.ptr, arr, len(dfuns), y.ctypes.data, saa.ctypes.data,
                                              saa.shape[1], g.ctypes.data))
     return g
+
+
+def _saa_matrix(vecs, n):
+    if isinstance(vecs, np.ndarray) and vecs.ndim == 2:
+        saa = np.asfortranarray(vecs, dtype=np.float64)
+    else:
+        saa = np.asfortranarray(np.stack([np.asarray(v, dtype=np.float64) for v in vecs], axis=1))
+    if saa.shape[0] != n:
+        raise ErrorException("Incorrect SAA vector size.")
+    return saa
+
+
+def stoch_fisher(K, HK, dfuns, vecs, plel=False, shuffle=False):
+    """HODLR.stoch_fisher (stochastichessian.jl:91-115) with the derivative
+    HODLRs given by their kernels."""
+    if not HK.is_factorized:
+        raise ErrorException("The matrix needs to be factorized.")
+    if len(dfuns) != len(K.parms):
+        raise ErrorException("You didn't supply the right number of gradient funs.")
+    saa = _saa_matrix(vecs, HK.n)
+    p = len(dfuns)
+    arr = (L.kmh_kernel * p)(*[dk.desc(K.parms) for dk in dfuns])
+    F = np.zeros((p, p), order="F")
+    _check(HK.lib, HK.lib.kmh_stoch_fisher(HK.ptr, arr, p, saa.ctypes.data, saa.shape[1], F.ctypes.data))
+    return F
+
+
+def stoch_hessian(K, HK, dat, d1funs, d2funs, vecs, plel=False, verbose=False, shuffle=False,
+                  profile=False, exact_second=False):
+    """HODLR.stoch_hessian (stochastichessian.jl:117-147).  d2funs[j][k-j] is
+    the (j,k) second-derivative kernel or None (the reference's
+    ZeroFunction).  exact_second=True uses the exact second derivative of
+    the Nystrom blocks instead of the reference's block terms."""
+    if profile:
+        raise ErrorException("the profile-likelihood Hessian is not provided")
+    if not HK.is_factorized:
+        raise ErrorException("The matrix needs to be factorized.")
+    p = len(d1funs)
+    if p != len(K.parms):
+        raise ErrorException("You didn't supply the right number of gradient funs.")
+    flat = [f for row in d2funs for f in row]
+    if len(flat) != p * (p + 1) // 2:
+        raise ErrorException("You didn't supply the right number of Hessian funs.")
+    saa = _saa_matrix(vecs, HK.n)
+    y = _f64(dat)
+    a1 = (L.kmh_kernel * p)(*[dk.desc(K.parms) for dk in d1funs])
+    a2 = (L.kmh_kernel * len(flat))()
+    for i, f in enumerate(flat):
+        if f is None:
+            a2[i].family = 0
+        else:
+            a2[i] = f.desc(K.parms)
+    Hm = np.zeros((p, p), order="F")
+    _check(HK.lib, HK.lib.kmh_stoch_hessian(HK.ptr, a1, p, a2, y.ctypes.data, saa.ctypes.data, saa.shape[1],
+                                            1 if exact_second else 0, Hm.ctypes.data))
+    return Hm

@@ -766,7 +766,7 @@ def deriv2mul(DKj, DKk, D2, src, exact=False):
     return tgt
 
 
-def stoch_hessian(K, HK, y, d1kernels, d2kernels, vecs):
+def stoch_hessian(K, HK, y, d1kernels, d2kernels, vecs, exact=False):
     """stochastichessian.jl:117-147 (non-profile).  d2kernels[j][k-j] is the
     (j, k) second-derivative kernel, or None for an identically zero one
     (the ZeroFunction shortcut, :135-143)."""
@@ -783,12 +783,12 @@ def stoch_hessian(K, HK, y, d1kernels, d2kernels, vecs):
                 D2 = SecondDerivative(K, djk, HK)
                 o = -HK.solve(DKs[k].matvec(HK.solve(DKs[j].matvec(HKsd))))   # :9-12
                 o = o - HK.solve(DKs[j].matvec(HK.solve(DKs[k].matvec(HKsd))))
-                o = o + HK.solve(deriv2mul(DKs[j], DKs[k], D2, HKsd))
+                o = o + HK.solve(deriv2mul(DKs[j], DKs[k], D2, HKsd, exact=exact))
                 sterm = float(np.dot(y, o))
                 t2 = 0.0
                 for v in vecs:                                                  # :79-89
                     t1 = HK.W.ldiv_t(v)
-                    t2 += float(np.dot(t1, deriv2mul(DKs[j], DKs[k], D2, t1)))
+                    t2 += float(np.dot(t1, deriv2mul(DKs[j], DKs[k], D2, t1, exact=exact)))
                 t2 /= len(vecs)
                 out[j, k] += 0.5 * t2 - 0.5 * sterm
             else:

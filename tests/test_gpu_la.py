@@ -87,7 +87,9 @@ def test_potrf_not_pd(ctx):
 def test_trsm(ctx, trans, n, ncols):
     print("trsm", trans, n, ncols, flush=True)
     rng = np.random.default_rng(n + ncols)
-    Lm = np.asfortranarray(np.tril(rng.standard_normal((n, n))) + 4 * np.eye(n))
+    # well-conditioned triangle (a random unit-scale triangle of order 257 is
+    # exponentially ill-conditioned)
+    Lm = np.asfortranarray(np.tril(rng.standard_normal((n, n))) / np.sqrt(n) + 2 * np.eye(n))
     B = np.asfortranarray(rng.standard_normal((n, ncols)))
     X = B.copy(order="F")
     rc = ctx.lib.kmh_test_trsm(ctx.ptr, trans, n, Lm.ctypes.data, n, ncols, X.ctypes.data, n)
