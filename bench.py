@@ -1,8 +1,8 @@
 """Benchmark: GP log-likelihood evaluations per second on one MI355X,
 served (continuous batching) vs plain (one request at a time), n = 65536.
 
-Workload: points uniform in [0,1]^2 (numpy PCG64 seed 0), KD-sorted,
-y ~ N(0,1); Matern nu=3/2 with a 1e-6 nugget; FixedLevel(8) (256 leaves of
+Workload: points uniform in [0,1]^2 (numpy PCG64 seed 0), KD-ordered by
+the library, y ~ N(0,1); Matern nu=3/2 with a 1e-6 nugget; FixedLevel(8) (256 leaves of
 256 points); Nystrom rank 72 (the served objective of the reference,
 nll_objective, always uses Nystrom).  A request is one log-likelihood
 evaluation, negloglik (maxlikfunctions.jl:2-9): HODLR build +
@@ -33,12 +33,14 @@ FP64_MFMA_PEAK_TFLOPS = 78.6
 
 
 def make_inputs(n, seed=0):
+    """Uniform points in [0,1]^2 and y ~ N(0,1), put in KD order by the
+    library (maxlikdata, HODLR/src/constructor.jl:171-181)."""
     import numpy as np
-    from oracle.kdsort import kd_sort   # input preparation only (host KD ordering)
+    import kmhodlr as km
     rng = np.random.default_rng(seed)
-    x = kd_sort(rng.random((n, 2)))
+    x = rng.random((n, 2))
     y = rng.standard_normal(n)
-    return x, y
+    return km.maxlikdata(x, y)
 
 
 def request_params(count, seed=1):

@@ -236,6 +236,13 @@ int kmh_stoch_fisher(kmh_hodlr* h, const kmh_kernel* dks, int32_t p, const doubl
 int kmh_stoch_hessian(kmh_hodlr* h, const kmh_kernel* d1, int32_t p, const kmh_kernel* d2,
                       const double* y, const double* saa, int64_t nv, int32_t exact_second, double* Hout);
 
+/* ---- point ordering: maxlikdata(pts=..., sortmethod=:KDTree)
+ * HODLR/src/constructor.jl:171-181 (NearestNeighbors.KDTree(pts).data).
+ * Host-only (no device work).  perm: n indices; pts[perm[i]] is the i-th
+ * sorted point.  Cells of more than leafsize points are split at their
+ * midpoint after a stable sort along the coordinate of largest spread.      */
+int kmh_kd_order(const double* pts, int64_t n, int32_t d, int32_t leafsize, int64_t* perm);
+
 #ifdef __cplusplus
 }
 #endif

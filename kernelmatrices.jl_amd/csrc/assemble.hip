@@ -62,8 +62,7 @@ void eval_blocks(Ctx& c, std::vector<EvalDesc> ds) {
     }
     if (act.empty()) return;
     EvalDesc* dd = c.upload_vec(act);
-    hipLaunchKernelGGL(k_eval, dim3(tiles), dim3(256), 0, c.stream, dd, (int)act.size());
-    KMH_HIP(hipGetLastError());
+    KMH_LAUNCH(c, k_eval, dim3(tiles), dim3(256), dd, (int)act.size());
 }
 
 // ---------------------------------------------------------------------------
@@ -324,8 +323,7 @@ void aca_launch(Ctx& c, const std::vector<AcaDesc>& ds) {
     AcaDesc* dd = c.upload_vec(ds);
     KMH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_aca),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    hipLaunchKernelGGL(k_aca, dim3((unsigned)ds.size()), dim3(AT), shm, c.stream, dd);
-    KMH_HIP(hipGetLastError());
+    c.launch_kernel(k_aca, dim3((unsigned)ds.size()), dim3(AT), (unsigned)shm, dd);
 }
 
 }  // namespace kmh

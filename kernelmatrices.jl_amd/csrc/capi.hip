@@ -11,20 +11,7 @@ struct kmh_hodlr {
 
 using kmh::Error;
 
-#define KMH_TRY(...)                                                        \
-    try {                                                                   \
-        __VA_ARGS__;                                                        \
-        return KMH_OK;                                                      \
-    } catch (const Error& e) {                                              \
-        kmh::set_last_error(e.what());                                      \
-        return e.code;                                                      \
-    } catch (const std::bad_alloc&) {                                       \
-        kmh::set_last_error("host allocation failed");                      \
-        return KMH_E_OOM;                                                   \
-    } catch (const std::exception& e) {                                     \
-        kmh::set_last_error(e.what());                                      \
-        return KMH_E_DEVICE;                                                \
-    }
+#define KMH_TRY(...) KMH_API_TRY(__VA_ARGS__)
 
 static kmh::Hodlr& H_of(kmh_hodlr* h) {
     KMH_CHECK(h && h->h, KMH_E_BAD_ARG, "null HODLR handle");
@@ -41,9 +28,9 @@ static void with_device_rhs(kmh::Hodlr& H, const double* x, double* y, int64_t n
     const size_t bytes = (size_t)H.n * nrhs * sizeof(double);
     double* dx = static_cast<double*>(c.scratch(bytes));
     double* dy = two ? static_cast<double*>(c.scratch(bytes)) : dx;
-    KMH_HIP(hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, c.stream));
+    KMH_HIP(hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, c.q()));
     f(dx, dy);
-    KMH_HIP(hipMemcpyAsync(y, dy, bytes, hipMemcpyDeviceToHost, c.stream));
+    KMH_HIP(hipMemcpyAsync(y, dy, bytes, hipMemcpyDeviceToHost, c.q()));
     if (two) c.scratch_free(dy);
     c.scratch_free(dx);
     c.sync();
@@ -77,17 +64,11 @@ int kmh_ctx_create(int device, uint64_t hbm_budget_bytes, kmh_ctx** out) {
         c->c.budget = hbm_budget_bytes;
         KMH_HIP(hipSetDevice(device));
         KMH_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
-        c->c.arena_size = size_t(16) << 20;
+        c->c.arena_size = size_t(64) << 20;
         KMH_HIP(hipMalloc(&c->c.arena, c->c.arena_size));
         KMH_HIP(hipHostMalloc(&c->c.staging, c->c.arena_size, hipHostMallocDefault));
         KMH_HIP(hipMalloc(&c->c.d_info, 256));
         KMH_HIP(hipMemset(c->c.d_info, 0, 256));
-        // keep freed stream-ordered scratch cached in the device pool
-        hipMemPool_t pool;
-        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
-            uint64_t thr = ~uint64_t(0);
-            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
-        }
         (void)hipGetLastError();
         *out = c.release();
     })
@@ -98,7 +79,8 @@ int kmh_ctx_destroy(kmh_ctx* ctx) {
         if (!ctx) return KMH_OK;
         kmh::Ctx& c = ctx->c;
         c.set_device();
-        (void)hipStreamSynchronize(c.stream);
+        c.release_pool();
+        kmh::forget_queued(&c);
         for (auto e : c.ev_pool) (void)hipEventDestroy(e);
         if (c.arena) (void)hipFree(c.arena);
         if (c.staging) (void)hipHostFree(c.staging);

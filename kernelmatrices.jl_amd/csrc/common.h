@@ -1,12 +1,17 @@
 // Shared host-side infrastructure of libkmhodlr: errors, the context (device,
-// stream, descriptor arena, stream-ordered scratch), device buffers.
+// stream, descriptor arena, launch queue, scratch pool), device buffers.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 #include <cstring>
+#include <functional>
+#include <map>
+#include <new>
 #include <stdexcept>
 #include <string>
+#include <tuple>
+#include <unordered_map>
 #include <vector>
 
 #include "kmhodlr.h"
@@ -57,11 +62,18 @@ struct Ctx {
     uint64_t in_use = 0;       // bytes held by handles
     hipStream_t stream = nullptr;
     // Descriptor arena: kernels read their problem lists from here.  Uploads
-    // are stream-ordered, so a region may be rewritten by the next launch.
+    // land in a pinned host mirror at the same offsets; the range written
+    // since the last flush goes to the device in one copy at the flush.
     char* arena = nullptr;
-    char* staging = nullptr;   // pinned host mirror of the arena
+    char* staging = nullptr;
     size_t arena_size = 0;
     size_t arena_off = 0;
+    size_t dirty_lo = 0, dirty_hi = 0;
+    // Launch queue: kernels are recorded and issued, in order and after the
+    // descriptor copy, at the next flush.  Every direct operation on the
+    // stream goes through q(), which flushes first, so stream order is the
+    // program order.
+    std::vector<std::function<void()>> pending;
     // Device flag for Cholesky failures (non-positive pivots).
     int* d_info = nullptr;
     // GEMM profiling: events bracketing every grouped-GEMM launch on this
@@ -71,6 +83,13 @@ struct Ctx {
     size_t ev_used = 0;
     std::vector<double> ev_flops;
     hipEvent_t prof_event();
+    // Scratch pool: device chunks carved first-fit on the host.  One in-order
+    // stream runs every kernel of the context, so a block freed here may be
+    // handed to a later kernel at once.
+    std::vector<std::pair<char*, size_t>> chunks;
+    std::map<char*, std::pair<size_t, int>> free_blocks;   // start -> (bytes, chunk)
+    std::unordered_map<void*, std::pair<size_t, int>> live;
+    size_t pool_bytes = 0;
 
     void set_device() const { KMH_HIP(hipSetDevice(device)); }
     // Copy `bytes` of host data into the arena; returns the device address.
@@ -78,12 +97,63 @@ struct Ctx {
     template <class T> T* upload_vec(const std::vector<T>& v) {
         return static_cast<T*>(upload(v.data(), v.size() * sizeof(T)));
     }
-    void arena_reset() { arena_off = 0; }
-    // Stream-ordered scratch (freed with scratch_free on the same stream).
+    bool queued = false;   // listed among the calling thread's contexts with pending launches
+    template <class F> void launch(F&& f) {
+        pending.emplace_back(std::forward<F>(f));
+        if (!queued) note_queued();
+    }
+    // Queue kernel<<<g, b, shm>>>(args...): the arguments are converted to the
+    // kernel's parameter types now and held by value until the launch.
+    template <class... P, class... A>
+    void launch_kernel(void (*kernel)(P...), dim3 g, dim3 b, unsigned shm, A&&... a) {
+        static_assert(sizeof...(P) == sizeof...(A) && sizeof...(P) > 0, "kernel argument count");
+        launch([kernel, g, b, shm, s = stream, args = std::tuple<P...>(static_cast<P>(a)...)]() mutable {
+            std::apply([&](auto&... x) {
+                void* argv[] = {static_cast<void*>(&x)...};
+                KMH_HIP(hipLaunchKernel(reinterpret_cast<const void*>(kernel), g, b, argv, shm, s));
+            }, args);
+        });
+    }
+    void note_queued();
+    void flush();
+    hipStream_t q() { flush(); return stream; }
     void* scratch(size_t bytes);
     void scratch_free(void* p);
-    void sync() { KMH_HIP(hipStreamSynchronize(stream)); }
+    void sync() { flush(); KMH_HIP(hipStreamSynchronize(stream)); }
+    // flush + synchronize without throwing (destructors, teardown)
+    void drain() noexcept;
+    void release_pool() noexcept;
 };
+
+// Issue the launches every context queued in this thread (end of a C-ABI
+// call), or drop them (a call that failed part-way; nothing it queued runs).
+void flush_queued();
+void discard_queued() noexcept;
+void forget_queued(Ctx* c) noexcept;
+
+// Body of a C-ABI entry point: status code from exceptions; the launches the
+// call queued are issued before it returns (dropped if it fails).
+#define KMH_API_TRY(...)                                                    \
+    try {                                                                   \
+        __VA_ARGS__;                                                        \
+        ::kmh::flush_queued();                                              \
+        return KMH_OK;                                                      \
+    } catch (const ::kmh::Error& e) {                                       \
+        ::kmh::discard_queued();                                            \
+        ::kmh::set_last_error(e.what());                                    \
+        return e.code;                                                      \
+    } catch (const std::bad_alloc&) {                                       \
+        ::kmh::discard_queued();                                            \
+        ::kmh::set_last_error("host allocation failed");                    \
+        return KMH_E_OOM;                                                   \
+    } catch (const std::exception& e) {                                     \
+        ::kmh::discard_queued();                                            \
+        ::kmh::set_last_error(e.what());                                    \
+        return KMH_E_DEVICE;                                                \
+    }
+
+// Record a kernel launch on the context's queue (see Ctx::launch).
+#define KMH_LAUNCH(ctx, kernel, grid, block, ...) (ctx).launch_kernel(kernel, grid, block, 0u, __VA_ARGS__)
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 

@@ -40,40 +40,98 @@ void DevBuf::alloc(Ctx* c, size_t nbytes) {
 void DevBuf::release() {
     if (p) {
         // frees are ordered after all work queued on the context's stream
-        (void)hipStreamSynchronize(ctx->stream);
+        ctx->drain();
         (void)hipFree(p);
         ctx->in_use -= bytes;
     }
     p = nullptr; bytes = 0; ctx = nullptr;
 }
 
-// The descriptor arena is mirrored by a pinned host staging buffer at the
-// same offsets.  Both wrap together; wrapping waits for the stream so that
-// neither the pending host->device copies nor the kernels reading older
-// descriptors can be overtaken.
+// The staging mirror and the arena wrap together; wrapping issues every
+// queued launch and waits for the stream, so neither a pending copy nor a
+// kernel reading older descriptors can be overtaken.
 void* Ctx::upload(const void* host, size_t bytes) {
     const size_t need = (bytes + 255) & ~size_t(255);
     if (need > arena_size / 4) {
         // grow (rare): drain the stream, reallocate both sides
-        KMH_HIP(hipStreamSynchronize(stream));
+        sync();
         size_t ns = std::max<size_t>(arena_size * 2, need * 8);
         if (arena) KMH_HIP(hipFree(arena));
         if (staging) KMH_HIP(hipHostFree(staging));
+        arena = nullptr;
+        staging = nullptr;
         KMH_HIP(hipMalloc(&arena, ns));
         KMH_HIP(hipHostMalloc(&staging, ns, hipHostMallocDefault));
         arena_size = ns;
-        arena_off = 0;
+        arena_off = dirty_lo = dirty_hi = 0;
     }
     if (arena_off + need > arena_size) {
-        KMH_HIP(hipStreamSynchronize(stream));
-        arena_off = 0;
+        sync();
+        arena_off = dirty_lo = dirty_hi = 0;
     }
-    char* h = staging + arena_off;
+    if (bytes) std::memcpy(staging + arena_off, host, bytes);
+    if (dirty_hi == dirty_lo) dirty_lo = arena_off;
+    dirty_hi = arena_off + need;
     char* d = arena + arena_off;
-    if (bytes) std::memcpy(h, host, bytes);
-    if (bytes) KMH_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
     arena_off += need;
     return d;
+}
+
+void Ctx::flush() {
+    if (dirty_hi > dirty_lo) {
+        KMH_HIP(hipMemcpyAsync(arena + dirty_lo, staging + dirty_lo, dirty_hi - dirty_lo,
+                               hipMemcpyHostToDevice, stream));
+        dirty_lo = dirty_hi = arena_off;
+    }
+    if (pending.empty()) return;
+    std::vector<std::function<void()>> run;
+    run.swap(pending);
+    for (auto& f : run) {
+        f();
+        KMH_HIP(hipGetLastError());
+    }
+}
+
+static thread_local std::vector<Ctx*> g_queued;
+
+void Ctx::note_queued() {
+    queued = true;
+    g_queued.push_back(this);
+}
+
+void flush_queued() {
+    while (!g_queued.empty()) {
+        Ctx* c = g_queued.back();
+        g_queued.pop_back();
+        c->queued = false;
+        c->flush();
+    }
+}
+
+void discard_queued() noexcept {
+    for (Ctx* c : g_queued) {
+        c->queued = false;
+        c->pending.clear();
+    }
+    g_queued.clear();
+}
+
+void forget_queued(Ctx* c) noexcept {
+    for (size_t i = 0; i < g_queued.size(); ++i)
+        if (g_queued[i] == c) {
+            g_queued.erase(g_queued.begin() + (long)i);
+            break;
+        }
+    c->queued = false;
+}
+
+void Ctx::drain() noexcept {
+    try {
+        flush();
+    } catch (...) {
+        pending.clear();
+    }
+    (void)hipStreamSynchronize(stream);
 }
 
 hipEvent_t Ctx::prof_event() {
@@ -86,18 +144,64 @@ hipEvent_t Ctx::prof_event() {
 }
 
 void* Ctx::scratch(size_t bytes) {
-    void* p = nullptr;
-    if (bytes == 0) bytes = 256;
-    hipError_t e = hipMallocAsync(&p, bytes, stream);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        throw Error(KMH_E_OOM, "scratch allocation of " + std::to_string(bytes) + " bytes failed");
+    bytes = std::max<size_t>(256, (bytes + 255) & ~size_t(255));
+    auto it = free_blocks.begin();
+    for (; it != free_blocks.end(); ++it)
+        if (it->second.first >= bytes) break;
+    if (it == free_blocks.end()) {
+        // new chunk: at least double the pool, at least 64 MiB
+        const size_t sz = std::max<size_t>({bytes, pool_bytes, size_t(64) << 20});
+        void* q = nullptr;
+        set_device();
+        hipError_t e = hipMalloc(&q, sz);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(KMH_E_OOM, "scratch allocation of " + std::to_string(bytes) + " bytes failed");
+        }
+        chunks.push_back({static_cast<char*>(q), sz});
+        pool_bytes += sz;
+        it = free_blocks.emplace(static_cast<char*>(q), std::make_pair(sz, (int)chunks.size() - 1)).first;
     }
+    char* p = it->first;
+    const size_t avail = it->second.first;
+    const int chunk = it->second.second;
+    free_blocks.erase(it);
+    if (avail > bytes) free_blocks.emplace(p + bytes, std::make_pair(avail - bytes, chunk));
+    live.emplace(p, std::make_pair(bytes, chunk));
     return p;
 }
 
-void Ctx::scratch_free(void* p) {
-    if (p) KMH_HIP(hipFreeAsync(p, stream));
+void Ctx::scratch_free(void* vp) {
+    if (!vp) return;
+    auto lt = live.find(vp);
+    KMH_CHECK(lt != live.end(), KMH_E_BAD_ARG, "scratch_free of an unknown block");
+    char* p = static_cast<char*>(vp);
+    size_t bytes = lt->second.first;
+    const int chunk = lt->second.second;
+    live.erase(lt);
+    // coalesce with the free neighbours of the same chunk
+    auto nx = free_blocks.lower_bound(p);
+    if (nx != free_blocks.end() && nx->first == p + bytes && nx->second.second == chunk) {
+        bytes += nx->second.first;
+        nx = free_blocks.erase(nx);
+    }
+    if (nx != free_blocks.begin()) {
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second.first == p && pv->second.second == chunk) {
+            pv->second.first += bytes;
+            return;
+        }
+    }
+    free_blocks.emplace(p, std::make_pair(bytes, chunk));
+}
+
+void Ctx::release_pool() noexcept {
+    drain();
+    for (auto& ch : chunks) (void)hipFree(ch.first);
+    chunks.clear();
+    free_blocks.clear();
+    live.clear();
+    pool_bytes = 0;
 }
 
 }  // namespace kmh

@@ -124,8 +124,8 @@ void hodlr_stoch_gradient(Hodlr& H, const kmh_kernel* dks, int p, const double* 
     double* Z = static_cast<double*>(c.scratch(2 * nk * sizeof(double) + (size_t)(k + 1) * sizeof(double)));
     double* O = Z + nk;
     double* dots = O + nk;
-    KMH_HIP(hipMemcpyAsync(Z, y_host, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
-    KMH_HIP(hipMemcpyAsync(Z + n, saa_host, (size_t)n * nv * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    KMH_HIP(hipMemcpyAsync(Z, y_host, n * sizeof(double), hipMemcpyHostToDevice, c.q()));
+    KMH_HIP(hipMemcpyAsync(Z + n, saa_host, (size_t)n * nv * sizeof(double), hipMemcpyHostToDevice, c.q()));
     hodlr_solve(H, Z, 1);                                  // K^-1 y
     hodlr_factor_apply(H, KMH_WTINV, Z + n, (int)nv);      // W^-T v for each SAA vector
     const int r = H.nys_r;
@@ -154,12 +154,12 @@ void hodlr_stoch_gradient(Hodlr& H, const kmh_kernel* dks, int p, const double* 
         // solve term: y' K^-1 dK K^-1 y   (stochasticgradient.jl:21)
         copy2d(c, {{O, (int)n, prv, (int)n, (int)n, 1, 1.0}});
         hodlr_solve(H, prv, 1);
-        KMH_HIP(hipMemcpyAsync(O, prv, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-        KMH_HIP(hipMemcpyAsync(prv, y_host, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        KMH_HIP(hipMemcpyAsync(O, prv, n * sizeof(double), hipMemcpyDeviceToDevice, c.q()));
+        KMH_HIP(hipMemcpyAsync(prv, y_host, n * sizeof(double), hipMemcpyHostToDevice, c.q()));
         column_dots(c, prv, (int)n, O, (int)n, (int)n, 1, dots);
         // trace term: mean over SAA vectors of t' dK t  (stochasticgradient.jl:2-10, 24)
         column_dots(c, Z + n, (int)n, O + n, (int)n, (int)n, (int)nv, dots + 1);
-        KMH_HIP(hipMemcpyAsync(hd.data(), dots, (size_t)k * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        KMH_HIP(hipMemcpyAsync(hd.data(), dots, (size_t)k * sizeof(double), hipMemcpyDeviceToHost, c.q()));
         c.sync();
         double tr = 0.0;
         for (int v = 1; v < k; ++v) tr += hd[v];

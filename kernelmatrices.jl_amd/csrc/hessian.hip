@@ -153,27 +153,33 @@ void dterm_matvec(Hodlr& H, const double* leaf, const std::vector<const double*>
         trsm_left(c, true, mt);
         copy2d(c, back);
     }
-    // combine per left panel: C_L = sum sign_t X_t
+    // combine per left panel: C_L = sum sign_t X_t, in term order.  The
+    // i-th further term of every C goes into launch i, so no launch has two
+    // problems updating the same C.
     std::vector<Copy2D> first;
-    std::vector<Axpby2D> rest;
+    std::vector<std::vector<Axpby2D>> rest;
     for (size_t jb = 0; jb < jobs.size(); ++jb) {
         const auto& j = jobs[jb];
         const auto& L = lp[j.dir];
         for (size_t q = 0; q < L.size(); ++q) {
             double* C = Cb + (job_c0[jb] + q) * rk;
-            bool any = false;
+            size_t cnt = 0;
             size_t x = job_x0[jb];
             for (const auto& t : *terms[j.dir]) {
                 const double* X = Xb + (x++) * rk;
                 if (t.left != L[q]) continue;
-                if (!any) first.push_back({X, r, C, r, r, k, t.sign});
-                else rest.push_back({X, r, C, r, r, k, 1.0, t.sign});
-                any = true;
+                if (cnt == 0) {
+                    first.push_back({X, r, C, r, r, k, t.sign});
+                } else {
+                    if (rest.size() < cnt) rest.resize(cnt);
+                    rest[cnt - 1].push_back({X, r, C, r, r, k, 1.0, t.sign});
+                }
+                ++cnt;
             }
         }
     }
     copy2d(c, first);
-    axpby2d(c, rest);
+    for (const auto& round : rest) axpby2d(c, round);
     // accumulate level by level, one launch per left panel (targets of one
     // level's forward and backward jobs are disjoint)
     for (int l = 0; l < H.L; ++l) {
@@ -247,7 +253,7 @@ static std::vector<double*> fisher_panels(Hodlr& H, const std::vector<DPanels>& 
 static double sum_sq_cols(Ctx& c, const double* X, int64_t n, int nv, double* dots) {
     column_dots(c, X, (int)n, X, (int)n, (int)n, nv, dots);
     std::vector<double> h(nv);
-    KMH_HIP(hipMemcpyAsync(h.data(), dots, nv * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    KMH_HIP(hipMemcpyAsync(h.data(), dots, nv * sizeof(double), hipMemcpyDeviceToHost, c.q()));
     c.sync();
     double s = 0.0;
     for (double v : h) s += v;
@@ -282,7 +288,7 @@ void hodlr_stoch_fisher(Hodlr& H, const kmh_kernel* dks, int p, const double* sa
     const int64_t n = H.n;
     double* T = static_cast<double*>(c.scratch((size_t)n * nv * sizeof(double) + (size_t)(nv + 8) * sizeof(double)));
     double* dots = T + (size_t)n * nv;
-    KMH_HIP(hipMemcpyAsync(T, saa_host, (size_t)n * nv * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    KMH_HIP(hipMemcpyAsync(T, saa_host, (size_t)n * nv * sizeof(double), hipMemcpyHostToDevice, c.q()));
     hodlr_factor_apply(H, KMH_WTINV, T, (int)nv);
     std::vector<DPanels> dp;
     for (int j = 0; j < p; ++j) dp.push_back(dpanels_make(H, dks[j]));
@@ -307,8 +313,8 @@ void hodlr_stoch_hessian(Hodlr& H, const kmh_kernel* d1, int p, const kmh_kernel
     double* u = O + nn * k;
     double* w = u + nn;
     double* dots = w + nn;
-    KMH_HIP(hipMemcpyAsync(Zb, y_host, nn * sizeof(double), hipMemcpyHostToDevice, c.stream));
-    KMH_HIP(hipMemcpyAsync(Zb + nn, saa_host, nn * nv * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    KMH_HIP(hipMemcpyAsync(Zb, y_host, nn * sizeof(double), hipMemcpyHostToDevice, c.q()));
+    KMH_HIP(hipMemcpyAsync(Zb + nn, saa_host, nn * nv * sizeof(double), hipMemcpyHostToDevice, c.q()));
     hodlr_solve(H, Zb, 1);
     hodlr_factor_apply(H, KMH_WTINV, Zb + nn, (int)nv);
     double* alpha = Zb;
@@ -335,7 +341,7 @@ void hodlr_stoch_hessian(Hodlr& H, const kmh_kernel* d1, int p, const kmh_kernel
     auto dot = [&](const double* a, const double* b) {
         column_dots(c, a, (int)n, b, (int)n, (int)n, 1, dots);
         double h = 0.0;
-        KMH_HIP(hipMemcpyAsync(&h, dots, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        KMH_HIP(hipMemcpyAsync(&h, dots, sizeof(double), hipMemcpyDeviceToHost, c.q()));
         c.sync();
         return h;
     };
@@ -357,7 +363,7 @@ void hodlr_stoch_hessian(Hodlr& H, const kmh_kernel* d1, int p, const kmh_kernel
             const double sterm = base + dot(alpha, O);
             column_dots(c, T, (int)n, O + nn, (int)n, (int)n, (int)nv, dots);
             std::vector<double> hd(nv);
-            KMH_HIP(hipMemcpyAsync(hd.data(), dots, nv * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+            KMH_HIP(hipMemcpyAsync(hd.data(), dots, nv * sizeof(double), hipMemcpyDeviceToHost, c.q()));
             c.sync();
             double t2 = 0.0;
             for (double v : hd) t2 += v;

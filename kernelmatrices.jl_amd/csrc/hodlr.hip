@@ -72,7 +72,7 @@ void check_kernel(const kmh_kernel& k, bool allow_deriv) {
 static void collect_info(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<Status>& st, const char* what) {
     std::vector<int> flags(hs.size(), 0);
     for (size_t i = 0; i < hs.size(); ++i)
-        if (hs[i]) KMH_HIP(hipMemcpyAsync(&flags[i], hs[i]->info.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        if (hs[i]) KMH_HIP(hipMemcpyAsync(&flags[i], hs[i]->info.p, sizeof(int), hipMemcpyDeviceToHost, c.q()));
     c.sync();
     for (size_t i = 0; i < hs.size(); ++i)
         if (hs[i] && flags[i] && st[i].code == KMH_OK) {
@@ -83,7 +83,7 @@ static void collect_info(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<Stat
 
 static void reset_info(Ctx& c, const std::vector<Hodlr*>& hs) {
     for (auto* h : hs)
-        if (h) KMH_HIP(hipMemsetAsync(h->info.p, 0, sizeof(int), c.stream));
+        if (h) KMH_HIP(hipMemsetAsync(h->info.p, 0, sizeof(int), c.q()));
 }
 
 // ---------------------------------------------------------------------------
@@ -172,7 +172,7 @@ static void build_aca_batch(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<S
             }
         }
         aca_launch(c, ds);
-        KMH_HIP(hipMemcpyAsync(h_out.data(), d_rank, 2 * sizeof(int) * nblk, hipMemcpyDeviceToHost, c.stream));
+        KMH_HIP(hipMemcpyAsync(h_out.data(), d_rank, 2 * sizeof(int) * nblk, hipMemcpyDeviceToHost, c.q()));
         c.sync();
         bool again = false;
         bi = 0;
@@ -221,7 +221,7 @@ static void build_aca_batch(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<S
         for (int l = 0; l < H.L; ++l) H.coff[l + 1] = H.coff[l] + H.rpad[l];
         H.Rtot = H.coff[H.L];
         H.P.alloc(&c, (size_t)H.n * H.Rtot * sizeof(double));
-        KMH_HIP(hipMemsetAsync(H.P.p, 0, H.P.bytes, c.stream));
+        KMH_HIP(hipMemsetAsync(H.P.p, 0, H.P.bytes, c.q()));
         double* P = H.P.as<double>();
         for (int l = 0; l < H.L; ++l) {
             double* Pl = P + (size_t)H.coff[l] * H.n;
@@ -258,7 +258,7 @@ std::vector<Hodlr*> hodlr_build_batch(Ctx& c, const std::vector<BuildReq>& reqs,
             H.opts = q.opts;
             tree_indices(H);
             H.info.alloc(&c, 256);
-            KMH_HIP(hipMemsetAsync(H.info.p, 0, sizeof(int), c.stream));
+            KMH_HIP(hipMemsetAsync(H.info.p, 0, sizeof(int), c.q()));
             H.pts.alloc(&c, 2 * (size_t)q.n * sizeof(double));
             double* px = H.pts.as<double>();
             if (!q.on_device) {
@@ -268,7 +268,7 @@ std::vector<Hodlr*> hodlr_build_batch(Ctx& c, const std::vector<BuildReq>& reqs,
                     soa[i][q.n + j] = q.pts[2 * j + 1];
                 }
                 KMH_HIP(hipMemcpyAsync(px, soa[i].data(), soa[i].size() * sizeof(double), hipMemcpyHostToDevice,
-                                       c.stream));
+                                       c.q()));
             } else {
                 dev_pts.push_back({q.pts, 2, px, (int)q.n, 2, (int)q.n, 1.0});
             }
@@ -600,7 +600,7 @@ void factor_end_batch(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<Status>
     std::vector<double> sums(hs.size());
     for (size_t i = 0; i < hs.size(); ++i)
         KMH_HIP(hipMemcpyAsync(&sums[i], hs[i]->parts + hs[i]->nparts, sizeof(double), hipMemcpyDeviceToHost,
-                               c.stream));
+                               c.q()));
     st.assign(hs.size(), Status());
     collect_info(c, hs, st, "Cholesky factorization failed.");
     for (size_t j = 0; j < hs.size(); ++j) {
@@ -610,7 +610,7 @@ void factor_end_batch(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<Status>
         H.AB = nullptr;
         H.parts = nullptr;
         if (st[j].code != KMH_OK) {
-            KMH_HIP(hipMemsetAsync(H.info.p, 0, sizeof(int), c.stream));
+            KMH_HIP(hipMemsetAsync(H.info.p, 0, sizeof(int), c.q()));
         } else {
             H.logdet = 2.0 * sums[j];
             H.factorized = true;
@@ -760,7 +760,7 @@ void hodlr_matvec(Hodlr& H, const double* x, double* y, int nrhs) {
 static void fetch_cols(Ctx& c, const double* src, int64_t ld, int64_t h, int r, double* dst) {
     if (h <= 0 || r <= 0) return;
     KMH_HIP(hipMemcpy2DAsync(dst, h * sizeof(double), src, ld * sizeof(double), h * sizeof(double), r,
-                             hipMemcpyDeviceToHost, c.stream));
+                             hipMemcpyDeviceToHost, c.q()));
 }
 
 void hodlr_full_host(Hodlr& H, double* out) {
@@ -768,7 +768,7 @@ void hodlr_full_host(Hodlr& H, double* out) {
     Ctx& c = *H.ctx;
     const int64_t n = H.n;
     std::vector<double> lv(H.leaves.bytes / sizeof(double));
-    KMH_HIP(hipMemcpyAsync(lv.data(), H.leaves.p, H.leaves.bytes, hipMemcpyDeviceToHost, c.stream));
+    KMH_HIP(hipMemcpyAsync(lv.data(), H.leaves.p, H.leaves.bytes, hipMemcpyDeviceToHost, c.q()));
     c.sync();
     for (int i = 0; i < H.nleaves(); ++i) {
         const int64_t s = H.leaf_off[i], m = H.leaf_size(i);

@@ -8,10 +8,10 @@
 //    few-tile problems are split along k into partial products that a second
 //    launch reduces in a fixed order (deterministic, no atomics).
 //  * POTRF / TRSM: right-looking blocked algorithms with 64-wide diagonal
-//    blocks factored or solved by one workgroup, trailing updates by GEMM.
-//  * QR: Householder QR of <=256 x 64 tiles held in LDS (130 KB), combined
-//    by a binary TSQR tree; columns beyond 64 by block Gram-Schmidt with one
-//    reorthogonalisation pass.
+//    blocks, trailing updates by GEMM.  A diagonal Cholesky block is one
+//    wave with a row per lane in registers; diagonal solves split the 64
+//    rows into four 16-row groups, one per wave.
+//  * QR of tall panels: qr.hip.
 #include <algorithm>
 #include <cmath>
 
@@ -34,7 +34,7 @@ struct GemmDesc {
 };
 
 constexpr int GT = 64;
-constexpr int GK = 16;
+constexpr int GK = 32;
 
 // Lane layout of V_MFMA_F64_16X16X4F64 on gfx950 (measured with one-hot
 // operands): A(i,k) is held by lane i + 16k, B(k,j) by lane j + 16k, and
@@ -42,8 +42,19 @@ constexpr int GK = 16;
 // 16x16x4 form instead holds D(4g + r, j).)
 __device__ __forceinline__ int mfma_d_row(int g, int r) { return 4 * r + g; }
 
+// Operand staging: each thread moves GT*GK/256 elements of A and of B per
+// k-chunk.  Element e of thread tid maps to (row-or-col index, k index) so
+// that consecutive threads read consecutive addresses of the column-major
+// operand.
+template <bool T>
+__device__ __forceinline__ void gemm_stage_idx(int tid, int q, int& i, int& kk) {
+    if (!T) { i = tid & 63; kk = (tid >> 6) + 4 * q; }          // i contiguous in memory
+    else    { kk = tid & (GK - 1); i = tid / GK + (256 / GK) * q; }   // k contiguous in memory
+}
+
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ ds, int nd) {
+    constexpr int NQ = GT * GK / 256;
     __shared__ double As[GK][GT + 1];
     __shared__ double Bs[GK][GT + 1];
     const int t = blockIdx.x;
@@ -65,32 +76,46 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ ds, i
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
 
-    for (int k0 = 0; k0 < d.k; k0 += GK) {
+    // prefetch registers for one k-chunk of A and B
+    double ra[NQ], rb[NQ];
+    auto fetch = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             int i, kk;
-            if (!TA) { i = tid & 63; kk = (tid >> 6) + 4 * q; }
-            else     { kk = tid & 15; i = (tid >> 4) + 16 * q; }
+            gemm_stage_idx<TA>(tid, q, i, kk);
             const int gi = row0 + i, gk = k0 + kk;
             double v = 0.0;
-            if (gi < d.m && gk < d.k)
-                v = TA ? d.A[gk + (size_t)gi * d.lda] : d.A[gi + (size_t)gk * d.lda];
-            As[kk][i] = v;
+            if (gi < d.m && gk < d.k) v = TA ? d.A[gk + (size_t)gi * d.lda] : d.A[gi + (size_t)gk * d.lda];
+            ra[q] = v;
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             int j, kk;
-            if (!TB) { kk = tid & 15; j = (tid >> 4) + 16 * q; }
-            else     { j = tid & 63; kk = (tid >> 6) + 4 * q; }
+            gemm_stage_idx<!TB>(tid, q, j, kk);
             const int gj = col0 + j, gk = k0 + kk;
             double v = 0.0;
-            if (gj < d.n && gk < d.k)
-                v = TB ? d.B[gj + (size_t)gk * d.ldb] : d.B[gk + (size_t)gj * d.ldb];
-            Bs[kk][j] = v;
+            if (gj < d.n && gk < d.k) v = TB ? d.B[gj + (size_t)gk * d.ldb] : d.B[gk + (size_t)gj * d.ldb];
+            rb[q] = v;
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < d.k; k0 += GK) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            int i, kk;
+            gemm_stage_idx<TA>(tid, q, i, kk);
+            As[kk][i] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            int j, kk;
+            gemm_stage_idx<!TB>(tid, q, j, kk);
+            Bs[kk][j] = rb[q];
         }
         __syncthreads();
+        if (k0 + GK < d.k) fetch(k0 + GK);   // next chunk in flight during the MFMAs
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
+        for (int ks = 0; ks < GK / 4; ++ks) {
             const int kk = ks * 4 + (lane >> 4);
             const double a0 = As[kk][wr + (lane & 15)];
             const double a1 = As[kk][wr + 16 + (lane & 15)];
@@ -212,19 +237,21 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
         double fl = 0.0;
         for (const auto& d : ds) fl += 2.0 * d.m * d.n * (double)d.k;
         c.ev_flops.push_back(fl);
-        KMH_HIP(hipEventRecord(c.prof_event(), c.stream));
+        hipEvent_t ev_ = c.prof_event();
+        c.launch([ev_, s_ = c.stream]() { KMH_HIP(hipEventRecord(ev_, s_)); });
     }
-    if (!ta && !tb) hipLaunchKernelGGL((k_gemm<false, false>), dim3(tiles), dim3(256), 0, c.stream, dd, nd);
-    else if (!ta && tb) hipLaunchKernelGGL((k_gemm<false, true>), dim3(tiles), dim3(256), 0, c.stream, dd, nd);
-    else if (ta && !tb) hipLaunchKernelGGL((k_gemm<true, false>), dim3(tiles), dim3(256), 0, c.stream, dd, nd);
-    else hipLaunchKernelGGL((k_gemm<true, true>), dim3(tiles), dim3(256), 0, c.stream, dd, nd);
-    KMH_HIP(hipGetLastError());
-    if (c.prof) KMH_HIP(hipEventRecord(c.prof_event(), c.stream));
+    if (!ta && !tb) KMH_LAUNCH(c, (k_gemm<false, false>), dim3(tiles), dim3(256), dd, nd);
+    else if (!ta && tb) KMH_LAUNCH(c, (k_gemm<false, true>), dim3(tiles), dim3(256), dd, nd);
+    else if (ta && !tb) KMH_LAUNCH(c, (k_gemm<true, false>), dim3(tiles), dim3(256), dd, nd);
+    else KMH_LAUNCH(c, (k_gemm<true, true>), dim3(tiles), dim3(256), dd, nd);
+    if (c.prof) {
+        hipEvent_t ev_ = c.prof_event();
+        c.launch([ev_, s_ = c.stream]() { KMH_HIP(hipEventRecord(ev_, s_)); });
+    }
     if (!rs.empty()) {
         RedDesc* dr = c.upload_vec(rs);
-        hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)ceil_div(relem, 256)), dim3(256), 0, c.stream,
-                           dr, (int)rs.size(), relem);
-        KMH_HIP(hipGetLastError());
+        KMH_LAUNCH(c, k_splitk_reduce, dim3((unsigned)ceil_div(relem, 256)), dim3(256),
+                   dr, (int)rs.size(), relem);
     }
     if (W) c.scratch_free(W);
 }
@@ -234,84 +261,58 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
 // ---------------------------------------------------------------------------
 constexpr int PB = 64;
 
-__global__ __launch_bounds__(256) void k_potrf_diag(const SqMat* __restrict__ ms, int* info0) {
-    __shared__ double S[PB][PB + 1];   // S[col][row]
+__device__ __forceinline__ double lane_bcast(double v, int lane) {
+    // value of v held by `lane` (compile-time constant after unrolling)
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// Cholesky of one <= 64 x 64 diagonal block per wave: lane r holds row r of
+// the lower triangle in registers; column j is scaled by 1/l_jj (LAPACK
+// dpotf2 order) and the trailing rows updated with l_kj broadcast from lane k.
+// No LDS and no barriers.
+__global__ __launch_bounds__(64) void k_potrf_diag(const SqMat* __restrict__ ms, int* info0) {
     const SqMat m = ms[blockIdx.x];
     int* info = m.info ? m.info : info0;
-    const int n = m.n, tid = threadIdx.x;
-    for (int e = tid; e < PB * PB; e += 256) {
-        const int i = e % PB, j = e / PB;
-        S[j][i] = (i < n && j < n && i >= j) ? m.A[i + (size_t)j * m.ld] : 0.0;
-    }
-    __syncthreads();
-    for (int j = 0; j < n; ++j) {
-        if (tid == 0) {
-            const double djj = S[j][j];
-            if (!(djj > 0.0)) atomicExch(info, 1);
-            S[j][j] = sqrt(djj);
-        }
-        __syncthreads();
-        const double ljj = S[j][j];
-        for (int i = j + 1 + tid; i < n; i += 256) S[j][i] = S[j][i] / ljj;
-        __syncthreads();
-        const int w = n - j - 1;
-        for (int e = tid; e < w * w; e += 256) {
-            const int l = j + 1 + e / w, i = j + 1 + e % w;
-            if (i >= l) S[l][i] -= S[j][i] * S[j][l];
-        }
-        __syncthreads();
-    }
-    for (int e = tid; e < n * n; e += 256) {
-        const int i = e % n, j = e / n;
-        m.A[i + (size_t)j * m.ld] = (i >= j) ? S[j][i] : 0.0;
-    }
-}
-
-// X L^T = B for the rows of B (one thread per row), L is nb x nb lower.
-struct RltDesc {
-    const double* L;
-    int nb, ldl;
-    double* B;
-    int m, ldb;
-    int chunk0, pad;
-};
-
-__global__ __launch_bounds__(256) void k_trsm_rlt(const RltDesc* __restrict__ ds, int nd) {
-    __shared__ double Ls[PB][PB + 1];   // Ls[j][l] = L(j, l), identity padding
-    int lo = 0, hi = nd - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (ds[mid].chunk0 <= (int)blockIdx.x) lo = mid;
-        else hi = mid - 1;
-    }
-    const RltDesc d = ds[lo];
-    const int tid = threadIdx.x;
-    for (int e = tid; e < PB * PB; e += 256) {
-        const int j = e % PB, l = e / PB;
-        double v;
-        if (j < d.nb && l < d.nb) v = (l <= j) ? d.L[j + (size_t)l * d.ldl] : 0.0;
-        else v = (j == l) ? 1.0 : 0.0;
-        Ls[j][l] = v;
-    }
-    __syncthreads();
-    const int row = ((int)blockIdx.x - d.chunk0) * 256 + tid;
-    if (row >= d.m) return;
-    double x[PB];
+    const int n = m.n, r = threadIdx.x;
+    double a[PB];
 #pragma unroll
-    for (int j = 0; j < PB; ++j) x[j] = (j < d.nb) ? d.B[row + (size_t)j * d.ldb] : 0.0;
+    for (int k = 0; k < PB; ++k) a[k] = (r < n && k < n && k <= r) ? m.A[r + (size_t)k * m.ld] : 0.0;
+    bool bad = false;
 #pragma unroll
     for (int j = 0; j < PB; ++j) {
-        double s = x[j];
+        if (j < n) {
+            const double ajj = lane_bcast(a[j], j);
+            bad |= !(ajj > 0.0);
+            const double ljj = sqrt(ajj);
+            const double inv = 1.0 / ljj;
+            a[j] = (r == j) ? ljj : ((r > j) ? a[j] * inv : 0.0);
 #pragma unroll
-        for (int l = 0; l < j; ++l) s -= x[l] * Ls[j][l];
-        x[j] = s / Ls[j][j];
+            for (int k = j + 1; k < PB; ++k) {
+                if (k < n) {
+                    const double lkj = lane_bcast(a[j], k);
+                    a[k] -= a[j] * lkj;
+                }
+            }
+        }
     }
+    if (bad && r == 0) atomicExch(info, 1);
+    if (r < n) {
 #pragma unroll
-    for (int j = 0; j < PB; ++j)
-        if (j < d.nb) d.B[row + (size_t)j * d.ldb] = x[j];
+        for (int k = 0; k < PB; ++k)
+            if (k < n) m.A[r + (size_t)k * m.ld] = (k <= r) ? a[k] : 0.0;
+    }
 }
 
-// Diagonal-block solve for the left TRSM: one thread per right-hand column.
+// Diagonal-block triangular solves, one 64 x 64 right-hand tile per
+// workgroup.  Wave g owns rows 16g..16g+15 of the tile (16 values per lane,
+// lane = right-hand column): the owning wave solves its 16 x 16 triangle,
+// publishes the result in LDS, and the waves after it (before it for the
+// transposed solve) subtract its contribution; one barrier per 16-row phase.
+//   MODE 0: L X = B        (B nb x ncols, tiles of 64 columns)
+//   MODE 1: L^T X = B      (as 0)
+//   MODE 2: X L^T = B      (B ncols x nb: tiles of 64 rows; L X^T = B^T)
 struct LdiagDesc {
     const double* L;
     int nb, ldl;
@@ -320,51 +321,106 @@ struct LdiagDesc {
     int chunk0, pad;
 };
 
-template <bool TRANS>
-__global__ __launch_bounds__(256) void k_trsm_ldiag(const LdiagDesc* __restrict__ ds, int nd) {
-    __shared__ double Ls[PB][PB + 1];   // Ls[j][l] = L(j, l), identity padding
+template <int MODE>
+__global__ __launch_bounds__(256) void k_trsm_diag(const LdiagDesc* __restrict__ ds, int nd) {
+    __shared__ double Ls[PB][PB + 1];   // Ls[i][l] = L(i, l), identity padding
+    __shared__ double Xs[PB][PB + 1];   // Xs[i][c]: tile row i, right-hand c
     int lo = 0, hi = nd - 1;
     while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
+        const int mid = (lo + hi + 1) >> 1;
         if (ds[mid].chunk0 <= (int)blockIdx.x) lo = mid;
         else hi = mid - 1;
     }
     const LdiagDesc d = ds[lo];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
+    const int c0 = ((int)blockIdx.x - d.chunk0) * PB;
+    const int nc = min(PB, d.ncols - c0);
     for (int e = tid; e < PB * PB; e += 256) {
-        const int j = e % PB, l = e / PB;
+        const int i = e & 63, l = e >> 6;
         double v;
-        if (j < d.nb && l < d.nb) v = (l <= j) ? d.L[j + (size_t)l * d.ldl] : 0.0;
-        else v = (j == l) ? 1.0 : 0.0;
-        Ls[j][l] = v;
+        if (i < d.nb && l < d.nb) v = (l <= i) ? d.L[i + (size_t)l * d.ldl] : 0.0;
+        else v = (i == l) ? 1.0 : 0.0;
+        Ls[i][l] = v;
     }
-    __syncthreads();
-    const int col = ((int)blockIdx.x - d.chunk0) * 256 + tid;
-    if (col >= d.ncols) return;
-    double* b = d.B + (size_t)col * d.ldb;
-    double x[PB];
+    double x[16];
+    if (MODE == 2) {
 #pragma unroll
-    for (int j = 0; j < PB; ++j) x[j] = (j < d.nb) ? b[j] : 0.0;
-    if (!TRANS) {
+        for (int i = 0; i < 16; ++i) {
+            const int row = 16 * g + i;
+            x[i] = (c < nc && row < d.nb) ? d.B[(size_t)(c0 + c) + (size_t)row * d.ldb] : 0.0;
+        }
+        __syncthreads();
+    } else {
+        for (int e = tid; e < PB * PB; e += 256) {
+            const int i = e & 63, cc = e >> 6;
+            Xs[i][cc] = (i < d.nb && cc < nc) ? d.B[i + (size_t)(c0 + cc) * d.ldb] : 0.0;
+        }
+        __syncthreads();
 #pragma unroll
-        for (int j = 0; j < PB; ++j) {
-            double s = x[j];
+        for (int i = 0; i < 16; ++i) x[i] = Xs[16 * g + i][c];
+    }
+    if (MODE != 1) {
 #pragma unroll
-            for (int l = 0; l < j; ++l) s -= Ls[j][l] * x[l];
-            x[j] = s / Ls[j][j];
+        for (int p = 0; p < 4; ++p) {
+            if (g == p) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    double s = x[i];
+#pragma unroll
+                    for (int l = 0; l < i; ++l) s -= Ls[16 * p + i][16 * p + l] * x[l];
+                    x[i] = s / Ls[16 * p + i][16 * p + i];
+                    Xs[16 * p + i][c] = x[i];
+                }
+            }
+            __syncthreads();
+            if (g > p) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    double s = x[i];
+#pragma unroll
+                    for (int l = 0; l < 16; ++l) s -= Ls[16 * g + i][16 * p + l] * Xs[16 * p + l][c];
+                    x[i] = s;
+                }
+            }
         }
     } else {
 #pragma unroll
-        for (int j = PB - 1; j >= 0; --j) {
-            double s = x[j];
+        for (int p = 3; p >= 0; --p) {
+            if (g == p) {
 #pragma unroll
-            for (int l = j + 1; l < PB; ++l) s -= Ls[l][j] * x[l];
-            x[j] = s / Ls[j][j];
+                for (int i = 15; i >= 0; --i) {
+                    double s = x[i];
+#pragma unroll
+                    for (int l = i + 1; l < 16; ++l) s -= Ls[16 * p + l][16 * p + i] * x[l];
+                    x[i] = s / Ls[16 * p + i][16 * p + i];
+                    Xs[16 * p + i][c] = x[i];
+                }
+            }
+            __syncthreads();
+            if (g < p) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    double s = x[i];
+#pragma unroll
+                    for (int l = 0; l < 16; ++l) s -= Ls[16 * p + l][16 * g + i] * Xs[16 * p + l][c];
+                    x[i] = s;
+                }
+            }
         }
     }
+    if (MODE == 2) {
 #pragma unroll
-    for (int j = 0; j < PB; ++j)
-        if (j < d.nb) b[j] = x[j];
+        for (int i = 0; i < 16; ++i) {
+            const int row = 16 * g + i;
+            if (c < nc && row < d.nb) d.B[(size_t)(c0 + c) + (size_t)row * d.ldb] = x[i];
+        }
+    } else {
+        // every row group has published its solution in Xs before the last barrier
+        for (int e = tid; e < PB * PB; e += 256) {
+            const int i = e & 63, cc = e >> 6;
+            if (i < d.nb && cc < nc) d.B[i + (size_t)(c0 + cc) * d.ldb] = Xs[i][cc];
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_zero_upper(const SqMat* __restrict__ ms) {
@@ -376,13 +432,24 @@ __global__ __launch_bounds__(256) void k_zero_upper(const SqMat* __restrict__ ms
     }
 }
 
+static void launch_trsm_diag(Ctx& c, int mode, const std::vector<LdiagDesc>& ds, int chunks) {
+    if (ds.empty()) return;
+    LdiagDesc* dd = c.upload_vec(ds);
+    if (mode == 0)
+        KMH_LAUNCH(c, k_trsm_diag<0>, dim3(chunks), dim3(256), dd, (int)ds.size());
+    else if (mode == 1)
+        KMH_LAUNCH(c, k_trsm_diag<1>, dim3(chunks), dim3(256), dd, (int)ds.size());
+    else
+        KMH_LAUNCH(c, k_trsm_diag<2>, dim3(chunks), dim3(256), dd, (int)ds.size());
+}
+
 void potrf(Ctx& c, const std::vector<SqMat>& mats) {
     if (mats.empty()) return;
     int nmax = 0;
     for (const auto& m : mats) nmax = std::max(nmax, m.n);
     for (int k0 = 0; k0 < nmax; k0 += PB) {
         std::vector<SqMat> diag;
-        std::vector<RltDesc> pan;
+        std::vector<LdiagDesc> pan;
         std::vector<GemmProb> upd;
         int chunks = 0;
         for (const auto& m : mats) {
@@ -393,24 +460,20 @@ void potrf(Ctx& c, const std::vector<SqMat>& mats) {
             const int rest = m.n - k0 - nb;
             if (rest > 0) {
                 pan.push_back({Akk, nb, m.ld, Akk + nb, rest, m.ld, chunks, 0});
-                chunks += (int)ceil_div(rest, 256);
+                chunks += (int)ceil_div(rest, PB);
                 upd.push_back({rest, rest, nb, Akk + nb, m.ld, Akk + nb, m.ld,
                                Akk + nb + (size_t)nb * m.ld, m.ld, -1.0, 1.0, 1});
             }
         }
         SqMat* dd = c.upload_vec(diag);
-        hipLaunchKernelGGL(k_potrf_diag, dim3((unsigned)diag.size()), dim3(256), 0, c.stream, dd, c.d_info);
-        KMH_HIP(hipGetLastError());
+        KMH_LAUNCH(c, k_potrf_diag, dim3((unsigned)diag.size()), dim3(64), dd, c.d_info);
         if (!pan.empty()) {
-            RltDesc* dp = c.upload_vec(pan);
-            hipLaunchKernelGGL(k_trsm_rlt, dim3(chunks), dim3(256), 0, c.stream, dp, (int)pan.size());
-            KMH_HIP(hipGetLastError());
+            launch_trsm_diag(c, 2, pan, chunks);
             gemm(c, false, true, upd);
         }
     }
     SqMat* dm = c.upload_vec(mats);
-    hipLaunchKernelGGL(k_zero_upper, dim3(16, (unsigned)mats.size()), dim3(256), 0, c.stream, dm);
-    KMH_HIP(hipGetLastError());
+    KMH_LAUNCH(c, k_zero_upper, dim3(16, (unsigned)mats.size()), dim3(256), dm);
 }
 
 void trsm_left(Ctx& c, bool trans, const std::vector<TrsmProb>& probs) {
@@ -427,7 +490,7 @@ void trsm_left(Ctx& c, bool trans, const std::vector<TrsmProb>& probs) {
             const int k0 = kb * PB;
             const int nb = std::min(PB, p.n - k0);
             diag.push_back({p.L + k0 + (size_t)k0 * p.ldl, nb, p.ldl, p.B + k0, p.ncols, p.ldb, chunks, 0});
-            chunks += (int)ceil_div(p.ncols, 256);
+            chunks += (int)ceil_div(p.ncols, PB);
             if (!trans) {
                 const int rest = p.n - k0 - nb;
                 if (rest > 0)
@@ -438,29 +501,10 @@ void trsm_left(Ctx& c, bool trans, const std::vector<TrsmProb>& probs) {
             }
         }
         if (diag.empty()) continue;
-        LdiagDesc* dd = c.upload_vec(diag);
-        if (!trans)
-            hipLaunchKernelGGL((k_trsm_ldiag<false>), dim3(chunks), dim3(256), 0, c.stream, dd, (int)diag.size());
-        else
-            hipLaunchKernelGGL((k_trsm_ldiag<true>), dim3(chunks), dim3(256), 0, c.stream, dd, (int)diag.size());
-        KMH_HIP(hipGetLastError());
+        launch_trsm_diag(c, trans ? 1 : 0, diag, chunks);
         if (!upd.empty()) gemm(c, trans, false, upd);
     }
 }
-
-// ---------------------------------------------------------------------------
-// Householder QR of small tiles (TSQR building block)
-// ---------------------------------------------------------------------------
-constexpr int QT = 256;   // max rows of a tile
-constexpr int QC = 64;    // max columns
-
-struct SqrDesc {
-    const double* src0; int rows0, ld0;
-    const double* src1; int rows1, ld1;
-    double* qout; int ldq;
-    double* rout; int ldr;
-    int nc, pad;
-};
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -476,229 +520,6 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
     if (lane == 0) sh[wave] = v;
     __syncthreads();
     return ((sh[0] + sh[1]) + (sh[2] + sh[3]));
-}
-
-__global__ __launch_bounds__(256) void k_small_qr(const SqrDesc* __restrict__ ds) {
-    __shared__ double A[QC][QT + 1];   // A[col][row]
-    __shared__ double tau[QC];
-    __shared__ double w[QC];
-    __shared__ double part[4][QC];
-    __shared__ double sh[4];
-    const SqrDesc d = ds[blockIdx.x];
-    const int mt = d.rows0 + d.rows1, nc = d.nc, tid = threadIdx.x;
-    for (int e = tid; e < mt * nc; e += 256) {
-        const int i = e % mt, j = e / mt;
-        A[j][i] = (i < d.rows0) ? d.src0[i + (size_t)j * d.ld0] : d.src1[(i - d.rows0) + (size_t)j * d.ld1];
-    }
-    __syncthreads();
-    const int cc = tid & 63, pt = tid >> 6;
-    for (int j = 0; j < nc; ++j) {
-        // xnorm^2 over rows j+1..mt-1 of column j
-        double s = 0.0;
-        for (int i = j + 1 + tid; i < mt; i += 256) s += A[j][i] * A[j][i];
-        const double x2 = block_sum(s, sh);
-        const double alpha = A[j][j];
-        double tj, beta, scal;
-        if (x2 == 0.0) {
-            tj = 0.0; beta = alpha; scal = 0.0;
-        } else {
-            beta = -copysign(sqrt(alpha * alpha + x2), alpha);
-            tj = (beta - alpha) / beta;
-            scal = 1.0 / (alpha - beta);
-        }
-        __syncthreads();
-        if (x2 != 0.0)
-            for (int i = j + 1 + tid; i < mt; i += 256) A[j][i] *= scal;
-        if (tid == 0) { A[j][j] = beta; tau[j] = tj; }
-        __syncthreads();
-        if (tj != 0.0 && j + 1 < nc) {
-            // w_c = A(j,c) + sum_{i>j} v_i A(i,c)
-            const int c = j + 1 + cc;
-            double p = 0.0;
-            if (c < nc)
-                for (int i = j + 1 + pt; i < mt; i += 4) p += A[j][i] * A[c][i];
-            part[pt][cc] = p;
-            __syncthreads();
-            if (tid < 64 && c < nc) w[cc] = A[c][j] + ((part[0][cc] + part[1][cc]) + (part[2][cc] + part[3][cc]));
-            __syncthreads();
-            for (int i = j + tid; i < mt; i += 256) {
-                const double vi = (i == j) ? 1.0 : A[j][i];
-                const double tv = tj * vi;
-                for (int c2 = j + 1; c2 < nc; ++c2) A[c2][i] -= tv * w[c2 - j - 1];
-            }
-            __syncthreads();
-        }
-    }
-    // R (nc x nc upper; lower zeroed)
-    for (int e = tid; e < nc * nc; e += 256) {
-        const int i = e % nc, j = e / nc;
-        d.rout[i + (size_t)j * d.ldr] = (i <= j) ? A[j][i] : 0.0;
-    }
-    __syncthreads();
-    // explicit Q in place (LAPACK dorg2r order)
-    for (int j = nc - 1; j >= 0; --j) {
-        const double tj = tau[j];
-        if (j + 1 < nc && tj != 0.0) {
-            const int c = j + 1 + cc;
-            double p = 0.0;
-            if (c < nc)
-                for (int i = j + 1 + pt; i < mt; i += 4) p += A[j][i] * A[c][i];
-            part[pt][cc] = p;
-            __syncthreads();
-            if (tid < 64 && c < nc) w[cc] = A[c][j] + ((part[0][cc] + part[1][cc]) + (part[2][cc] + part[3][cc]));
-            __syncthreads();
-            for (int i = j + tid; i < mt; i += 256) {
-                const double vi = (i == j) ? 1.0 : A[j][i];
-                const double tv = tj * vi;
-                for (int c2 = j + 1; c2 < nc; ++c2) A[c2][i] -= tv * w[c2 - j - 1];
-            }
-            __syncthreads();
-        } else if (j + 1 < nc) {
-            // H_j = I: the rows above and at j of later columns are untouched
-            __syncthreads();
-        }
-        for (int i = tid; i < mt; i += 256) {
-            double v;
-            if (i < j) v = 0.0;
-            else if (i == j) v = 1.0 - tj;
-            else v = -tj * A[j][i];
-            A[j][i] = v;
-        }
-        __syncthreads();
-    }
-    for (int e = tid; e < mt * nc; e += 256) {
-        const int i = e % mt, j = e / mt;
-        d.qout[i + (size_t)j * d.ldq] = A[j][i];
-    }
-}
-
-struct TsqrPanel {
-    double* A; int m; int nb; int lda;
-    double* R; int ldr;
-};
-
-// Thin QR of m x nb panels (nb <= 64, m >= nb) by TSQR.
-static void tsqr(Ctx& c, const std::vector<TsqrPanel>& ps) {
-    if (ps.empty()) return;
-    struct Node { int panel; const double* rptr; int rld; };
-    struct Tile { int panel; int row0; int rows; int node; };
-    std::vector<Tile> tiles;
-    std::vector<std::vector<int>> cur(ps.size());
-    int nnodes = 0;
-    for (size_t p = 0; p < ps.size(); ++p) {
-        const int nt = std::max<int>(1, (int)ceil_div(ps[p].m, QT));
-        const int base = ps[p].m / nt, rem = ps[p].m % nt;
-        int r0 = 0;
-        for (int t = 0; t < nt; ++t) {
-            const int rows = base + (t < rem ? 1 : 0);
-            tiles.push_back({(int)p, r0, rows, nnodes});
-            cur[p].push_back(nnodes++);
-            r0 += rows;
-        }
-    }
-    // count internal nodes
-    int ninternal = 0;
-    for (size_t p = 0; p < ps.size(); ++p) {
-        int k = (int)cur[p].size();
-        while (k > 1) { ninternal += k / 2; k = k - k / 2; }
-    }
-    const int total_nodes = nnodes + ninternal;
-    const size_t RS = (size_t)QC * QC;          // R slot, ld 64
-    const size_t QS = (size_t)2 * QC * QC;      // Q' slot, 128 x 64, ld 128
-    double* RT = static_cast<double*>(c.scratch(std::max<size_t>(1, total_nodes) * RS * sizeof(double)));
-    double* QTb = static_cast<double*>(c.scratch(std::max(1, ninternal) * QS * sizeof(double)));
-    double* CS = static_cast<double*>(c.scratch(std::max<size_t>(1, total_nodes) * RS * sizeof(double)));
-    // tile QRs
-    {
-        std::vector<SqrDesc> ds;
-        for (const auto& t : tiles) {
-            const auto& P = ps[t.panel];
-            const bool single = cur[t.panel].size() == 1;
-            double* a = P.A + t.row0;
-            ds.push_back({a, t.rows, P.lda, nullptr, 0, 0, a, P.lda,
-                          single ? P.R : RT + (size_t)t.node * RS, single ? P.ldr : QC, P.nb, 0});
-        }
-        SqrDesc* dd = c.upload_vec(ds);
-        hipLaunchKernelGGL(k_small_qr, dim3((unsigned)ds.size()), dim3(256), 0, c.stream, dd);
-        KMH_HIP(hipGetLastError());
-    }
-    // reduction tree
-    struct Comb { int panel; int node; int a; int b; int level; };
-    std::vector<Comb> combs;
-    int next_internal = 0;
-    int level = 0;
-    std::vector<int> root(ps.size(), -1);
-    for (size_t p = 0; p < ps.size(); ++p)
-        if (cur[p].size() == 1) root[p] = cur[p][0];
-    for (;;) {
-        std::vector<SqrDesc> ds;
-        for (size_t p = 0; p < ps.size(); ++p) {
-            auto& L = cur[p];
-            if (L.size() <= 1) continue;
-            std::vector<int> nxt;
-            const bool final_level = (L.size() == 2);
-            for (size_t i = 0; i + 1 < L.size(); i += 2) {
-                const int node = nnodes + next_internal;
-                const int qi = next_internal++;
-                combs.push_back({(int)p, node, L[i], L[i + 1], level});
-                const bool is_root = final_level;
-                ds.push_back({RT + (size_t)L[i] * RS, ps[p].nb, QC, RT + (size_t)L[i + 1] * RS, ps[p].nb, QC,
-                              QTb + (size_t)qi * QS, 2 * QC,
-                              is_root ? ps[p].R : RT + (size_t)node * RS, is_root ? ps[p].ldr : QC,
-                              ps[p].nb, 0});
-                nxt.push_back(node);
-                if (is_root) root[p] = node;
-            }
-            if (L.size() % 2) nxt.push_back(L.back());
-            L = nxt;
-        }
-        if (ds.empty()) break;
-        SqrDesc* dd = c.upload_vec(ds);
-        hipLaunchKernelGGL(k_small_qr, dim3((unsigned)ds.size()), dim3(256), 0, c.stream, dd);
-        KMH_HIP(hipGetLastError());
-        ++level;
-    }
-    // down pass: coefficient of each node (pointer + ld); root = identity
-    std::vector<const double*> cptr(total_nodes, nullptr);
-    std::vector<int> cld(total_nodes, 0);
-    std::vector<char> is_ident(total_nodes, 0);
-    for (size_t p = 0; p < ps.size(); ++p) is_ident[root[p]] = 1;
-    for (int lv = level - 1; lv >= 0; --lv) {
-        std::vector<GemmProb> gp;
-        for (const auto& cb : combs) {
-            if (cb.level != lv) continue;
-            const int nb = ps[cb.panel].nb;
-            const int qi = cb.node - nnodes;
-            const double* Q = QTb + (size_t)qi * QS;
-            for (int h = 0; h < 2; ++h) {
-                const int child = h ? cb.b : cb.a;
-                if (is_ident[cb.node]) {
-                    cptr[child] = Q + h * nb;
-                    cld[child] = 2 * QC;
-                } else {
-                    double* dst = CS + (size_t)child * RS;
-                    gp.push_back({nb, nb, nb, Q + h * nb, 2 * QC, cptr[cb.node], cld[cb.node], dst, QC,
-                                  1.0, 0.0, 0});
-                    cptr[child] = dst;
-                    cld[child] = QC;
-                }
-            }
-        }
-        gemm(c, false, false, gp);
-    }
-    // tiles: A_tile <- Q_tile * C_tile (in place; n = nb <= 64 keeps each
-    // output row block inside one workgroup, which reads it before writing)
-    std::vector<GemmProb> gp;
-    for (const auto& t : tiles) {
-        if (is_ident[t.node]) continue;
-        const auto& P = ps[t.panel];
-        double* a = P.A + t.row0;
-        gp.push_back({t.rows, P.nb, P.nb, a, P.lda, cptr[t.node], cld[t.node], a, P.lda, 1.0, 0.0, 0});
-    }
-    gemm(c, false, false, gp);
-    c.scratch_free(CS);
-    c.scratch_free(QTb);
-    c.scratch_free(RT);
 }
 
 // ---------------------------------------------------------------------------
@@ -724,9 +545,8 @@ void copy2d(Ctx& c, const std::vector<Copy2D>& probs, bool transpose) {
         long long mx = 0;
         for (auto& p : part) mx = std::max<long long>(mx, (long long)p.m * p.n);
         const unsigned gx = (unsigned)std::min<long long>(std::max<long long>(1, ceil_div(mx, 256)), 1024);
-        if (transpose) hipLaunchKernelGGL((k_copy2d<true>), dim3(gx, (unsigned)part.size()), dim3(256), 0, c.stream, d);
-        else hipLaunchKernelGGL((k_copy2d<false>), dim3(gx, (unsigned)part.size()), dim3(256), 0, c.stream, d);
-        KMH_HIP(hipGetLastError());
+        if (transpose) KMH_LAUNCH(c, (k_copy2d<true>), dim3(gx, (unsigned)part.size()), dim3(256), d);
+        else KMH_LAUNCH(c, (k_copy2d<false>), dim3(gx, (unsigned)part.size()), dim3(256), d);
     }
 }
 
@@ -749,8 +569,7 @@ void axpby2d(Ctx& c, const std::vector<Axpby2D>& probs) {
         long long mx = 0;
         for (auto& p : part) mx = std::max<long long>(mx, (long long)p.m * p.n);
         const unsigned gx = (unsigned)std::min<long long>(std::max<long long>(1, ceil_div(mx, 256)), 1024);
-        hipLaunchKernelGGL(k_axpby2d, dim3(gx, (unsigned)part.size()), dim3(256), 0, c.stream, d);
-        KMH_HIP(hipGetLastError());
+        KMH_LAUNCH(c, k_axpby2d, dim3(gx, (unsigned)part.size()), dim3(256), d);
     }
 }
 
@@ -776,9 +595,8 @@ static void diag_launch(Ctx& c, const std::vector<DiagOp>& mats, bool set) {
         long long mx = 0;
         for (auto& p : part) mx = std::max<long long>(mx, set ? (long long)p.n * p.n : p.n);
         const unsigned gx = (unsigned)std::min<long long>(std::max<long long>(1, ceil_div(mx, 256)), 1024);
-        if (set) hipLaunchKernelGGL((k_diag_op<true>), dim3(gx, (unsigned)part.size()), dim3(256), 0, c.stream, d);
-        else hipLaunchKernelGGL((k_diag_op<false>), dim3(gx, (unsigned)part.size()), dim3(256), 0, c.stream, d);
-        KMH_HIP(hipGetLastError());
+        if (set) KMH_LAUNCH(c, (k_diag_op<true>), dim3(gx, (unsigned)part.size()), dim3(256), d);
+        else KMH_LAUNCH(c, (k_diag_op<false>), dim3(gx, (unsigned)part.size()), dim3(256), d);
     }
 }
 void set_identity(Ctx& c, const std::vector<DiagOp>& mats) { diag_launch(c, mats, true); }
@@ -804,8 +622,7 @@ void log_diag_sums(Ctx& c, const std::vector<SqMat>& mats, const std::vector<dou
     std::vector<LogDiagDesc> ds;
     for (size_t j = 0; j < mats.size(); ++j) ds.push_back({mats[j].A, mats[j].n, mats[j].ld, outs[j]});
     LogDiagDesc* d = c.upload_vec(ds);
-    hipLaunchKernelGGL(k_log_diag, dim3((unsigned)ds.size()), dim3(256), 0, c.stream, d);
-    KMH_HIP(hipGetLastError());
+    KMH_LAUNCH(c, k_log_diag, dim3((unsigned)ds.size()), dim3(256), d);
 }
 
 void log_diag_sums(Ctx& c, const std::vector<SqMat>& mats, double* d_out) {
@@ -826,8 +643,7 @@ __global__ __launch_bounds__(256) void k_sum_vec(const SumJob* __restrict__ jobs
 void sum_vectors(Ctx& c, const std::vector<SumJob>& jobs) {
     if (jobs.empty()) return;
     SumJob* d = c.upload_vec(jobs);
-    hipLaunchKernelGGL(k_sum_vec, dim3((unsigned)jobs.size()), dim3(256), 0, c.stream, d);
-    KMH_HIP(hipGetLastError());
+    KMH_LAUNCH(c, k_sum_vec, dim3((unsigned)jobs.size()), dim3(256), d);
 }
 
 void sum_vector(Ctx& c, const double* d_in, int count, double* d_out) { sum_vectors(c, {{d_in, count, d_out}}); }
@@ -845,58 +661,10 @@ __global__ __launch_bounds__(256) void k_coldots(const double* X, int ldx, const
 void column_dots(Ctx& c, const double* X, int ldx, const double* Y, int ldy, int m, int ncols,
                  double* d_out) {
     if (ncols <= 0) return;
-    hipLaunchKernelGGL(k_coldots, dim3(ncols), dim3(256), 0, c.stream, X, ldx, Y, ldy, m, d_out);
-    KMH_HIP(hipGetLastError());
+    KMH_LAUNCH(c, k_coldots, dim3(ncols), dim3(256), X, ldx, Y, ldy, m, d_out);
 }
 
 // ---------------------------------------------------------------------------
 // panel QR: block Gram-Schmidt (two passes) over 64-column blocks + TSQR
 // ---------------------------------------------------------------------------
-void qr_panels(Ctx& c, const std::vector<QrPanel>& ps) {
-    if (ps.empty()) return;
-    std::vector<DiagOp> z;
-    int rmax = 0;
-    for (const auto& p : ps) {
-        KMH_CHECK(p.m >= p.r, KMH_E_BAD_ARG, "qr_panels: panel with more columns than rows");
-        if (p.r > 0) z.push_back({p.R, p.r, p.ldr, 0.0});
-        rmax = std::max(rmax, p.r);
-    }
-    set_identity(c, z);
-    for (int c0 = 0; c0 < rmax; c0 += QC) {
-        std::vector<TsqrPanel> tp;
-        std::vector<GemmProb> p1, u1, p2, u2;
-        std::vector<Axpby2D> acc;
-        size_t tsz = 0;
-        for (const auto& p : ps)
-            if (p.r > c0 && c0 > 0) tsz += (size_t)c0 * std::min(QC, p.r - c0);
-        double* T2 = tsz ? static_cast<double*>(c.scratch(tsz * sizeof(double))) : nullptr;
-        size_t toff = 0;
-        for (const auto& p : ps) {
-            if (p.r <= c0) continue;
-            const int nb = std::min(QC, p.r - c0);
-            double* Aj = p.A + (size_t)c0 * p.lda;
-            double* Rj = p.R + (size_t)c0 * p.ldr;
-            if (c0 > 0) {
-                double* t2 = T2 + toff;
-                toff += (size_t)c0 * nb;
-                p1.push_back({c0, nb, p.m, p.A, p.lda, Aj, p.lda, Rj, p.ldr, 1.0, 0.0, 0});
-                u1.push_back({p.m, nb, c0, p.A, p.lda, Rj, p.ldr, Aj, p.lda, -1.0, 1.0, 0});
-                p2.push_back({c0, nb, p.m, p.A, p.lda, Aj, p.lda, t2, c0, 1.0, 0.0, 0});
-                u2.push_back({p.m, nb, c0, p.A, p.lda, t2, c0, Aj, p.lda, -1.0, 1.0, 0});
-                acc.push_back({t2, c0, Rj, p.ldr, c0, nb, 1.0, 1.0});
-            }
-            tp.push_back({Aj, p.m, nb, p.lda, Rj + c0, p.ldr});
-        }
-        if (!p1.empty()) {
-            gemm(c, true, false, p1);
-            gemm(c, false, false, u1);
-            gemm(c, true, false, p2);
-            gemm(c, false, false, u2);
-            axpby2d(c, acc);
-        }
-        tsqr(c, tp);
-        if (T2) c.scratch_free(T2);
-    }
-}
-
 }  // namespace kmh

@@ -103,11 +103,11 @@ static void finish_batch(Server& S, std::vector<Req*>& rs) {
         for (size_t i = 0; i < ok.size(); ++i) {
             const int64_t n = ok[i]->n;
             if (okr[i]->dy)
-                KMH_HIP(hipMemcpyAsync(buf + off, okr[i]->dy, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+                KMH_HIP(hipMemcpyAsync(buf + off, okr[i]->dy, n * sizeof(double), hipMemcpyDeviceToDevice, c.q()));
             else
                 KMH_HIP(hipMemcpyAsync(buf + off, okr[i]->y.data(), n * sizeof(double), hipMemcpyHostToDevice,
-                                       c.stream));
-            KMH_HIP(hipMemcpyAsync(buf + off + n, buf + off, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+                                       c.q()));
+            KMH_HIP(hipMemcpyAsync(buf + off + n, buf + off, n * sizeof(double), hipMemcpyDeviceToDevice, c.q()));
             T.push_back(buf + off + n);
             k.push_back(1);
             off += 2 * n;
@@ -120,7 +120,7 @@ static void finish_batch(Server& S, std::vector<Req*>& rs) {
             off += 2 * n;
         }
         std::vector<double> hd(ok.size());
-        KMH_HIP(hipMemcpyAsync(hd.data(), dots, ok.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        KMH_HIP(hipMemcpyAsync(hd.data(), dots, ok.size() * sizeof(double), hipMemcpyDeviceToHost, c.q()));
         c.sync();
         c.scratch_free(buf);
         for (size_t i = 0; i < ok.size(); ++i) {
@@ -245,17 +245,7 @@ struct kmh_server {
     kmh::Server s;
 };
 
-#define SRV_TRY(...)                                                \
-    try {                                                           \
-        __VA_ARGS__;                                                \
-        return KMH_OK;                                              \
-    } catch (const kmh::Error& e) {                                 \
-        kmh::set_last_error(e.what());                              \
-        return e.code;                                              \
-    } catch (const std::exception& e) {                             \
-        kmh::set_last_error(e.what());                              \
-        return KMH_E_DEVICE;                                        \
-    }
+#define SRV_TRY(...) KMH_API_TRY(__VA_ARGS__)
 
 extern "C" {
 

@@ -13,27 +13,17 @@ struct Dev {
     Dev(kmh::Ctx& cc, const double* host, size_t count) : c(cc) {
         bytes = std::max<size_t>(8, count * sizeof(double));
         p = static_cast<double*>(c.scratch(bytes));
-        if (host && count) KMH_HIP(hipMemcpyAsync(p, host, count * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        if (host && count) KMH_HIP(hipMemcpyAsync(p, host, count * sizeof(double), hipMemcpyHostToDevice, c.q()));
     }
     void to_host(double* host, size_t count) {
-        KMH_HIP(hipMemcpyAsync(host, p, count * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        KMH_HIP(hipMemcpyAsync(host, p, count * sizeof(double), hipMemcpyDeviceToHost, c.q()));
     }
     ~Dev() { c.scratch_free(p); }
 };
 
 }  // namespace
 
-#define TRY_(...)                                                   \
-    try {                                                           \
-        __VA_ARGS__;                                                \
-        return KMH_OK;                                              \
-    } catch (const kmh::Error& e) {                                 \
-        kmh::set_last_error(e.what());                              \
-        return e.code;                                              \
-    } catch (const std::exception& e) {                             \
-        kmh::set_last_error(e.what());                              \
-        return KMH_E_DEVICE;                                        \
-    }
+#define TRY_(...) KMH_API_TRY(__VA_ARGS__)
 
 extern "C" {
 
@@ -54,15 +44,15 @@ int kmh_test_potrf(kmh_ctx* ctx, int32_t n, double* A, int32_t lda) {
     TRY_({
         kmh::Ctx& c = ctx->c;
         c.set_device();
-        KMH_HIP(hipMemsetAsync(c.d_info, 0, sizeof(int), c.stream));
+        KMH_HIP(hipMemsetAsync(c.d_info, 0, sizeof(int), c.q()));
         Dev a(c, A, (size_t)lda * n);
         kmh::potrf(c, {{a.p, n, lda}});
         a.to_host(A, (size_t)lda * n);
         int info = 0;
-        KMH_HIP(hipMemcpyAsync(&info, c.d_info, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        KMH_HIP(hipMemcpyAsync(&info, c.d_info, sizeof(int), hipMemcpyDeviceToHost, c.q()));
         c.sync();
         if (info) {
-            KMH_HIP(hipMemsetAsync(c.d_info, 0, sizeof(int), c.stream));
+            KMH_HIP(hipMemsetAsync(c.d_info, 0, sizeof(int), c.q()));
             c.sync();
             throw kmh::Error(KMH_E_NOT_PD, "PosDefException: matrix is not positive definite");
         }

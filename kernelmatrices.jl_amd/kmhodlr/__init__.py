@@ -7,7 +7,7 @@ from . import _lib
 from ._lib import load, LibraryMissing
 from .api import (Context, ErrorException, FactorHODLR, FixedLevel, HBMBudgetExceeded, Kernel,
                   KernelHODLR, KernelMatrix, LogLevel, PosDefException, cauchy_kernel, default_context,
-                  full, ldiv, logdet, matern12, matern32, negloglik, rational_kernel, stoch_fisher, stoch_gradient, stoch_hessian,
+                  full, kd_order, maxlikdata, ldiv, logdet, matern12, matern32, negloglik, rational_kernel, stoch_fisher, stoch_gradient, stoch_hessian,
                   symmetricfactorize)
 
 

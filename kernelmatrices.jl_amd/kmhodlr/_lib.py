@@ -103,6 +103,7 @@ SIGNATURES = {
     "kmh_server_step": [_P, C.POINTER(C.c_int32)],
     "kmh_server_pending": [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_uint64)],
     "kmh_server_result": [_P, C.c_int64, C.POINTER(kmh_serve_result)],
+    "kmh_kd_order": [_P, _I64, C.c_int32, C.c_int32, _P],
 }
 
 # include/kmhodlr_testing.h

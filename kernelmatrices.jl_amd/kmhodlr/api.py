@@ -189,6 +189,33 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def kd_order(pts, leafsize=8):
+    """Permutation that KD-orders the n x d points (host code of the
+    library; see kmh_kd_order)."""
+    pts = np.ascontiguousarray(np.asarray(pts, dtype=np.float64).reshape(len(pts), -1))
+    n, d = pts.shape
+    perm = np.empty(n, dtype=np.int64)
+    lib = L.load()
+    _check(lib, lib.kmh_kd_order(pts.ctypes.data, n, d, int(leafsize), perm.ctypes.data))
+    return perm
+
+
+def maxlikdata(pts, data, sortmethod="KDTree", warn=True):
+    """maxlikdata(; pts, data, sortmethod) (HODLR/src/constructor.jl:171-181):
+    the points in KD order and the data reordered with them."""
+    pts = np.asarray(pts, dtype=np.float64)
+    data = np.asarray(data, dtype=np.float64)
+    if sortmethod in ("Nothing", None):
+        if warn:
+            import warnings
+            warnings.warn("Not sorting data can adversely affect approximation quality...")
+        return pts, data
+    if sortmethod != "KDTree":
+        raise ErrorException("The options are :Nothing and :KDTree")
+    perm = kd_order(pts)
+    return np.ascontiguousarray(pts[perm]), np.ascontiguousarray(data[perm])
+
+
 def _as_cols(x, n):
     """Vector or matrix -> column-major n x k buffer and k."""
     x = np.asarray(x, dtype=np.float64)

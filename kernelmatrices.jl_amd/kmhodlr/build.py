@@ -16,7 +16,7 @@ INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(HERE, "libkmhodlr.so")
 
-SOURCES = ["ctx.hip", "la.hip", "assemble.hip", "hodlr.hip", "gradient.hip", "hessian.hip", "capi.hip", "serve.hip", "testing.hip"]
+SOURCES = ["ctx.hip", "la.hip", "qr.hip", "assemble.hip", "hodlr.hip", "gradient.hip", "hessian.hip", "capi.hip", "serve.hip", "ordering.hip", "testing.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
@@ -24,8 +24,19 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
          "-Wno-unused-command-line-argument", "-I" + INCLUDE, "-I" + CSRC]
 
 
-def _compile(src):
+def _stale(src, obj):
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [os.path.join(CSRC, src)] + [os.path.join(d, f) for d in (CSRC, INCLUDE)
+                                        for f in os.listdir(d) if f.endswith(".h")]
+    return any(os.path.getmtime(x) > t for x in deps)
+
+
+def _compile(src, incremental=False):
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    if incremental and not _stale(src, obj):
+        return obj, ""
     cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     out = "\n".join(l for l in (p.stdout + p.stderr).splitlines()
@@ -35,10 +46,12 @@ def _compile(src):
     return obj, out
 
 
-def build(verbose=False):
+def build(verbose=False, incremental=False):
+    """Compile every source (incremental=True: only those older than their
+    source or a header) and link libkmhodlr.so."""
     os.makedirs(BUILD, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        results = list(ex.map(_compile, SOURCES))
+        results = list(ex.map(lambda s: _compile(s, incremental), SOURCES))
     for src, (obj, out) in zip(SOURCES, results):
         if verbose and out:
             print("[%s]\n%s" % (src, out), file=sys.stderr)
@@ -52,4 +65,4 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, incremental="--incremental" in sys.argv))

@@ -219,3 +219,6 @@ def test_stoch_fisher_hessian_parity(km, exact):
     Hg = km.stoch_hessian(Kg, HKg, y, [k.d1, k.d2], [[None, k.d12], [k.d22]], vecs, exact_second=exact)
     assert np.allclose(Fg, Fo, rtol=1e-9, atol=0), (Fg, Fo)
     assert np.allclose(Hg, Ho, rtol=1e-8, atol=1e-10 * np.abs(Ho).max()), (Hg, Ho)
+    # deterministic: a second call gives bitwise the same matrix
+    Hg2 = km.stoch_hessian(Kg, HKg, y, [k.d1, k.d2], [[None, k.d12], [k.d22]], vecs, exact_second=exact)
+    assert np.array_equal(Hg, Hg2)
