@@ -20,28 +20,24 @@ DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
 
 DevBuf::~DevBuf() { release(); }
 
+// Handle buffers come from the context's pool: on the context's one
+// in-order stream a block can be reused by later work as soon as it is
+// released, so neither allocation nor release touches the device.
 void DevBuf::alloc(Ctx* c, size_t nbytes) {
     release();
     if (nbytes == 0) return;
     KMH_CHECK(c->budget == 0 || c->in_use + nbytes <= c->budget, KMH_E_BUDGET,
               "HBM budget exceeded: need " + std::to_string(nbytes) + " more bytes, " +
                   std::to_string(c->budget - std::min<uint64_t>(c->budget, c->in_use)) + " available");
-    c->set_device();
-    void* q = nullptr;
-    hipError_t e = hipMalloc(&q, nbytes);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        throw Error(KMH_E_OOM, "hipMalloc of " + std::to_string(nbytes) + " bytes failed");
-    }
-    p = q; bytes = nbytes; ctx = c;
+    p = c->scratch(nbytes);
+    bytes = nbytes;
+    ctx = c;
     c->in_use += nbytes;
 }
 
 void DevBuf::release() {
     if (p) {
-        // frees are ordered after all work queued on the context's stream
-        ctx->drain();
-        (void)hipFree(p);
+        ctx->scratch_free(p);
         ctx->in_use -= bytes;
     }
     p = nullptr; bytes = 0; ctx = nullptr;

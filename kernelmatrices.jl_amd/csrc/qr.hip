@@ -58,12 +58,20 @@ __device__ __forceinline__ double tile_colsum(const double* part, int k) {
     return (p[0] + p[1]) + (p[2] + p[3]);
 }
 
+// value of v held by `lane` (a compile-time constant after unrolling)
+__device__ __forceinline__ double qr_bcast(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
 // Register-resident Householder QR (LAPACK dgeqr2 + dorg2r arithmetic):
 // thread t holds row t of the tile.  The column loop runs at run time with a
 // fixed body: after step j the registers shift left by one, so the pivot
 // column is always a[0] and register indices stay static.  Step j needs
 // S_k = sum_{rows > j} a(row, j) a(row, k); the four wave partials meet in
-// LDS (one barrier per column).  The Householder vector of step j is parked
+// LDS (one barrier per column), lane k of every wave completes S_k and
+// v^T a_k, and readlane hands them to the other lanes.  The Householder vector of step j is parked
 // in column j of the Q output (each thread its own row) for the backward
 // accumulation of Q, which shifts right one column per step.
 template <int NC>
@@ -93,8 +101,11 @@ __global__ __launch_bounds__(256) void k_tile_qr(const SqrDesc* __restrict__ ds)
         }
         tile_reduce<NC>(a, r > j ? a[0] : 0.0, pw, part[buf], lane, wave);
         __syncthreads();
-        const double alpha = rowj[buf][0];
-        const double x2 = tile_colsum(part[buf], 0);
+        // lane l finishes column l % NC; the other lanes read it by readlane
+        const double sk = tile_colsum(part[buf], lane % NC);
+        const double rk = rowj[buf][lane % NC];
+        const double alpha = qr_bcast(rk, 0);
+        const double x2 = qr_bcast(sk, 0);
         double tau, beta, scal;
         if (x2 == 0.0) {
             tau = 0.0; beta = alpha; scal = 0.0;
@@ -105,9 +116,10 @@ __global__ __launch_bounds__(256) void k_tile_qr(const SqrDesc* __restrict__ ds)
         }
         const double vr = (r > j) ? scal * a[0] : (r == j ? 1.0 : 0.0);
         if (tau != 0.0) {
+            const double wk = rk + scal * sk;   // v^T a_k for this lane's column
             const double tv = tau * vr;
 #pragma unroll
-            for (int k = 1; k < NC; ++k) a[k] -= tv * (rowj[buf][k] + scal * tile_colsum(part[buf], k));
+            for (int k = 1; k < NC; ++k) a[k] -= tv * qr_bcast(wk, k);
         }
         if (r < mt) qcol[(size_t)j * d.ldq] = vr;
         if (r == j) {
@@ -139,9 +151,10 @@ __global__ __launch_bounds__(256) void k_tile_qr(const SqrDesc* __restrict__ ds)
         tile_reduce<NC>(a, vr, pw, part[buf], lane, wave);
         __syncthreads();
         if (tau != 0.0) {
+            const double sk = tile_colsum(part[buf], lane % NC);
             const double tv = tau * vr;
 #pragma unroll
-            for (int k = 1; k < NC; ++k) a[k] -= tv * tile_colsum(part[buf], k);
+            for (int k = 1; k < NC; ++k) a[k] -= tv * qr_bcast(sk, k);
         }
         a[0] = (r > j) ? -tau * vr : (r == j ? 1.0 - tau : 0.0);
     }
