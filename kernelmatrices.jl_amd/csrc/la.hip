@@ -14,6 +14,7 @@
 //  * QR of tall panels: qr.hip.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "la.h"
 
@@ -121,28 +122,145 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ ds, i
             const double a1 = As[kk][wr + 16 + (lane & 15)];
             const double b0 = Bs[kk][wc + (lane & 15)];
             const double b1 = Bs[kk][wc + 16 + (lane & 15)];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+            // operands swapped: each accumulator holds a 16x16 block of C^T,
+            // so a store instruction covers whole column segments of C
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
         }
         __syncthreads();
     }
+    // epilogue: all loads of C before any store, so they are in flight together
+    double cv[2][2][4];
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int row = row0 + wr + mb * 16 + mfma_d_row(lane >> 4, j);
-                const int col = col0 + wc + nb * 16 + (lane & 15);
+                const int row = row0 + wr + mb * 16 + (lane & 15);
+                const int col = col0 + wc + nb * 16 + mfma_d_row(lane >> 4, j);
+                cv[mb][nb][j] = (d.beta != 0.0 && row < d.m && col < d.n) ? d.C[row + (size_t)col * d.ldc] : 0.0;
+            }
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int row = row0 + wr + mb * 16 + (lane & 15);
+                const int col = col0 + wc + nb * 16 + mfma_d_row(lane >> 4, j);
                 if (row < d.m && col < d.n) {
-                    double* cp = d.C + row + (size_t)col * d.ldc;
                     double v = d.alpha * acc[mb][nb][j];
-                    if (d.beta != 0.0) v += d.beta * (*cp);
-                    *cp = v;
+                    if (d.beta != 0.0) v += d.beta * cv[mb][nb][j];
+                    d.C[row + (size_t)col * d.ldc] = v;
                 }
             }
+}
+
+// Wide tiles for skinny problems (64 < n <= 160): a workgroup computes
+// 64 rows x 160 columns, so A is read once per row block and C once; wave w
+// owns rows 16w..16w+15 across ten 16-column MFMA blocks.  Same
+// accumulation order as k_gemm, and the same transposed accumulators.
+constexpr int WN = 160;
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm_w(const GemmDesc* __restrict__ ds, int nd) {
+    constexpr int KW = 16;   // k-chunk (the accumulation order does not depend on it)
+    constexpr int NQA = GT * KW / 256;
+    constexpr int NQB = WN * KW / 256;
+    __shared__ double As[KW][GT + 1];
+    __shared__ double Bs[KW][WN + 1];
+    const int t = blockIdx.x;
+    int lo = 0, hi = nd - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (ds[mid].tile0 <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const GemmDesc d = ds[lo];
+    const int lt = t - d.tile0;
+    const int row0 = (lt / d.tn) * GT, col0 = (lt % d.tn) * WN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    d4 acc[WN / 16];
+#pragma unroll
+    for (int b = 0; b < WN / 16; ++b) acc[b] = d4{0.0, 0.0, 0.0, 0.0};
+    double ra[NQA], rb[NQB];
+    // element e of a chunk: (i, kk) with the operand's contiguous index fastest
+    auto a_idx = [&](int e, int& i, int& kk) {
+        if (!TA) { i = e % GT; kk = e / GT; } else { kk = e % KW; i = e / KW; }
+    };
+    auto b_idx = [&](int e, int& j, int& kk) {
+        if (!TB) { kk = e % KW; j = e / KW; } else { j = e % WN; kk = e / WN; }
+    };
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < NQA; ++q) {
+            int i, kk;
+            a_idx(tid + 256 * q, i, kk);
+            const int gi = row0 + i, gk = k0 + kk;
+            double v = 0.0;
+            if (gi < d.m && gk < d.k) v = TA ? d.A[gk + (size_t)gi * d.lda] : d.A[gi + (size_t)gk * d.lda];
+            ra[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < NQB; ++q) {
+            int j, kk;
+            b_idx(tid + 256 * q, j, kk);
+            const int gj = col0 + j, gk = k0 + kk;
+            double v = 0.0;
+            if (gj < d.n && gk < d.k) v = TB ? d.B[gj + (size_t)gk * d.ldb] : d.B[gk + (size_t)gj * d.ldb];
+            rb[q] = v;
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < d.k; k0 += KW) {
+#pragma unroll
+        for (int q = 0; q < NQA; ++q) {
+            int i, kk;
+            a_idx(tid + 256 * q, i, kk);
+            As[kk][i] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < NQB; ++q) {
+            int j, kk;
+            b_idx(tid + 256 * q, j, kk);
+            Bs[kk][j] = rb[q];
+        }
+        __syncthreads();
+        if (k0 + KW < d.k) fetch(k0 + KW);
+#pragma unroll
+        for (int ks = 0; ks < KW / 4; ++ks) {
+            const int kk = ks * 4 + (lane >> 4);
+            const double a = As[kk][16 * wave + (lane & 15)];
+#pragma unroll
+            for (int b = 0; b < WN / 16; ++b)
+                acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(Bs[kk][16 * b + (lane & 15)], a, acc[b], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    double cv[WN / 16][4];
+#pragma unroll
+    for (int b = 0; b < WN / 16; ++b)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = row0 + 16 * wave + (lane & 15);
+            const int col = col0 + 16 * b + mfma_d_row(lane >> 4, j);
+            cv[b][j] = (d.beta != 0.0 && row < d.m && col < d.n) ? d.C[row + (size_t)col * d.ldc] : 0.0;
+        }
+#pragma unroll
+    for (int b = 0; b < WN / 16; ++b)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = row0 + 16 * wave + (lane & 15);
+            const int col = col0 + 16 * b + mfma_d_row(lane >> 4, j);
+            if (row < d.m && col < d.n) {
+                double v = d.alpha * acc[b][j];
+                if (d.beta != 0.0) v += d.beta * cv[b][j];
+                d.C[row + (size_t)col * d.ldc] = v;
+            }
+        }
 }
 
 struct RedDesc {
@@ -175,19 +293,30 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(const RedDesc* __restrict
     *cp = v;
 }
 
+// Problems with 64 < n <= 160 (and no triangle restriction) run on the wide
+// tiles, the others on 64 x 64 tiles; the choice and the split-k decision
+// depend on the problem's own shape only, so a problem gives bitwise the
+// same result whatever else shares the launch.
+// KMH_GEMM_WIDE=0 turns the wide tiles off (for timing comparisons).
+static bool gemm_wide_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("KMH_GEMM_WIDE");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+static bool gemm_wide(const GemmProb& p) { return gemm_wide_enabled() && !p.lower && p.n > GT && p.n <= WN; }
+
 void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
-    long long total_tiles = 0;
-    for (const auto& p : probs)
-        if (p.m > 0 && p.n > 0) total_tiles += ceil_div(p.m, GT) * ceil_div(p.n, GT);
-    if (total_tiles == 0) return;
-    // Split-k decision from the problem's own shape only, so a problem gives
-    // bitwise the same result whatever else shares the launch.
+    bool any = false;
+    for (const auto& p : probs) any = any || (p.m > 0 && p.n > 0);
+    if (!any) return;
     std::vector<int> split(probs.size(), 1), kchunk(probs.size(), 0);
     size_t wsz = 0;
     for (size_t i = 0; i < probs.size(); ++i) {
         const auto& p = probs[i];
         if (p.m <= 0 || p.n <= 0 || p.lower) continue;
-        const long long own = ceil_div(p.m, GT) * ceil_div(p.n, GT);
+        const long long own = ceil_div(p.m, GT) * ceil_div(p.n, gemm_wide(p) ? WN : GT);
         if (p.k >= 1024 && own < 64) {
             long long s = std::min<long long>(ceil_div(p.k, 512), std::max<long long>(1, 256 / own));
             if (s > 1) {
@@ -200,20 +329,23 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
         }
     }
     double* W = wsz ? static_cast<double*>(c.scratch(wsz * sizeof(double))) : nullptr;
-    std::vector<GemmDesc> ds;
+    std::vector<GemmDesc> ds[2];
+    int tiles[2] = {0, 0};
     std::vector<RedDesc> rs;
-    int tiles = 0;
     size_t woff = 0;
     long long relem = 0;
+    double fl = 0.0;
     for (size_t i = 0; i < probs.size(); ++i) {
         const auto& p = probs[i];
         if (p.m <= 0 || p.n <= 0) continue;
-        const int tn = (int)ceil_div(p.n, GT);
+        const int w = gemm_wide(p) ? 1 : 0;
+        const int tn = (int)ceil_div(p.n, w ? WN : GT);
         const int nt = (int)ceil_div(p.m, GT) * tn;
+        fl += 2.0 * p.m * p.n * (double)p.k;
         if (split[i] == 1) {
-            ds.push_back({p.A, p.B, p.C, p.m, p.n, p.k, p.lda, p.ldb, p.ldc, p.alpha, p.beta,
-                          tiles, tn, p.lower, 0});
-            tiles += nt;
+            ds[w].push_back({p.A, p.B, p.C, p.m, p.n, p.k, p.lda, p.ldb, p.ldc, p.alpha, p.beta,
+                             tiles[w], tn, p.lower, 0});
+            tiles[w] += nt;
         } else {
             const int s = split[i];
             const int kc = kchunk[i];
@@ -222,28 +354,38 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
                 const int kk = std::min(kc, p.k - k0);
                 const double* A = ta ? p.A + k0 : p.A + (size_t)k0 * p.lda;
                 const double* B = tb ? p.B + (size_t)k0 * p.ldb : p.B + k0;
-                ds.push_back({A, B, W + woff + (size_t)q * p.m * p.n, p.m, p.n, kk, p.lda, p.ldb,
-                              p.m, 1.0, 0.0, tiles, tn, 0, 0});
-                tiles += nt;
+                ds[w].push_back({A, B, W + woff + (size_t)q * p.m * p.n, p.m, p.n, kk, p.lda, p.ldb,
+                                 p.m, 1.0, 0.0, tiles[w], tn, 0, 0});
+                tiles[w] += nt;
             }
             rs.push_back({p.C, W + woff, p.ldc, p.m, p.n, s, p.alpha, p.beta, relem});
             woff += (size_t)s * p.m * p.n;
             relem += (long long)p.m * p.n;
         }
     }
-    GemmDesc* dd = c.upload_vec(ds);
-    const int nd = (int)ds.size();
     if (c.prof) {
-        double fl = 0.0;
-        for (const auto& d : ds) fl += 2.0 * d.m * d.n * (double)d.k;
         c.ev_flops.push_back(fl);
         hipEvent_t ev_ = c.prof_event();
         c.launch([ev_, s_ = c.stream]() { KMH_HIP(hipEventRecord(ev_, s_)); });
     }
-    if (!ta && !tb) KMH_LAUNCH(c, (k_gemm<false, false>), dim3(tiles), dim3(256), dd, nd);
-    else if (!ta && tb) KMH_LAUNCH(c, (k_gemm<false, true>), dim3(tiles), dim3(256), dd, nd);
-    else if (ta && !tb) KMH_LAUNCH(c, (k_gemm<true, false>), dim3(tiles), dim3(256), dd, nd);
-    else KMH_LAUNCH(c, (k_gemm<true, true>), dim3(tiles), dim3(256), dd, nd);
+    if (!ds[0].empty()) {
+        GemmDesc* dd = c.upload_vec(ds[0]);
+        const int nd = (int)ds[0].size();
+        const dim3 g(tiles[0]);
+        if (!ta && !tb) KMH_LAUNCH(c, (k_gemm<false, false>), g, dim3(256), dd, nd);
+        else if (!ta && tb) KMH_LAUNCH(c, (k_gemm<false, true>), g, dim3(256), dd, nd);
+        else if (ta && !tb) KMH_LAUNCH(c, (k_gemm<true, false>), g, dim3(256), dd, nd);
+        else KMH_LAUNCH(c, (k_gemm<true, true>), g, dim3(256), dd, nd);
+    }
+    if (!ds[1].empty()) {
+        GemmDesc* dd = c.upload_vec(ds[1]);
+        const int nd = (int)ds[1].size();
+        const dim3 g(tiles[1]);
+        if (!ta && !tb) KMH_LAUNCH(c, (k_gemm_w<false, false>), g, dim3(256), dd, nd);
+        else if (!ta && tb) KMH_LAUNCH(c, (k_gemm_w<false, true>), g, dim3(256), dd, nd);
+        else if (ta && !tb) KMH_LAUNCH(c, (k_gemm_w<true, false>), g, dim3(256), dd, nd);
+        else KMH_LAUNCH(c, (k_gemm_w<true, true>), g, dim3(256), dd, nd);
+    }
     if (c.prof) {
         hipEvent_t ev_ = c.prof_event();
         c.launch([ev_, s_ = c.stream]() { KMH_HIP(hipEventRecord(ev_, s_)); });
@@ -528,9 +670,12 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 template <bool TR>
 __global__ __launch_bounds__(256) void k_copy2d(const Copy2D* __restrict__ ds) {
     const Copy2D d = ds[blockIdx.y];
-    const long long tot = (long long)d.m * d.n;
-    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < tot; e += (long long)gridDim.x * 256) {
-        const int i = (int)(e % d.m), j = (int)(e / d.m);
+    // 256-row segments of one column per step (uniform index arithmetic)
+    const int rb = (d.m + 255) >> 8;
+    const long long segs = (long long)rb * d.n;
+    for (long long t = blockIdx.x; t < segs; t += gridDim.x) {
+        const int j = (int)(t / rb), i = (int)(t - (long long)j * rb) * 256 + (int)threadIdx.x;
+        if (i >= d.m) continue;
         const double v = d.alpha * d.src[i + (size_t)j * d.lds];
         if (TR) d.dst[j + (size_t)i * d.ldd] = v;
         else d.dst[i + (size_t)j * d.ldd] = v;
@@ -552,9 +697,11 @@ void copy2d(Ctx& c, const std::vector<Copy2D>& probs, bool transpose) {
 
 __global__ __launch_bounds__(256) void k_axpby2d(const Axpby2D* __restrict__ ds) {
     const Axpby2D d = ds[blockIdx.y];
-    const long long tot = (long long)d.m * d.n;
-    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < tot; e += (long long)gridDim.x * 256) {
-        const int i = (int)(e % d.m), j = (int)(e / d.m);
+    const int rb = (d.m + 255) >> 8;
+    const long long segs = (long long)rb * d.n;
+    for (long long t = blockIdx.x; t < segs; t += gridDim.x) {
+        const int j = (int)(t / rb), i = (int)(t - (long long)j * rb) * 256 + (int)threadIdx.x;
+        if (i >= d.m) continue;
         double* p = d.dst + i + (size_t)j * d.ldd;
         const double s = d.src[i + (size_t)j * d.lds];
         *p = (d.a == 0.0 ? 0.0 : d.a * (*p)) + d.b * s;

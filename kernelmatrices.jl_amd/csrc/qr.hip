@@ -71,12 +71,15 @@ __device__ __forceinline__ double qr_bcast(double v, int lane) {
 // column is always a[0] and register indices stay static.  Step j needs
 // S_k = sum_{rows > j} a(row, j) a(row, k); the four wave partials meet in
 // LDS (one barrier per column), lane k of every wave completes S_k and
-// v^T a_k, and readlane hands them to the other lanes.  The Householder vector of step j is parked
-// in column j of the Q output (each thread its own row) for the backward
-// accumulation of Q, which shifts right one column per step.
+// v^T a_k, and readlane hands them to the other lanes.  The Householder
+// vectors and R stay in LDS, so no global memory operation is outstanding
+// at a barrier; the backward accumulation of Q shifts right one column per
+// step, and Q and R are written once at the end.
 template <int NC>
 __global__ __launch_bounds__(256) void k_tile_qr(const SqrDesc* __restrict__ ds) {
     __shared__ double pw_all[4][64 * (NC + 1)];
+    __shared__ double Vs[NC][QT];
+    __shared__ double Rs[NC][NC + 1];
     __shared__ __align__(16) double part[2][NC * 4];
     __shared__ double rowj[2][NC];
     __shared__ double taus[NC];
@@ -84,14 +87,12 @@ __global__ __launch_bounds__(256) void k_tile_qr(const SqrDesc* __restrict__ ds)
     const int nc = d.nc, mt = d.nstack * d.nbs;
     const int r = threadIdx.x, lane = r & 63, wave = r >> 6;
     double* pw = pw_all[wave];
-    double* qcol = d.qout + (r < mt ? r : 0);   // this row of Q (stride ldq)
     double a[NC];
     {
         const double* src = (r < mt) ? d.src + (size_t)(r / d.nbs) * d.slot + (r % d.nbs) : d.src;
 #pragma unroll
         for (int k = 0; k < NC; ++k) a[k] = (r < mt && k < nc) ? src[(size_t)k * d.ld] : 0.0;
     }
-    __syncthreads();   // a tile read in place: every row is loaded before column 0 is overwritten
 #pragma unroll 1
     for (int j = 0; j < nc; ++j) {
         const int buf = j & 1;
@@ -121,29 +122,31 @@ __global__ __launch_bounds__(256) void k_tile_qr(const SqrDesc* __restrict__ ds)
 #pragma unroll
             for (int k = 1; k < NC; ++k) a[k] -= tv * qr_bcast(wk, k);
         }
-        if (r < mt) qcol[(size_t)j * d.ldq] = vr;
+        Vs[j][r] = vr;
         if (r == j) {
             // row j of R: zeros left of the diagonal, beta, then the updated row
-            for (int cc = 0; cc < j; ++cc) d.rout[j + (size_t)cc * d.ldr] = 0.0;
-            d.rout[j + (size_t)j * d.ldr] = beta;
+            for (int cc = 0; cc < j; ++cc) Rs[j][cc] = 0.0;
+            Rs[j][j] = beta;
 #pragma unroll
             for (int k = 1; k < NC; ++k)
-                if (j + k < nc) d.rout[j + (size_t)(j + k) * d.ldr] = a[k];
+                if (j + k < nc) Rs[j][j + k] = a[k];
+            taus[j] = tau;
         }
-        if (r == 0) taus[j] = tau;
 #pragma unroll
         for (int k = 0; k < NC - 1; ++k) a[k] = a[k + 1];
         a[NC - 1] = 0.0;
     }
-    __syncthreads();   // taus visible; the last step's LDS reads are done
+    __syncthreads();   // V, R, taus visible; the last step's LDS reads are done
+    for (int e = r; e < nc * nc; e += 256) {
+        const int i = e % nc, cc = e / nc;
+        d.rout[i + (size_t)cc * d.ldr] = Rs[i][cc];
+    }
 #pragma unroll
     for (int k = 0; k < NC; ++k) a[k] = 0.0;
-    double vnext = (r < mt && nc > 0) ? qcol[(size_t)(nc - 1) * d.ldq] : 0.0;
 #pragma unroll 1
     for (int j = nc - 1; j >= 0; --j) {
         const int buf = j & 1;
-        const double vr = vnext;
-        if (j > 0) vnext = (r < mt) ? qcol[(size_t)(j - 1) * d.ldq] : 0.0;
+        const double vr = Vs[j][r];
 #pragma unroll
         for (int k = NC - 1; k > 0; --k) a[k] = a[k - 1];
         a[0] = 0.0;
@@ -161,7 +164,7 @@ __global__ __launch_bounds__(256) void k_tile_qr(const SqrDesc* __restrict__ ds)
     if (r < mt) {
 #pragma unroll
         for (int k = 0; k < NC; ++k)
-            if (k < nc) qcol[(size_t)k * d.ldq] = a[k];
+            if (k < nc) d.qout[r + (size_t)k * d.ldq] = a[k];
     }
 }
 
