@@ -82,6 +82,32 @@ def cpu_baseline(n, lvl, params):
     return dt, nll
 
 
+GEMM_SHAPES = [  # (ta, tb, m, n, k): cascade update, projections, long-k split, TSQR tiles
+    (0, 0, 65536, 144, 72), (1, 0, 72, 144, 256), (1, 0, 72, 144, 32768), (0, 0, 256, 32, 32),
+    (1, 0, 32, 32, 256), (0, 0, 4096, 144, 72), (0, 0, 65536, 64, 72)]
+
+
+def gemm_shapes():
+    import numpy as np
+    import kmhodlr as km
+    lib = km.load()
+    ctx = km.Context(0)
+    rng = np.random.default_rng(0)
+    for ta, tb, m, n, k in GEMM_SHAPES:
+        A = np.asfortranarray(rng.random((k, m) if ta else (m, k)))
+        B = np.asfortranarray(rng.random((n, k) if tb else (k, n)))
+        Cm = np.asfortranarray(rng.random((m, n)))
+        ref = (A.T if ta else A) @ (B.T if tb else B)
+        C0 = Cm.copy(order="F")
+        for rep in range(3):
+            rc = lib.kmh_test_gemm(ctx.ptr, ta, tb, m, n, k, A.ctypes.data, A.shape[0], B.ctypes.data, B.shape[0],
+                                   Cm.ctypes.data, m, 1.0, 1.0, 0)
+            if rc != 0:
+                raise RuntimeError(lib.last_error())
+        err = np.abs(Cm - (C0 + 3 * ref)).max() / np.abs(C0 + 3 * ref).max()
+        print("gemm shape ta=%d tb=%d m=%d n=%d k=%d: 3 launches, rel err %.1e" % (ta, tb, m, n, k, err), flush=True)
+
+
 def pct(v, q):
     s = sorted(v)
     return s[min(len(s) - 1, max(0, int(math.ceil(q * len(s))) - 1))]
@@ -97,7 +123,13 @@ def main():
     ap.add_argument("--inflight", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--legs", default="plain,served")
+    ap.add_argument("--gemm-shapes", action="store_true",
+                    help="time the grouped GEMM alone on the skinny shapes of the factorization "
+                         "(one problem per launch, 3 launches each; read kernel times with rocprofv3)")
     args = ap.parse_args()
+    if args.gemm_shapes:
+        gemm_shapes()
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(independent GP requests shard across GPUs without collectives)", file=sys.stderr)

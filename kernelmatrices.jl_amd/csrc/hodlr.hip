@@ -450,6 +450,60 @@ static void woodbury_batch(Ctx& c, int mode, const std::vector<Rhs>& rs) {
 // symmetricfactorize!  (symfact.jl:6-50), across handles
 // ---------------------------------------------------------------------------
 
+// W_lv^{-1} applied to the shared Nystrom panels [A B] (n x 2r).  With
+// M = blkdiag(Q_V, Q_U), V = B rows of the first child and U = A rows of the
+// second child, M^T [A B] has the blocks Q_V^T B_c = R_V and Q_U^T A_a = R_U
+// already: only Q_V^T A_c and Q_U^T B_a need a product.  Rb holds R_V, R_U
+// (r x r each, then an r x r slot) per block, in block order.
+static void woodbury_shared_basis(Ctx& c, const std::vector<Hodlr*>& hs, const std::vector<int>& lvs,
+                                  const std::vector<double*>& rbs) {
+    size_t ssz = 0;
+    for (size_t i = 0; i < hs.size(); ++i)
+        for (int r : hs[i]->ranks[lvs[i]]) ssz += (size_t)4 * r * r;
+    if (ssz == 0) return;
+    double* S = static_cast<double*>(c.scratch(2 * ssz * sizeof(double)));
+    double* S2 = S + ssz;
+    std::vector<GemmProb> tn, op, up;
+    std::vector<Copy2D> rc;
+    size_t off = 0;
+    for (size_t i = 0; i < hs.size(); ++i) {
+        Hodlr& H = *hs[i];
+        const int lv = lvs[i];
+        const int64_t n = H.n;
+        const double* P = H.P.as<double>() + (size_t)H.coff[lv] * n;
+        const double* G = H.G.as<double>();
+        double* AB = H.AB;
+        const double* rb = rbs[i];
+        for (size_t k = 0; k < H.blocks[lv].size(); ++k) {
+            const auto& b = H.blocks[lv][k];
+            const int r = H.ranks[lv][k];
+            KMH_CHECK(r == H.nys_r, KMH_E_BAD_ARG, "shared-basis panels need the full Nystrom rank");
+            const int h1 = (int)(b.d1 - b.c0), h2 = (int)(b.b1 - b.a0);
+            const double* Q1 = P + b.c0;
+            const double* Q2 = P + b.a0;
+            const double* R1 = rb;
+            const double* R2 = rb + (size_t)r * r;
+            rb += (size_t)3 * r * r;
+            double* s = S + off;     // 2r x 2r, ld 2r: [Q_V^T A_c, R_V; R_U, Q_U^T B_a]
+            double* s2 = S2 + off;
+            off += (size_t)4 * r * r;
+            const int l2 = 2 * r;
+            tn.push_back({r, r, h1, Q1, (int)n, AB + b.c0, (int)n, s, l2, 1.0, 0.0, 0});
+            tn.push_back({r, r, h2, Q2, (int)n, AB + (size_t)n * r + b.a0, (int)n, s + r + (size_t)r * l2, l2, 1.0, 0.0, 0});
+            rc.push_back({R1, r, s + (size_t)r * l2, l2, r, r, 1.0});
+            rc.push_back({R2, r, s + r, l2, r, r, 1.0});
+            op.push_back({l2, l2, l2, G + H.coff_blk[lv][k], l2, s, l2, s2, l2, 1.0, 0.0, 0});
+            up.push_back({h1, l2, r, Q1, (int)n, s2, l2, AB + b.c0, (int)n, -1.0, 1.0, 0});
+            up.push_back({h2, l2, r, Q2, (int)n, s2 + r, l2, AB + b.a0, (int)n, -1.0, 1.0, 0});
+        }
+    }
+    gemm(c, true, false, tn);
+    copy2d(c, rc);
+    gemm(c, false, false, op);
+    gemm(c, false, false, up);
+    c.scratch_free(S);
+}
+
 // One level per handle (levels may differ across handles).
 void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vector<int>& lvs) {
     // Nystrom: this level's panels are row ranges of the cascade panels
@@ -524,18 +578,30 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
     axpby2d(c, neg);
     add_diag(c, plus1);                          // G = I - chol(C)^{-1}
     add_diag(c, minus1);                         // X = chol(C) - I
-    c.scratch_free(Rb);
     // Woodbury update of every higher level (symfact.jl:34-37)
     std::vector<Rhs> wr;
+    std::vector<Hodlr*> nys;
+    std::vector<int> nys_lv;
+    std::vector<double*> nys_r;
+    off = 0;
     for (size_t ai = 0; ai < act.size(); ++ai) {
         Hodlr* h = act[ai];
         Hodlr& H = *h;
         const int lv = lvs[ai];
+        double* r0 = Rb + off;
+        for (int r : H.ranks[lv]) off += (size_t)3 * r * r;
         if (lv + 1 >= H.L) continue;
-        if (H.AB) wr.push_back({h, H.AB, H.n, 2 * H.nys_r, lv});
-        else wr.push_back({h, H.P.as<double>() + (size_t)H.coff[lv + 1] * H.n, H.n, (int)(H.Rtot - H.coff[lv + 1]), lv});
+        if (H.AB) {
+            nys.push_back(h);
+            nys_lv.push_back(lv);
+            nys_r.push_back(r0);
+        } else {
+            wr.push_back({h, H.P.as<double>() + (size_t)H.coff[lv + 1] * H.n, H.n, (int)(H.Rtot - H.coff[lv + 1]), lv});
+        }
     }
     woodbury_batch(c, W_INV, wr);
+    woodbury_shared_basis(c, nys, nys_lv, nys_r);
+    c.scratch_free(Rb);
 }
 
 // Leaf stage: storage, leaf Cholesky, leaf solves on the panels.

@@ -85,7 +85,16 @@ struct SumJob {
     double* out;
 };
 void sum_vectors(Ctx& c, const std::vector<SumJob>& jobs);
-// out[j] = dot(X[:,j], Y[:,j]) for j < ncols (deterministic order).
+// *out = dot(x[0..m), y[0..m)) for each job, in a fixed order (slices of
+// 8192 elements, then the slices in sequence).
+struct DotJob {
+    const double* x;
+    const double* y;
+    int64_t m;
+    double* out;
+};
+void dots_batch(Ctx& c, const std::vector<DotJob>& jobs);
+// out[j] = dot(X[:,j], Y[:,j]) for j < ncols (as dots_batch).
 void column_dots(Ctx& c, const double* X, int ldx, const double* Y, int ldy,
                  int m, int ncols, double* d_out);
 

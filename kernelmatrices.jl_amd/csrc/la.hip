@@ -159,110 +159,6 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ ds, i
             }
 }
 
-// Wide tiles for skinny problems (64 < n <= 160): a workgroup computes
-// 64 rows x 160 columns, so A is read once per row block and C once; wave w
-// owns rows 16w..16w+15 across ten 16-column MFMA blocks.  Same
-// accumulation order as k_gemm, and the same transposed accumulators.
-constexpr int WN = 160;
-
-template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void k_gemm_w(const GemmDesc* __restrict__ ds, int nd) {
-    constexpr int KW = 16;   // k-chunk (the accumulation order does not depend on it)
-    constexpr int NQA = GT * KW / 256;
-    constexpr int NQB = WN * KW / 256;
-    __shared__ double As[KW][GT + 1];
-    __shared__ double Bs[KW][WN + 1];
-    const int t = blockIdx.x;
-    int lo = 0, hi = nd - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (ds[mid].tile0 <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    const GemmDesc d = ds[lo];
-    const int lt = t - d.tile0;
-    const int row0 = (lt / d.tn) * GT, col0 = (lt % d.tn) * WN;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    d4 acc[WN / 16];
-#pragma unroll
-    for (int b = 0; b < WN / 16; ++b) acc[b] = d4{0.0, 0.0, 0.0, 0.0};
-    double ra[NQA], rb[NQB];
-    // element e of a chunk: (i, kk) with the operand's contiguous index fastest
-    auto a_idx = [&](int e, int& i, int& kk) {
-        if (!TA) { i = e % GT; kk = e / GT; } else { kk = e % KW; i = e / KW; }
-    };
-    auto b_idx = [&](int e, int& j, int& kk) {
-        if (!TB) { kk = e % KW; j = e / KW; } else { j = e % WN; kk = e / WN; }
-    };
-    auto fetch = [&](int k0) {
-#pragma unroll
-        for (int q = 0; q < NQA; ++q) {
-            int i, kk;
-            a_idx(tid + 256 * q, i, kk);
-            const int gi = row0 + i, gk = k0 + kk;
-            double v = 0.0;
-            if (gi < d.m && gk < d.k) v = TA ? d.A[gk + (size_t)gi * d.lda] : d.A[gi + (size_t)gk * d.lda];
-            ra[q] = v;
-        }
-#pragma unroll
-        for (int q = 0; q < NQB; ++q) {
-            int j, kk;
-            b_idx(tid + 256 * q, j, kk);
-            const int gj = col0 + j, gk = k0 + kk;
-            double v = 0.0;
-            if (gj < d.n && gk < d.k) v = TB ? d.B[gj + (size_t)gk * d.ldb] : d.B[gk + (size_t)gj * d.ldb];
-            rb[q] = v;
-        }
-    };
-    fetch(0);
-    for (int k0 = 0; k0 < d.k; k0 += KW) {
-#pragma unroll
-        for (int q = 0; q < NQA; ++q) {
-            int i, kk;
-            a_idx(tid + 256 * q, i, kk);
-            As[kk][i] = ra[q];
-        }
-#pragma unroll
-        for (int q = 0; q < NQB; ++q) {
-            int j, kk;
-            b_idx(tid + 256 * q, j, kk);
-            Bs[kk][j] = rb[q];
-        }
-        __syncthreads();
-        if (k0 + KW < d.k) fetch(k0 + KW);
-#pragma unroll
-        for (int ks = 0; ks < KW / 4; ++ks) {
-            const int kk = ks * 4 + (lane >> 4);
-            const double a = As[kk][16 * wave + (lane & 15)];
-#pragma unroll
-            for (int b = 0; b < WN / 16; ++b)
-                acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(Bs[kk][16 * b + (lane & 15)], a, acc[b], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-    double cv[WN / 16][4];
-#pragma unroll
-    for (int b = 0; b < WN / 16; ++b)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int row = row0 + 16 * wave + (lane & 15);
-            const int col = col0 + 16 * b + mfma_d_row(lane >> 4, j);
-            cv[b][j] = (d.beta != 0.0 && row < d.m && col < d.n) ? d.C[row + (size_t)col * d.ldc] : 0.0;
-        }
-#pragma unroll
-    for (int b = 0; b < WN / 16; ++b)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int row = row0 + 16 * wave + (lane & 15);
-            const int col = col0 + 16 * b + mfma_d_row(lane >> 4, j);
-            if (row < d.m && col < d.n) {
-                double v = d.alpha * acc[b][j];
-                if (d.beta != 0.0) v += d.beta * cv[b][j];
-                d.C[row + (size_t)col * d.ldc] = v;
-            }
-        }
-}
-
 struct RedDesc {
     double* C;
     const double* W;
@@ -293,20 +189,8 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(const RedDesc* __restrict
     *cp = v;
 }
 
-// Problems with 64 < n <= 160 (and no triangle restriction) run on the wide
-// tiles, the others on 64 x 64 tiles; the choice and the split-k decision
-// depend on the problem's own shape only, so a problem gives bitwise the
-// same result whatever else shares the launch.
-// KMH_GEMM_WIDE=0 turns the wide tiles off (for timing comparisons).
-static bool gemm_wide_enabled() {
-    static const bool on = [] {
-        const char* e = std::getenv("KMH_GEMM_WIDE");
-        return !(e && e[0] == '0');
-    }();
-    return on;
-}
-static bool gemm_wide(const GemmProb& p) { return gemm_wide_enabled() && !p.lower && p.n > GT && p.n <= WN; }
-
+// The split-k decision depends on the problem's own shape only, so a
+// problem gives bitwise the same result whatever else shares the launch.
 void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
     bool any = false;
     for (const auto& p : probs) any = any || (p.m > 0 && p.n > 0);
@@ -316,7 +200,7 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
     for (size_t i = 0; i < probs.size(); ++i) {
         const auto& p = probs[i];
         if (p.m <= 0 || p.n <= 0 || p.lower) continue;
-        const long long own = ceil_div(p.m, GT) * ceil_div(p.n, gemm_wide(p) ? WN : GT);
+        const long long own = ceil_div(p.m, GT) * ceil_div(p.n, GT);
         if (p.k >= 1024 && own < 64) {
             long long s = std::min<long long>(ceil_div(p.k, 512), std::max<long long>(1, 256 / own));
             if (s > 1) {
@@ -329,8 +213,8 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
         }
     }
     double* W = wsz ? static_cast<double*>(c.scratch(wsz * sizeof(double))) : nullptr;
-    std::vector<GemmDesc> ds[2];
-    int tiles[2] = {0, 0};
+    std::vector<GemmDesc> ds;
+    int tiles = 0;
     std::vector<RedDesc> rs;
     size_t woff = 0;
     long long relem = 0;
@@ -338,14 +222,13 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
     for (size_t i = 0; i < probs.size(); ++i) {
         const auto& p = probs[i];
         if (p.m <= 0 || p.n <= 0) continue;
-        const int w = gemm_wide(p) ? 1 : 0;
-        const int tn = (int)ceil_div(p.n, w ? WN : GT);
+        const int tn = (int)ceil_div(p.n, GT);
         const int nt = (int)ceil_div(p.m, GT) * tn;
         fl += 2.0 * p.m * p.n * (double)p.k;
         if (split[i] == 1) {
-            ds[w].push_back({p.A, p.B, p.C, p.m, p.n, p.k, p.lda, p.ldb, p.ldc, p.alpha, p.beta,
-                             tiles[w], tn, p.lower, 0});
-            tiles[w] += nt;
+            ds.push_back({p.A, p.B, p.C, p.m, p.n, p.k, p.lda, p.ldb, p.ldc, p.alpha, p.beta,
+                          tiles, tn, p.lower, 0});
+            tiles += nt;
         } else {
             const int s = split[i];
             const int kc = kchunk[i];
@@ -354,9 +237,9 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
                 const int kk = std::min(kc, p.k - k0);
                 const double* A = ta ? p.A + k0 : p.A + (size_t)k0 * p.lda;
                 const double* B = tb ? p.B + (size_t)k0 * p.ldb : p.B + k0;
-                ds[w].push_back({A, B, W + woff + (size_t)q * p.m * p.n, p.m, p.n, kk, p.lda, p.ldb,
-                                 p.m, 1.0, 0.0, tiles[w], tn, 0, 0});
-                tiles[w] += nt;
+                ds.push_back({A, B, W + woff + (size_t)q * p.m * p.n, p.m, p.n, kk, p.lda, p.ldb,
+                              p.m, 1.0, 0.0, tiles, tn, 0, 0});
+                tiles += nt;
             }
             rs.push_back({p.C, W + woff, p.ldc, p.m, p.n, s, p.alpha, p.beta, relem});
             woff += (size_t)s * p.m * p.n;
@@ -368,23 +251,14 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
         hipEvent_t ev_ = c.prof_event();
         c.launch([ev_, s_ = c.stream]() { KMH_HIP(hipEventRecord(ev_, s_)); });
     }
-    if (!ds[0].empty()) {
-        GemmDesc* dd = c.upload_vec(ds[0]);
-        const int nd = (int)ds[0].size();
-        const dim3 g(tiles[0]);
+    if (!ds.empty()) {
+        GemmDesc* dd = c.upload_vec(ds);
+        const int nd = (int)ds.size();
+        const dim3 g(tiles);
         if (!ta && !tb) KMH_LAUNCH(c, (k_gemm<false, false>), g, dim3(256), dd, nd);
         else if (!ta && tb) KMH_LAUNCH(c, (k_gemm<false, true>), g, dim3(256), dd, nd);
         else if (ta && !tb) KMH_LAUNCH(c, (k_gemm<true, false>), g, dim3(256), dd, nd);
         else KMH_LAUNCH(c, (k_gemm<true, true>), g, dim3(256), dd, nd);
-    }
-    if (!ds[1].empty()) {
-        GemmDesc* dd = c.upload_vec(ds[1]);
-        const int nd = (int)ds[1].size();
-        const dim3 g(tiles[1]);
-        if (!ta && !tb) KMH_LAUNCH(c, (k_gemm_w<false, false>), g, dim3(256), dd, nd);
-        else if (!ta && tb) KMH_LAUNCH(c, (k_gemm_w<false, true>), g, dim3(256), dd, nd);
-        else if (ta && !tb) KMH_LAUNCH(c, (k_gemm_w<true, false>), g, dim3(256), dd, nd);
-        else KMH_LAUNCH(c, (k_gemm_w<true, true>), g, dim3(256), dd, nd);
     }
     if (c.prof) {
         hipEvent_t ev_ = c.prof_event();
@@ -795,23 +669,77 @@ void sum_vectors(Ctx& c, const std::vector<SumJob>& jobs) {
 
 void sum_vector(Ctx& c, const double* d_in, int count, double* d_out) { sum_vectors(c, {{d_in, count, d_out}}); }
 
-__global__ __launch_bounds__(256) void k_coldots(const double* X, int ldx, const double* Y, int ldy, int m,
-                                                 double* out) {
+// Dot products in two deterministic stages: fixed 8192-element slices, one
+// workgroup each, then one workgroup per job adds its slices in order.
+constexpr int DOT_SLICE = 8192;
+
+struct DotSlice {
+    const double* x;
+    const double* y;
+    int len;
+    int pad;
+    double* part;
+};
+
+__global__ __launch_bounds__(256) void k_dot_slices(const DotSlice* __restrict__ ds) {
     __shared__ double sh[4];
-    const int j = blockIdx.x;
+    const DotSlice d = ds[blockIdx.x];
+    double s0 = 0.0, s1 = 0.0;
+    int i = threadIdx.x;
+    for (; i + 256 < d.len; i += 512) {
+        s0 += d.x[i] * d.y[i];
+        s1 += d.x[i + 256] * d.y[i + 256];
+    }
+    if (i < d.len) s0 += d.x[i] * d.y[i];
+    const double t = block_sum(s0 + s1, sh);
+    if (threadIdx.x == 0) *d.part = t;
+}
+
+struct DotSum {
+    const double* part;
+    int count;
+    int pad;
+    double* out;
+};
+
+__global__ __launch_bounds__(64) void k_dot_sums(const DotSum* __restrict__ ds) {
+    const DotSum d = ds[blockIdx.x];
+    if (threadIdx.x != 0) return;
     double s = 0.0;
-    for (int i = threadIdx.x; i < m; i += 256) s += X[i + (size_t)j * ldx] * Y[i + (size_t)j * ldy];
-    const double t = block_sum(s, sh);
-    if (threadIdx.x == 0) out[j] = t;
+    for (int q = 0; q < d.count; ++q) s += d.part[q];
+    *d.out = s;
+}
+
+void dots_batch(Ctx& c, const std::vector<DotJob>& jobs) {
+    if (jobs.empty()) return;
+    std::vector<DotSlice> sl;
+    std::vector<DotSum> sm;
+    size_t np = 0;
+    for (const auto& j : jobs) np += std::max<int64_t>(1, ceil_div(j.m, DOT_SLICE));
+    double* part = static_cast<double*>(c.scratch(np * sizeof(double)));
+    size_t q = 0;
+    for (const auto& j : jobs) {
+        const int ns = (int)std::max<int64_t>(1, ceil_div(j.m, DOT_SLICE));
+        sm.push_back({part + q, ns, 0, j.out});
+        for (int s = 0; s < ns; ++s) {
+            const int64_t i0 = (int64_t)s * DOT_SLICE;
+            sl.push_back({j.x + i0, j.y + i0, (int)std::max<int64_t>(0, std::min<int64_t>(DOT_SLICE, j.m - i0)), 0,
+                          part + q + s});
+        }
+        q += ns;
+    }
+    DotSlice* dsl = c.upload_vec(sl);
+    KMH_LAUNCH(c, k_dot_slices, dim3((unsigned)sl.size()), dim3(256), dsl);
+    DotSum* dsm = c.upload_vec(sm);
+    KMH_LAUNCH(c, k_dot_sums, dim3((unsigned)sm.size()), dim3(64), dsm);
+    c.scratch_free(part);
 }
 
 void column_dots(Ctx& c, const double* X, int ldx, const double* Y, int ldy, int m, int ncols,
                  double* d_out) {
-    if (ncols <= 0) return;
-    KMH_LAUNCH(c, k_coldots, dim3(ncols), dim3(256), X, ldx, Y, ldy, m, d_out);
+    std::vector<DotJob> jobs;
+    for (int j = 0; j < ncols; ++j) jobs.push_back({X + (size_t)j * ldx, Y + (size_t)j * ldy, m, d_out + j});
+    dots_batch(c, jobs);
 }
 
-// ---------------------------------------------------------------------------
-// panel QR: block Gram-Schmidt (two passes) over 64-column blocks + TSQR
-// ---------------------------------------------------------------------------
 }  // namespace kmh

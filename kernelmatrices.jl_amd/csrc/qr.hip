@@ -1,7 +1,6 @@
 // Thin QR of batched tall panels on gfx950 (see la.h): Householder QR of
-// <=256 x 32 tiles held in registers (a row per thread, butterfly
-// reductions across lanes, one barrier per column), combined by a TSQR tree
-// of fan-in 8; columns beyond 32 by block Gram-Schmidt with one
+// <=128 x 32 tiles, one wave per tile with the rows in registers, combined by
+// a TSQR tree of fan-in 4; columns beyond 32 by block Gram-Schmidt with one
 // reorthogonalisation pass whose projections run on the MFMA GEMM.
 #include <algorithm>
 #include <cmath>
@@ -13,9 +12,9 @@ namespace kmh {
 // ---------------------------------------------------------------------------
 // Householder QR of small tiles (TSQR building block)
 // ---------------------------------------------------------------------------
-constexpr int QT = 256;   // max rows of a tile
+constexpr int QT = 128;   // max rows of a tile: one wave, two rows per lane
 constexpr int QC = 32;    // max columns of a tile (wider panels: block Gram-Schmidt)
-constexpr int QF = 8;     // TSQR fan-in: QF stacked QC x QC factors fill one tile
+constexpr int QF = 4;     // TSQR fan-in: QF stacked QC x QC factors fill one tile
 
 // A tile is nstack blocks of nbs rows stacked vertically; block i starts at
 // src + i * slot (column-major, leading dimension ld).  A leaf tile is one
@@ -27,84 +26,83 @@ struct SqrDesc {
     int nc, pad;
 };
 
-constexpr int ilog2(int p) { return p <= 1 ? 0 : 1 + ilog2(p / 2); }
-
-// Per-wave partial sums over the wave's 64 rows of aj * a[k], k < NC, into
-// part[k * 4 + wave].  The lanes write their products to the wave's own LDS
-// block pw (64 x (NC+1), padded against bank conflicts); lane l then sums
-// column l % NC over a segment of 64 / (64 / NC) rows, and the segments meet
-// by xor-shuffles.  LDS operations of one wave complete in issue order, so
-// the wave reads its own writes without a block barrier.  Fixed order of
-// summation (deterministic).
-template <int NC>
-__device__ __forceinline__ void tile_reduce(const double (&a)[NC], double aj, double* pw, double* part, int lane,
-                                            int wave) {
-    constexpr int SEGS = 64 / NC, RPS = 64 / SEGS;
-#pragma unroll
-    for (int k = 0; k < NC; ++k) pw[lane * (NC + 1) + k] = aj * a[k];
-    __builtin_amdgcn_wave_barrier();
-    const int k = lane % NC, seg = lane / NC;
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < RPS; ++i) s += pw[(seg * RPS + i) * (NC + 1) + k];
-#pragma unroll
-    for (int m = NC; m < 64; m <<= 1) s += __shfl_xor(s, m);
-    if (seg == 0) part[k * 4 + wave] = s;
-    __builtin_amdgcn_wave_barrier();
-}
-
-__device__ __forceinline__ double tile_colsum(const double* part, int k) {
-    const double* p = part + 4 * k;
-    return (p[0] + p[1]) + (p[2] + p[3]);
-}
-
-// value of v held by `lane` (a compile-time constant after unrolling)
+// value of v held by lane `lane` (uniform lane index)
 __device__ __forceinline__ double qr_bcast(double v, int lane) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
     return __hiloint2double(hi, lo);
 }
 
-// Register-resident Householder QR (LAPACK dgeqr2 + dorg2r arithmetic):
-// thread t holds row t of the tile.  The column loop runs at run time with a
-// fixed body: after step j the registers shift left by one, so the pivot
-// column is always a[0] and register indices stay static.  Step j needs
-// S_k = sum_{rows > j} a(row, j) a(row, k); the four wave partials meet in
-// LDS (one barrier per column), lane k of every wave completes S_k and
-// v^T a_k, and readlane hands them to the other lanes.  The Householder
-// vectors and R stay in LDS, so no global memory operation is outstanding
-// at a barrier; the backward accumulation of Q shifts right one column per
-// step, and Q and R are written once at the end.
+// Column sums over the wave's 128 rows: lane l ends with
+// S_{l % NC} = sum_rows p(row, l % NC).  The lanes write their per-lane
+// products (two rows already added) to LDS, lane l sums a segment of column
+// l % NC, and the segments meet by xor-shuffles.  LDS operations of one wave
+// complete in issue order, so no barrier is needed.  Fixed summation order.
 template <int NC>
-__global__ __launch_bounds__(256) void k_tile_qr(const SqrDesc* __restrict__ ds) {
-    __shared__ double pw_all[4][64 * (NC + 1)];
-    __shared__ double Vs[NC][QT];
-    __shared__ double Rs[NC][NC + 1];
-    __shared__ __align__(16) double part[2][NC * 4];
-    __shared__ double rowj[2][NC];
+__device__ __forceinline__ double wave_colsum(const double (&p)[NC], double* pw, int lane) {
+    constexpr int SEGS = 64 / NC, RPS = 64 / SEGS;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) pw[lane * (NC + 1) + k] = p[k];
+    __builtin_amdgcn_wave_barrier();
+    const int k = lane % NC, seg = lane / NC;
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int i = 0; i < RPS; i += 2) {
+        s0 += pw[(seg * RPS + i) * (NC + 1) + k];
+        s1 += pw[(seg * RPS + i + 1) * (NC + 1) + k];
+    }
+    double s = s0 + s1;
+#pragma unroll
+    for (int m = NC; m < 64; m <<= 1) s += __shfl_xor(s, m);
+    __builtin_amdgcn_wave_barrier();
+    return s;
+}
+
+// Householder QR (LAPACK dgeqr2 + dorg2r arithmetic) of a tile of <= 128
+// rows and nc <= NC columns by one wave: lane l holds rows l and l + 64 in
+// registers.  The column loop has a fixed body: after step j the registers
+// shift left by one, so the pivot column is always a[0] and all register
+// indices are static.  Pivot-row values come from lane j by readlane, column
+// sums from the wave-local reduction; the Householder vector of step j is
+// parked in column j of the Q output (each lane its own rows) for the
+// backward accumulation of Q, which shifts right one column per step.
+template <int NC>
+__global__ __launch_bounds__(64) void k_tile_qr(const SqrDesc* __restrict__ ds) {
+    __shared__ double pw[64 * (NC + 1)];
+    __shared__ double Rs[NC][2 * NC];   // row j of R at Rs[j][j..j+NC): the pivot lane writes it whole
+    __shared__ double rowj[NC];
     __shared__ double taus[NC];
     const SqrDesc d = ds[blockIdx.x];
     const int nc = d.nc, mt = d.nstack * d.nbs;
-    const int r = threadIdx.x, lane = r & 63, wave = r >> 6;
-    double* pw = pw_all[wave];
-    double a[NC];
+    const int lane = threadIdx.x;
+    const int r0 = lane, r1 = lane + 64;
+    const int kl = lane % NC;   // the column this lane completes
+    for (int e = lane; e < NC * 2 * NC; e += 64) (&Rs[0][0])[e] = 0.0;
+    double a0[NC], a1[NC];
     {
-        const double* src = (r < mt) ? d.src + (size_t)(r / d.nbs) * d.slot + (r % d.nbs) : d.src;
+        const double* s0 = (r0 < mt) ? d.src + (size_t)(r0 / d.nbs) * d.slot + (r0 % d.nbs) : d.src;
+        const double* s1 = (r1 < mt) ? d.src + (size_t)(r1 / d.nbs) * d.slot + (r1 % d.nbs) : d.src;
 #pragma unroll
-        for (int k = 0; k < NC; ++k) a[k] = (r < mt && k < nc) ? src[(size_t)k * d.ld] : 0.0;
+        for (int k = 0; k < NC; ++k) {
+            a0[k] = (r0 < mt && k < nc) ? s0[(size_t)k * d.ld] : 0.0;
+            a1[k] = (r1 < mt && k < nc) ? s1[(size_t)k * d.ld] : 0.0;
+        }
     }
+    double* q0 = d.qout + (r0 < mt ? r0 : 0);
+    double* q1 = d.qout + (r1 < mt ? r1 : 0);
 #pragma unroll 1
     for (int j = 0; j < nc; ++j) {
-        const int buf = j & 1;
-        if (r == j) {
+        // S_k = sum over rows > j of a(row, 0) a(row, k)   (rows of a1 are >= 64 > j)
+        double p[NC];
+        const double h0 = r0 > j ? a0[0] : 0.0;
 #pragma unroll
-            for (int k = 0; k < NC; ++k) rowj[buf][k] = a[k];
+        for (int k = 0; k < NC; ++k) p[k] = h0 * a0[k] + a1[0] * a1[k];
+        if (r0 == j) {
+#pragma unroll
+            for (int k = 0; k < NC; ++k) rowj[k] = a0[k];
         }
-        tile_reduce<NC>(a, r > j ? a[0] : 0.0, pw, part[buf], lane, wave);
-        __syncthreads();
-        // lane l finishes column l % NC; the other lanes read it by readlane
-        const double sk = tile_colsum(part[buf], lane % NC);
-        const double rk = rowj[buf][lane % NC];
+        const double sk = wave_colsum<NC>(p, pw, lane);   // its LDS traffic orders rowj too
+        const double rk = rowj[kl];
         const double alpha = qr_bcast(rk, 0);
         const double x2 = qr_bcast(sk, 0);
         double tau, beta, scal;
@@ -115,56 +113,83 @@ __global__ __launch_bounds__(256) void k_tile_qr(const SqrDesc* __restrict__ ds)
             tau = (beta - alpha) / beta;
             scal = 1.0 / (alpha - beta);
         }
-        const double vr = (r > j) ? scal * a[0] : (r == j ? 1.0 : 0.0);
+        const double v0 = (r0 > j) ? scal * a0[0] : (r0 == j ? 1.0 : 0.0);
+        const double v1 = scal * a1[0];
         if (tau != 0.0) {
-            const double wk = rk + scal * sk;   // v^T a_k for this lane's column
-            const double tv = tau * vr;
+            const double wl = rk + scal * sk;   // v^T a_k for this lane's column
+            const double t0 = tau * v0, t1 = tau * v1;
 #pragma unroll
-            for (int k = 1; k < NC; ++k) a[k] -= tv * qr_bcast(wk, k);
+            for (int k = 1; k < NC; ++k) {
+                const double wk = qr_bcast(wl, k);
+                a0[k] -= t0 * wk;
+                a1[k] -= t1 * wk;
+            }
         }
-        Vs[j][r] = vr;
-        if (r == j) {
-            // row j of R: zeros left of the diagonal, beta, then the updated row
-            for (int cc = 0; cc < j; ++cc) Rs[j][cc] = 0.0;
+        if (r0 < mt) q0[(size_t)j * d.ldq] = v0;
+        if (r1 < mt) q1[(size_t)j * d.ldq] = v1;
+        if (r0 == j) {
             Rs[j][j] = beta;
 #pragma unroll
-            for (int k = 1; k < NC; ++k)
-                if (j + k < nc) Rs[j][j + k] = a[k];
+            for (int k = 1; k < NC; ++k) Rs[j][j + k] = a0[k];
             taus[j] = tau;
         }
 #pragma unroll
-        for (int k = 0; k < NC - 1; ++k) a[k] = a[k + 1];
-        a[NC - 1] = 0.0;
+        for (int k = 0; k < NC - 1; ++k) {
+            a0[k] = a0[k + 1];
+            a1[k] = a1[k + 1];
+        }
+        a0[NC - 1] = 0.0;
+        a1[NC - 1] = 0.0;
     }
-    __syncthreads();   // V, R, taus visible; the last step's LDS reads are done
-    for (int e = r; e < nc * nc; e += 256) {
+    __builtin_amdgcn_wave_barrier();
+    for (int e = lane; e < nc * nc; e += 64) {
         const int i = e % nc, cc = e / nc;
         d.rout[i + (size_t)cc * d.ldr] = Rs[i][cc];
     }
 #pragma unroll
-    for (int k = 0; k < NC; ++k) a[k] = 0.0;
+    for (int k = 0; k < NC; ++k) {
+        a0[k] = 0.0;
+        a1[k] = 0.0;
+    }
+    double n0 = (nc > 0 && r0 < mt) ? q0[(size_t)(nc - 1) * d.ldq] : 0.0;
+    double n1 = (nc > 0 && r1 < mt) ? q1[(size_t)(nc - 1) * d.ldq] : 0.0;
 #pragma unroll 1
     for (int j = nc - 1; j >= 0; --j) {
-        const int buf = j & 1;
-        const double vr = Vs[j][r];
-#pragma unroll
-        for (int k = NC - 1; k > 0; --k) a[k] = a[k - 1];
-        a[0] = 0.0;
-        const double tau = taus[j];
-        tile_reduce<NC>(a, vr, pw, part[buf], lane, wave);
-        __syncthreads();
-        if (tau != 0.0) {
-            const double sk = tile_colsum(part[buf], lane % NC);
-            const double tv = tau * vr;
-#pragma unroll
-            for (int k = 1; k < NC; ++k) a[k] -= tv * qr_bcast(sk, k);
+        const double v0 = n0, v1 = n1;
+        if (j > 0) {
+            n0 = (r0 < mt) ? q0[(size_t)(j - 1) * d.ldq] : 0.0;
+            n1 = (r1 < mt) ? q1[(size_t)(j - 1) * d.ldq] : 0.0;
         }
-        a[0] = (r > j) ? -tau * vr : (r == j ? 1.0 - tau : 0.0);
-    }
-    if (r < mt) {
 #pragma unroll
-        for (int k = 0; k < NC; ++k)
-            if (k < nc) d.qout[r + (size_t)k * d.ldq] = a[k];
+        for (int k = NC - 1; k > 0; --k) {
+            a0[k] = a0[k - 1];
+            a1[k] = a1[k - 1];
+        }
+        a0[0] = 0.0;
+        a1[0] = 0.0;
+        const double tau = taus[j];
+        double p[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) p[k] = v0 * a0[k] + v1 * a1[k];
+        const double dk = wave_colsum<NC>(p, pw, lane);
+        if (tau != 0.0) {
+            const double t0 = tau * v0, t1 = tau * v1;
+#pragma unroll
+            for (int k = 1; k < NC; ++k) {
+                const double w = qr_bcast(dk, k);
+                a0[k] -= t0 * w;
+                a1[k] -= t1 * w;
+            }
+        }
+        a0[0] = (r0 > j) ? -tau * v0 : (r0 == j ? 1.0 - tau : 0.0);
+        a1[0] = -tau * v1;
+    }
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        if (k < nc) {
+            if (r0 < mt) q0[(size_t)k * d.ldq] = a0[k];
+            if (r1 < mt) q1[(size_t)k * d.ldq] = a1[k];
+        }
     }
 }
 
@@ -172,16 +197,17 @@ __global__ __launch_bounds__(256) void k_tile_qr(const SqrDesc* __restrict__ ds)
 static void launch_tile_qr(Ctx& c, const std::vector<SqrDesc>& ds) {
     std::vector<SqrDesc> cls[3];
     for (const auto& d : ds) {
-        KMH_CHECK(d.nc <= QC && d.nstack * d.nbs <= QT, KMH_E_BAD_ARG, "tile QR beyond its size limits");
+        KMH_CHECK(d.nc <= QC && d.nstack * d.nbs <= QT && d.nstack * d.nbs >= d.nc, KMH_E_BAD_ARG,
+                  "tile QR beyond its size limits");
         cls[d.nc <= 8 ? 0 : d.nc <= 16 ? 1 : 2].push_back(d);
     }
     for (int k = 0; k < 3; ++k) {
         if (cls[k].empty()) continue;
         SqrDesc* dd = c.upload_vec(cls[k]);
         const dim3 g((unsigned)cls[k].size());
-        if (k == 0) KMH_LAUNCH(c, k_tile_qr<8>, g, dim3(256), dd);
-        else if (k == 1) KMH_LAUNCH(c, k_tile_qr<16>, g, dim3(256), dd);
-        else KMH_LAUNCH(c, k_tile_qr<32>, g, dim3(256), dd);
+        if (k == 0) KMH_LAUNCH(c, k_tile_qr<8>, g, dim3(64), dd);
+        else if (k == 1) KMH_LAUNCH(c, k_tile_qr<16>, g, dim3(64), dd);
+        else KMH_LAUNCH(c, k_tile_qr<32>, g, dim3(64), dd);
     }
 }
 
@@ -190,7 +216,7 @@ struct TsqrPanel {
     double* R; int ldr;
 };
 
-// Thin QR of m x nb panels (nb <= QC, m >= nb) by TSQR: leaf tiles of <= 256
+// Thin QR of m x nb panels (nb <= QC, m >= nb) by TSQR: leaf tiles of <= 128
 // rows, then levels of inner nodes each stacking up to QF children's R
 // factors; the leaf Q factors are finally multiplied by the product of the
 // inner Q blocks on the path to the root.

@@ -99,26 +99,40 @@ static void finish_batch(Server& S, std::vector<Req*>& rs) {
         double* dots = buf + tot;
         std::vector<double*> T;
         std::vector<int> k;
+        std::vector<Copy2D> cp;
         size_t off = 0;
         for (size_t i = 0; i < ok.size(); ++i) {
             const int64_t n = ok[i]->n;
-            if (okr[i]->dy)
-                KMH_HIP(hipMemcpyAsync(buf + off, okr[i]->dy, n * sizeof(double), hipMemcpyDeviceToDevice, c.q()));
-            else
+            if (okr[i]->dy) {
+                cp.push_back({okr[i]->dy, (int)n, buf + off, (int)n, (int)n, 1, 1.0});
+            } else {
                 KMH_HIP(hipMemcpyAsync(buf + off, okr[i]->y.data(), n * sizeof(double), hipMemcpyHostToDevice,
                                        c.q()));
-            KMH_HIP(hipMemcpyAsync(buf + off + n, buf + off, n * sizeof(double), hipMemcpyDeviceToDevice, c.q()));
+                cp.push_back({buf + off, (int)n, buf + off + n, (int)n, (int)n, 1, 1.0});
+            }
             T.push_back(buf + off + n);
             k.push_back(1);
             off += 2 * n;
         }
-        hodlr_solve_batch(c, ok, T, k);
+        // y into the right-hand side (and, from device inputs, into buf too)
+        std::vector<Copy2D> cp2;
         off = 0;
         for (size_t i = 0; i < ok.size(); ++i) {
             const int64_t n = ok[i]->n;
-            column_dots(c, buf + off, (int)n, buf + off + n, (int)n, (int)n, 1, dots + i);
+            if (okr[i]->dy) cp2.push_back({okr[i]->dy, (int)n, buf + off + n, (int)n, (int)n, 1, 1.0});
             off += 2 * n;
         }
+        copy2d(c, cp);
+        copy2d(c, cp2);
+        hodlr_solve_batch(c, ok, T, k);
+        std::vector<DotJob> dj;
+        off = 0;
+        for (size_t i = 0; i < ok.size(); ++i) {
+            const int64_t n = ok[i]->n;
+            dj.push_back({buf + off, buf + off + n, n, dots + i});
+            off += 2 * n;
+        }
+        dots_batch(c, dj);
         std::vector<double> hd(ok.size());
         KMH_HIP(hipMemcpyAsync(hd.data(), dots, ok.size() * sizeof(double), hipMemcpyDeviceToHost, c.q()));
         c.sync();
