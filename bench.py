@@ -122,7 +122,8 @@ def main():
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--inflight", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--legs", default="plain,served")
+    ap.add_argument("--legs", default="plain,served,plain_grad,served_grad")
+    ap.add_argument("--grad-vecs", type=int, default=32, help="SAA vectors of the gradient legs")
     ap.add_argument("--gemm-shapes", action="store_true",
                     help="time the grouped GEMM alone on the skinny shapes of the factorization "
                          "(one problem per launch, 3 launches each; read kernel times with rocprofv3)")
@@ -158,13 +159,18 @@ def main():
         if rc != 0:
             raise RuntimeError(lib.last_error())
 
-    def kernel_of(p):
+    def kernel_of(p, deriv=0):
         k = L.kmh_kernel()
         k.family = L.KMH_MATERN32
-        k.deriv = 0
+        k.deriv = deriv
         k.params[0], k.params[1] = p
-        k.nugget = 1e-6
+        k.nugget = 1e-6 if deriv == 0 else 0.0
         return k
+
+    # gradient legs: nll_objective (maxlikfunctions.jl:29-50) = negloglik +
+    # stoch_gradient with Rademacher SAA vectors
+    saa = np.asfortranarray(np.random.default_rng(2).choice([-1.0, 1.0], size=(n, args.grad_vecs)))
+    y_host = np.ascontiguousarray(y)
 
     opts = L.kmh_build_opts()
     opts.method = L.KMH_NYSTROM
@@ -195,6 +201,22 @@ def main():
         check(lib.kmh_hodlr_destroy(h))
         return 0.5 * ld.value + 0.5 * quad, ld.value
 
+    def plain_grad_eval(p):
+        h = C.c_void_p()
+        kern = kernel_of(p)
+        check(lib.kmh_hodlr_build_dev(ctx.ptr, pts_d.data_ptr(), n, 2, C.byref(kern), C.byref(opts), C.byref(h)))
+        check(lib.kmh_hodlr_factorize(h))
+        ld = C.c_double()
+        check(lib.kmh_hodlr_logdet(h, C.byref(ld)))
+        check(lib.kmh_hodlr_solve_dev(h, y_d.data_ptr(), sol_d.data_ptr(), 1))
+        check(lib.kmh_ctx_synchronize(ctx.ptr))
+        quad = float(torch.dot(y_d, sol_d))
+        dks = (L.kmh_kernel * 2)(kernel_of(p, L.KMH_D1), kernel_of(p, L.KMH_D2))
+        g = np.zeros(2)
+        check(lib.kmh_stoch_gradient(h, dks, 2, y_host.ctypes.data, saa.ctypes.data, saa.shape[1], g.ctypes.data))
+        check(lib.kmh_hodlr_destroy(h))
+        return 0.5 * ld.value + 0.5 * quad, g
+
     legs = args.legs.split(",")
     plain_ld = {}
     if "plain" in legs:
@@ -218,11 +240,28 @@ def main():
                         "gemm_flops_per_eval": gfl.value / args.steps, "gemm_ms_per_eval": gms.value / args.steps,
                         "gemm_launches_per_eval": gl.value / args.steps}
 
+    plain_grad = {}
+    if "plain_grad" in legs:
+        for i in range(args.warmup):
+            plain_grad_eval(params[i])
+        torch.cuda.synchronize()
+        lat = []
+        t0 = time.perf_counter()
+        for i in range(args.warmup, total):
+            s_ = time.perf_counter()
+            plain_grad[i] = plain_grad_eval(params[i])
+            lat.append(1e3 * (time.perf_counter() - s_))
+        el = time.perf_counter() - t0
+        out["plain_grad"] = {"evals_per_s": args.steps / el, "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99),
+                             "saa_vectors": args.grad_vecs}
+
     # ---- served: continuous batching -----------------------------------
-    if "served" in legs:
+    if "served" in legs or "served_grad" in legs:
         srv = C.c_void_p()
         check(lib.kmh_server_create(ctx.ptr, args.inflight, 0, C.byref(srv)))
         reqs = []
+
+        grad_mode = [False]
 
         def submit(i):
             r = L.kmh_serve_req()
@@ -234,6 +273,12 @@ def main():
             r.kernel = kernel_of(params[i])
             r.opts = opts
             r.y = y_d.data_ptr()
+            if grad_mode[0]:
+                r.ndk = 2
+                r.saa = saa.ctypes.data
+                r.nv = saa.shape[1]
+                r.dks[0] = kernel_of(params[i], L.KMH_D1)
+                r.dks[1] = kernel_of(params[i], L.KMH_D2)
             reqs.append(r)
             rid = C.c_int64()
             check(lib.kmh_server_submit(srv, C.byref(r), C.byref(rid)))
@@ -262,7 +307,7 @@ def main():
                         check(rc)
                         if o.status != 0:
                             raise RuntimeError("request failed: %s" % lib.last_error())
-                        res[live.pop(rid)] = (o.nll, o.logdet, o.latency_us * 1e-3)
+                        res[live.pop(rid)] = (o.nll, o.logdet, o.latency_us * 1e-3, [o.grad[0], o.grad[1]])
             el = time.perf_counter() - t0
             prof = None
             if profile:
@@ -271,17 +316,30 @@ def main():
                 prof = (gfl.value, gms.value, gl.value)
             return res, el, prof
 
-        run(range(args.warmup), False)
-        torch.cuda.synchronize()
-        res, el, prof = run(range(args.warmup, total), True)
-        lat = [v[2] for v in res.values()]
-        out["served"] = {"evals_per_s": args.steps / el, "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99),
-                         "gemm_tflops": prof[0] / (prof[1] * 1e-3) / 1e12 if prof[1] else 0.0,
-                         "gemm_flops_per_eval": prof[0] / args.steps, "gemm_ms_per_eval": prof[1] / args.steps,
-                         "gemm_launches_total": prof[2], "inflight": args.inflight}
-        if plain_ld:
-            errs = [abs(res[i][1] - plain_ld[i]) / abs(plain_ld[i]) for i in plain_ld]
-            out["served"]["max_rel_logdet_err_vs_plain"] = max(errs)
+        if "served" in legs:
+            run(range(args.warmup), False)
+            torch.cuda.synchronize()
+            res, el, prof = run(range(args.warmup, total), True)
+            lat = [v[2] for v in res.values()]
+            out["served"] = {"evals_per_s": args.steps / el, "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99),
+                             "gemm_tflops": prof[0] / (prof[1] * 1e-3) / 1e12 if prof[1] else 0.0,
+                             "gemm_flops_per_eval": prof[0] / args.steps, "gemm_ms_per_eval": prof[1] / args.steps,
+                             "gemm_launches_total": prof[2], "inflight": args.inflight}
+            if plain_ld:
+                errs = [abs(res[i][1] - plain_ld[i]) / abs(plain_ld[i]) for i in plain_ld]
+                out["served"]["max_rel_logdet_err_vs_plain"] = max(errs)
+        if "served_grad" in legs:
+            grad_mode[0] = True
+            run(range(args.warmup), False)
+            torch.cuda.synchronize()
+            res, el, _ = run(range(args.warmup, total), False)
+            lat = [v[2] for v in res.values()]
+            out["served_grad"] = {"evals_per_s": args.steps / el, "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99),
+                                  "saa_vectors": args.grad_vecs, "inflight": args.inflight}
+            if plain_grad:
+                gerr = max(abs(res[i][3][j] - plain_grad[i][1][j]) / abs(plain_grad[i][1][j])
+                           for i in plain_grad for j in range(2))
+                out["served_grad"]["max_rel_grad_err_vs_plain"] = gerr
         lib.kmh_server_destroy(srv)
 
     head = out.get("served") or out.get("plain")
@@ -317,6 +375,9 @@ def main():
     }
     if "served" in out and "plain" in out:
         line["speedup_served_vs_plain"] = round(out["served"]["evals_per_s"] / out["plain"]["evals_per_s"], 3)
+    if "served_grad" in out and "plain_grad" in out:
+        line["speedup_served_vs_plain_with_gradient"] = round(
+            out["served_grad"]["evals_per_s"] / out["plain_grad"]["evals_per_s"], 3)
     if not args.no_cpu:
         # bounded CPU sample: one eval of the same workload shape at n=16384
         # (FixedLevel(6), same 256-point leaves), scaled to n=65536 by the

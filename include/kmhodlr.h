@@ -222,6 +222,13 @@ int kmh_server_result(kmh_server* srv, int64_t id, kmh_serve_result* out);
 int kmh_stoch_gradient(kmh_hodlr* h, const kmh_kernel* dks, int32_t p,
                        const double* y, const double* saa, int64_t nv, double* grad);
 
+/* ---- HODLR.stoch_profile_gradient(K, HK, y, dfuns, vecs)  stochasticgradient.jl:48-86
+ * gradient of the profile negative log-likelihood
+ * 0.5 (logdet K + n log(y' K^-1 y)):
+ * grad_j = 0.5 (trace_j - n y' K^-1 dK_j K^-1 y / y' K^-1 y)              */
+int kmh_stoch_profile_gradient(kmh_hodlr* h, const kmh_kernel* dks, int32_t p,
+                               const double* y, const double* saa, int64_t nv, double* grad);
+
 /* ---- HODLR.stoch_fisher(K, HK, DKs, vecs)  stochastichessian.jl:91-115
  * F: p x p column-major (symmetric)                                         */
 int kmh_stoch_fisher(kmh_hodlr* h, const kmh_kernel* dks, int32_t p, const double* saa, int64_t nv, double* F);

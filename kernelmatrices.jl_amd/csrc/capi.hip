@@ -366,6 +366,17 @@ int kmh_stoch_gradient(kmh_hodlr* h, const kmh_kernel* dks, int32_t p, const dou
     })
 }
 
+int kmh_stoch_profile_gradient(kmh_hodlr* h, const kmh_kernel* dks, int32_t p, const double* y, const double* saa,
+                               int64_t nv, double* grad) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        KMH_CHECK(H.factorized, KMH_E_NOT_FACTORIZED, "The matrix needs to be factorized for this to work.");
+        KMH_CHECK(dks && y && grad && p >= 1, KMH_E_BAD_ARG, "null argument");
+        KMH_CHECK(nv >= 1 && saa, KMH_E_BAD_ARG, "need at least one SAA vector");
+        kmh::hodlr_stoch_gradient(H, dks, p, y, saa, nv, grad, true);
+    })
+}
+
 int kmh_stoch_fisher(kmh_hodlr* h, const kmh_kernel* dks, int32_t p, const double* saa, int64_t nv, double* F) {
     KMH_TRY({
         kmh::Hodlr& H = H_of(h);

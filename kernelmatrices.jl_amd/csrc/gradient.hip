@@ -109,7 +109,7 @@ static void deriv_matvec(Hodlr& H, const double* dleaf, const double* dkxp, cons
 }
 
 void hodlr_stoch_gradient(Hodlr& H, const kmh_kernel* dks, int p, const double* y_host, const double* saa_host,
-                          int64_t nv, double* grad) {
+                          int64_t nv, double* grad, bool profile) {
     KMH_CHECK(H.opts.method == KMH_NYSTROM, KMH_E_BAD_ARG, "This is only valid for Nystrom-block matrices.");
     KMH_CHECK(H.factorized, KMH_E_NOT_FACTORIZED, "The matrix needs to be factorized for this to work.");
     KMH_CHECK(H.L == 0 || H.nys_r > 0, KMH_E_STATE, "missing Nystrom state");
@@ -128,6 +128,15 @@ void hodlr_stoch_gradient(Hodlr& H, const kmh_kernel* dks, int p, const double* 
     KMH_HIP(hipMemcpyAsync(Z + n, saa_host, (size_t)n * nv * sizeof(double), hipMemcpyHostToDevice, c.q()));
     hodlr_solve(H, Z, 1);                                  // K^-1 y
     hodlr_factor_apply(H, KMH_WTINV, Z + n, (int)nv);      // W^-T v for each SAA vector
+    double yky = 0.0;                                      // y' K^-1 y (profile form only)
+    if (profile) {
+        double* yd = static_cast<double*>(c.scratch((size_t)(n + 1) * sizeof(double)));
+        KMH_HIP(hipMemcpyAsync(yd, y_host, n * sizeof(double), hipMemcpyHostToDevice, c.q()));
+        column_dots(c, yd, (int)n, Z, (int)n, (int)n, 1, yd + n);
+        KMH_HIP(hipMemcpyAsync(&yky, yd + n, sizeof(double), hipMemcpyDeviceToHost, c.q()));
+        c.sync();
+        c.scratch_free(yd);
+    }
     const int r = H.nys_r;
     double* dleaf = static_cast<double*>(c.scratch(H.leaves.bytes));
     double* dkxp = r ? static_cast<double*>(c.scratch((size_t)n * r * sizeof(double))) : nullptr;
@@ -164,7 +173,8 @@ void hodlr_stoch_gradient(Hodlr& H, const kmh_kernel* dks, int p, const double* 
         double tr = 0.0;
         for (int v = 1; v < k; ++v) tr += hd[v];
         tr /= (double)nv;
-        grad[j] = 0.5 * (tr - hd[0]);
+        // stochasticgradient.jl:24 ; profile form :60-62
+        grad[j] = profile ? 0.5 * (tr - (double)n * hd[0] / yky) : 0.5 * (tr - hd[0]);
     }
     c.scratch_free(prv);
     if (Sj) c.scratch_free(Sj);

@@ -121,7 +121,7 @@ void hodlr_solve(Hodlr& H, double* T, int nrhs);
 void hodlr_full_host(Hodlr& H, double* out);
 void hodlr_block_uv_host(Hodlr& H, int level, int block, double* U, double* V);
 void hodlr_stoch_gradient(Hodlr& H, const kmh_kernel* dks, int p, const double* y_host, const double* saa_host,
-                          int64_t nv, double* grad);
+                          int64_t nv, double* grad, bool profile = false);
 void check_kernel(const kmh_kernel& k, bool allow_deriv);
 
 }  // namespace kmh

@@ -81,9 +81,12 @@ family(::CauchyKernel)   = Int32(2)
 deriv(k::MaternKernel) = k.deriv
 deriv(::GPUKernel) = Int32(0)
 
-# derivative kernels with respect to parms[1] (t0) and parms[2] (rho)
+# derivative kernels with respect to parms[1] (t0) and parms[2] (rho), and
+# the second derivatives (src/covariancefunctions.jl:268-330)
 d1(k::MaternKernel{N}) where {N} = MaternKernel{N}(0.0, 1)
 d2(k::MaternKernel{N}) where {N} = MaternKernel{N}(0.0, 2)
+d12(k::MaternKernel{N}) where {N} = MaternKernel{N}(0.0, 12)
+d22(k::MaternKernel{N}) where {N} = MaternKernel{N}(0.0, 22)
 
 _cor(::Val{1}, x) = x <= 1e-8 ? 1.0 : exp(-x)
 _cor(::Val{3}, x) = x <= 1e-8 ? 1.0 : (1 + sqrt(3)x) * exp(-sqrt(3)x)
@@ -101,6 +104,8 @@ descriptor(k::GPUKernel, parms) =
              k isa MaternKernel ? k.nugget : (k.deriv == 0 ? k.nugget : 0.0))
 descriptor(k::Union{RationalKernel,CauchyKernel}, parms) =
   kmh_kernel(family(k), 0, ntuple(i -> i <= length(parms) ? Float64(parms[i]) : 0.0, 4), k.nugget)
+# family 0: the reference's ZeroFunction in stoch_hessian's second-derivative list
+zero_descriptor() = kmh_kernel(Int32(0), Int32(0), (0.0, 0.0, 0.0, 0.0), 0.0)
 
 # ---- context ----
 mutable struct Context
@@ -128,7 +133,7 @@ Base.size(K::KernelMatrix) = (length(K.x1), length(K.x2))
 
 module HODLR
   using LinearAlgebra
-  import ..KernelMatricesHIP: check, libkmhodlr, kmh_build_opts, kmh_info, descriptor, default_context,
+  import ..KernelMatricesHIP: check, libkmhodlr, kmh_build_opts, kmh_info, descriptor, zero_descriptor, default_context,
                               KernelMatrix, KMH_W, KMH_WT, KMH_WINV, KMH_WTINV
   import LinearAlgebra: logdet, mul!, ldiv!
 
@@ -246,7 +251,66 @@ module HODLR
     return g
   end
 
+  # stochasticgradient.jl:48-86
+  function stoch_profile_gradient(K::KernelMatrix, HK::KernelHODLR, dat::Vector{Float64},
+                                  dfuns::Vector, vecs::Vector{Vector{Float64}};
+                                  plel::Bool=false, verbose::Bool=false, shuffle::Bool=false)
+    HK.U == nothing                  || error("The matrix needs to be factorized for this to work.")
+    length(dfuns) == length(K.parms) || error("You didn't supply the right number of gradient funs.")
+    dk = [descriptor(f, K.parms) for f in dfuns]
+    saa = reduce(hcat, vecs)
+    g = zeros(length(dfuns))
+    check(ccall((:kmh_stoch_profile_gradient, libkmhodlr), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Float64}),
+                HK.ptr, dk, length(dk), dat, saa, length(vecs), g))
+    return g
+  end
+
+  # stochastichessian.jl:91-115 (derivative HODLRs given by their kernels)
+  function stoch_fisher(K::KernelMatrix, HK::KernelHODLR, dfuns::Vector, vecs::Vector{Vector{Float64}};
+                        plel::Bool=false, shuffle::Bool=false)
+    HK.U == nothing || error("The matrix needs to be factorized.")
+    dk = [descriptor(f, K.parms) for f in dfuns]
+    saa = reduce(hcat, vecs)
+    F = zeros(length(dk), length(dk))
+    check(ccall((:kmh_stoch_fisher, libkmhodlr), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Float64}, Int64, Ptr{Float64}),
+                HK.ptr, dk, length(dk), saa, length(vecs), F))
+    return Symmetric(F)
+  end
+
+  # stochastichessian.jl:117-147; d2funs[j][k-j] is the (j,k) kernel or nothing (ZeroFunction)
+  function stoch_hessian(K::KernelMatrix, HK::KernelHODLR, dat::Vector{Float64}, d1funs::Vector,
+                         d2funs::Vector, vecs::Vector{Vector{Float64}}; plel::Bool=false,
+                         verbose::Bool=false, shuffle::Bool=false, profile::Bool=false,
+                         exact_second::Bool=false)
+    profile && error("the profile-likelihood Hessian is not provided")
+    HK.U == nothing || error("The matrix needs to be factorized.")
+    d1 = [descriptor(f, K.parms) for f in d1funs]
+    d2 = [f === nothing ? zero_descriptor() : descriptor(f, K.parms) for row in d2funs for f in row]
+    saa = reduce(hcat, vecs)
+    Hm = zeros(length(d1), length(d1))
+    check(ccall((:kmh_stoch_hessian, libkmhodlr), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Int32, Ptr{Float64}),
+                HK.ptr, d1, length(d1), d2, dat, saa, length(vecs), Int32(exact_second), Hm))
+    return Symmetric(Hm)
+  end
+
   negloglik(HK::KernelHODLR, dat::Vector{Float64}) = 0.5 * logdet(HK) + 0.5 * dot(dat, HK \ dat)
+  # maxlikfunctions.jl:11-19
+  profile_negloglik(HK::KernelHODLR, dat::Vector{Float64}) =
+    0.5 * (logdet(HK) + length(dat) * log(dot(dat, HK \ dat)))
+
+  # maxlikdata(; pts, data, sortmethod=:KDTree)  constructor.jl:171-181
+  function kd_order(pts::Vector; leafsize::Integer=8)
+    d = length(first(pts))
+    P = Matrix{Float64}(undef, d, length(pts))        # row-major n x d for the C side
+    for (i, x) in enumerate(pts); P[:, i] .= x; end
+    perm = Vector{Int64}(undef, length(pts))
+    check(ccall((:kmh_kd_order, libkmhodlr), Cint, (Ptr{Float64}, Int64, Int32, Int32, Ptr{Int64}),
+                P, length(pts), d, leafsize, perm))
+    return perm .+ 1
+  end
 end # module HODLR
 
 end # module KernelMatricesHIP

@@ -94,6 +94,7 @@ SIGNATURES = {
     "kmh_hodlr_build_batch": [_P, C.c_int32, C.POINTER(kmh_build_req), C.POINTER(_P), C.POINTER(C.c_int32)],
     "kmh_hodlr_factorize_batch": [_P, C.c_int32, C.POINTER(_P), C.POINTER(C.c_int32)],
     "kmh_hodlr_solve_batch_dev": [_P, C.c_int32, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int64)],
+    "kmh_stoch_profile_gradient": [_P, C.POINTER(kmh_kernel), C.c_int32, _P, _P, _I64, _P],
     "kmh_stoch_fisher": [_P, C.POINTER(kmh_kernel), C.c_int32, _P, _I64, _P],
     "kmh_stoch_hessian": [_P, C.POINTER(kmh_kernel), C.c_int32, C.POINTER(kmh_kernel), _P, _P, _I64, C.c_int32,
                           _P],

@@ -447,6 +447,32 @@ def stoch_gradient(K, HK, dat, dfuns, vecs, plel=False, verbose=False, shuffle=F
     return g
 
 
+def stoch_profile_gradient(K, HK, dat, dfuns, vecs, plel=False, verbose=False, shuffle=False):
+    """HODLR.stoch_profile_gradient (stochasticgradient.jl:48-86): gradient of
+    the profile negative log-likelihood 0.5 (logdet K + n log(y'K^-1 y))."""
+    if not HK.is_factorized:
+        raise ErrorException("The matrix needs to be factorized for this to work.")
+    if len(dfuns) != len(K.parms):
+        raise ErrorException("You didn't supply the right number of gradient funs.")
+    if shuffle:
+        raise ErrorException("shuffle=true is not supported (the reference's path references an undefined opts)")
+    saa = _saa_matrix(vecs, HK.n)
+    y = _f64(dat)
+    arr = (L.kmh_kernel * len(dfuns))(*[dk.desc(K.parms) for dk in dfuns])
+    g = np.zeros(len(dfuns))
+    _check(HK.lib, HK.lib.kmh_stoch_profile_gradient(HK.ptr, arr, len(dfuns), y.ctypes.data, saa.ctypes.data,
+                                                     saa.shape[1], g.ctypes.data))
+    return g
+
+
+def profile_negloglik(HK, y):
+    """maxlikfunctions.jl:11-19: 0.5 (logdet K + n log(y' K^-1 y))."""
+    if not HK.is_factorized:
+        raise ErrorException("The HODLR matrix needs to be factorized for this.")
+    y = np.asarray(y, dtype=np.float64)
+    return 0.5 * (HK.logdet() + len(y) * float(np.log(np.dot(y, HK.solve(y)))))
+
+
 def _saa_matrix(vecs, n):
     if isinstance(vecs, np.ndarray) and vecs.ndim == 2:
         saa = np.asfortranarray(vecs, dtype=np.float64)

@@ -632,6 +632,28 @@ def stoch_gradient(K, HK, y, dkernels, vecs, nparams=None):
     return np.array([hodlr_grad_term(K, HK, dk, y, vecs) for dk in dkernels])
 
 
+def profile_negloglik(HK, y):
+    """maxlikfunctions.jl:11-19: 0.5 (logdet K + n log(y' K^-1 y))."""
+    if not HK.is_factorized:
+        raise NotFactorizedError("The HODLR matrix needs to be factorized for this.")
+    return 0.5 * (HK.logdet() + len(y) * math.log(float(np.dot(y, HK.solve(y)))))
+
+
+def hodlr_p_grad_term(K, HK, dkernel, y, vecs):
+    """stochasticgradient.jl:48-64: 0.5 (trace - n y'K^-1 dK K^-1 y / y'K^-1 y)."""
+    HKj = DerivativeHODLR(K, dkernel, HK)
+    prvec = HK.solve(HKj.matvec(HK.solve(y)))
+    trtm = sum(hodlr_trace_apply(HK, HKj, v) for v in vecs) / len(vecs)
+    return 0.5 * (trtm - len(y) * float(np.dot(y, prvec)) / float(np.dot(y, HK.solve(y))))
+
+
+def stoch_profile_gradient(K, HK, y, dkernels, vecs):
+    """stochasticgradient.jl:68-86."""
+    if not HK.is_factorized:
+        raise NotFactorizedError("The matrix needs to be factorized for this to work.")
+    return np.array([hodlr_p_grad_term(K, HK, dk, y, vecs) for dk in dkernels])
+
+
 def givesaa(nvec, n, rng):
     """Rademacher SAA vectors (HODLR/src/utils.jl:15-27), from a numpy RNG."""
     return [rng.choice(np.array([-1.0, 1.0]), size=n) for _ in range(nvec)]

@@ -139,6 +139,39 @@ def test_stoch_gradient_exact_trace(fam):
     assert np.allclose(g, _dense_grad(Kf, dKs, y), rtol=1e-9)
 
 
+@pytest.mark.parametrize("profile", [False, True])
+def test_gradients_match_finite_differences_of_the_likelihood(profile):
+    """With exact trace vectors, stoch_gradient / stoch_profile_gradient
+    (stochasticgradient.jl:14-86) equal central differences of negloglik /
+    profile_negloglik (maxlikfunctions.jl:2-19) of the Nystrom HODLR
+    approximant rebuilt at perturbed parameters."""
+    n = 256
+    K, rng = make(n, params=(1.3, 0.15), seed=11)
+    y = rng.standard_normal(n)
+    x = K.x1
+
+    def nll(params):
+        Kq = H.KernelMatrix(x, x, kk.Kernel(K.kernel.family, params, nugget=K.kernel.nugget))
+        HKq = H.KernelHODLR(Kq, 0.0, 24, H.FixedLevel(2), nystrom=True)
+        H.symmetricfactorize(HKq)
+        return H.profile_negloglik(HKq, y) if profile else H.negloglik(HKq, y)
+
+    HK = H.KernelHODLR(K, 0.0, 24, H.FixedLevel(2), nystrom=True)
+    H.symmetricfactorize(HK)
+    dks = [K.kernel.derivative(kk.D1), K.kernel.derivative(kk.D2)]
+    vecs = [np.sqrt(n) * e for e in np.eye(n)]
+    g = (H.stoch_profile_gradient(K, HK, y, dks, vecs) if profile
+         else H.stoch_gradient(K, HK, y, dks, vecs))
+    p0 = list(K.kernel.params)
+    for i in range(2):
+        h = 1e-4 * abs(p0[i])   # truncation O(h^2) ~ 1e-8; rounding of the nll ~ 1e-11 / h
+        up, dn = list(p0), list(p0)
+        up[i] += h
+        dn[i] -= h
+        fd = (nll(up) - nll(dn)) / (2 * h)
+        assert abs(g[i] - fd) <= 1e-5 * max(1.0, abs(fd)), (i, g[i], fd)
+
+
 def test_derivative_hodlr_is_exact_derivative():
     """full(DerivativeHODLR) is the parameter derivative of full(HK) (finite
     differences of the Nystrom approximant itself)."""
