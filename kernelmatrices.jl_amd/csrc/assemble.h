@@ -3,6 +3,8 @@
 #include "common.h"
 #include "kernelfn.h"
 
+#include <memory>
+
 namespace kmh {
 
 // Dense block dst(i, j) = k(row point i, col point j); points are SoA.
@@ -17,10 +19,10 @@ struct EvalDesc {
 void eval_blocks(Ctx& c, const kmh_kernel& kern, std::vector<EvalDesc> ds);   // sets every k
 void eval_blocks(Ctx& c, std::vector<EvalDesc> ds);                           // per-descriptor k
 
-// One off-diagonal block for the ACA kernel: rows a..a+nro (second child),
-// cols c..c+ncl (first child).  U (nro x cap) and V (ncl x cap) share the
-// leading dimension ld.  Outputs: rank, status (0 ok, 1 start-row error,
-// 2 capacity reached before convergence).
+// One off-diagonal block for ACA: rows a..a+nro (second child), cols
+// c..c+ncl (first child).  U (nro x cap) and V (ncl x cap) share the leading
+// dimension ld.  Outputs: rank, status (0 ok, 1 start-row error, 2 capacity
+// reached before convergence).
 struct AcaDesc {
     const double* px; const double* py;   // the request's points (SoA)
     int a, nro, c, ncl;
@@ -32,7 +34,23 @@ struct AcaDesc {
     double rtol;
     KDev k;
 };
-size_t aca_shared_bytes(int nro, int ncl, int cap);
-void aca_launch(Ctx& c, const std::vector<AcaDesc>& ds);
+
+// ACA of a set of blocks.  The constructor queues the first cross of every
+// block; run() steps all blocks until each has converged or stopped (status
+// written to the descriptors' rank/status).  A block that stopped at its
+// capacity (status 2) can be given wider panels holding the columns found so
+// far with widen(), and the next run() continues it where it stopped.
+class AcaRun {
+public:
+    AcaRun(Ctx& c, const std::vector<AcaDesc>& ds);
+    ~AcaRun();
+    AcaRun(const AcaRun&) = delete;
+    AcaRun& operator=(const AcaRun&) = delete;
+    void run();
+    void widen(int block, double* U, double* V, int cap);
+private:
+    struct Impl;
+    std::unique_ptr<Impl> m_;
+};
 
 }  // namespace kmh

@@ -2,13 +2,12 @@
 //
 //  * k_eval: dense blocks K(rows, cols) of uneven sizes in one launch; the
 //    64 column points of a tile are staged in LDS, each thread owns rows.
-//  * k_aca:  one workgroup (1024 threads) runs the full ACA of one
-//    off-diagonal block, following src/factorizations.jl:9-84 step by step:
-//    pivot search with the first-index tie rule of restrictedmaxabs
-//    (src/utils.jl:2-15), sequential residual subtraction, and the norm
-//    update accumulated in the reference's order.  Rows and columns are
-//    written straight into the level's panel buffer.
+//  * AcaRun: partial-pivot ACA of every off-diagonal block of a build, run
+//    step by step with all blocks' steps grouped into shared launches
+//    (src/factorizations.jl:9-84).  Rows and columns are written straight
+//    into the level's panel buffer.
 #include <algorithm>
+#include <cstring>
 
 #include "assemble.h"
 
@@ -65,265 +64,517 @@ void eval_blocks(Ctx& c, std::vector<EvalDesc> ds) {
     KMH_LAUNCH(c, k_eval, dim3(tiles), dim3(256), dd, (int)act.size());
 }
 
-// ---------------------------------------------------------------------------
-// ACA
-// ---------------------------------------------------------------------------
-constexpr int AT = 1024;          // threads per ACA workgroup
-constexpr int AW = AT / 64;       // waves
+__device__ __forceinline__ bool bit_get(const unsigned* m, int i) { return (m[i >> 5] >> (i & 31)) & 1u; }
+__device__ __forceinline__ void bit_set(unsigned* m, int i) { m[i >> 5] |= 1u << (i & 31); }
 
-struct ArgMax {
-    double a;   // |value|
-    double v;   // signed value
-    int i;      // index
+// ---------------------------------------------------------------------------
+// ACA (src/factorizations.jl:9-84), step-parallel across workgroups.
+//
+// Every block of every level runs the reference's loop.  One loop iteration
+// of all running blocks is five grouped launches, so a large block's
+// residual rows and columns and its norm-update products spread over many
+// workgroups:
+//   k_aca_row   residual row at the row pivot (:48-50), per 1024-column
+//               chunk, with the chunk's argmax of |v| over unused columns
+//   k_aca_piv   column pivot (:52-55, restrictedmaxabs over the chunks)
+//   k_aca_col   v /= pivot (:56) and its squared norm per column chunk; the
+//               residual column at the column pivot (:58-61), its squared
+//               norm and the argmax of |u| over unused rows per row chunk
+//   k_aca_dots  u.U_t and v.V_t for the earlier terms t (:63-65), one wave
+//               per term
+//   k_aca_fin   ||Z||_F^2 update (:63-67), rank + 1, stop test, next row
+//               pivot (:43-46)
+// The element arithmetic, pivot tie rule and stop test are the reference's;
+// sums over chunks are added in chunk order, so results are deterministic.
+// ---------------------------------------------------------------------------
+constexpr int ACH = 1024;   // rows or columns per chunk (256 threads x 4)
+constexpr int ACA_COEF = 4096;   // LDS coefficients: the rank limit
+
+struct AcaArg {
+    double a, v;   // |value|, value
+    int i, ff;     // index of the maximum, first admissible index
 };
 
-__device__ __forceinline__ bool better(const ArgMax& x, const ArgMax& y) {
-    // larger |value| wins; ties go to the lower index
-    return (x.a > y.a) || (x.a == y.a && x.i < y.i);
+__device__ __forceinline__ bool arg_better(double a1, int i1, double a2, int i2) {
+    return (a1 > a2) || (a1 == a2 && i1 < i2);
 }
 
-__device__ __forceinline__ ArgMax wave_argmax(ArgMax m) {
+__device__ __forceinline__ AcaArg arg_none() { return AcaArg{-1.0, 0.0, 0x7fffffff, 0x7fffffff}; }
+
+struct AcaState {
+    const double *rx, *ry, *cx, *cy;
+    double *U, *V;
+    int nro, ncl, ld, maxsz, cap, rch, cch, pad0;
+    int* rank;
+    int* status;
+    double rtol;
+    KDev k;
+    unsigned *urow, *ucol;
+    AcaArg *rpart, *cpart;   // per row / column chunk argmax
+    double *rsum, *csum;     // per column / row chunk squared norms
+    double *du, *dv;         // norm-update products of the current step
+    // evolving state
+    int frnk, done, st, ri, ci, pad1;
+    double mv, rn2, cn2, znorm2;
+};
+
+// A running block and the offsets of its chunks in the grouped grids.
+struct AcaPlan {
+    int b, cch0, rch0, pad;
+};
+
+// workgroup-wide argmax over (a, i) and min over ff; every thread gets it
+__device__ AcaArg block_argmax(AcaArg m, AcaArg* sh) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-        ArgMax o;
+        AcaArg o;
         o.a = __shfl_xor(m.a, off);
         o.v = __shfl_xor(m.v, off);
         o.i = __shfl_xor(m.i, off);
-        if (better(o, m)) m = o;
+        o.ff = __shfl_xor(m.ff, off);
+        if (arg_better(o.a, o.i, m.a, m.i)) { m.a = o.a; m.v = o.v; m.i = o.i; }
+        m.ff = min(m.ff, o.ff);
     }
-    return m;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = m;
+    __syncthreads();
+    AcaArg b = sh[0];
+    for (int q = 1; q < nw; ++q) {
+        if (arg_better(sh[q].a, sh[q].i, b.a, b.i)) { b.a = sh[q].a; b.v = sh[q].v; b.i = sh[q].i; }
+        b.ff = min(b.ff, sh[q].ff);
+    }
+    return b;
 }
 
-__device__ __forceinline__ double wave_sum_d(double v) {
+// workgroup-wide sum in a fixed order; every thread gets it
+__device__ double block_sum_fixed(double v, double* sh) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
-}
-
-__device__ __forceinline__ int wave_min_i(int v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
-    return v;
-}
-
-struct AcaShared {
-    double sd[AW];
-    double sa[AW], sv[AW];
-    int si[AW], sm[AW];
-};
-
-__device__ double block_sum_aca(double v, AcaShared& sh) {
-    v = wave_sum_d(v);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     __syncthreads();
-    if (lane == 0) sh.sd[w] = v;
+    if (lane == 0) sh[w] = v;
     __syncthreads();
     double s = 0.0;
-#pragma unroll
-    for (int q = 0; q < AW; ++q) s += sh.sd[q];
+    for (int q = 0; q < nw; ++q) s += sh[q];
     return s;
 }
 
-__device__ __forceinline__ bool bit_get(const unsigned* m, int i) { return (m[i >> 5] >> (i & 31)) & 1u; }
-
-// restrictedmaxabs over x[0..len) (global, ld-contiguous) excluding mask.
-// Returns (signed value, index); (0, first free index) if all are zero.
-__device__ void restricted_maxabs(const double* x, int len, const unsigned* mask, AcaShared& sh,
-                                  double* val, int* idx) {
-    ArgMax m{-1.0, 0.0, 0x7fffffff};
-    int ffree = 0x7fffffff;
-    for (int j = threadIdx.x; j < len; j += AT) {
-        if (bit_get(mask, j)) continue;
-        if (j < ffree) ffree = j;
-        const double v = x[j];
-        const double a = fabs(v);
-        if (a > m.a) { m.a = a; m.v = v; m.i = j; }
+// restrictedmaxabs (src/utils.jl:2-15) from chunk partials: the signed value
+// and index of the largest |x| over admissible indices, ties to the lower
+// index; (0, first admissible index) if all are zero, where the first
+// admissible index is the last index when every index is excluded.
+__device__ void pick(const AcaArg* part, int n, int len, double* val, int* idx) {
+    double ba = -1.0, bv = 0.0;
+    int bi = 0x7fffffff, ff = 0x7fffffff;
+    for (int q = 0; q < n; ++q) {
+        if (arg_better(part[q].a, part[q].i, ba, bi)) { ba = part[q].a; bv = part[q].v; bi = part[q].i; }
+        ff = min(ff, part[q].ff);
     }
-    m = wave_argmax(m);
-    ffree = wave_min_i(ffree);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) { sh.sa[w] = m.a; sh.sv[w] = m.v; sh.si[w] = m.i; sh.sm[w] = ffree; }
-    __syncthreads();
-    ArgMax b{sh.sa[0], sh.sv[0], sh.si[0]};
-    int ff = sh.sm[0];
-    for (int q = 1; q < AW; ++q) {
-        ArgMax o{sh.sa[q], sh.sv[q], sh.si[q]};
-        if (better(o, b)) b = o;
-        ff = min(ff, sh.sm[q]);
-    }
-    if (ff == 0x7fffffff) ff = len - 1;   // every index excluded
-    if (b.a > 0.0) { *val = b.v; *idx = b.i; }
+    if (ff == 0x7fffffff) ff = len - 1;
+    if (ba > 0.0) { *val = bv; *idx = bi; }
     else { *val = 0.0; *idx = ff; }
 }
 
-__global__ __launch_bounds__(AT) void k_aca(const AcaDesc* __restrict__ ds) {
-    extern __shared__ char smem[];
-    __shared__ AcaShared sh;
-    __shared__ double s_z;
-    const AcaDesc d = ds[blockIdx.x];
-    const KDev k = d.k;
-    const double rtol = d.rtol;
-    const double* px = d.px;
-    const double* py = d.py;
-    const int tid = threadIdx.x;
-    const int wr = (d.nro + 31) / 32, wc = (d.ncl + 31) / 32;
-    unsigned* urow = reinterpret_cast<unsigned*>(smem);
-    unsigned* ucol = urow + wr;
-    double* coef = reinterpret_cast<double*>(smem + (((size_t)(wr + wc) * 4 + 15) & ~size_t(15)));
-    double* du = coef + d.cap;
-    double* dv = du + d.cap;
-    for (int i = tid; i < wr + wc; i += AT) urow[i] = 0u;
-    const double* rx = px + d.a;
-    const double* ry = py + d.a;
-    const double* cxp = px + d.c;
-    const double* cyp = py + d.c;
-    double* U = d.U;
-    double* V = d.V;
-    const size_t ld = (size_t)d.ld;
-    __syncthreads();
+__device__ void aca_stop(AcaState& s, int st) {
+    s.done = 1;
+    s.st = st;
+    *s.status = st;
+    *s.rank = s.frnk;
+}
 
-    // first row: skip all-zero rows (src/factorizations.jl:22-27)
+// loop head (:43-46): stop test, capacity, next row pivot
+__device__ void aca_loop_head(AcaState& s) {
+    if (!(sqrt(s.rn2) * sqrt(s.cn2) > s.rtol * sqrt(s.znorm2) && s.frnk < s.maxsz)) {
+        aca_stop(s, 0);
+        return;
+    }
+    if (s.frnk >= s.cap) {   // more columns than the panel holds: the host widens it
+        aca_stop(s, 2);
+        return;
+    }
+    double v;
+    int ri;
+    pick(s.rpart, s.rch, s.nro, &v, &ri);
+    s.ri = ri;
+    bit_set(s.urow, ri);
+}
+
+__device__ __forceinline__ int find_plan(const AcaPlan* pl, int np, int x, bool rows) {
+    int lo = 0, hi = np - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((rows ? pl[mid].rch0 : pl[mid].cch0) <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// First pair (:18-41) and the first loop head, one workgroup per block.
+__global__ __launch_bounds__(256) void k_aca_init(AcaState* __restrict__ sts) {
+    __shared__ AcaArg sha[4];
+    __shared__ double shd[4];
+    AcaState& s = sts[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int wr = (s.nro + 31) / 32, wc = (s.ncl + 31) / 32;
+    for (int i = tid; i < wr; i += 256) s.urow[i] = 0u;
+    for (int i = tid; i < wc; i += 256) s.ucol[i] = 0u;
     int strt = 0;
-    for (;;) {
-        const double xr = rx[strt], yr = ry[strt];
+    for (;;) {   // :22-25 first non-zero row
+        const double xr = s.rx[strt], yr = s.ry[strt];
         double mx = 0.0;
-        for (int j = tid; j < d.ncl; j += AT) {
-            const double v = kernel_eval(k, xr, yr, cxp[j], cyp[j]);
-            V[j] = v;
+        for (int j = tid; j < s.ncl; j += 256) {
+            const double v = kernel_eval(s.k, xr, yr, s.cx[j], s.cy[j]);
+            s.V[j] = v;
             mx = fmax(mx, fabs(v));
         }
-        // block max
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
         __syncthreads();
-        if ((tid & 63) == 0) sh.sd[tid >> 6] = mx;
+        if ((tid & 63) == 0) shd[tid >> 6] = mx;
         __syncthreads();
-        double m = 0.0;
-        for (int q = 0; q < AW; ++q) m = fmax(m, sh.sd[q]);
-        if (m == 0.0 && strt + 1 < d.maxsz) { ++strt; continue; }
+        const double m = fmax(fmax(shd[0], shd[1]), fmax(shd[2], shd[3]));
+        __syncthreads();
+        if (m == 0.0 && strt + 1 < s.maxsz) {
+            ++strt;
+            continue;
+        }
         break;
     }
-    if (strt + 1 == d.maxsz) {
-        if (tid == 0) { *d.status = 1; *d.rank = 0; }
+    if (strt + 1 == s.maxsz) {   // :26
+        if (tid == 0) {
+            s.frnk = 0;
+            aca_stop(s, 1);
+        }
         return;
     }
-    if (tid == 0) urow[strt >> 5] |= 1u << (strt & 31);
-    __syncthreads();
+    __syncthreads();   // the bit masks are cleared
+    if (tid == 0) bit_set(s.urow, strt);
+    AcaArg m = arg_none();
+    for (int j = tid; j < s.ncl; j += 256) {
+        if (j < m.ff) m.ff = j;
+        const double v = s.V[j], a = fabs(v);
+        if (a > m.a) { m.a = a; m.v = v; m.i = j; }
+    }
+    const AcaArg bc = block_argmax(m, sha);
     double mv;
     int mi;
-    restricted_maxabs(V, d.ncl, ucol, sh, &mv, &mi);
+    pick(&bc, 1, s.ncl, &mv, &mi);   // :33
     double rs = 0.0;
-    for (int j = tid; j < d.ncl; j += AT) {
-        const double v = V[j] / mv;
-        V[j] = v;
+    for (int j = tid; j < s.ncl; j += 256) {
+        const double v = s.V[j] / mv;
+        s.V[j] = v;
         rs += v * v;
     }
     double cs = 0.0;
-    {
-        const double xc = cxp[mi], yc = cyp[mi];
-        for (int i = tid; i < d.nro; i += AT) {
-            const double v = kernel_eval(k, rx[i], ry[i], xc, yc);
-            U[i] = v;
-            cs += v * v;
-        }
+    AcaArg mr = arg_none();
+    const double xc = s.cx[mi], yc = s.cy[mi];
+    for (int i = tid; i < s.nro; i += 256) {
+        const double v = kernel_eval(s.k, s.rx[i], s.ry[i], xc, yc);
+        s.U[i] = v;
+        cs += v * v;
+        if (i == strt) continue;
+        if (i < mr.ff) mr.ff = i;
+        const double a = fabs(v);
+        if (a > mr.a) { mr.a = a; mr.v = v; mr.i = i; }
     }
-    double rn2 = block_sum_aca(rs, sh);
-    double cn2 = block_sum_aca(cs, sh);
-    double znorm2 = rn2 * cn2;
-    if (tid == 0) ucol[mi >> 5] |= 1u << (mi & 31);
-    int frnk = 1;
-    int status = 0;
-    __syncthreads();
+    const double rn2 = block_sum_fixed(rs, shd);
+    const double cn2 = block_sum_fixed(cs, shd);
+    const AcaArg br = block_argmax(mr, sha);
+    if (tid == 0) {
+        bit_set(s.ucol, mi);
+        s.rn2 = rn2;
+        s.cn2 = cn2;
+        s.znorm2 = rn2 * cn2;
+        s.frnk = 1;
+        s.st = 0;
+        s.done = 0;
+        s.rpart[0] = br;   // the next row pivot comes from U[:, 0]
+        for (int q = 1; q < s.rch; ++q) s.rpart[q] = arg_none();
+        aca_loop_head(s);
+    }
+}
 
-    while (sqrt(rn2) * sqrt(cn2) > rtol * sqrt(znorm2) && frnk < d.maxsz) {
-        if (frnk >= d.cap) { status = 2; break; }
-        const double* ucur = U + (size_t)(frnk - 1) * ld;
-        double* vnew = V + (size_t)frnk * ld;
-        double* unew = U + (size_t)frnk * ld;
-        // next row index (src/factorizations.jl:45-46)
-        int ri;
-        restricted_maxabs(ucur, d.nro, urow, sh, &mv, &ri);
-        if (tid == 0) urow[ri >> 5] |= 1u << (ri & 31);
-        for (int t = tid; t < frnk; t += AT) coef[t] = U[ri + (size_t)t * ld];
-        __syncthreads();
-        // residual row (src/factorizations.jl:48-50)
-        {
-            const double xr = rx[ri], yr = ry[ri];
-            for (int j = tid; j < d.ncl; j += AT) {
-                double v = kernel_eval(k, xr, yr, cxp[j], cyp[j]);
-                for (int t = 0; t < frnk; ++t) v = __dsub_rn(v, __dmul_rn(coef[t], V[j + (size_t)t * ld]));
-                vnew[j] = v;
-            }
-        }
-        __syncthreads();
-        int ci;
-        restricted_maxabs(vnew, d.ncl, ucol, sh, &mv, &ci);
-        if (mv == 0.0) break;                         // :55
-        if (tid == 0) ucol[ci >> 5] |= 1u << (ci & 31);
-        rs = 0.0;
-        for (int j = tid; j < d.ncl; j += AT) {
+// A block stopped at capacity continues with wider panels.
+__global__ void k_aca_resume(AcaState* __restrict__ sts, const AcaPlan* __restrict__ pl) {
+    if (threadIdx.x != 0) return;
+    AcaState& s = sts[pl[blockIdx.x].b];
+    s.done = 0;
+    s.st = 0;
+    aca_loop_head(s);
+}
+
+__global__ __launch_bounds__(256) void k_aca_row(AcaState* __restrict__ sts, const AcaPlan* __restrict__ pl,
+                                                 int np) {
+    __shared__ double coef[ACA_COEF];
+    __shared__ AcaArg sha[4];
+    const int p = find_plan(pl, np, blockIdx.x, false);
+    AcaState& s = sts[pl[p].b];
+    if (s.done) return;
+    const int chunk = blockIdx.x - pl[p].cch0;
+    const int f = s.frnk, ri = s.ri;
+    const size_t ld = (size_t)s.ld;
+    for (int t = threadIdx.x; t < f; t += 256) coef[t] = s.U[ri + (size_t)t * ld];
+    __syncthreads();
+    const double xr = s.rx[ri], yr = s.ry[ri];
+    double* vnew = s.V + (size_t)f * ld;
+    AcaArg m = arg_none();
+    const int j0 = chunk * ACH, j1 = min(s.ncl, j0 + ACH);
+    for (int j = j0 + threadIdx.x; j < j1; j += 256) {
+        double v = kernel_eval(s.k, xr, yr, s.cx[j], s.cy[j]);
+        for (int t = 0; t < f; ++t) v = __dsub_rn(v, __dmul_rn(coef[t], s.V[j + (size_t)t * ld]));
+        vnew[j] = v;
+        if (bit_get(s.ucol, j)) continue;
+        if (j < m.ff) m.ff = j;
+        const double a = fabs(v);
+        if (a > m.a) { m.a = a; m.v = v; m.i = j; }
+    }
+    const AcaArg r = block_argmax(m, sha);
+    if (threadIdx.x == 0) s.cpart[chunk] = r;
+}
+
+__global__ void k_aca_piv(AcaState* __restrict__ sts, const AcaPlan* __restrict__ pl) {
+    if (threadIdx.x != 0) return;
+    AcaState& s = sts[pl[blockIdx.x].b];
+    if (s.done) return;
+    double mv;
+    int ci;
+    pick(s.cpart, s.cch, s.ncl, &mv, &ci);
+    if (mv == 0.0) {   // :55
+        aca_stop(s, 0);
+        return;
+    }
+    s.mv = mv;
+    s.ci = ci;
+    bit_set(s.ucol, ci);
+}
+
+__global__ __launch_bounds__(256) void k_aca_col(AcaState* __restrict__ sts, const AcaPlan* __restrict__ pl,
+                                                 int np, int ncch) {
+    __shared__ double coef[ACA_COEF];
+    __shared__ AcaArg sha[4];
+    __shared__ double shd[4];
+    const bool colpart = (int)blockIdx.x < ncch;
+    const int x = colpart ? (int)blockIdx.x : (int)blockIdx.x - ncch;
+    const int p = find_plan(pl, np, x, !colpart);
+    AcaState& s = sts[pl[p].b];
+    if (s.done) return;
+    const int f = s.frnk;
+    const size_t ld = (size_t)s.ld;
+    if (colpart) {
+        const int chunk = x - pl[p].cch0;
+        double* vnew = s.V + (size_t)f * ld;
+        const double mv = s.mv;
+        double rs = 0.0;
+        const int j0 = chunk * ACH, j1 = min(s.ncl, j0 + ACH);
+        for (int j = j0 + threadIdx.x; j < j1; j += 256) {
             const double v = vnew[j] / mv;
             vnew[j] = v;
             rs += v * v;
         }
-        for (int t = tid; t < frnk; t += AT) coef[t] = V[ci + (size_t)t * ld];
-        __syncthreads();
-        // residual column (src/factorizations.jl:58-61)
-        cs = 0.0;
-        {
-            const double xc = cxp[ci], yc = cyp[ci];
-            for (int i = tid; i < d.nro; i += AT) {
-                double v = kernel_eval(k, rx[i], ry[i], xc, yc);
-                for (int t = 0; t < frnk; ++t) v = __dsub_rn(v, __dmul_rn(coef[t], U[i + (size_t)t * ld]));
-                unew[i] = v;
-                cs += v * v;
-            }
-        }
-        rn2 = block_sum_aca(rs, sh);
-        cn2 = block_sum_aca(cs, sh);   // includes a barrier: unew/vnew complete
-        // norm update terms (src/factorizations.jl:63-67): one wave per term
-        {
-            const int lane = tid & 63, w = tid >> 6;
-            for (int t = w; t < frnk; t += AW) {
-                const double* ut = U + (size_t)t * ld;
-                const double* vt = V + (size_t)t * ld;
-                double a = 0.0, b = 0.0;
-                for (int i = lane; i < d.nro; i += 64) a += ut[i] * unew[i];
-                for (int j = lane; j < d.ncl; j += 64) b += vt[j] * vnew[j];
-                a = wave_sum_d(a);
-                b = wave_sum_d(b);
-                if (lane == 0) { du[t] = a; dv[t] = b; }
-            }
-        }
-        __syncthreads();
-        if (tid == 0) {
-            double z = znorm2;
-            for (int t = 0; t < frnk; ++t) z += 2.0 * fabs(du[t]) * fabs(dv[t]);
-            z += rn2 * cn2;
-            s_z = z;
-        }
-        __syncthreads();
-        znorm2 = s_z;
-        ++frnk;
+        const double t = block_sum_fixed(rs, shd);
+        if (threadIdx.x == 0) s.rsum[chunk] = t;
+        return;
     }
-    if (tid == 0) { *d.status = status; *d.rank = frnk; }
+    const int chunk = x - pl[p].rch0;
+    const int ci = s.ci;
+    for (int t = threadIdx.x; t < f; t += 256) coef[t] = s.V[ci + (size_t)t * ld];
+    __syncthreads();
+    const double xc = s.cx[ci], yc = s.cy[ci];
+    double* unew = s.U + (size_t)f * ld;
+    double cs = 0.0;
+    AcaArg m = arg_none();
+    const int i0 = chunk * ACH, i1 = min(s.nro, i0 + ACH);
+    for (int i = i0 + threadIdx.x; i < i1; i += 256) {
+        double v = kernel_eval(s.k, s.rx[i], s.ry[i], xc, yc);
+        for (int t = 0; t < f; ++t) v = __dsub_rn(v, __dmul_rn(coef[t], s.U[i + (size_t)t * ld]));
+        unew[i] = v;
+        cs += v * v;
+        if (bit_get(s.urow, i)) continue;
+        if (i < m.ff) m.ff = i;
+        const double a = fabs(v);
+        if (a > m.a) { m.a = a; m.v = v; m.i = i; }
+    }
+    const double t = block_sum_fixed(cs, shd);
+    const AcaArg r = block_argmax(m, sha);
+    if (threadIdx.x == 0) {
+        s.csum[chunk] = t;
+        s.rpart[chunk] = r;
+    }
 }
 
-size_t aca_shared_bytes(int nro, int ncl, int cap) {
-    const size_t words = (size_t)(nro + 31) / 32 + (size_t)(ncl + 31) / 32;
-    return ((words * 4 + 15) & ~size_t(15)) + (size_t)3 * cap * sizeof(double);
+// wave w of the grid: plan entry w / fb, earlier term t = w % fb
+__global__ __launch_bounds__(256) void k_aca_dots(AcaState* __restrict__ sts, const AcaPlan* __restrict__ pl,
+                                                  int np, int fb) {
+    const int wid = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int p = wid / fb, t = wid - p * fb;
+    if (p >= np) return;
+    AcaState& s = sts[pl[p].b];
+    if (s.done || t >= s.frnk) return;
+    const size_t ld = (size_t)s.ld;
+    const double* ut = s.U + (size_t)t * ld;
+    const double* vt = s.V + (size_t)t * ld;
+    const double* un = s.U + (size_t)s.frnk * ld;
+    const double* vn = s.V + (size_t)s.frnk * ld;
+    double a = 0.0, c = 0.0;
+    for (int i = lane; i < s.nro; i += 64) a += ut[i] * un[i];
+    for (int j = lane; j < s.ncl; j += 64) c += vt[j] * vn[j];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        c += __shfl_xor(c, off);
+    }
+    if (lane == 0) {
+        s.du[t] = a;
+        s.dv[t] = c;
+    }
 }
 
-void aca_launch(Ctx& c, const std::vector<AcaDesc>& ds) {
-    if (ds.empty()) return;
-    size_t shm = 0;
-    for (const auto& d : ds) shm = std::max(shm, aca_shared_bytes(d.nro, d.ncl, d.cap));
-    KMH_CHECK(shm <= 120 * 1024, KMH_E_BAD_ARG, "ACA block too large for one workgroup's LDS");
-    AcaDesc* dd = c.upload_vec(ds);
-    KMH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_aca),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    c.launch_kernel(k_aca, dim3((unsigned)ds.size()), dim3(AT), (unsigned)shm, dd);
+__global__ void k_aca_fin(AcaState* __restrict__ sts, const AcaPlan* __restrict__ pl) {
+    if (threadIdx.x != 0) return;
+    AcaState& s = sts[pl[blockIdx.x].b];
+    if (s.done) return;
+    double rn2 = 0.0, cn2 = 0.0;
+    for (int q = 0; q < s.cch; ++q) rn2 += s.rsum[q];
+    for (int q = 0; q < s.rch; ++q) cn2 += s.csum[q];
+    double z = s.znorm2;
+    for (int t = 0; t < s.frnk; ++t) z += 2.0 * fabs(s.du[t]) * fabs(s.dv[t]);
+    z += rn2 * cn2;
+    s.rn2 = rn2;
+    s.cn2 = cn2;
+    s.znorm2 = z;
+    s.frnk += 1;
+    aca_loop_head(s);
+}
+
+struct AcaRun::Impl {
+    Ctx& c;
+    int nb = 0;
+    char* mem = nullptr;
+    AcaState* d_st = nullptr;
+    std::vector<AcaState> h;
+    std::vector<int> widen;   // blocks to continue
+    explicit Impl(Ctx& cc) : c(cc) {}
+};
+
+AcaRun::AcaRun(Ctx& c, const std::vector<AcaDesc>& ds) : m_(new Impl(c)) {
+    Impl& m = *m_;
+    m.nb = (int)ds.size();
+    if (m.nb == 0) return;
+    size_t bits = 0, rparts = 0, cparts = 0, ts = 0;
+    m.h.resize(m.nb);
+    for (int b = 0; b < m.nb; ++b) {
+        const auto& d = ds[b];
+        KMH_CHECK(d.cap <= ACA_COEF, KMH_E_BAD_ARG,
+                  "ACA rank exceeds the supported maximum of 4096");
+        AcaState& s = m.h[b];
+        std::memset(&s, 0, sizeof(s));
+        s.rx = d.px + d.a; s.ry = d.py + d.a;
+        s.cx = d.px + d.c; s.cy = d.py + d.c;
+        s.U = d.U; s.V = d.V;
+        s.nro = d.nro; s.ncl = d.ncl; s.ld = d.ld; s.maxsz = d.maxsz; s.cap = d.cap;
+        s.rch = (int)ceil_div(d.nro, ACH);
+        s.cch = (int)ceil_div(d.ncl, ACH);
+        s.rank = d.rank; s.status = d.status;
+        s.rtol = d.rtol; s.k = d.k;
+        bits += (size_t)(d.nro + 31) / 32 + (size_t)(d.ncl + 31) / 32;
+        rparts += s.rch;
+        cparts += s.cch;
+        ts += std::min(d.maxsz, ACA_COEF);
+    }
+    auto al = [](size_t n) { return (n + 255) & ~size_t(255); };
+    const size_t bytes = al(sizeof(AcaState) * m.nb) + al(bits * 4) + al((rparts + cparts) * sizeof(AcaArg)) +
+                         al((rparts + cparts) * sizeof(double)) + al(2 * ts * sizeof(double));
+    m.mem = static_cast<char*>(c.scratch(bytes));
+    char* w = m.mem;
+    auto take = [&](size_t n) { char* p = w; w += al(n); return p; };
+    m.d_st = reinterpret_cast<AcaState*>(take(sizeof(AcaState) * m.nb));
+    unsigned* bitp = reinterpret_cast<unsigned*>(take(bits * 4));
+    AcaArg* rp = reinterpret_cast<AcaArg*>(take((rparts + cparts) * sizeof(AcaArg)));
+    AcaArg* cp = rp + rparts;
+    double* rsum = reinterpret_cast<double*>(take((rparts + cparts) * sizeof(double)));
+    double* csum = rsum + cparts;
+    double* du = reinterpret_cast<double*>(take(2 * ts * sizeof(double)));
+    double* dv = du + ts;
+    for (auto& s : m.h) {
+        s.urow = bitp; bitp += (s.nro + 31) / 32;
+        s.ucol = bitp; bitp += (s.ncl + 31) / 32;
+        s.rpart = rp; rp += s.rch;
+        s.cpart = cp; cp += s.cch;
+        s.rsum = rsum; rsum += s.cch;
+        s.csum = csum; csum += s.rch;
+        s.du = du; du += std::min(s.maxsz, ACA_COEF);
+        s.dv = dv; dv += std::min(s.maxsz, ACA_COEF);
+    }
+    KMH_HIP(hipMemcpyAsync(m.d_st, m.h.data(), sizeof(AcaState) * m.nb, hipMemcpyHostToDevice, c.q()));
+    KMH_LAUNCH(c, k_aca_init, dim3(m.nb), dim3(256), m.d_st);
+}
+
+AcaRun::~AcaRun() {
+    if (m_->mem) m_->c.scratch_free(m_->mem);
+}
+
+void AcaRun::widen(int b, double* U, double* V, int cap) {
+    Impl& m = *m_;
+    KMH_CHECK(b >= 0 && b < m.nb && cap <= ACA_COEF, KMH_E_BAD_ARG,
+              "ACA rank exceeds the supported maximum of 4096");
+    AcaState& s = m.h[b];
+    s.U = U;
+    s.V = V;
+    s.cap = cap;
+    m.widen.push_back(b);
+}
+
+void AcaRun::run() {
+    Impl& m = *m_;
+    if (m.nb == 0) return;
+    Ctx& c = m.c;
+    constexpr int SYNC_STEPS = 16;
+    const size_t st_bytes = sizeof(AcaState) * m.nb;
+    if (!m.widen.empty()) {
+        // the device holds the evolving state; patch the widened panels in
+        std::vector<AcaState> cur(m.nb);
+        KMH_HIP(hipMemcpyAsync(cur.data(), m.d_st, st_bytes, hipMemcpyDeviceToHost, c.q()));
+        c.sync();
+        std::vector<AcaPlan> pl;
+        for (int b : m.widen) {
+            KMH_CHECK(cur[b].st == 2 && cur[b].done, KMH_E_BAD_ARG, "ACA widen of a block that did not stop at capacity");
+            cur[b].U = m.h[b].U;
+            cur[b].V = m.h[b].V;
+            cur[b].cap = m.h[b].cap;
+            pl.push_back({b, 0, 0, 0});
+        }
+        m.widen.clear();
+        KMH_HIP(hipMemcpyAsync(m.d_st, cur.data(), st_bytes, hipMemcpyHostToDevice, c.q()));
+        c.sync();   // cur is a host temporary
+        AcaPlan* d_pl = c.upload_vec(pl);
+        KMH_LAUNCH(c, k_aca_resume, dim3((unsigned)pl.size()), dim3(64), m.d_st, d_pl);
+    }
+    std::vector<AcaState> cur(m.nb);
+    for (;;) {
+        KMH_HIP(hipMemcpyAsync(cur.data(), m.d_st, st_bytes, hipMemcpyDeviceToHost, c.q()));
+        c.sync();
+        std::vector<AcaPlan> pl;
+        int cch = 0, rch = 0, fb = 1;
+        for (int b = 0; b < m.nb; ++b) {
+            if (cur[b].done) continue;
+            pl.push_back({b, cch, rch, 0});
+            cch += cur[b].cch;
+            rch += cur[b].rch;
+            fb = std::max(fb, cur[b].frnk);
+        }
+        if (pl.empty()) break;
+        const int np = (int)pl.size();
+        AcaPlan* d_pl = c.upload_vec(pl);
+        for (int q = 0; q < SYNC_STEPS; ++q, ++fb) {
+            KMH_LAUNCH(c, k_aca_row, dim3(cch), dim3(256), m.d_st, d_pl, np);
+            KMH_LAUNCH(c, k_aca_piv, dim3(np), dim3(64), m.d_st, d_pl);
+            KMH_LAUNCH(c, k_aca_col, dim3(cch + rch), dim3(256), m.d_st, d_pl, np, cch);
+            KMH_LAUNCH(c, k_aca_dots, dim3((unsigned)ceil_div((int64_t)np * fb, 4)), dim3(256), m.d_st, d_pl, np,
+                       fb);
+            KMH_LAUNCH(c, k_aca_fin, dim3(np), dim3(64), m.d_st, d_pl);
+        }
+    }
 }
 
 }  // namespace kmh

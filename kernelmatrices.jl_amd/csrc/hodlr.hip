@@ -128,11 +128,13 @@ static void nystrom_setup(Hodlr& H) {
     for (int l = 0; l < H.L; ++l) H.ranks[l].assign(H.blocks[l].size(), r);
 }
 
-// ACA of every block of every level of every ACA handle, one launch per
-// attempt; a level whose rank reached its column capacity reruns wider.
+// ACA of every block of every level of every ACA handle, all blocks
+// stepping together.  A level whose block reached the panel's column
+// capacity gets a panel twice as wide holding the columns found so far, and
+// its stopped blocks continue where they stopped.
 static void build_aca_batch(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<Status>& st,
                             const std::vector<int>& idx) {
-    struct Lv { int h; int l; int cap; int maxsz; DevBuf tmp; bool redo; };
+    struct Lv { int h; int l; int cap; int maxsz; int b0; DevBuf tmp; };
     std::vector<Lv> lvs;
     int nblk = 0;
     for (int i : idx) {
@@ -143,7 +145,7 @@ static void build_aca_batch(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<S
                 const int h = (int)std::min(b.b1 - b.a0, b.d1 - b.c0);
                 ms_max = std::max(ms_max, H.opts.maxrank == 0 ? h : std::min(H.opts.maxrank, h));
             }
-            lvs.push_back({i, l, std::min(ms_max, 256), ms_max, DevBuf(), true});
+            lvs.push_back({i, l, std::min(ms_max, 256), ms_max, nblk, DevBuf()});
             nblk += (int)H.blocks[l].size();
         }
     }
@@ -153,52 +155,61 @@ static void build_aca_batch(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<S
     int* d_rank = out.as<int>();
     int* d_stat = d_rank + nblk;
     std::vector<int> h_out(2 * nblk);
-    for (;;) {
-        std::vector<AcaDesc> ds;
-        int bi = 0;
-        for (auto& v : lvs) {
-            Hodlr& H = *hs[v.h];
-            if (!v.redo) { bi += (int)H.blocks[v.l].size(); continue; }
-            v.tmp.alloc(&c, (size_t)H.n * v.cap * sizeof(double));
-            double* T = v.tmp.as<double>();
-            const double* px = H.pts.as<double>();
-            for (const auto& b : H.blocks[v.l]) {
-                const int nro = (int)(b.b1 - b.a0), ncl = (int)(b.d1 - b.c0);
-                const int h = std::min(nro, ncl);
-                const int ms = H.opts.maxrank == 0 ? h : std::min(H.opts.maxrank, h);
-                ds.push_back({px, px + H.n, (int)b.a0, nro, (int)b.c0, ncl, T + b.a0, T + b.c0, (int)H.n, ms,
-                              v.cap, 0, d_rank + bi, d_stat + bi, H.opts.tol, to_kdev(H.kern)});
-                ++bi;
-            }
+    std::vector<AcaDesc> ds;
+    for (auto& v : lvs) {
+        Hodlr& H = *hs[v.h];
+        v.tmp.alloc(&c, (size_t)H.n * v.cap * sizeof(double));
+        double* T = v.tmp.as<double>();
+        const double* px = H.pts.as<double>();
+        int bi = v.b0;
+        for (const auto& b : H.blocks[v.l]) {
+            const int nro = (int)(b.b1 - b.a0), ncl = (int)(b.d1 - b.c0);
+            const int h = std::min(nro, ncl);
+            const int ms = H.opts.maxrank == 0 ? h : std::min(H.opts.maxrank, h);
+            ds.push_back({px, px + H.n, (int)b.a0, nro, (int)b.c0, ncl, T + b.a0, T + b.c0, (int)H.n, ms,
+                          v.cap, 0, d_rank + bi, d_stat + bi, H.opts.tol, to_kdev(H.kern)});
+            ++bi;
         }
-        aca_launch(c, ds);
+    }
+    AcaRun run(c, ds);
+    for (;;) {
+        run.run();
         KMH_HIP(hipMemcpyAsync(h_out.data(), d_rank, 2 * sizeof(int) * nblk, hipMemcpyDeviceToHost, c.q()));
         c.sync();
         bool again = false;
-        bi = 0;
         for (auto& v : lvs) {
             Hodlr& H = *hs[v.h];
+            const int nbl = (int)H.blocks[v.l].size();
             bool wider = false;
-            for (size_t k = 0; k < H.blocks[v.l].size(); ++k, ++bi) {
-                if (!v.redo) continue;
-                const int s = h_out[nblk + bi];
+            for (int k = 0; k < nbl; ++k) {
+                const int s = h_out[nblk + v.b0 + k];
                 if (s == 1 && st[v.h].code == KMH_OK) {
                     st[v.h].code = KMH_E_ACA;
                     st[v.h].msg = "Incompatible maxrank or matrix dimensions.";
                 }
                 if (s == 2) wider = true;
             }
-            if (v.redo && wider && st[v.h].code == KMH_OK) {
-                KMH_CHECK(v.cap < v.maxsz, KMH_E_DEVICE, "ACA capacity logic error");
-                v.cap = std::min(v.maxsz, v.cap * 2);
-                if ((size_t)3 * v.cap * 8 >= 96 * 1024) {
-                    st[v.h].code = KMH_E_BAD_ARG;
-                    st[v.h].msg = "ACA rank exceeds the supported maximum of 4096";
-                    wider = false;
-                }
+            if (!wider || st[v.h].code != KMH_OK) continue;
+            KMH_CHECK(v.cap < v.maxsz, KMH_E_DEVICE, "ACA capacity logic error");
+            const int cap = std::min(v.maxsz, v.cap * 2);
+            if (cap > 4096) {
+                st[v.h].code = KMH_E_BAD_ARG;
+                st[v.h].msg = "ACA rank exceeds the supported maximum of 4096";
+                continue;
             }
-            v.redo = wider && st[v.h].code == KMH_OK;
-            again = again || v.redo;
+            DevBuf wide;
+            wide.alloc(&c, (size_t)H.n * cap * sizeof(double));
+            KMH_HIP(hipMemcpyAsync(wide.p, v.tmp.p, (size_t)H.n * v.cap * sizeof(double), hipMemcpyDeviceToDevice,
+                                   c.q()));
+            v.tmp = std::move(wide);
+            v.cap = cap;
+            double* T = v.tmp.as<double>();
+            for (int k = 0; k < nbl; ++k) {
+                if (h_out[nblk + v.b0 + k] != 2) continue;
+                const auto& b = H.blocks[v.l][k];
+                run.widen(v.b0 + k, T + b.a0, T + b.c0, cap);
+            }
+            again = true;
         }
         if (!again) break;
     }
