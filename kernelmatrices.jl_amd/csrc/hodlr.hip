@@ -338,24 +338,27 @@ std::vector<Hodlr*> hodlr_build_batch(Ctx& c, const std::vector<BuildReq>& reqs,
     if (!nys.empty()) {
         potrf(c, kpp);
         collect_info(c, hs, st, "Cholesky of the Nystrom landmark matrix K(P,P) failed.");
-        // Y = K(x,P) K(P,P)^{-1}: transpose, two triangular solves, transpose back
-        std::vector<Copy2D> t1, t2;
+        // Y = K(x,P) K(P,P)^{-1} (constructor.jl / factorizations.jl:92):
+        // K(P,P)^{-1} = L^{-T} L^{-1} from two triangular solves on the r x r
+        // identity, then one n x r x r product.
+        std::vector<DiagOp> eye;
         std::vector<TrsmProb> tr;
-        std::vector<double*> yts;
+        std::vector<GemmProb> yg;
+        std::vector<double*> kis;
         for (int i : nys) {
             Hodlr& H = *hs[i];
             const int r = H.nys_r;
-            double* yt = static_cast<double*>(c.scratch((size_t)r * H.n * sizeof(double)));
-            yts.push_back(yt);
-            t1.push_back({H.kxp.as<double>(), (int)H.n, yt, r, (int)H.n, r, 1.0});
-            tr.push_back({H.kpp.as<double>(), r, r, yt, (int)H.n, r});
-            t2.push_back({yt, r, H.ynys.as<double>(), (int)H.n, r, (int)H.n, 1.0});
+            double* ki = static_cast<double*>(c.scratch((size_t)r * r * sizeof(double)));
+            kis.push_back(ki);
+            eye.push_back({ki, r, r, 1.0});
+            tr.push_back({H.kpp.as<double>(), r, r, ki, r, r});
+            yg.push_back({(int)H.n, r, r, H.kxp.as<double>(), (int)H.n, ki, r, H.ynys.as<double>(), (int)H.n, 1.0, 0.0, 0});
         }
-        copy2d(c, t1, true);
+        set_identity(c, eye);
         trsm_left(c, false, tr);
         trsm_left(c, true, tr);
-        copy2d(c, t2, true);
-        for (double* p : yts) c.scratch_free(p);
+        gemm(c, false, false, yg);
+        for (double* p : kis) c.scratch_free(p);
     }
     if (!aca.empty()) build_aca_batch(c, hs, st, aca);
     c.sync();

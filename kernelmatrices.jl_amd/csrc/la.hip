@@ -189,17 +189,193 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(const RedDesc* __restrict
     *cp = v;
 }
 
-// The split-k decision depends on the problem's own shape only, so a
-// problem gives bitwise the same result whatever else shares the launch.
-void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
-    bool any = false;
-    for (const auto& p : probs) any = any || (p.m > 0 && p.n > 0);
-    if (!any) return;
+// Tall TN products C = alpha A^T B + beta C with m, n <= 80 and long k (the
+// projections Q^T X of the panel QR and of the Woodbury cascade): one
+// workgroup holds the whole (16-padded) output for a slab of TT_KS rows of k,
+// so A and B are read once; slabs of one problem are summed by the split-k
+// reduction in slab order.  The 16x16 output blocks are dealt round-robin to
+// the four waves.
+constexpr int TT_MAX = 80;
+constexpr int TT_KS = 256;
+constexpr int TT_KC = 16;
+
+__global__ __launch_bounds__(256, 2) void k_gemm_tn_tall(const GemmDesc* __restrict__ ds) {
+    __shared__ double As[TT_KC][TT_MAX + 1];
+    __shared__ double Bs[TT_KC][TT_MAX + 1];
+    const GemmDesc d = ds[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int NB = (d.n + 15) >> 4, nblk = ((d.m + 15) >> 4) * NB;
+    d4 acc[7];
+#pragma unroll
+    for (int t = 0; t < 7; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
+    // staging: thread tid moves (k = tid & 15, column (tid >> 4) + 16 q)
+    const int skk = tid & 15, scol = tid >> 4;
+    double ra[5], rb[5];
+    auto fetch = [&](int k0) {
+        const int gk = k0 + skk;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            const int col = scol + 16 * q;
+            ra[q] = (col < d.m && gk < d.k) ? d.A[gk + (size_t)col * d.lda] : 0.0;
+            rb[q] = (col < d.n && gk < d.k) ? d.B[gk + (size_t)col * d.ldb] : 0.0;
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < d.k; k0 += TT_KC) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            As[skk][scol + 16 * q] = ra[q];
+            Bs[skk][scol + 16 * q] = rb[q];
+        }
+        __syncthreads();
+        if (k0 + TT_KC < d.k) fetch(k0 + TT_KC);
+#pragma unroll
+        for (int ks = 0; ks < TT_KC / 4; ++ks) {
+            const int kk = ks * 4 + (lane >> 4);
+#pragma unroll
+            for (int t = 0; t < 7; ++t) {
+                const int b = wave + 4 * t;
+                if (b < nblk) {
+                    const int mb = b / NB, nb = b - mb * NB;
+                    // operands swapped: the accumulator holds a block of C^T
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(Bs[kk][nb * 16 + (lane & 15)],
+                                                                  As[kk][mb * 16 + (lane & 15)], acc[t], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double cv[7][4];
+#pragma unroll
+    for (int t = 0; t < 7; ++t) {
+        const int b = wave + 4 * t;
+        const int mb = b / NB, nb = b - mb * NB;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = mb * 16 + (lane & 15), col = nb * 16 + mfma_d_row(lane >> 4, r);
+            cv[t][r] = (b < nblk && d.beta != 0.0 && row < d.m && col < d.n) ? d.C[row + (size_t)col * d.ldc] : 0.0;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 7; ++t) {
+        const int b = wave + 4 * t;
+        const int mb = b / NB, nb = b - mb * NB;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = mb * 16 + (lane & 15), col = nb * 16 + mfma_d_row(lane >> 4, r);
+            if (b < nblk && row < d.m && col < d.n) {
+                double v = d.alpha * acc[t][r];
+                if (d.beta != 0.0) v += d.beta * cv[t][r];
+                d.C[row + (size_t)col * d.ldc] = v;
+            }
+        }
+    }
+}
+
+// Wide NN products C = alpha A B + beta C with n <= 160 (panel updates
+// X -= Q R, the Woodbury apply, the TSQR Q assembly): a workgroup owns 64
+// rows and every output column, so each row of A is read once; wave w takes
+// rows 16w..16w+15.  A is loaded straight from memory (16 consecutive rows
+// per k index), B through LDS in 16-deep chunks.  A wave reads all of its
+// own rows of A before it writes the same rows of C, so C may alias A (the
+// in-place TSQR assembly Q <- Q C with n = k).
+constexpr int WN_MAX = 160;
+constexpr int WN_KC = 16;
+
+__global__ __launch_bounds__(256, 2) void k_gemm_nn_wide(const GemmDesc* __restrict__ ds, int nd) {
+    __shared__ double Bs[WN_KC][WN_MAX + 1];
+    const int t = blockIdx.x;
+    int lo = 0, hi = nd - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (ds[mid].tile0 <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const GemmDesc d = ds[lo];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int NB = (d.n + 15) >> 4;
+    const int arow = (t - d.tile0) * 64 + wave * 16 + (lane & 15);
+    const bool arow_ok = arow < d.m;
+    const double* ap = d.A + (arow_ok ? arow : 0);
+    d4 acc[10];
+#pragma unroll
+    for (int b = 0; b < 10; ++b) acc[b] = d4{0.0, 0.0, 0.0, 0.0};
+    const int skk = tid & 15, scol = tid >> 4;
+    double ra[4], rb[10];
+    auto fetch = [&](int k0) {
+        const int gk = k0 + skk;
+#pragma unroll
+        for (int q = 0; q < 10; ++q) {
+            const int col = scol + 16 * q;
+            rb[q] = (col < d.n && gk < d.k) ? d.B[gk + (size_t)col * d.ldb] : 0.0;
+        }
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            const int ka = k0 + ks * 4 + (lane >> 4);
+            ra[ks] = (arow_ok && ka < d.k) ? ap[(size_t)ka * d.lda] : 0.0;
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < d.k; k0 += WN_KC) {
+#pragma unroll
+        for (int q = 0; q < 10; ++q) Bs[skk][scol + 16 * q] = rb[q];
+        double a[4];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) a[ks] = ra[ks];
+        __syncthreads();
+        if (k0 + WN_KC < d.k) fetch(k0 + WN_KC);
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            const int kk = ks * 4 + (lane >> 4);
+#pragma unroll
+            for (int b = 0; b < 10; ++b)
+                if (b < NB)   // operands swapped: the accumulator holds a block of C^T
+                    acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(Bs[kk][b * 16 + (lane & 15)], a[ks], acc[b], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // epilogue in two halves: the loads of C of a half before its stores
+#pragma unroll
+    for (int h = 0; h < 10; h += 5) {
+        double cv[5][4];
+#pragma unroll
+        for (int b = 0; b < 5; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = (h + b) * 16 + mfma_d_row(lane >> 4, r);
+                cv[b][r] = (h + b < NB && d.beta != 0.0 && arow_ok && col < d.n) ? d.C[arow + (size_t)col * d.ldc]
+                                                                                   : 0.0;
+            }
+#pragma unroll
+        for (int b = 0; b < 5; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = (h + b) * 16 + mfma_d_row(lane >> 4, r);
+                if (h + b < NB && arow_ok && col < d.n) {
+                    double v = d.alpha * acc[h + b][r];
+                    if (d.beta != 0.0) v += d.beta * cv[b][r];
+                    d.C[arow + (size_t)col * d.ldc] = v;
+                }
+            }
+    }
+}
+
+// Which kernel takes a problem, and how it is split, depends on the problem's
+// own shape only, so a problem gives bitwise the same result whatever else
+// shares the launch.
+static bool tall_tn(bool ta, bool tb, const GemmProb& p) {
+    return ta && !tb && !p.lower && p.m <= TT_MAX && p.n <= TT_MAX;
+}
+static bool wide_nn(bool ta, bool tb, const GemmProb& p) {
+    return !ta && !tb && !p.lower && p.n <= WN_MAX && (p.k <= 512 || p.m >= 64 * 64);
+}
+
+static void gemm_generic(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
     std::vector<int> split(probs.size(), 1), kchunk(probs.size(), 0);
     size_t wsz = 0;
     for (size_t i = 0; i < probs.size(); ++i) {
         const auto& p = probs[i];
-        if (p.m <= 0 || p.n <= 0 || p.lower) continue;
+        if (p.lower) continue;
         const long long own = ceil_div(p.m, GT) * ceil_div(p.n, GT);
         if (p.k >= 1024 && own < 64) {
             long long s = std::min<long long>(ceil_div(p.k, 512), std::max<long long>(1, 256 / own));
@@ -218,13 +394,10 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
     std::vector<RedDesc> rs;
     size_t woff = 0;
     long long relem = 0;
-    double fl = 0.0;
     for (size_t i = 0; i < probs.size(); ++i) {
         const auto& p = probs[i];
-        if (p.m <= 0 || p.n <= 0) continue;
         const int tn = (int)ceil_div(p.n, GT);
         const int nt = (int)ceil_div(p.m, GT) * tn;
-        fl += 2.0 * p.m * p.n * (double)p.k;
         if (split[i] == 1) {
             ds.push_back({p.A, p.B, p.C, p.m, p.n, p.k, p.lda, p.ldb, p.ldc, p.alpha, p.beta,
                           tiles, tn, p.lower, 0});
@@ -246,11 +419,6 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
             relem += (long long)p.m * p.n;
         }
     }
-    if (c.prof) {
-        c.ev_flops.push_back(fl);
-        hipEvent_t ev_ = c.prof_event();
-        c.launch([ev_, s_ = c.stream]() { KMH_HIP(hipEventRecord(ev_, s_)); });
-    }
     if (!ds.empty()) {
         GemmDesc* dd = c.upload_vec(ds);
         const int nd = (int)ds.size();
@@ -260,16 +428,84 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
         else if (ta && !tb) KMH_LAUNCH(c, (k_gemm<true, false>), g, dim3(256), dd, nd);
         else KMH_LAUNCH(c, (k_gemm<true, true>), g, dim3(256), dd, nd);
     }
-    if (c.prof) {
-        hipEvent_t ev_ = c.prof_event();
-        c.launch([ev_, s_ = c.stream]() { KMH_HIP(hipEventRecord(ev_, s_)); });
-    }
     if (!rs.empty()) {
         RedDesc* dr = c.upload_vec(rs);
         KMH_LAUNCH(c, k_splitk_reduce, dim3((unsigned)ceil_div(relem, 256)), dim3(256),
                    dr, (int)rs.size(), relem);
     }
     if (W) c.scratch_free(W);
+}
+
+static void gemm_tall_tn(Ctx& c, const std::vector<GemmProb>& probs) {
+    size_t wsz = 0;
+    for (const auto& p : probs) {
+        const int s = (int)ceil_div(p.k, TT_KS);
+        if (s > 1) wsz += (size_t)s * p.m * p.n;
+    }
+    double* W = wsz ? static_cast<double*>(c.scratch(wsz * sizeof(double))) : nullptr;
+    std::vector<GemmDesc> ds;
+    std::vector<RedDesc> rs;
+    size_t woff = 0;
+    long long relem = 0;
+    for (const auto& p : probs) {
+        const int s = (int)ceil_div(p.k, TT_KS);
+        if (s <= 1) {
+            ds.push_back({p.A, p.B, p.C, p.m, p.n, p.k, p.lda, p.ldb, p.ldc, p.alpha, p.beta, 0, 0, 0, 0});
+            continue;
+        }
+        for (int q = 0; q < s; ++q) {
+            const int k0 = q * TT_KS;
+            ds.push_back({p.A + k0, p.B + k0, W + woff + (size_t)q * p.m * p.n, p.m, p.n,
+                          std::min(TT_KS, p.k - k0), p.lda, p.ldb, p.m, 1.0, 0.0, 0, 0, 0, 0});
+        }
+        rs.push_back({p.C, W + woff, p.ldc, p.m, p.n, s, p.alpha, p.beta, relem});
+        woff += (size_t)s * p.m * p.n;
+        relem += (long long)p.m * p.n;
+    }
+    GemmDesc* dd = c.upload_vec(ds);
+    KMH_LAUNCH(c, k_gemm_tn_tall, dim3((unsigned)ds.size()), dim3(256), dd);
+    if (!rs.empty()) {
+        RedDesc* dr = c.upload_vec(rs);
+        KMH_LAUNCH(c, k_splitk_reduce, dim3((unsigned)ceil_div(relem, 256)), dim3(256),
+                   dr, (int)rs.size(), relem);
+    }
+    if (W) c.scratch_free(W);
+}
+
+static void gemm_wide_nn(Ctx& c, const std::vector<GemmProb>& probs) {
+    std::vector<GemmDesc> ds;
+    int tiles = 0;
+    for (const auto& p : probs) {
+        ds.push_back({p.A, p.B, p.C, p.m, p.n, p.k, p.lda, p.ldb, p.ldc, p.alpha, p.beta, tiles, 0, 0, 0});
+        tiles += (int)ceil_div(p.m, 64);
+    }
+    GemmDesc* dd = c.upload_vec(ds);
+    KMH_LAUNCH(c, k_gemm_nn_wide, dim3(tiles), dim3(256), dd, (int)ds.size());
+}
+
+void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
+    std::vector<GemmProb> gen, tall, wide;
+    double fl = 0.0;
+    for (const auto& p : probs) {
+        if (p.m <= 0 || p.n <= 0) continue;
+        fl += 2.0 * p.m * p.n * (double)p.k;
+        if (tall_tn(ta, tb, p)) tall.push_back(p);
+        else if (wide_nn(ta, tb, p)) wide.push_back(p);
+        else gen.push_back(p);
+    }
+    if (gen.empty() && tall.empty() && wide.empty()) return;
+    if (c.prof) {
+        c.ev_flops.push_back(fl);
+        hipEvent_t ev_ = c.prof_event();
+        c.launch([ev_, s_ = c.stream]() { KMH_HIP(hipEventRecord(ev_, s_)); });
+    }
+    if (!gen.empty()) gemm_generic(c, ta, tb, gen);
+    if (!tall.empty()) gemm_tall_tn(c, tall);
+    if (!wide.empty()) gemm_wide_nn(c, wide);
+    if (c.prof) {
+        hipEvent_t ev_ = c.prof_event();
+        c.launch([ev_, s_ = c.stream]() { KMH_HIP(hipEventRecord(ev_, s_)); });
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -33,29 +33,37 @@ __device__ __forceinline__ double qr_bcast(double v, int lane) {
     return __hiloint2double(hi, lo);
 }
 
-// Column sums over the wave's 128 rows: lane l ends with
-// S_{l % NC} = sum_rows p(row, l % NC).  The lanes write their per-lane
+// Column sums over the wave's 128 rows of p(row, k) = c(row) a(row, k):
+// lane l ends with S_{l % NC}.  Lane l holds rows l and l + 64 (c0, a0 and
+// c1, a1).  In rounds of H <= 16 columns the lanes write their per-lane
 // products (two rows already added) to LDS, lane l sums a segment of column
-// l % NC, and the segments meet by xor-shuffles.  LDS operations of one wave
+// l % H, and the segments meet by xor-shuffles.  LDS operations of one wave
 // complete in issue order, so no barrier is needed.  Fixed summation order.
+// The LDS buffer holds 64 x (H + 1) doubles.
 template <int NC>
-__device__ __forceinline__ double wave_colsum(const double (&p)[NC], double* pw, int lane) {
-    constexpr int SEGS = 64 / NC, RPS = 64 / SEGS;
+__device__ __forceinline__ double wave_colsum(double c0, const double (&a0)[NC], double c1, const double (&a1)[NC],
+                                              double* pw, int lane) {
+    constexpr int H = NC > 16 ? 16 : NC, SEGS = 64 / H, RPS = 64 / SEGS;
+    double res = 0.0;
 #pragma unroll
-    for (int k = 0; k < NC; ++k) pw[lane * (NC + 1) + k] = p[k];
-    __builtin_amdgcn_wave_barrier();
-    const int k = lane % NC, seg = lane / NC;
-    double s0 = 0.0, s1 = 0.0;
+    for (int rd = 0; rd < NC / H; ++rd) {
 #pragma unroll
-    for (int i = 0; i < RPS; i += 2) {
-        s0 += pw[(seg * RPS + i) * (NC + 1) + k];
-        s1 += pw[(seg * RPS + i + 1) * (NC + 1) + k];
+        for (int k = 0; k < H; ++k) pw[lane * (H + 1) + k] = c0 * a0[rd * H + k] + c1 * a1[rd * H + k];
+        __builtin_amdgcn_wave_barrier();
+        const int k = lane % H, seg = lane / H;
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int i = 0; i < RPS; i += 2) {
+            s0 += pw[(seg * RPS + i) * (H + 1) + k];
+            s1 += pw[(seg * RPS + i + 1) * (H + 1) + k];
+        }
+        double s = s0 + s1;
+#pragma unroll
+        for (int m = H; m < 64; m <<= 1) s += __shfl_xor(s, m);
+        __builtin_amdgcn_wave_barrier();
+        if ((lane % NC) / H == rd) res = s;
     }
-    double s = s0 + s1;
-#pragma unroll
-    for (int m = NC; m < 64; m <<= 1) s += __shfl_xor(s, m);
-    __builtin_amdgcn_wave_barrier();
-    return s;
+    return res;
 }
 
 // Householder QR (LAPACK dgeqr2 + dorg2r arithmetic) of a tile of <= 128
@@ -65,11 +73,12 @@ __device__ __forceinline__ double wave_colsum(const double (&p)[NC], double* pw,
 // indices are static.  Pivot-row values come from lane j by readlane, column
 // sums from the wave-local reduction; the Householder vector of step j is
 // parked in column j of the Q output (each lane its own rows) for the
-// backward accumulation of Q, which shifts right one column per step.
+// backward accumulation of Q, which shifts right one column per step.  The
+// pivot lane writes row j of R straight to the R output.
+// Three waves per SIMD: the tile (2 x NC doubles per lane) stays in registers.
 template <int NC>
-__global__ __launch_bounds__(64) void k_tile_qr(const SqrDesc* __restrict__ ds) {
-    __shared__ double pw[64 * (NC + 1)];
-    __shared__ double Rs[NC][2 * NC];   // row j of R at Rs[j][j..j+NC): the pivot lane writes it whole
+__global__ __launch_bounds__(64, 2) void k_tile_qr(const SqrDesc* __restrict__ ds) {
+    __shared__ double pw[64 * ((NC > 16 ? 16 : NC) + 1)];
     __shared__ double rowj[NC];
     __shared__ double taus[NC];
     const SqrDesc d = ds[blockIdx.x];
@@ -77,15 +86,19 @@ __global__ __launch_bounds__(64) void k_tile_qr(const SqrDesc* __restrict__ ds) 
     const int lane = threadIdx.x;
     const int r0 = lane, r1 = lane + 64;
     const int kl = lane % NC;   // the column this lane completes
-    for (int e = lane; e < NC * 2 * NC; e += 64) (&Rs[0][0])[e] = 0.0;
     double a0[NC], a1[NC];
     {
-        const double* s0 = (r0 < mt) ? d.src + (size_t)(r0 / d.nbs) * d.slot + (r0 % d.nbs) : d.src;
-        const double* s1 = (r1 < mt) ? d.src + (size_t)(r1 / d.nbs) * d.slot + (r1 % d.nbs) : d.src;
+        // branch-free loads: out-of-range rows and columns read a valid
+        // element and are replaced by zero
+        const bool in0 = r0 < mt, in1 = r1 < mt;
+        const double* s0 = d.src + (in0 ? (size_t)(r0 / d.nbs) * d.slot + (r0 % d.nbs) : 0);
+        const double* s1 = d.src + (in1 ? (size_t)(r1 / d.nbs) * d.slot + (r1 % d.nbs) : 0);
 #pragma unroll
         for (int k = 0; k < NC; ++k) {
-            a0[k] = (r0 < mt && k < nc) ? s0[(size_t)k * d.ld] : 0.0;
-            a1[k] = (r1 < mt && k < nc) ? s1[(size_t)k * d.ld] : 0.0;
+            const size_t off = (size_t)(k < nc ? k : 0) * d.ld;
+            const double x0 = s0[off], x1 = s1[off];
+            a0[k] = (in0 && k < nc) ? x0 : 0.0;
+            a1[k] = (in1 && k < nc) ? x1 : 0.0;
         }
     }
     double* q0 = d.qout + (r0 < mt ? r0 : 0);
@@ -93,15 +106,12 @@ __global__ __launch_bounds__(64) void k_tile_qr(const SqrDesc* __restrict__ ds) 
 #pragma unroll 1
     for (int j = 0; j < nc; ++j) {
         // S_k = sum over rows > j of a(row, 0) a(row, k)   (rows of a1 are >= 64 > j)
-        double p[NC];
         const double h0 = r0 > j ? a0[0] : 0.0;
-#pragma unroll
-        for (int k = 0; k < NC; ++k) p[k] = h0 * a0[k] + a1[0] * a1[k];
         if (r0 == j) {
 #pragma unroll
             for (int k = 0; k < NC; ++k) rowj[k] = a0[k];
         }
-        const double sk = wave_colsum<NC>(p, pw, lane);   // its LDS traffic orders rowj too
+        const double sk = wave_colsum<NC>(h0, a0, a1[0], a1, pw, lane);   // its LDS traffic orders rowj too
         const double rk = rowj[kl];
         const double alpha = qr_bcast(rk, 0);
         const double x2 = qr_bcast(sk, 0);
@@ -115,24 +125,25 @@ __global__ __launch_bounds__(64) void k_tile_qr(const SqrDesc* __restrict__ ds) 
         }
         const double v0 = (r0 > j) ? scal * a0[0] : (r0 == j ? 1.0 : 0.0);
         const double v1 = scal * a1[0];
+        const double wl = rk + scal * sk;   // v^T a_k for this lane's column
         if (tau != 0.0) {
-            const double wl = rk + scal * sk;   // v^T a_k for this lane's column
             const double t0 = tau * v0, t1 = tau * v1;
 #pragma unroll
             for (int k = 1; k < NC; ++k) {
                 const double wk = qr_bcast(wl, k);
-                a0[k] -= t0 * wk;
-                a1[k] -= t1 * wk;
+                a0[k] = __fma_rn(-t0, wk, a0[k]);
+                a1[k] = __fma_rn(-t1, wk, a1[k]);
             }
         }
         if (r0 < mt) q0[(size_t)j * d.ldq] = v0;
         if (r1 < mt) q1[(size_t)j * d.ldq] = v1;
-        if (r0 == j) {
-            Rs[j][j] = beta;
-#pragma unroll
-            for (int k = 1; k < NC; ++k) Rs[j][j + k] = a0[k];
-            taus[j] = tau;
+        // row j of R: lane k < NC holds column j + k (the pivot row's updated
+        // entry, the same arithmetic as the pivot lane's update above)
+        if (lane < nc - j) {
+            const double rv = kl == 0 ? beta : (tau != 0.0 ? __fma_rn(-tau, wl, rk) : rk);
+            d.rout[j + (size_t)(j + lane) * d.ldr] = rv;
         }
+        if (r0 == j) taus[j] = tau;
 #pragma unroll
         for (int k = 0; k < NC - 1; ++k) {
             a0[k] = a0[k + 1];
@@ -142,9 +153,9 @@ __global__ __launch_bounds__(64) void k_tile_qr(const SqrDesc* __restrict__ ds) 
         a1[NC - 1] = 0.0;
     }
     __builtin_amdgcn_wave_barrier();
-    for (int e = lane; e < nc * nc; e += 64) {
+    for (int e = lane; e < nc * nc; e += 64) {   // zero strict lower triangle of R
         const int i = e % nc, cc = e / nc;
-        d.rout[i + (size_t)cc * d.ldr] = Rs[i][cc];
+        if (i > cc) d.rout[i + (size_t)cc * d.ldr] = 0.0;
     }
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
@@ -168,10 +179,7 @@ __global__ __launch_bounds__(64) void k_tile_qr(const SqrDesc* __restrict__ ds) 
         a0[0] = 0.0;
         a1[0] = 0.0;
         const double tau = taus[j];
-        double p[NC];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) p[k] = v0 * a0[k] + v1 * a1[k];
-        const double dk = wave_colsum<NC>(p, pw, lane);
+        const double dk = wave_colsum<NC>(v0, a0, v1, a1, pw, lane);
         if (tau != 0.0) {
             const double t0 = tau * v0, t1 = tau * v1;
 #pragma unroll
