@@ -4,6 +4,7 @@
 // reorthogonalisation pass whose projections run on the MFMA GEMM.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "la.h"
 
@@ -33,59 +34,75 @@ __device__ __forceinline__ double qr_bcast(double v, int lane) {
     return __hiloint2double(hi, lo);
 }
 
-// Column sums over the wave's 128 rows of p(row, k) = c(row) a(row, k):
-// lane l ends with S_{l % NC}.  Lane l holds rows l and l + 64 (c0, a0 and
-// c1, a1).  In rounds of H <= 16 columns the lanes write their per-lane
-// products (two rows already added) to LDS, lane l sums a segment of column
-// l % H, and the segments meet by xor-shuffles.  LDS operations of one wave
-// complete in issue order, so no barrier is needed.  Fixed summation order.
-// The LDS buffer holds 64 x (H + 1) doubles.
-template <int NC>
-__device__ __forceinline__ double wave_colsum(double c0, const double (&a0)[NC], double c1, const double (&a1)[NC],
-                                              double* pw, int lane) {
-    constexpr int H = NC > 16 ? 16 : NC, SEGS = 64 / H, RPS = 64 / SEGS;
-    double res = 0.0;
+// One round of column sums over the wave's 128 rows of p(row, k) =
+// c(row) a(row, k) for the H columns cb..cb+H-1: lane l ends with the sum of
+// column cb + l % H.  Lane l holds rows l and l + 64 (c0, a0 and c1, a1).
+// The lanes write their per-lane products (two rows already added) to LDS,
+// lane l sums a segment of column l % H, and the segments meet by
+// xor-shuffles.  LDS operations of one wave complete in issue order, so no
+// barrier is needed.  Fixed summation order.  LDS: 64 x (H + 1) doubles.
+template <int NC, int H, int CB>
+__device__ __forceinline__ double colsum_round(double c0, const double (&a0)[NC], double c1, const double (&a1)[NC],
+                                               double* pw, int lane) {
+    constexpr int SEGS = 64 / H, RPS = 64 / SEGS;
 #pragma unroll
-    for (int rd = 0; rd < NC / H; ++rd) {
+    for (int k = 0; k < H; ++k) pw[lane * (H + 1) + k] = c0 * a0[CB + k] + c1 * a1[CB + k];
+    __builtin_amdgcn_wave_barrier();
+    const int k = lane % H, seg = lane / H;
+    double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-        for (int k = 0; k < H; ++k) pw[lane * (H + 1) + k] = c0 * a0[rd * H + k] + c1 * a1[rd * H + k];
-        __builtin_amdgcn_wave_barrier();
-        const int k = lane % H, seg = lane / H;
-        double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-        for (int i = 0; i < RPS; i += 2) {
-            s0 += pw[(seg * RPS + i) * (H + 1) + k];
-            s1 += pw[(seg * RPS + i + 1) * (H + 1) + k];
-        }
-        double s = s0 + s1;
-#pragma unroll
-        for (int m = H; m < 64; m <<= 1) s += __shfl_xor(s, m);
-        __builtin_amdgcn_wave_barrier();
-        if ((lane % NC) / H == rd) res = s;
+    for (int i = 0; i < RPS; i += 2) {
+        s0 += pw[(seg * RPS + i) * (H + 1) + k];
+        s1 += pw[(seg * RPS + i + 1) * (H + 1) + k];
     }
-    return res;
+    double s = s0 + s1;
+#pragma unroll
+    for (int m = H; m < 64; m <<= 1) s += __shfl_xor(s, m);
+    __builtin_amdgcn_wave_barrier();
+    return s;
 }
+
+// Column sums of the first W columns (W a multiple of 8): lane l < W ends
+// with S_l (rounds of 16 columns, then one of 8 if W is odd in eighths).
+template <int NC, int W, int CB = 0>
+__device__ __forceinline__ double wave_colsum(double c0, const double (&a0)[NC], double c1, const double (&a1)[NC],
+                                              double* pw, int lane, double res = 0.0) {
+    if constexpr (CB + 16 <= W) {
+        const double s = colsum_round<NC, 16, CB>(c0, a0, c1, a1, pw, lane);
+        if (lane >= CB && lane < CB + 16) res = s;
+        return wave_colsum<NC, W, CB + 16>(c0, a0, c1, a1, pw, lane, res);
+    } else if constexpr (CB + 8 <= W) {
+        const double s = colsum_round<NC, 8, CB>(c0, a0, c1, a1, pw, lane);
+        if (lane >= CB && lane < CB + 8) res = s;
+        return res;
+    } else {
+        return res;
+    }
+}
+
+template <int N> using ic = std::integral_constant<int, N>;
 
 // Householder QR (LAPACK dgeqr2 + dorg2r arithmetic) of a tile of <= 128
 // rows and nc <= NC columns by one wave: lane l holds rows l and l + 64 in
 // registers.  The column loop has a fixed body: after step j the registers
 // shift left by one, so the pivot column is always a[0] and all register
-// indices are static.  Pivot-row values come from lane j by readlane, column
-// sums from the wave-local reduction; the Householder vector of step j is
-// parked in column j of the Q output (each lane its own rows) for the
-// backward accumulation of Q, which shifts right one column per step.  The
-// pivot lane writes row j of R straight to the R output.
-// Three waves per SIMD: the tile (2 x NC doubles per lane) stays in registers.
+// indices are static.  The steps run in phases of 8 whose body covers only
+// the W columns that can still be non-zero (W = NC, NC - 8, ...).  Pivot-row
+// values come from lane j by readlane, column sums from the wave-local
+// reduction; the Householder vector of step j is parked in column j of the Q
+// output (each lane its own rows) for the backward accumulation of Q, which
+// shifts right one column per step (phases of growing W).  Lanes k < nc - j
+// write row j of R straight to the R output.  Two waves per SIMD: the tile
+// (2 x NC doubles per lane) stays in registers.
 template <int NC>
 __global__ __launch_bounds__(64, 2) void k_tile_qr(const SqrDesc* __restrict__ ds) {
-    __shared__ double pw[64 * ((NC > 16 ? 16 : NC) + 1)];
+    __shared__ double pw[64 * 17];
     __shared__ double rowj[NC];
     __shared__ double taus[NC];
     const SqrDesc d = ds[blockIdx.x];
     const int nc = d.nc, mt = d.nstack * d.nbs;
     const int lane = threadIdx.x;
     const int r0 = lane, r1 = lane + 64;
-    const int kl = lane % NC;   // the column this lane completes
     double a0[NC], a1[NC];
     {
         // branch-free loads: out-of-range rows and columns read a valid
@@ -103,55 +120,64 @@ __global__ __launch_bounds__(64, 2) void k_tile_qr(const SqrDesc* __restrict__ d
     }
     double* q0 = d.qout + (r0 < mt ? r0 : 0);
     double* q1 = d.qout + (r1 < mt ? r1 : 0);
+
+    // factorization steps j0 <= j < j1 with W live columns
+    auto fwd = [&](auto wc, int j0, int j1) {
+        constexpr int W = decltype(wc)::value;
 #pragma unroll 1
-    for (int j = 0; j < nc; ++j) {
-        // S_k = sum over rows > j of a(row, 0) a(row, k)   (rows of a1 are >= 64 > j)
-        const double h0 = r0 > j ? a0[0] : 0.0;
-        if (r0 == j) {
+        for (int j = j0; j < j1; ++j) {
+            // S_k = sum over rows > j of a(row, 0) a(row, k)   (rows of a1 are >= 64 > j)
+            const double h0 = r0 > j ? a0[0] : 0.0;
+            if (r0 == j) {
 #pragma unroll
-            for (int k = 0; k < NC; ++k) rowj[k] = a0[k];
-        }
-        const double sk = wave_colsum<NC>(h0, a0, a1[0], a1, pw, lane);   // its LDS traffic orders rowj too
-        const double rk = rowj[kl];
-        const double alpha = qr_bcast(rk, 0);
-        const double x2 = qr_bcast(sk, 0);
-        double tau, beta, scal;
-        if (x2 == 0.0) {
-            tau = 0.0; beta = alpha; scal = 0.0;
-        } else {
-            beta = -copysign(sqrt(alpha * alpha + x2), alpha);
-            tau = (beta - alpha) / beta;
-            scal = 1.0 / (alpha - beta);
-        }
-        const double v0 = (r0 > j) ? scal * a0[0] : (r0 == j ? 1.0 : 0.0);
-        const double v1 = scal * a1[0];
-        const double wl = rk + scal * sk;   // v^T a_k for this lane's column
-        if (tau != 0.0) {
-            const double t0 = tau * v0, t1 = tau * v1;
-#pragma unroll
-            for (int k = 1; k < NC; ++k) {
-                const double wk = qr_bcast(wl, k);
-                a0[k] = __fma_rn(-t0, wk, a0[k]);
-                a1[k] = __fma_rn(-t1, wk, a1[k]);
+                for (int k = 0; k < W; ++k) rowj[k] = a0[k];
             }
-        }
-        if (r0 < mt) q0[(size_t)j * d.ldq] = v0;
-        if (r1 < mt) q1[(size_t)j * d.ldq] = v1;
-        // row j of R: lane k < NC holds column j + k (the pivot row's updated
-        // entry, the same arithmetic as the pivot lane's update above)
-        if (lane < nc - j) {
-            const double rv = kl == 0 ? beta : (tau != 0.0 ? __fma_rn(-tau, wl, rk) : rk);
-            d.rout[j + (size_t)(j + lane) * d.ldr] = rv;
-        }
-        if (r0 == j) taus[j] = tau;
+            const double sk = wave_colsum<NC, W>(h0, a0, a1[0], a1, pw, lane);   // its LDS traffic orders rowj too
+            const double rk = rowj[lane < W ? lane : 0];
+            const double alpha = qr_bcast(rk, 0);
+            const double x2 = qr_bcast(sk, 0);
+            double tau, beta, scal;
+            if (x2 == 0.0) {
+                tau = 0.0; beta = alpha; scal = 0.0;
+            } else {
+                beta = -copysign(sqrt(alpha * alpha + x2), alpha);
+                tau = (beta - alpha) / beta;
+                scal = 1.0 / (alpha - beta);
+            }
+            const double v0 = (r0 > j) ? scal * a0[0] : (r0 == j ? 1.0 : 0.0);
+            const double v1 = scal * a1[0];
+            const double wl = rk + scal * sk;   // v^T a_k for this lane's column
+            if (tau != 0.0) {
+                const double t0 = tau * v0, t1 = tau * v1;
 #pragma unroll
-        for (int k = 0; k < NC - 1; ++k) {
-            a0[k] = a0[k + 1];
-            a1[k] = a1[k + 1];
+                for (int k = 1; k < W; ++k) {
+                    const double wk = qr_bcast(wl, k);
+                    a0[k] = __fma_rn(-t0, wk, a0[k]);
+                    a1[k] = __fma_rn(-t1, wk, a1[k]);
+                }
+            }
+            if (r0 < mt) q0[(size_t)j * d.ldq] = v0;
+            if (r1 < mt) q1[(size_t)j * d.ldq] = v1;
+            // row j of R: lane k holds column j + k (the pivot row's updated
+            // entry, the same arithmetic as the pivot lane's update above)
+            if (lane < nc - j) {
+                const double rv = lane == 0 ? beta : (tau != 0.0 ? __fma_rn(-tau, wl, rk) : rk);
+                d.rout[j + (size_t)(j + lane) * d.ldr] = rv;
+            }
+            if (r0 == j) taus[j] = tau;
+#pragma unroll
+            for (int k = 0; k < W - 1; ++k) {
+                a0[k] = a0[k + 1];
+                a1[k] = a1[k + 1];
+            }
+            a0[W - 1] = 0.0;
+            a1[W - 1] = 0.0;
         }
-        a0[NC - 1] = 0.0;
-        a1[NC - 1] = 0.0;
-    }
+    };
+    fwd(ic<NC>{}, 0, min(nc, 8));
+    if constexpr (NC > 8) fwd(ic<NC - 8>{}, 8, min(nc, 16));
+    if constexpr (NC > 16) fwd(ic<NC - 16>{}, 16, min(nc, 24));
+    if constexpr (NC > 24) fwd(ic<NC - 24>{}, 24, min(nc, 32));
     __builtin_amdgcn_wave_barrier();
     for (int e = lane; e < nc * nc; e += 64) {   // zero strict lower triangle of R
         const int i = e % nc, cc = e / nc;
@@ -164,34 +190,43 @@ __global__ __launch_bounds__(64, 2) void k_tile_qr(const SqrDesc* __restrict__ d
     }
     double n0 = (nc > 0 && r0 < mt) ? q0[(size_t)(nc - 1) * d.ldq] : 0.0;
     double n1 = (nc > 0 && r1 < mt) ? q1[(size_t)(nc - 1) * d.ldq] : 0.0;
+    // accumulation steps jhi >= j >= jlo with W live columns
+    auto bwd = [&](auto wc, int jhi, int jlo) {
+        constexpr int W = decltype(wc)::value;
 #pragma unroll 1
-    for (int j = nc - 1; j >= 0; --j) {
-        const double v0 = n0, v1 = n1;
-        if (j > 0) {
-            n0 = (r0 < mt) ? q0[(size_t)(j - 1) * d.ldq] : 0.0;
-            n1 = (r1 < mt) ? q1[(size_t)(j - 1) * d.ldq] : 0.0;
-        }
-#pragma unroll
-        for (int k = NC - 1; k > 0; --k) {
-            a0[k] = a0[k - 1];
-            a1[k] = a1[k - 1];
-        }
-        a0[0] = 0.0;
-        a1[0] = 0.0;
-        const double tau = taus[j];
-        const double dk = wave_colsum<NC>(v0, a0, v1, a1, pw, lane);
-        if (tau != 0.0) {
-            const double t0 = tau * v0, t1 = tau * v1;
-#pragma unroll
-            for (int k = 1; k < NC; ++k) {
-                const double w = qr_bcast(dk, k);
-                a0[k] -= t0 * w;
-                a1[k] -= t1 * w;
+        for (int j = jhi; j >= jlo; --j) {
+            const double v0 = n0, v1 = n1;
+            if (j > 0) {
+                n0 = (r0 < mt) ? q0[(size_t)(j - 1) * d.ldq] : 0.0;
+                n1 = (r1 < mt) ? q1[(size_t)(j - 1) * d.ldq] : 0.0;
             }
+#pragma unroll
+            for (int k = W - 1; k > 0; --k) {
+                a0[k] = a0[k - 1];
+                a1[k] = a1[k - 1];
+            }
+            a0[0] = 0.0;
+            a1[0] = 0.0;
+            const double tau = taus[j];
+            const double dk = wave_colsum<NC, W>(v0, a0, v1, a1, pw, lane);
+            if (tau != 0.0) {
+                const double t0 = tau * v0, t1 = tau * v1;
+#pragma unroll
+                for (int k = 1; k < W; ++k) {
+                    const double w = qr_bcast(dk, k);
+                    a0[k] -= t0 * w;
+                    a1[k] -= t1 * w;
+                }
+            }
+            a0[0] = (r0 > j) ? -tau * v0 : (r0 == j ? 1.0 - tau : 0.0);
+            a1[0] = -tau * v1;
         }
-        a0[0] = (r0 > j) ? -tau * v0 : (r0 == j ? 1.0 - tau : 0.0);
-        a1[0] = -tau * v1;
-    }
+    };
+    // step j touches columns j..nc-1: W >= NC - j
+    bwd(ic<8>{}, nc - 1, NC - 8);
+    if constexpr (NC > 8) bwd(ic<16>{}, min(nc - 1, NC - 9), NC - 16);
+    if constexpr (NC > 16) bwd(ic<24>{}, min(nc - 1, NC - 17), NC - 24);
+    if constexpr (NC > 24) bwd(ic<32>{}, min(nc - 1, NC - 25), NC - 32);
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
         if (k < nc) {
