@@ -30,6 +30,23 @@ for p in (ROOT, os.path.join(ROOT, "kernelmatrices.jl_amd")):
         sys.path.insert(0, p)
 
 FP64_MFMA_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBPS = 8000.0
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "k_gemm_nn_wide_served_n65536_traffic.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (a line 'traffic_bytes_per_launch <kernel> <bytes>'), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                    return int(float(parts[2]))
+    except OSError:
+        pass
+    return None
 
 
 def make_inputs(n, seed=0):
@@ -170,6 +187,8 @@ def main():
     # gradient legs: nll_objective (maxlikfunctions.jl:29-50) = negloglik +
     # stoch_gradient with Rademacher SAA vectors
     saa = np.asfortranarray(np.random.default_rng(2).choice([-1.0, 1.0], size=(n, args.grad_vecs)))
+    # the SAA vectors are fixed across evaluations: resident in HBM like y
+    saa_d = torch.from_numpy(np.ascontiguousarray(saa.T)).to(dev)   # column-major n x nv
     y_host = np.ascontiguousarray(y)
 
     opts = L.kmh_build_opts()
@@ -213,7 +232,7 @@ def main():
         quad = float(torch.dot(y_d, sol_d))
         dks = (L.kmh_kernel * 2)(kernel_of(p, L.KMH_D1), kernel_of(p, L.KMH_D2))
         g = np.zeros(2)
-        check(lib.kmh_stoch_gradient(h, dks, 2, y_host.ctypes.data, saa.ctypes.data, saa.shape[1], g.ctypes.data))
+        check(lib.kmh_stoch_gradient_dev(h, dks, 2, y_d.data_ptr(), saa_d.data_ptr(), saa.shape[1], g.ctypes.data))
         check(lib.kmh_hodlr_destroy(h))
         return 0.5 * ld.value + 0.5 * quad, g
 
@@ -275,7 +294,7 @@ def main():
             r.y = y_d.data_ptr()
             if grad_mode[0]:
                 r.ndk = 2
-                r.saa = saa.ctypes.data
+                r.saa = saa_d.data_ptr()
                 r.nv = saa.shape[1]
                 r.dks[0] = kernel_of(params[i], L.KMH_D1)
                 r.dks[1] = kernel_of(params[i], L.KMH_D2)
@@ -313,7 +332,9 @@ def main():
             if profile:
                 gfl, gms, gl = C.c_double(), C.c_double(), C.c_int64()
                 check(lib.kmh_ctx_profile_end(ctx.ptr, C.byref(gfl), C.byref(gms), C.byref(gl)))
-                prof = (gfl.value, gms.value, gl.value)
+                wb, wms, wl = C.c_double(), C.c_double(), C.c_int64()
+                check(lib.kmh_ctx_profile_query(ctx.ptr, L.KMH_PROF_NN_WIDE, C.byref(wb), C.byref(wms), C.byref(wl)))
+                prof = (gfl.value, gms.value, gl.value, wb.value, wms.value, wl.value)
             return res, el, prof
 
         if "served" in legs:
@@ -324,7 +345,11 @@ def main():
             out["served"] = {"evals_per_s": args.steps / el, "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99),
                              "gemm_tflops": prof[0] / (prof[1] * 1e-3) / 1e12 if prof[1] else 0.0,
                              "gemm_flops_per_eval": prof[0] / args.steps, "gemm_ms_per_eval": prof[1] / args.steps,
-                             "gemm_launches_total": prof[2], "inflight": args.inflight}
+                             "gemm_launches_total": prof[2], "inflight": args.inflight,
+                             "nn_wide_bytes_per_launch": prof[3] / max(1, prof[5]),
+                             "nn_wide_us_per_launch": 1e3 * prof[4] / max(1, prof[5]),
+                             "nn_wide_gbps": prof[3] / (prof[4] * 1e-3) / 1e9 if prof[4] else 0.0,
+                             "nn_wide_launches": prof[5]}
             if plain_ld:
                 errs = [abs(res[i][1] - plain_ld[i]) / abs(plain_ld[i]) for i in plain_ld]
                 out["served"]["max_rel_logdet_err_vs_plain"] = max(errs)
@@ -342,7 +367,26 @@ def main():
                 out["served_grad"]["max_rel_grad_err_vs_plain"] = gerr
         lib.kmh_server_destroy(srv)
 
-    head = out.get("served") or out.get("plain")
+    head = out.get("served") or out.get("plain") or out.get("served_grad") or out.get("plain_grad")
+
+    def roofline(h):
+        """Dominant kernel of the served leg: k_gemm_nn_wide (HBM-bound skinny
+        FP64 GEMM).  achieved = its algorithmic bytes per launch / its average
+        launch time, both measured live with HIP events on the library's
+        stream; traffic = measured HBM bytes per launch (rocprofv3 FETCH_SIZE
+        + WRITE_SIZE of the same command, calibrated on a copy kernel) when
+        the committed summary holds it."""
+        if "nn_wide_gbps" not in h:
+            return {"bound": "hbm", "kernel": "k_gemm_nn_wide", "achieved": None, "peak": HBM_PEAK_GBPS,
+                    "unit": "GB/s", "frac": None, "traffic": None}
+        return {"bound": "hbm", "kernel": "k_gemm_nn_wide (skinny FP64 GEMM, v_mfma_f64_16x16x4f64)",
+                "achieved": round(h["nn_wide_gbps"], 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(h["nn_wide_gbps"] / HBM_PEAK_GBPS, 4),
+                "algorithmic_bytes_per_launch": round(h["nn_wide_bytes_per_launch"]),
+                "us_per_launch": round(h["nn_wide_us_per_launch"], 2),
+                "traffic": measured_traffic("k_gemm_nn_wide"),
+                "gemm_all_tflops": round(h["gemm_tflops"], 3), "fp64_mfma_peak_tflops": FP64_MFMA_PEAK_TFLOPS,
+                "factor_flops_structured_per_eval": ff}
     L_ = info.get("levels", args.level)
     nblocks = [2 ** (L_ - l - 1) for l in range(L_)]
     m = n // (2 ** L_)
@@ -368,10 +412,7 @@ def main():
                    "n": n, "leaf": m, "nystrom_rank": 72, "levels": L_, "inflight": args.inflight},
         "legs": {k: {kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in d.items()}
                  for k, d in out.items()},
-        "roofline": {"bound": "mfma", "kernel": "k_gemm (grouped FP64 GEMM, v_mfma_f64_16x16x4f64)",
-                     "achieved": round(head["gemm_tflops"], 3), "peak": FP64_MFMA_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(head["gemm_tflops"] / FP64_MFMA_PEAK_TFLOPS, 4),
-                     "traffic": None, "factor_flops_structured_per_eval": ff},
+        "roofline": roofline(head),
     }
     if "served" in out and "plain" in out:
         line["speedup_served_vs_plain"] = round(out["served"]["evals_per_s"] / out["plain"]["evals_per_s"], 3)

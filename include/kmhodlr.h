@@ -107,11 +107,18 @@ int kmh_ctx_destroy(kmh_ctx* ctx);
 int kmh_ctx_synchronize(kmh_ctx* ctx);
 int kmh_ctx_bytes_in_use(kmh_ctx* ctx, uint64_t* out);
 /* Measurement hooks (no counterpart in the reference): between begin and end
- * every grouped FP64 GEMM launch on the context's stream is bracketed by HIP
- * events; end returns the launches' algorithmic flops (sum of 2mnk), their
- * summed duration in ms and the launch count.                               */
+ * the launches below are bracketed by HIP events on the context's stream.
+ * end returns the KMH_PROF_GEMM totals (every grouped FP64 GEMM call: the
+ * algorithmic flops 2mnk, summed duration in ms, call count); query returns
+ * any tag's totals of the last begin..end window:
+ *   KMH_PROF_GEMM     grouped GEMM calls, amount = flops
+ *   KMH_PROF_NN_WIDE  k_gemm_nn_wide launches, amount = algorithmic HBM bytes
+ *                     (A, B and C written, plus C read when beta != 0)
+ *   KMH_PROF_TN_TALL  k_gemm_tn_tall launches, amount = bytes of A, B, C     */
+enum { KMH_PROF_GEMM = 0, KMH_PROF_NN_WIDE = 1, KMH_PROF_TN_TALL = 2, KMH_PROF_TAGS = 3 };
 int kmh_ctx_profile_begin(kmh_ctx* ctx);
 int kmh_ctx_profile_end(kmh_ctx* ctx, double* gemm_flops, double* gemm_ms, int64_t* gemm_launches);
+int kmh_ctx_profile_query(kmh_ctx* ctx, int32_t tag, double* amount, double* ms, int64_t* spans);
 
 /* ---- HODLR.KernelHODLR(K, ep, maxrank, lvl; nystrom)  constructor.jl:6-58 ----
  * K = KernelMatrix(pts, pts, params, kernel)  src/structstypes.jl:17-23      */
@@ -189,9 +196,8 @@ typedef struct kmh_serve_req {
     int64_t n;
     int32_t d;
     int32_t ndk;              /* number of derivative kernels (0: no gradient) */
-    int32_t on_device;        /* pts and y are device pointers the caller keeps
-                                 alive until the request completes; saa stays
-                                 on the host                                   */
+    int32_t on_device;        /* pts, y and saa are device pointers the caller
+                                 keeps alive until the request completes      */
     int32_t reserved;
     kmh_kernel kernel;
     kmh_build_opts opts;
@@ -221,6 +227,10 @@ int kmh_server_result(kmh_server* srv, int64_t id, kmh_serve_result* out);
  * dks: p derivative kernels; y: n; saa: n x nv column-major; grad: p        */
 int kmh_stoch_gradient(kmh_hodlr* h, const kmh_kernel* dks, int32_t p,
                        const double* y, const double* saa, int64_t nv, double* grad);
+/* same with y and saa in device memory (SAA vectors are fixed across an
+ * optimisation, so callers keep them resident); grad stays on the host      */
+int kmh_stoch_gradient_dev(kmh_hodlr* h, const kmh_kernel* dks, int32_t p,
+                           const double* y, const double* saa, int64_t nv, double* grad);
 
 /* ---- HODLR.stoch_profile_gradient(K, HK, y, dfuns, vecs)  stochasticgradient.jl:48-86
  * gradient of the profile negative log-likelihood

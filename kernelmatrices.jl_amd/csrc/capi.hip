@@ -113,7 +113,8 @@ int kmh_ctx_profile_begin(kmh_ctx* ctx) {
         c.sync();
         c.prof = true;
         c.ev_used = 0;
-        c.ev_flops.clear();
+        c.spans.clear();
+        for (auto& t : c.prof_sum) t[0] = t[1] = t[2] = 0.0;
     })
 }
 
@@ -123,19 +124,31 @@ int kmh_ctx_profile_end(kmh_ctx* ctx, double* gemm_flops, double* gemm_ms, int64
         kmh::Ctx& c = ctx->c;
         c.set_device();
         c.sync();
-        double fl = 0.0, ms = 0.0;
-        for (size_t i = 0; i < c.ev_flops.size(); ++i) {
+        for (auto& t : c.prof_sum) t[0] = t[1] = t[2] = 0.0;
+        for (const auto& sp : c.spans) {
             float t = 0.f;
-            KMH_HIP(hipEventElapsedTime(&t, c.ev_pool[2 * i], c.ev_pool[2 * i + 1]));
-            fl += c.ev_flops[i];
-            ms += t;
+            KMH_HIP(hipEventElapsedTime(&t, sp.a, sp.b));
+            c.prof_sum[sp.tag][0] += sp.amount;
+            c.prof_sum[sp.tag][1] += t;
+            c.prof_sum[sp.tag][2] += 1.0;
         }
-        *gemm_flops = fl;
-        *gemm_ms = ms;
-        *gemm_launches = (int64_t)c.ev_flops.size();
+        *gemm_flops = c.prof_sum[KMH_PROF_GEMM][0];
+        *gemm_ms = c.prof_sum[KMH_PROF_GEMM][1];
+        *gemm_launches = (int64_t)c.prof_sum[KMH_PROF_GEMM][2];
         c.prof = false;
         c.ev_used = 0;
-        c.ev_flops.clear();
+        c.spans.clear();
+    })
+}
+
+int kmh_ctx_profile_query(kmh_ctx* ctx, int32_t tag, double* amount, double* ms, int64_t* spans) {
+    KMH_TRY({
+        KMH_CHECK(ctx && amount && ms && spans, KMH_E_BAD_ARG, "null argument");
+        KMH_CHECK(tag >= 0 && tag < KMH_PROF_TAGS, KMH_E_BAD_ARG, "unknown profiling tag");
+        const kmh::Ctx& c = ctx->c;
+        *amount = c.prof_sum[tag][0];
+        *ms = c.prof_sum[tag][1];
+        *spans = (int64_t)c.prof_sum[tag][2];
     })
 }
 
@@ -363,6 +376,17 @@ int kmh_stoch_gradient(kmh_hodlr* h, const kmh_kernel* dks, int32_t p, const dou
         KMH_CHECK(dks && y && grad && p >= 1, KMH_E_BAD_ARG, "null argument");
         KMH_CHECK(nv >= 1 && saa, KMH_E_BAD_ARG, "need at least one SAA vector");
         kmh::hodlr_stoch_gradient(H, dks, p, y, saa, nv, grad);
+    })
+}
+
+int kmh_stoch_gradient_dev(kmh_hodlr* h, const kmh_kernel* dks, int32_t p, const double* y, const double* saa,
+                           int64_t nv, double* grad) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        KMH_CHECK(H.factorized, KMH_E_NOT_FACTORIZED, "The matrix needs to be factorized for this to work.");
+        KMH_CHECK(dks && y && grad && p >= 1, KMH_E_BAD_ARG, "null argument");
+        KMH_CHECK(nv >= 1 && saa, KMH_E_BAD_ARG, "need at least one SAA vector");
+        kmh::hodlr_stoch_gradient(H, dks, p, y, saa, nv, grad, false, true);
     })
 }
 

@@ -76,12 +76,21 @@ struct Ctx {
     std::vector<std::function<void()>> pending;
     // Device flag for Cholesky failures (non-positive pivots).
     int* d_info = nullptr;
-    // GEMM profiling: events bracketing every grouped-GEMM launch on this
-    // stream and the algorithmic flops (2mnk) of its problems.
+    // Profiling spans (measurement hooks): HIP events recorded in stream
+    // order around a launch or group of launches, with a tag and the
+    // algorithmic amount (flops or bytes) of the work between them.
+    struct ProfSpan {
+        int tag;
+        double amount;
+        hipEvent_t a, b;
+    };
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
-    std::vector<double> ev_flops;
+    std::vector<ProfSpan> spans;
+    double prof_sum[KMH_PROF_TAGS][3] = {};   // per tag: amount, ms, spans
+    size_t prof_begin_span(int tag, double amount);
+    void prof_end_span(size_t i);
     hipEvent_t prof_event();
     // Scratch pool: device chunks carved first-fit on the host.  One in-order
     // stream runs every kernel of the context, so a block freed here may be

@@ -139,6 +139,18 @@ hipEvent_t Ctx::prof_event() {
     return ev_pool[ev_used++];
 }
 
+size_t Ctx::prof_begin_span(int tag, double amount) {
+    hipEvent_t a = prof_event(), b = prof_event();
+    spans.push_back({tag, amount, a, b});
+    launch([a, s = stream]() { KMH_HIP(hipEventRecord(a, s)); });
+    return spans.size() - 1;
+}
+
+void Ctx::prof_end_span(size_t i) {
+    hipEvent_t b = spans[i].b;
+    launch([b, s = stream]() { KMH_HIP(hipEventRecord(b, s)); });
+}
+
 void* Ctx::scratch(size_t bytes) {
     bytes = std::max<size_t>(256, (bytes + 255) & ~size_t(255));
     auto it = free_blocks.begin();

@@ -120,8 +120,29 @@ void hodlr_factor_apply(Hodlr& H, int op, double* T, int nrhs);
 void hodlr_solve(Hodlr& H, double* T, int nrhs);
 void hodlr_full_host(Hodlr& H, double* out);
 void hodlr_block_uv_host(Hodlr& H, int level, int block, double* U, double* V);
-void hodlr_stoch_gradient(Hodlr& H, const kmh_kernel* dks, int p, const double* y_host, const double* saa_host,
-                          int64_t nv, double* grad, bool profile = false);
+void hodlr_stoch_gradient(Hodlr& H, const kmh_kernel* dks, int p, const double* y, const double* saa, int64_t nv,
+                          double* grad, bool profile = false, bool on_device = false);
+// W^-1 (KMH_WINV) or W^-T (KMH_WTINV) of each handle's device block T in place.
+void hodlr_factor_apply_batch(Ctx& c, int op, const std::vector<Hodlr*>& hs, const std::vector<double*>& T,
+                              const std::vector<int>& nrhs);
+// One handle's stochastic gradient in a batch: y and the SAA vectors
+// (n x nv) on the host or the device, K^-1 y optionally already computed on
+// the device (it is then not solved again).
+struct GradJob {
+    Hodlr* H;
+    const kmh_kernel* dks;
+    int p;
+    const double* y_host;
+    const double* y_dev;
+    const double* kinv_y_dev;
+    const double* saa_host;
+    const double* saa_dev;
+    int64_t nv;
+    double* grad;
+    bool profile;
+};
+void check_grad_job(const GradJob& jb);   // the checks of the batch, for one job
+void hodlr_stoch_gradient_batch(Ctx& c, const std::vector<GradJob>& jobs);
 void check_kernel(const kmh_kernel& k, bool allow_deriv);
 
 }  // namespace kmh

@@ -761,6 +761,31 @@ void hodlr_solve_batch(Ctx& c, const std::vector<Hodlr*>& hs, const std::vector<
     leaf_trsm_batch(c, true, rs);
 }
 
+void hodlr_factor_apply_batch(Ctx& c, int op, const std::vector<Hodlr*>& hs, const std::vector<double*>& T,
+                              const std::vector<int>& nrhs) {
+    KMH_CHECK(op == KMH_WINV || op == KMH_WTINV, KMH_E_BAD_ARG, "batched factor apply: W^-1 or W^-T only");
+    std::vector<Rhs> rs;
+    int maxL = 0;
+    for (size_t i = 0; i < hs.size(); ++i) {
+        KMH_CHECK(hs[i]->factorized, KMH_E_NOT_FACTORIZED, "No solves without factorization.");
+        rs.push_back({hs[i], T[i], hs[i]->n, nrhs[i]});
+        maxL = std::max(maxL, hs[i]->L);
+    }
+    auto at = [&](int l) {
+        std::vector<Rhs> a;
+        for (auto x : rs)
+            if (x.H->L > l) { x.lv = l; a.push_back(x); }
+        return a;
+    };
+    if (op == KMH_WINV) {
+        leaf_trsm_batch(c, false, rs);
+        for (int l = 0; l < maxL; ++l) woodbury_batch(c, W_INV, at(l));
+    } else {
+        for (int l = maxL - 1; l >= 0; --l) woodbury_batch(c, W_INVT, at(l));
+        leaf_trsm_batch(c, true, rs);
+    }
+}
+
 void hodlr_factor_apply(Hodlr& H, int op, double* T, int nrhs) {
     KMH_CHECK(H.factorized, KMH_E_NOT_FACTORIZED, "No solves without factorization.");
     Ctx& c = *H.ctx;

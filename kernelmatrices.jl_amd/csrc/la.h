@@ -65,6 +65,14 @@ struct Axpby2D {
     double a, b;
 };
 void axpby2d(Ctx& c, const std::vector<Axpby2D>& probs);
+// out[i] = x[i] + y[i] for i < n (contiguous vectors; out may alias neither)
+struct Add2 {
+    const double* x;
+    const double* y;
+    double* out;
+    int64_t n;
+};
+void add2(Ctx& c, const std::vector<Add2>& probs);
 // dst = identity (n x n) ; or dst(i,i) += v
 struct DiagOp {
     double* A; int n; int ld;

@@ -463,7 +463,15 @@ static void gemm_tall_tn(Ctx& c, const std::vector<GemmProb>& probs) {
         relem += (long long)p.m * p.n;
     }
     GemmDesc* dd = c.upload_vec(ds);
+    size_t span = 0;
+    if (c.prof) {
+        double bytes = 0.0;
+        for (const auto& d : ds)
+            bytes += 8.0 * ((double)d.k * d.m + (double)d.k * d.n + (double)d.m * d.n * (d.beta != 0.0 ? 2 : 1));
+        span = c.prof_begin_span(KMH_PROF_TN_TALL, bytes);
+    }
     KMH_LAUNCH(c, k_gemm_tn_tall, dim3((unsigned)ds.size()), dim3(256), dd);
+    if (c.prof) c.prof_end_span(span);
     if (!rs.empty()) {
         RedDesc* dr = c.upload_vec(rs);
         KMH_LAUNCH(c, k_splitk_reduce, dim3((unsigned)ceil_div(relem, 256)), dim3(256),
@@ -480,7 +488,15 @@ static void gemm_wide_nn(Ctx& c, const std::vector<GemmProb>& probs) {
         tiles += (int)ceil_div(p.m, 64);
     }
     GemmDesc* dd = c.upload_vec(ds);
+    size_t span = 0;
+    if (c.prof) {
+        double bytes = 0.0;
+        for (const auto& p : probs)
+            bytes += 8.0 * ((double)p.m * p.k + (double)p.k * p.n + (double)p.m * p.n * (p.beta != 0.0 ? 2 : 1));
+        span = c.prof_begin_span(KMH_PROF_NN_WIDE, bytes);
+    }
     KMH_LAUNCH(c, k_gemm_nn_wide, dim3(tiles), dim3(256), dd, (int)ds.size());
+    if (c.prof) c.prof_end_span(span);
 }
 
 void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
@@ -494,18 +510,11 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
         else gen.push_back(p);
     }
     if (gen.empty() && tall.empty() && wide.empty()) return;
-    if (c.prof) {
-        c.ev_flops.push_back(fl);
-        hipEvent_t ev_ = c.prof_event();
-        c.launch([ev_, s_ = c.stream]() { KMH_HIP(hipEventRecord(ev_, s_)); });
-    }
+    const size_t span = c.prof ? c.prof_begin_span(KMH_PROF_GEMM, fl) : 0;
     if (!gen.empty()) gemm_generic(c, ta, tb, gen);
     if (!tall.empty()) gemm_tall_tn(c, tall);
     if (!wide.empty()) gemm_wide_nn(c, wide);
-    if (c.prof) {
-        hipEvent_t ev_ = c.prof_event();
-        c.launch([ev_, s_ = c.stream]() { KMH_HIP(hipEventRecord(ev_, s_)); });
-    }
+    if (c.prof) c.prof_end_span(span);
 }
 
 // ---------------------------------------------------------------------------
@@ -827,6 +836,24 @@ void axpby2d(Ctx& c, const std::vector<Axpby2D>& probs) {
         for (auto& p : part) mx = std::max<long long>(mx, (long long)p.m * p.n);
         const unsigned gx = (unsigned)std::min<long long>(std::max<long long>(1, ceil_div(mx, 256)), 1024);
         KMH_LAUNCH(c, k_axpby2d, dim3(gx, (unsigned)part.size()), dim3(256), d);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_add2(const Add2* __restrict__ ds) {
+    const Add2 d = ds[blockIdx.y];
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < d.n; i += (int64_t)gridDim.x * 256)
+        d.out[i] = d.x[i] + d.y[i];
+}
+
+void add2(Ctx& c, const std::vector<Add2>& probs) {
+    if (probs.empty()) return;
+    for (size_t off = 0; off < probs.size(); off += 65535) {
+        std::vector<Add2> part(probs.begin() + off, probs.begin() + std::min(probs.size(), off + 65535));
+        Add2* d = c.upload_vec(part);
+        int64_t mx = 0;
+        for (auto& p : part) mx = std::max(mx, p.n);
+        const unsigned gx = (unsigned)std::min<int64_t>(std::max<int64_t>(1, ceil_div(mx, 256)), 1024);
+        KMH_LAUNCH(c, k_add2, dim3(gx, (unsigned)part.size()), dim3(256), d);
     }
 }
 

@@ -36,6 +36,7 @@ struct Req {
     std::vector<double> pts, y, saa;
     const double* dpts = nullptr;   // device-resident inputs (not owned)
     const double* dy = nullptr;
+    const double* dsaa = nullptr;
     int64_t n = 0;
     int nv = 0;
     kmh_kernel kern{};
@@ -76,6 +77,7 @@ uint64_t request_footprint(int64_t n, const kmh_build_opts& o, int nv) {
     b += 8 * (uint64_t)(nleaf - 1) * 2 * (uint64_t)(4 * r * r);   // X and G
     if (o.method == KMH_NYSTROM) b += 8 * (uint64_t)n * r * 4 + 16 * (uint64_t)r * r;
     b += 8 * (uint64_t)n * (2 + nv) * 2;                          // right-hand sides
+    if (nv > 0) b += 8 * (uint64_t)n * (m + (uint64_t)r + 2);     // derivative leaves and panel
     return b;
 }
 
@@ -136,27 +138,43 @@ static void finish_batch(Server& S, std::vector<Req*>& rs) {
         std::vector<double> hd(ok.size());
         KMH_HIP(hipMemcpyAsync(hd.data(), dots, ok.size() * sizeof(double), hipMemcpyDeviceToHost, c.q()));
         c.sync();
-        c.scratch_free(buf);
         for (size_t i = 0; i < ok.size(); ++i) {
             okr[i]->logdet = ok[i]->logdet;
             okr[i]->nll = 0.5 * ok[i]->logdet + 0.5 * hd[i];
         }
-        // stochastic gradient (stochasticgradient.jl:30-45)
-        for (Req* q : okr) {
-            if (q->dks.empty()) continue;
-            if (q->y.empty()) {
-                q->y.resize(q->n);
-                KMH_HIP(hipMemcpy(q->y.data(), q->dy, q->n * sizeof(double), hipMemcpyDeviceToHost));
+        // stochastic gradient (stochasticgradient.jl:30-45) of every request
+        // that asks for one, together; y and K^-1 y are still in buf
+        std::vector<GradJob> gj;
+        off = 0;
+        for (size_t i = 0; i < ok.size(); ++i) {
+            Req* q = okr[i];
+            const int64_t n = ok[i]->n;
+            if (!q->dks.empty()) {
+                q->grad.assign(q->dks.size(), 0.0);
+                GradJob jb{ok[i], q->dks.data(), (int)q->dks.size(), nullptr, buf + off, buf + off + n,
+                           q->dsaa ? nullptr : q->saa.data(), q->dsaa, q->nv, q->grad.data(), false};
+                try {
+                    check_grad_job(jb);
+                    gj.push_back(jb);
+                } catch (const Error& e) {
+                    q->st.code = e.code;
+                    q->st.msg = e.what();
+                }
             }
-            q->grad.assign(q->dks.size(), 0.0);
+            off += 2 * n;
+        }
+        if (!gj.empty()) {
             try {
-                hodlr_stoch_gradient(*q->H, q->dks.data(), (int)q->dks.size(), q->y.data(), q->saa.data(), q->nv,
-                                     q->grad.data());
+                hodlr_stoch_gradient_batch(c, gj);
             } catch (const Error& e) {
-                q->st.code = e.code;
-                q->st.msg = e.what();
+                for (size_t i = 0; i < okr.size(); ++i)
+                    if (!okr[i]->dks.empty() && okr[i]->st.code == KMH_OK) {
+                        okr[i]->st.code = e.code;
+                        okr[i]->st.msg = e.what();
+                    }
             }
         }
+        c.scratch_free(buf);
     }
     for (Req* q : rs) q->stage = ST_DONE;
 }
@@ -304,7 +322,8 @@ int kmh_server_submit(kmh_server* srv, const kmh_serve_req* rq, int64_t* id) {
         q->opts = rq->opts;
         q->nv = rq->ndk ? (int)rq->nv : 0;
         if (rq->ndk) {
-            q->saa.assign(rq->saa, rq->saa + (size_t)rq->n * rq->nv);
+            if (rq->on_device) q->dsaa = rq->saa;
+            else q->saa.assign(rq->saa, rq->saa + (size_t)rq->n * rq->nv);
             for (int j = 0; j < rq->ndk; ++j) {
                 kmh::check_kernel(rq->dks[j], true);
                 q->dks.push_back(rq->dks[j]);

@@ -21,6 +21,7 @@ KMH_ACA, KMH_NYSTROM = 0, 1
 KMH_FIXED_LEVEL, KMH_LOG_LEVEL = 0, 1
 KMH_FP64 = 0
 KMH_W, KMH_WT, KMH_WINV, KMH_WTINV = 0, 1, 2, 3
+KMH_PROF_GEMM, KMH_PROF_NN_WIDE, KMH_PROF_TN_TALL = 0, 1, 2
 
 
 class kmh_kernel(C.Structure):
@@ -74,6 +75,7 @@ SIGNATURES = {
     "kmh_ctx_bytes_in_use": [_P, C.POINTER(C.c_uint64)],
     "kmh_ctx_profile_begin": [_P],
     "kmh_ctx_profile_end": [_P, _D, _D, C.POINTER(C.c_int64)],
+    "kmh_ctx_profile_query": [_P, C.c_int32, _D, _D, C.POINTER(C.c_int64)],
     "kmh_hodlr_build": [_P, _P, _I64, C.c_int32, C.POINTER(kmh_kernel), C.POINTER(kmh_build_opts), C.POINTER(_P)],
     "kmh_hodlr_build_dev": [_P, _P, _I64, C.c_int32, C.POINTER(kmh_kernel), C.POINTER(kmh_build_opts), C.POINTER(_P)],
     "kmh_hodlr_destroy": [_P],
@@ -91,6 +93,7 @@ SIGNATURES = {
     "kmh_hodlr_solve_dev": [_P, _P, _P, _I64],
     "kmh_hodlr_logdet": [_P, _D],
     "kmh_stoch_gradient": [_P, C.POINTER(kmh_kernel), C.c_int32, _P, _P, _I64, _P],
+    "kmh_stoch_gradient_dev": [_P, C.POINTER(kmh_kernel), C.c_int32, _P, _P, _I64, _P],
     "kmh_hodlr_build_batch": [_P, C.c_int32, C.POINTER(kmh_build_req), C.POINTER(_P), C.POINTER(C.c_int32)],
     "kmh_hodlr_factorize_batch": [_P, C.c_int32, C.POINTER(_P), C.POINTER(C.c_int32)],
     "kmh_hodlr_solve_batch_dev": [_P, C.c_int32, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int64)],

@@ -420,10 +420,21 @@ def negloglik(HK, y):
     return 0.5 * HK.logdet() + 0.5 * float(np.dot(y, HK.solve(y)))
 
 
+def _is_device_tensor(x):
+    """A float64 torch tensor in GPU memory (torch is imported lazily)."""
+    mod = type(x).__module__
+    if not mod.startswith("torch"):
+        return False
+    import torch
+    return isinstance(x, torch.Tensor) and x.is_cuda and x.dtype == torch.float64
+
+
 def stoch_gradient(K, HK, dat, dfuns, vecs, plel=False, verbose=False, shuffle=False):
     """HODLR.stoch_gradient (stochasticgradient.jl:30-45).  dfuns are
     derivative kernels (Kernel.derivative); vecs a list of n-vectors or an
-    n x nv array of SAA vectors."""
+    n x nv array of SAA vectors.  If dat and vecs are float64 torch tensors
+    on the GPU (vecs n x nv, column-major i.e. vecs.t() contiguous), they are
+    used in place (kmh_stoch_gradient_dev)."""
     if not HK.is_factorized:
         raise ErrorException("The matrix needs to be factorized for this to work.")
     if len(dfuns) != len(K.parms):
@@ -431,6 +442,18 @@ def stoch_gradient(K, HK, dat, dfuns, vecs, plel=False, verbose=False, shuffle=F
     if shuffle:
         raise ErrorException("shuffle=true is not supported (the reference's path references an undefined opts)")
     n = HK.n
+    arr = (L.kmh_kernel * len(dfuns))()
+    for i, dk in enumerate(dfuns):
+        arr[i] = dk.desc(K.parms)
+    g = np.zeros(len(dfuns))
+    if _is_device_tensor(dat) and _is_device_tensor(vecs):
+        if vecs.dim() != 2 or vecs.shape[0] != n or vecs.stride(0) != 1 or vecs.stride(1) != n:
+            raise ErrorException("Incorrect SAA Vector size.")
+        if dat.dim() != 1 or dat.shape[0] != n or not dat.is_contiguous():
+            raise ErrorException("data vector must be a contiguous n-vector")
+        _check(HK.lib, HK.lib.kmh_stoch_gradient_dev(HK.ptr, arr, len(dfuns), dat.data_ptr(), vecs.data_ptr(),
+                                                     vecs.shape[1], g.ctypes.data))
+        return g
     if isinstance(vecs, np.ndarray) and vecs.ndim == 2:
         saa = np.asfortranarray(vecs, dtype=np.float64)
     else:
@@ -438,10 +461,6 @@ def stoch_gradient(K, HK, dat, dfuns, vecs, plel=False, verbose=False, shuffle=F
     if saa.shape[0] != n:
         raise ErrorException("Incorrect SAA Vector size.")
     y = _f64(dat)
-    arr = (L.kmh_kernel * len(dfuns))()
-    for i, dk in enumerate(dfuns):
-        arr[i] = dk.desc(K.parms)
-    g = np.zeros(len(dfuns))
     _check(HK.lib, HK.lib.kmh_stoch_gradient(HK.ptr, arr, len(dfuns), y.ctypes.data, saa.ctypes.data,
                                              saa.shape[1], g.ctypes.data))
     return g
