@@ -13,16 +13,10 @@
 
 namespace kmh {
 
-__global__ __launch_bounds__(256) void k_eval(const EvalDesc* __restrict__ ds, int nd) {
+__global__ __launch_bounds__(256) void k_eval(const EvalDesc* __restrict__ ds, const int* __restrict__ map) {
     __shared__ double cx[64], cy[64];
-    int lo = 0, hi = nd - 1;
     const int t = blockIdx.x;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (ds[mid].tile0 <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    const EvalDesc d = ds[lo];
+    const EvalDesc d = ds[map[t]];
     const KDev k = d.k;
     const int lt = t - d.tile0;
     const int tn = (d.n + 63) / 64;
@@ -53,15 +47,19 @@ void eval_blocks(Ctx& c, const kmh_kernel& kern, std::vector<EvalDesc> ds) {
 void eval_blocks(Ctx& c, std::vector<EvalDesc> ds) {
     int tiles = 0;
     std::vector<EvalDesc> act;
+    std::vector<int> map;
     for (auto d : ds) {
         if (d.m <= 0 || d.n <= 0) continue;
         d.tile0 = tiles;
-        tiles += (int)(ceil_div(d.m, 64) * ceil_div(d.n, 64));
+        const int nt = (int)(ceil_div(d.m, 64) * ceil_div(d.n, 64));
+        map_append(map, (int)act.size(), nt);
+        tiles += nt;
         act.push_back(d);
     }
     if (act.empty()) return;
     EvalDesc* dd = c.upload_vec(act);
-    KMH_LAUNCH(c, k_eval, dim3(tiles), dim3(256), dd, (int)act.size());
+    const int* dm = c.upload_vec(map);
+    KMH_LAUNCH(c, k_eval, dim3(tiles), dim3(256), dd, dm);
 }
 
 __device__ __forceinline__ bool bit_get(const unsigned* m, int i) { return (m[i >> 5] >> (i & 31)) & 1u; }

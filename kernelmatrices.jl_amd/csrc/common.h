@@ -166,6 +166,13 @@ void forget_queued(Ctx* c) noexcept;
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Grouped launches of uneven problems: workgroup b works on problem map[b]
+// (problems own consecutive workgroups), so a kernel finds its problem with
+// one load instead of a search over the descriptors.
+inline void map_append(std::vector<int>& map, int problem, int64_t count) {
+    map.insert(map.end(), (size_t)count, problem);
+}
+
 }  // namespace kmh
 
 // Opaque C-ABI context handle.

@@ -54,18 +54,12 @@ __device__ __forceinline__ void gemm_stage_idx(int tid, int q, int& i, int& kk) 
 }
 
 template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ ds, int nd) {
+__global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ ds, const int* __restrict__ map) {
     constexpr int NQ = GT * GK / 256;
     __shared__ double As[GK][GT + 1];
     __shared__ double Bs[GK][GT + 1];
     const int t = blockIdx.x;
-    int lo = 0, hi = nd - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (ds[mid].tile0 <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    const GemmDesc d = ds[lo];
+    const GemmDesc d = ds[map[t]];
     const int lt = t - d.tile0;
     const int row0 = (lt / d.tn) * GT, col0 = (lt % d.tn) * GT;
     if (d.lower && col0 > row0 + GT - 1) return;
@@ -164,21 +158,14 @@ struct RedDesc {
     const double* W;
     int ldc, m, n, s;
     double alpha, beta;
-    long long elem0;
+    long long elem0;   // first workgroup of the problem in the reduction launch
 };
 
-__global__ __launch_bounds__(256) void k_splitk_reduce(const RedDesc* __restrict__ ds, int nd,
-                                                       long long total) {
-    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= total) return;
-    int lo = 0, hi = nd - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (ds[mid].elem0 <= e) lo = mid;
-        else hi = mid - 1;
-    }
-    const RedDesc d = ds[lo];
-    const long long loc = e - d.elem0;
+// elem0 is the problem's first workgroup; each workgroup sums 256 elements
+__global__ __launch_bounds__(256) void k_splitk_reduce(const RedDesc* __restrict__ ds, const int* __restrict__ map) {
+    const RedDesc d = ds[map[blockIdx.x]];
+    const long long loc = ((long long)blockIdx.x - d.elem0) * 256 + threadIdx.x;
+    if (loc >= (long long)d.m * d.n) return;
     const int i = (int)(loc % d.m), j = (int)(loc / d.m);
     const size_t mn = (size_t)d.m * d.n;
     double s = 0.0;
@@ -282,16 +269,11 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn_tall(const GemmDesc* __restr
 constexpr int WN_MAX = 160;
 constexpr int WN_KC = 16;
 
-__global__ __launch_bounds__(256, 2) void k_gemm_nn_wide(const GemmDesc* __restrict__ ds, int nd) {
+__global__ __launch_bounds__(256, 2) void k_gemm_nn_wide(const GemmDesc* __restrict__ ds,
+                                                        const int* __restrict__ map) {
     __shared__ double Bs[WN_KC][WN_MAX + 1];
     const int t = blockIdx.x;
-    int lo = 0, hi = nd - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (ds[mid].tile0 <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    const GemmDesc d = ds[lo];
+    const GemmDesc d = ds[map[t]];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int NB = (d.n + 15) >> 4;
     const int arow = (t - d.tile0) * 64 + wave * 16 + (lane & 15);
@@ -390,15 +372,18 @@ static void gemm_generic(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& 
     }
     double* W = wsz ? static_cast<double*>(c.scratch(wsz * sizeof(double))) : nullptr;
     std::vector<GemmDesc> ds;
+    std::vector<int> map;
     int tiles = 0;
     std::vector<RedDesc> rs;
+    std::vector<int> rmap;
     size_t woff = 0;
-    long long relem = 0;
+    long long rblk = 0;
     for (size_t i = 0; i < probs.size(); ++i) {
         const auto& p = probs[i];
         const int tn = (int)ceil_div(p.n, GT);
         const int nt = (int)ceil_div(p.m, GT) * tn;
         if (split[i] == 1) {
+            map_append(map, (int)ds.size(), nt);
             ds.push_back({p.A, p.B, p.C, p.m, p.n, p.k, p.lda, p.ldb, p.ldc, p.alpha, p.beta,
                           tiles, tn, p.lower, 0});
             tiles += nt;
@@ -410,28 +395,31 @@ static void gemm_generic(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& 
                 const int kk = std::min(kc, p.k - k0);
                 const double* A = ta ? p.A + k0 : p.A + (size_t)k0 * p.lda;
                 const double* B = tb ? p.B + (size_t)k0 * p.ldb : p.B + k0;
+                map_append(map, (int)ds.size(), nt);
                 ds.push_back({A, B, W + woff + (size_t)q * p.m * p.n, p.m, p.n, kk, p.lda, p.ldb,
                               p.m, 1.0, 0.0, tiles, tn, 0, 0});
                 tiles += nt;
             }
-            rs.push_back({p.C, W + woff, p.ldc, p.m, p.n, s, p.alpha, p.beta, relem});
+            const long long nb = ceil_div((long long)p.m * p.n, 256);
+            map_append(rmap, (int)rs.size(), nb);
+            rs.push_back({p.C, W + woff, p.ldc, p.m, p.n, s, p.alpha, p.beta, rblk});
             woff += (size_t)s * p.m * p.n;
-            relem += (long long)p.m * p.n;
+            rblk += nb;
         }
     }
     if (!ds.empty()) {
         GemmDesc* dd = c.upload_vec(ds);
-        const int nd = (int)ds.size();
+        const int* dm = c.upload_vec(map);
         const dim3 g(tiles);
-        if (!ta && !tb) KMH_LAUNCH(c, (k_gemm<false, false>), g, dim3(256), dd, nd);
-        else if (!ta && tb) KMH_LAUNCH(c, (k_gemm<false, true>), g, dim3(256), dd, nd);
-        else if (ta && !tb) KMH_LAUNCH(c, (k_gemm<true, false>), g, dim3(256), dd, nd);
-        else KMH_LAUNCH(c, (k_gemm<true, true>), g, dim3(256), dd, nd);
+        if (!ta && !tb) KMH_LAUNCH(c, (k_gemm<false, false>), g, dim3(256), dd, dm);
+        else if (!ta && tb) KMH_LAUNCH(c, (k_gemm<false, true>), g, dim3(256), dd, dm);
+        else if (ta && !tb) KMH_LAUNCH(c, (k_gemm<true, false>), g, dim3(256), dd, dm);
+        else KMH_LAUNCH(c, (k_gemm<true, true>), g, dim3(256), dd, dm);
     }
     if (!rs.empty()) {
         RedDesc* dr = c.upload_vec(rs);
-        KMH_LAUNCH(c, k_splitk_reduce, dim3((unsigned)ceil_div(relem, 256)), dim3(256),
-                   dr, (int)rs.size(), relem);
+        const int* drm = c.upload_vec(rmap);
+        KMH_LAUNCH(c, k_splitk_reduce, dim3((unsigned)rblk), dim3(256), dr, drm);
     }
     if (W) c.scratch_free(W);
 }
@@ -445,8 +433,9 @@ static void gemm_tall_tn(Ctx& c, const std::vector<GemmProb>& probs) {
     double* W = wsz ? static_cast<double*>(c.scratch(wsz * sizeof(double))) : nullptr;
     std::vector<GemmDesc> ds;
     std::vector<RedDesc> rs;
+    std::vector<int> rmap;
     size_t woff = 0;
-    long long relem = 0;
+    long long rblk = 0;
     for (const auto& p : probs) {
         const int s = (int)ceil_div(p.k, TT_KS);
         if (s <= 1) {
@@ -458,9 +447,11 @@ static void gemm_tall_tn(Ctx& c, const std::vector<GemmProb>& probs) {
             ds.push_back({p.A + k0, p.B + k0, W + woff + (size_t)q * p.m * p.n, p.m, p.n,
                           std::min(TT_KS, p.k - k0), p.lda, p.ldb, p.m, 1.0, 0.0, 0, 0, 0, 0});
         }
-        rs.push_back({p.C, W + woff, p.ldc, p.m, p.n, s, p.alpha, p.beta, relem});
+        const long long nb = ceil_div((long long)p.m * p.n, 256);
+        map_append(rmap, (int)rs.size(), nb);
+        rs.push_back({p.C, W + woff, p.ldc, p.m, p.n, s, p.alpha, p.beta, rblk});
         woff += (size_t)s * p.m * p.n;
-        relem += (long long)p.m * p.n;
+        rblk += nb;
     }
     GemmDesc* dd = c.upload_vec(ds);
     size_t span = 0;
@@ -474,20 +465,24 @@ static void gemm_tall_tn(Ctx& c, const std::vector<GemmProb>& probs) {
     if (c.prof) c.prof_end_span(span);
     if (!rs.empty()) {
         RedDesc* dr = c.upload_vec(rs);
-        KMH_LAUNCH(c, k_splitk_reduce, dim3((unsigned)ceil_div(relem, 256)), dim3(256),
-                   dr, (int)rs.size(), relem);
+        const int* drm = c.upload_vec(rmap);
+        KMH_LAUNCH(c, k_splitk_reduce, dim3((unsigned)rblk), dim3(256), dr, drm);
     }
     if (W) c.scratch_free(W);
 }
 
 static void gemm_wide_nn(Ctx& c, const std::vector<GemmProb>& probs) {
     std::vector<GemmDesc> ds;
+    std::vector<int> map;
     int tiles = 0;
     for (const auto& p : probs) {
+        const int nt = (int)ceil_div(p.m, 64);
+        map_append(map, (int)ds.size(), nt);
         ds.push_back({p.A, p.B, p.C, p.m, p.n, p.k, p.lda, p.ldb, p.ldc, p.alpha, p.beta, tiles, 0, 0, 0});
-        tiles += (int)ceil_div(p.m, 64);
+        tiles += nt;
     }
     GemmDesc* dd = c.upload_vec(ds);
+    const int* dm = c.upload_vec(map);
     size_t span = 0;
     if (c.prof) {
         double bytes = 0.0;
@@ -495,7 +490,7 @@ static void gemm_wide_nn(Ctx& c, const std::vector<GemmProb>& probs) {
             bytes += 8.0 * ((double)p.m * p.k + (double)p.k * p.n + (double)p.m * p.n * (p.beta != 0.0 ? 2 : 1));
         span = c.prof_begin_span(KMH_PROF_NN_WIDE, bytes);
     }
-    KMH_LAUNCH(c, k_gemm_nn_wide, dim3(tiles), dim3(256), dd, (int)ds.size());
+    KMH_LAUNCH(c, k_gemm_nn_wide, dim3(tiles), dim3(256), dd, dm);
     if (c.prof) c.prof_end_span(span);
 }
 
@@ -583,16 +578,10 @@ struct LdiagDesc {
 };
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_trsm_diag(const LdiagDesc* __restrict__ ds, int nd) {
+__global__ __launch_bounds__(256) void k_trsm_diag(const LdiagDesc* __restrict__ ds, const int* __restrict__ map) {
     __shared__ double Ls[PB][PB + 1];   // Ls[i][l] = L(i, l), identity padding
     __shared__ double Xs[PB][PB + 1];   // Xs[i][c]: tile row i, right-hand c
-    int lo = 0, hi = nd - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (ds[mid].chunk0 <= (int)blockIdx.x) lo = mid;
-        else hi = mid - 1;
-    }
-    const LdiagDesc d = ds[lo];
+    const LdiagDesc d = ds[map[blockIdx.x]];
     const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
     const int c0 = ((int)blockIdx.x - d.chunk0) * PB;
     const int nc = min(PB, d.ncols - c0);
@@ -695,13 +684,18 @@ __global__ __launch_bounds__(256) void k_zero_upper(const SqMat* __restrict__ ms
 
 static void launch_trsm_diag(Ctx& c, int mode, const std::vector<LdiagDesc>& ds, int chunks) {
     if (ds.empty()) return;
+    std::vector<int> map;
+    map.reserve(chunks);
+    for (size_t i = 0; i < ds.size(); ++i)
+        map_append(map, (int)i, (i + 1 < ds.size() ? ds[i + 1].chunk0 : chunks) - ds[i].chunk0);
     LdiagDesc* dd = c.upload_vec(ds);
+    const int* dm = c.upload_vec(map);
     if (mode == 0)
-        KMH_LAUNCH(c, k_trsm_diag<0>, dim3(chunks), dim3(256), dd, (int)ds.size());
+        KMH_LAUNCH(c, k_trsm_diag<0>, dim3(chunks), dim3(256), dd, dm);
     else if (mode == 1)
-        KMH_LAUNCH(c, k_trsm_diag<1>, dim3(chunks), dim3(256), dd, (int)ds.size());
+        KMH_LAUNCH(c, k_trsm_diag<1>, dim3(chunks), dim3(256), dd, dm);
     else
-        KMH_LAUNCH(c, k_trsm_diag<2>, dim3(chunks), dim3(256), dd, (int)ds.size());
+        KMH_LAUNCH(c, k_trsm_diag<2>, dim3(chunks), dim3(256), dd, dm);
 }
 
 void potrf(Ctx& c, const std::vector<SqMat>& mats) {
