@@ -114,8 +114,10 @@ int kmh_ctx_bytes_in_use(kmh_ctx* ctx, uint64_t* out);
  *   KMH_PROF_GEMM     grouped GEMM calls, amount = flops
  *   KMH_PROF_NN_WIDE  k_gemm_nn_wide launches, amount = algorithmic HBM bytes
  *                     (A, B and C written, plus C read when beta != 0)
- *   KMH_PROF_TN_TALL  k_gemm_tn_tall launches, amount = bytes of A, B, C     */
-enum { KMH_PROF_GEMM = 0, KMH_PROF_NN_WIDE = 1, KMH_PROF_TN_TALL = 2, KMH_PROF_TAGS = 3 };
+ *   KMH_PROF_TN_TALL  k_gemm_tn_tall launches, amount = bytes of A, B, C
+ *   KMH_PROF_NN_ROWS  k_gemm_nn_rows launches (one span per grouped call),
+ *                     amount = bytes as for KMH_PROF_NN_WIDE               */
+enum { KMH_PROF_GEMM = 0, KMH_PROF_NN_WIDE = 1, KMH_PROF_TN_TALL = 2, KMH_PROF_NN_ROWS = 3, KMH_PROF_TAGS = 4 };
 int kmh_ctx_profile_begin(kmh_ctx* ctx);
 int kmh_ctx_profile_end(kmh_ctx* ctx, double* gemm_flops, double* gemm_ms, int64_t* gemm_launches);
 int kmh_ctx_profile_query(kmh_ctx* ctx, int32_t tag, double* amount, double* ms, int64_t* spans);

@@ -342,11 +342,63 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn_wide(const GemmDesc* __restr
     }
 }
 
+// Narrow NN products C = alpha A B + beta C with n <= 32 and k <= 64 (the
+// panel QR updates X -= Q R and the TSQR assembly Q <- Q C): one thread per
+// row of C, B in LDS (broadcast reads), the row of A streamed once with
+// coalesced loads and n accumulators in registers.  The multiply-adds run
+// in k order for each element.  A thread reads its whole row of A before it
+// writes its row of C, so C may alias A.
+constexpr int NR_K = 64;
+
+template <int NMAX>
+__global__ __launch_bounds__(256) void k_gemm_nn_rows(const GemmDesc* __restrict__ ds, const int* __restrict__ map) {
+    __shared__ double Bs[NR_K * NMAX];
+    const GemmDesc d = ds[map[blockIdx.x]];
+    for (int e = threadIdx.x; e < NR_K * NMAX; e += 256) {
+        const int kk = e / NMAX, j = e - kk * NMAX;
+        Bs[e] = (kk < d.k && j < d.n) ? d.B[kk + (size_t)j * d.ldb] : 0.0;
+    }
+    __syncthreads();
+    const int row = (blockIdx.x - d.tile0) * 256 + threadIdx.x;
+    if (row >= d.m) return;
+    double acc[NMAX];
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) acc[j] = 0.0;
+    const double* ap = d.A + row;
+    for (int k0 = 0; k0 < d.k; k0 += 8) {
+        double a[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) a[q] = (k0 + q < d.k) ? ap[(size_t)(k0 + q) * d.lda] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const double* b = Bs + (k0 + q) * NMAX;
+#pragma unroll
+            for (int j = 0; j < NMAX; ++j) acc[j] = __fma_rn(a[q], b[j], acc[j]);
+        }
+    }
+    double* cp = d.C + row;
+    if (d.beta != 0.0) {
+        double cv[NMAX];
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j) cv[j] = j < d.n ? cp[(size_t)j * d.ldc] : 0.0;
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j)
+            if (j < d.n) cp[(size_t)j * d.ldc] = d.alpha * acc[j] + d.beta * cv[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j)
+            if (j < d.n) cp[(size_t)j * d.ldc] = d.alpha * acc[j];
+    }
+}
+
 // Which kernel takes a problem, and how it is split, depends on the problem's
 // own shape only, so a problem gives bitwise the same result whatever else
 // shares the launch.
 static bool tall_tn(bool ta, bool tb, const GemmProb& p) {
     return ta && !tb && !p.lower && p.m <= TT_MAX && p.n <= TT_MAX;
+}
+static bool rows_nn(bool ta, bool tb, const GemmProb& p) {
+    return !ta && !tb && !p.lower && p.n <= 32 && p.k <= NR_K;
 }
 static bool wide_nn(bool ta, bool tb, const GemmProb& p) {
     return !ta && !tb && !p.lower && p.n <= WN_MAX && (p.k <= 512 || p.m >= 64 * 64);
@@ -494,21 +546,49 @@ static void gemm_wide_nn(Ctx& c, const std::vector<GemmProb>& probs) {
     if (c.prof) c.prof_end_span(span);
 }
 
+static void gemm_rows_nn(Ctx& c, const std::vector<GemmProb>& probs) {
+    // one launch per width class (8, 16, 32)
+    std::vector<GemmDesc> ds[3];
+    std::vector<int> map[3];
+    int blocks[3] = {0, 0, 0};
+    double bytes = 0.0;
+    for (const auto& p : probs) {
+        const int cl = p.n <= 8 ? 0 : p.n <= 16 ? 1 : 2;
+        const int nb = (int)ceil_div(p.m, 256);
+        map_append(map[cl], (int)ds[cl].size(), nb);
+        ds[cl].push_back({p.A, p.B, p.C, p.m, p.n, p.k, p.lda, p.ldb, p.ldc, p.alpha, p.beta, blocks[cl], 0, 0, 0});
+        blocks[cl] += nb;
+        bytes += 8.0 * ((double)p.m * p.k + (double)p.k * p.n + (double)p.m * p.n * (p.beta != 0.0 ? 2 : 1));
+    }
+    const size_t span = c.prof ? c.prof_begin_span(KMH_PROF_NN_ROWS, bytes) : 0;
+    for (int cl = 0; cl < 3; ++cl) {
+        if (ds[cl].empty()) continue;
+        GemmDesc* dd = c.upload_vec(ds[cl]);
+        const int* dm = c.upload_vec(map[cl]);
+        if (cl == 0) KMH_LAUNCH(c, k_gemm_nn_rows<8>, dim3(blocks[cl]), dim3(256), dd, dm);
+        else if (cl == 1) KMH_LAUNCH(c, k_gemm_nn_rows<16>, dim3(blocks[cl]), dim3(256), dd, dm);
+        else KMH_LAUNCH(c, k_gemm_nn_rows<32>, dim3(blocks[cl]), dim3(256), dd, dm);
+    }
+    if (c.prof) c.prof_end_span(span);
+}
+
 void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
-    std::vector<GemmProb> gen, tall, wide;
+    std::vector<GemmProb> gen, tall, wide, rows;
     double fl = 0.0;
     for (const auto& p : probs) {
         if (p.m <= 0 || p.n <= 0) continue;
         fl += 2.0 * p.m * p.n * (double)p.k;
         if (tall_tn(ta, tb, p)) tall.push_back(p);
+        else if (rows_nn(ta, tb, p)) rows.push_back(p);
         else if (wide_nn(ta, tb, p)) wide.push_back(p);
         else gen.push_back(p);
     }
-    if (gen.empty() && tall.empty() && wide.empty()) return;
+    if (gen.empty() && tall.empty() && wide.empty() && rows.empty()) return;
     const size_t span = c.prof ? c.prof_begin_span(KMH_PROF_GEMM, fl) : 0;
     if (!gen.empty()) gemm_generic(c, ta, tb, gen);
     if (!tall.empty()) gemm_tall_tn(c, tall);
     if (!wide.empty()) gemm_wide_nn(c, wide);
+    if (!rows.empty()) gemm_rows_nn(c, rows);
     if (c.prof) c.prof_end_span(span);
 }
 

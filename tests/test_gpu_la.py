@@ -34,7 +34,9 @@ def _gemm(ctx, ta, tb, A, B, Cm, alpha=1.0, beta=0.0, lower=0):
                                    # tall A^T B (k split in 256-row slabs) and wide A B shapes
                                    (72, 72, 256), (72, 72, 257), (80, 80, 4097), (32, 8, 32768),
                                    (80, 1, 513), (1, 80, 255), (81, 80, 300),
-                                   (4097, 144, 72), (8192, 8, 64), (4096, 16, 600), (63, 32, 17)])
+                                   (4097, 144, 72), (8192, 8, 64), (4096, 16, 600), (63, 32, 17),
+                                   # row-per-thread narrow A B shapes
+                                   (300, 32, 64), (5000, 16, 33), (77, 8, 8), (257, 9, 1)])
 def test_gemm_matches_numpy(ctx, ta, tb, m, n, k):
     print("gemm", ta, tb, m, n, k, flush=True)
     rng = np.random.default_rng(m * 1000 + n * 10 + k)
