@@ -26,6 +26,10 @@ int kmh_test_trsm(kmh_ctx* ctx, int32_t trans, int32_t n, const double* Lm, int3
                   int32_t ncols, double* B, int32_t ldb);
 /* thin QR of an m x r panel: A <- Q, R (r x r) */
 int kmh_test_qr(kmh_ctx* ctx, int32_t m, int32_t r, double* A, int32_t lda, double* R, int32_t ldr);
+/* descriptor-arena ring: `count` uploads of `words` 64-bit words, each read
+ * back by a kernel that checks its contents; *bad = uploads whose kernel saw
+ * other data (0 when the ring's fences hold) */
+int kmh_test_arena(kmh_ctx* ctx, int32_t count, int32_t words, int64_t* bad);
 
 #ifdef __cplusplus
 }

@@ -82,6 +82,8 @@ int kmh_ctx_destroy(kmh_ctx* ctx) {
         c.release_pool();
         kmh::forget_queued(&c);
         for (auto e : c.ev_pool) (void)hipEventDestroy(e);
+        for (auto& f : c.seg_fence)
+            if (f.ev) (void)hipEventDestroy(f.ev);
         if (c.arena) (void)hipFree(c.arena);
         if (c.staging) (void)hipHostFree(c.staging);
         if (c.d_info) (void)hipFree(c.d_info);

@@ -69,6 +69,13 @@ struct Ctx {
     size_t arena_size = 0;
     size_t arena_off = 0;
     size_t dirty_lo = 0, dirty_hi = 0;
+    static constexpr int ARENA_SEGS = 4;
+    struct Fence {
+        hipEvent_t ev = nullptr;
+        bool pending = false;
+    };
+    Fence seg_fence[ARENA_SEGS];
+    int arena_seg = 0;   // segment arena_off lies in (arena_off may equal its end)
     // Launch queue: kernels are recorded and issued, in order and after the
     // descriptor copy, at the next flush.  Every direct operation on the
     // stream goes through q(), which flushes first, so stream order is the

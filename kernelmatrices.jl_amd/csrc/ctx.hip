@@ -43,15 +43,19 @@ void DevBuf::release() {
     p = nullptr; bytes = 0; ctx = nullptr;
 }
 
-// The staging mirror and the arena wrap together; wrapping issues every
-// queued launch and waits for the stream, so neither a pending copy nor a
-// kernel reading older descriptors can be overtaken.
+// The staging mirror and the arena are a ring of ARENA_SEGS segments.
+// Leaving a segment issues every queued launch and records an event after
+// them; entering a segment again waits for its event, i.e. for the copies
+// and kernels that read the segment's previous contents (issued at least a
+// whole ring of uploads earlier, so the wait rarely blocks).
 void* Ctx::upload(const void* host, size_t bytes) {
     const size_t need = (bytes + 255) & ~size_t(255);
-    if (need > arena_size / 4) {
+    const size_t seg = arena_size / ARENA_SEGS;
+    if (need > seg) {
         // grow (rare): drain the stream, reallocate both sides
         sync();
-        size_t ns = std::max<size_t>(arena_size * 2, need * 8);
+        size_t ns = std::max<size_t>(arena_size * 2, need * ARENA_SEGS * 2);
+        ns = (ns + 255 * ARENA_SEGS) / (256 * ARENA_SEGS) * (256 * ARENA_SEGS);
         if (arena) KMH_HIP(hipFree(arena));
         if (staging) KMH_HIP(hipHostFree(staging));
         arena = nullptr;
@@ -60,10 +64,25 @@ void* Ctx::upload(const void* host, size_t bytes) {
         KMH_HIP(hipHostMalloc(&staging, ns, hipHostMallocDefault));
         arena_size = ns;
         arena_off = dirty_lo = dirty_hi = 0;
+        arena_seg = 0;
+        for (auto& f : seg_fence) f.pending = false;
+        return upload(host, bytes);
     }
-    if (arena_off + need > arena_size) {
-        sync();
-        arena_off = dirty_lo = dirty_hi = 0;
+    const int cur = arena_seg;
+    if (arena_off + need > (size_t)(cur + 1) * seg) {
+        flush();
+        Fence& leave = seg_fence[cur];
+        if (!leave.ev) KMH_HIP(hipEventCreateWithFlags(&leave.ev, hipEventDisableTiming));
+        KMH_HIP(hipEventRecord(leave.ev, stream));
+        leave.pending = true;
+        const int nxt = (cur + 1) % ARENA_SEGS;
+        Fence& enter = seg_fence[nxt];
+        if (enter.pending) {
+            KMH_HIP(hipEventSynchronize(enter.ev));
+            enter.pending = false;
+        }
+        arena_seg = nxt;
+        arena_off = dirty_lo = dirty_hi = (size_t)nxt * seg;
     }
     if (bytes) std::memcpy(staging + arena_off, host, bytes);
     if (dirty_hi == dirty_lo) dirty_lo = arena_off;

@@ -68,22 +68,40 @@ void check_kernel(const kmh_kernel& k, bool allow_deriv) {
     if (k.family != KMH_CAUCHY) KMH_CHECK(k.params[1] != 0.0, KMH_E_BAD_ARG, "kernel length scale must be nonzero");
 }
 
-// Read the per-handle Cholesky flags (one sync) and mark failures.
-static void collect_info(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<Status>& st, const char* what) {
-    std::vector<int> flags(hs.size(), 0);
+// Read the per-handle Cholesky flags (one copy, one sync) and mark failures;
+// extra doubles (e.g. per-handle sums) can ride along in the same copy.
+static void collect_info(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<Status>& st, const char* what,
+                         const std::vector<const double*>& extra = {}, double* extra_out = nullptr) {
+    std::vector<const int*> fp;
+    std::vector<size_t> at;
     for (size_t i = 0; i < hs.size(); ++i)
-        if (hs[i]) KMH_HIP(hipMemcpyAsync(&flags[i], hs[i]->info.p, sizeof(int), hipMemcpyDeviceToHost, c.q()));
+        if (hs[i]) { fp.push_back(hs[i]->info.as<int>()); at.push_back(i); }
+    const size_t nd = extra.size(), ni = fp.size();
+    if (ni + nd == 0) return;
+    char* buf = static_cast<char*>(c.scratch(nd * sizeof(double) + ni * sizeof(int)));
+    gather_doubles(c, extra, reinterpret_cast<double*>(buf));
+    gather_ints(c, fp, reinterpret_cast<int*>(buf + nd * sizeof(double)));
+    std::vector<char> h(nd * sizeof(double) + ni * sizeof(int));
+    KMH_HIP(hipMemcpyAsync(h.data(), buf, h.size(), hipMemcpyDeviceToHost, c.q()));
     c.sync();
-    for (size_t i = 0; i < hs.size(); ++i)
-        if (hs[i] && flags[i] && st[i].code == KMH_OK) {
+    c.scratch_free(buf);
+    if (nd) std::memcpy(extra_out, h.data(), nd * sizeof(double));
+    for (size_t q = 0; q < ni; ++q) {
+        int flag;
+        std::memcpy(&flag, h.data() + nd * sizeof(double) + q * sizeof(int), sizeof(int));
+        const size_t i = at[q];
+        if (flag && st[i].code == KMH_OK) {
             st[i].code = KMH_E_NOT_PD;
             st[i].msg = std::string("PosDefException: matrix is not positive definite; ") + what;
         }
+    }
 }
 
 static void reset_info(Ctx& c, const std::vector<Hodlr*>& hs) {
+    std::vector<int*> p;
     for (auto* h : hs)
-        if (h) KMH_HIP(hipMemsetAsync(h->info.p, 0, sizeof(int), c.q()));
+        if (h) p.push_back(h->info.as<int>());
+    zero_ints(c, p);
 }
 
 // ---------------------------------------------------------------------------
@@ -257,6 +275,7 @@ std::vector<Hodlr*> hodlr_build_batch(Ctx& c, const std::vector<BuildReq>& reqs,
     std::vector<Hodlr*> hs(R, nullptr);
     std::vector<std::vector<double>> soa(R);
     std::vector<Copy2D> dev_pts;
+    std::vector<int*> info0;
     for (size_t i = 0; i < R; ++i) {
         try {
             const BuildReq& q = reqs[i];
@@ -269,7 +288,6 @@ std::vector<Hodlr*> hodlr_build_batch(Ctx& c, const std::vector<BuildReq>& reqs,
             H.opts = q.opts;
             tree_indices(H);
             H.info.alloc(&c, 256);
-            KMH_HIP(hipMemsetAsync(H.info.p, 0, sizeof(int), c.q()));
             H.pts.alloc(&c, 2 * (size_t)q.n * sizeof(double));
             double* px = H.pts.as<double>();
             if (!q.on_device) {
@@ -293,12 +311,14 @@ std::vector<Hodlr*> hodlr_build_batch(Ctx& c, const std::vector<BuildReq>& reqs,
                 H.Rtot = 0;
             }
             hs[i] = own[i].get();
+            info0.push_back(H.info.as<int>());   // only handles that survive validation
         } catch (const Error& e) {
             st[i].code = e.code;
             st[i].msg = e.what();
             own[i].reset();
         }
     }
+    zero_ints(c, info0);
     copy2d(c, dev_pts, true);
     // landmark gathers (Nystrom)
     std::vector<Copy2D> g;
@@ -678,11 +698,11 @@ void factor_end_batch(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<Status>
     for (Hodlr* h : hs) sj.push_back({h->parts, h->nparts, h->parts + h->nparts});
     sum_vectors(c, sj);
     std::vector<double> sums(hs.size());
-    for (size_t i = 0; i < hs.size(); ++i)
-        KMH_HIP(hipMemcpyAsync(&sums[i], hs[i]->parts + hs[i]->nparts, sizeof(double), hipMemcpyDeviceToHost,
-                               c.q()));
+    std::vector<const double*> sp;
+    for (Hodlr* h : hs) sp.push_back(h->parts + h->nparts);
     st.assign(hs.size(), Status());
-    collect_info(c, hs, st, "Cholesky factorization failed.");
+    collect_info(c, hs, st, "Cholesky factorization failed.", sp, sums.data());
+    std::vector<int*> failed;
     for (size_t j = 0; j < hs.size(); ++j) {
         Hodlr& H = *hs[j];
         if (H.AB) c.scratch_free(H.AB);
@@ -690,12 +710,13 @@ void factor_end_batch(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<Status>
         H.AB = nullptr;
         H.parts = nullptr;
         if (st[j].code != KMH_OK) {
-            KMH_HIP(hipMemsetAsync(H.info.p, 0, sizeof(int), c.q()));
+            failed.push_back(H.info.as<int>());
         } else {
             H.logdet = 2.0 * sums[j];
             H.factorized = true;
         }
     }
+    zero_ints(c, failed);
 }
 
 void hodlr_factorize_batch(Ctx& c, const std::vector<Hodlr*>& hs_in, std::vector<Status>& st) {

@@ -73,6 +73,12 @@ struct Add2 {
     int64_t n;
 };
 void add2(Ctx& c, const std::vector<Add2>& probs);
+// d_out[i] = *src[i] for every pointer (one launch), e.g. per-handle flags or
+// sums read back with one copy.
+void gather_doubles(Ctx& c, const std::vector<const double*>& src, double* d_out);
+void gather_ints(Ctx& c, const std::vector<const int*>& src, int* d_out);
+// *p = 0 for every pointer (one launch).
+void zero_ints(Ctx& c, const std::vector<int*>& ptrs);
 // dst = identity (n x n) ; or dst(i,i) += v
 struct DiagOp {
     double* A; int n; int ld;

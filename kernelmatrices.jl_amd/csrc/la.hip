@@ -931,6 +931,37 @@ void add2(Ctx& c, const std::vector<Add2>& probs) {
     }
 }
 
+template <class T>
+__global__ __launch_bounds__(256) void k_gather(const T* const* __restrict__ src, T* __restrict__ dst, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = *src[i];
+}
+
+__global__ __launch_bounds__(256) void k_zero_ints(int* const* __restrict__ p, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) *p[i] = 0;
+}
+
+void gather_doubles(Ctx& c, const std::vector<const double*>& src, double* d_out) {
+    if (src.empty()) return;
+    const double* const* d = c.upload_vec(src);
+    KMH_LAUNCH(c, k_gather<double>, dim3((unsigned)ceil_div((int64_t)src.size(), 256)), dim3(256), d, d_out,
+               (int)src.size());
+}
+
+void gather_ints(Ctx& c, const std::vector<const int*>& src, int* d_out) {
+    if (src.empty()) return;
+    const int* const* d = c.upload_vec(src);
+    KMH_LAUNCH(c, k_gather<int>, dim3((unsigned)ceil_div((int64_t)src.size(), 256)), dim3(256), d, d_out,
+               (int)src.size());
+}
+
+void zero_ints(Ctx& c, const std::vector<int*>& ptrs) {
+    if (ptrs.empty()) return;
+    int* const* d = c.upload_vec(ptrs);
+    KMH_LAUNCH(c, k_zero_ints, dim3((unsigned)ceil_div((int64_t)ptrs.size(), 256)), dim3(256), d, (int)ptrs.size());
+}
+
 template <bool SET>
 __global__ __launch_bounds__(256) void k_diag_op(const DiagOp* __restrict__ ds) {
     const DiagOp d = ds[blockIdx.y];

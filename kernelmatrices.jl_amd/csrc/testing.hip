@@ -21,6 +21,15 @@ struct Dev {
     ~Dev() { c.scratch_free(p); }
 };
 
+// one workgroup checks the pattern word i = seed * 1000003 + i of its upload
+__global__ __launch_bounds__(256) void k_check_upload(const unsigned long long* __restrict__ d, int words,
+                                                      unsigned long long seed, unsigned long long* __restrict__ bad) {
+    int wrong = 0;
+    for (int i = threadIdx.x; i < words; i += 256) wrong |= d[i] != seed * 1000003ull + (unsigned long long)i;
+    wrong = __syncthreads_or(wrong);
+    if (threadIdx.x == 0 && wrong) atomicAdd(bad, 1ull);
+}
+
 }  // namespace
 
 #define TRY_(...) KMH_API_TRY(__VA_ARGS__)
@@ -80,6 +89,27 @@ int kmh_test_qr(kmh_ctx* ctx, int32_t m, int32_t r, double* A, int32_t lda, doub
         a.to_host(A, (size_t)lda * r);
         rr.to_host(R, (size_t)ldr * r);
         c.sync();
+    })
+}
+
+int kmh_test_arena(kmh_ctx* ctx, int32_t count, int32_t words, int64_t* bad) {
+    TRY_({
+        KMH_CHECK(ctx && bad && count >= 0 && words >= 1, KMH_E_BAD_ARG, "invalid arena test arguments");
+        kmh::Ctx& c = ctx->c;
+        c.set_device();
+        unsigned long long* d_bad = static_cast<unsigned long long*>(c.scratch(sizeof(unsigned long long)));
+        KMH_HIP(hipMemsetAsync(d_bad, 0, sizeof(unsigned long long), c.q()));
+        std::vector<unsigned long long> h(words);
+        for (int q = 0; q < count; ++q) {
+            for (int i = 0; i < words; ++i) h[i] = (unsigned long long)q * 1000003ull + (unsigned long long)i;
+            const unsigned long long* d = c.upload_vec(h);
+            KMH_LAUNCH(c, k_check_upload, dim3(1), dim3(256), d, words, (unsigned long long)q, d_bad);
+        }
+        unsigned long long hb = 0;
+        KMH_HIP(hipMemcpyAsync(&hb, d_bad, sizeof(hb), hipMemcpyDeviceToHost, c.q()));
+        c.sync();
+        c.scratch_free(d_bad);
+        *bad = (int64_t)hb;
     })
 }
 

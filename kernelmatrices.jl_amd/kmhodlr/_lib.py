@@ -117,6 +117,7 @@ TEST_SIGNATURES = {
     "kmh_test_potrf": [_P, C.c_int32, _P, C.c_int32],
     "kmh_test_trsm": [_P, C.c_int32, C.c_int32, _P, C.c_int32, C.c_int32, _P, C.c_int32],
     "kmh_test_qr": [_P, C.c_int32, C.c_int32, _P, C.c_int32, _P, C.c_int32],
+    "kmh_test_arena": [_P, C.c_int32, C.c_int32, C.POINTER(C.c_int64)],
 }
 
 

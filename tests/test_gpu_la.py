@@ -134,3 +134,19 @@ def test_qr_rank_deficient(ctx):
     assert ctx.lib.kmh_test_qr(ctx.ptr, m, r, A.ctypes.data, m, R.ctypes.data, r) == 0
     assert np.abs(A.T @ A - np.eye(r)).max() < 1e-10
     assert np.abs(A @ R - A0).max() <= 1e-13 * np.abs(A0).max()
+
+
+def test_descriptor_arena_ring(ctx):
+    """Uploads cycle several times through the descriptor arena's segment
+    ring (64 MiB); every kernel must still read its own upload."""
+    lib = ctx.lib
+    bad = C.c_int64(-1)
+    # 2600 uploads of 128 KiB = 325 MiB: about five passes around the ring,
+    # segment ends hit exactly (128 KiB divides the 16 MiB segments)
+    rc = lib.kmh_test_arena(ctx.ptr, 2600, 16384, C.byref(bad))
+    assert rc == 0, lib.last_error()
+    assert bad.value == 0
+    # uneven sizes (not dividing the segments)
+    rc = lib.kmh_test_arena(ctx.ptr, 3000, 12345, C.byref(bad))
+    assert rc == 0, lib.last_error()
+    assert bad.value == 0
