@@ -540,21 +540,8 @@ static void woodbury_shared_basis(Ctx& c, const std::vector<Hodlr*>& hs, const s
 
 // One level per handle (levels may differ across handles).
 void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vector<int>& lvs) {
-    // Nystrom: this level's panels are row ranges of the cascade panels
-    std::vector<Copy2D> cp;
-    for (size_t ai = 0; ai < act.size(); ++ai) {
-        Hodlr& H = *act[ai];
-        const int lv = lvs[ai];
-        if (!H.AB) continue;
-        const int64_t n = H.n;
-        const int r = H.nys_r;
-        double* P = H.P.as<double>() + (size_t)H.coff[lv] * n;
-        for (const auto& b : H.blocks[lv]) {
-            cp.push_back({H.AB + (size_t)n * r + b.c0, (int)n, P + b.c0, (int)n, (int)(b.d1 - b.c0), r, 1.0});
-            cp.push_back({H.AB + b.a0, (int)n, P + b.a0, (int)n, (int)(b.b1 - b.a0), r, 1.0});
-        }
-    }
-    copy2d(c, cp);
+    // Nystrom: this level's panels are row ranges of the cascade panels; the
+    // panel QR reads them there and writes Q into the level's slab of P
     size_t rsz = 0;
     for (size_t ai = 0; ai < act.size(); ++ai)
         for (int r : act[ai]->ranks[lvs[ai]]) rsz += (size_t)3 * r * r;
@@ -585,8 +572,16 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
             double* Bm = R2 + (size_t)r * r;
             off += (size_t)3 * r * r;
             // lrsymfact(V, U): V (first child) first (symfact.jl:31, HODLR/src/utils.jl:29-35)
-            qp.push_back({P + b.c0, (int)(b.d1 - b.c0), r, (int)n, R1, r});
-            qp.push_back({P + b.a0, (int)(b.b1 - b.a0), r, (int)n, R2, r});
+            QrPanel qv{P + b.c0, (int)(b.d1 - b.c0), r, (int)n, R1, r};
+            QrPanel qu{P + b.a0, (int)(b.b1 - b.a0), r, (int)n, R2, r};
+            if (H.AB) {   // V: B half of the first child's rows; U: A half of the second child's
+                qv.src = H.AB + (size_t)n * r + b.c0;
+                qv.lds = (int)n;
+                qu.src = H.AB + b.a0;
+                qu.lds = (int)n;
+            }
+            qp.push_back(qv);
+            qp.push_back(qu);
             bm.push_back({r, r, r, R1, r, R2, r, Bm, r, 1.0, 0.0, 0});
             double* Xb = X + H.coff_blk[lv][k];
             double* Gb = G + H.coff_blk[lv][k];

@@ -11,6 +11,9 @@ namespace kmh {
 // C = alpha * op(A) * op(B) + beta * C, op(A) is m x k, op(B) is k x n.
 // beta == 0 never reads C.  lower != 0 skips 64x64 output tiles strictly
 // above the diagonal (symmetric rank-k updates of a lower triangle).
+// Cin (optional, same leading dimension as C): the beta term reads Cin
+// instead of C (out-of-place update); only for untransposed products with
+// n <= 32 and k <= 64.
 struct GemmProb {
     int m, n, k;
     const double* A; int lda;
@@ -18,6 +21,7 @@ struct GemmProb {
     double* C; int ldc;
     double alpha, beta;
     int lower;
+    const double* Cin = nullptr;
 };
 void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs);
 
@@ -41,11 +45,15 @@ void trsm_left(Ctx& c, bool trans, const std::vector<TrsmProb>& probs);
 
 // Thin QR of each m x r panel (m >= r): A <- Q (orthonormal columns),
 // R (r x r upper, leading dimension ldr) written in full (lower part zero).
-// Block classical Gram-Schmidt with reorthogonalisation across 64-column
-// blocks, TSQR (Householder tiles in LDS + binary reduction tree) inside.
+// Block classical Gram-Schmidt with reorthogonalisation across 32-column
+// blocks, TSQR (single-wave Householder tiles, fan-in-4 tree) inside.  With
+// src set, the panel is read from src (leading dimension lds) and only Q is
+// written to A.
 struct QrPanel {
     double* A; int m; int r; int lda;
     double* R; int ldr;
+    const double* src = nullptr;
+    int lds = 0;
 };
 void qr_panels(Ctx& c, const std::vector<QrPanel>& panels);
 

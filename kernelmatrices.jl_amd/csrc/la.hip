@@ -32,6 +32,7 @@ struct GemmDesc {
     int m, n, k, lda, ldb, ldc;
     double alpha, beta;
     int tile0, tn, lower, pad;
+    const double* Cin;   // beta term source (k_gemm_nn_rows only; others read C)
 };
 
 constexpr int GT = 64;
@@ -378,9 +379,10 @@ __global__ __launch_bounds__(256) void k_gemm_nn_rows(const GemmDesc* __restrict
     }
     double* cp = d.C + row;
     if (d.beta != 0.0) {
+        const double* ci = (d.Cin ? d.Cin : d.C) + row;
         double cv[NMAX];
 #pragma unroll
-        for (int j = 0; j < NMAX; ++j) cv[j] = j < d.n ? cp[(size_t)j * d.ldc] : 0.0;
+        for (int j = 0; j < NMAX; ++j) cv[j] = j < d.n ? ci[(size_t)j * d.ldc] : 0.0;
 #pragma unroll
         for (int j = 0; j < NMAX; ++j)
             if (j < d.n) cp[(size_t)j * d.ldc] = d.alpha * acc[j] + d.beta * cv[j];
@@ -556,7 +558,8 @@ static void gemm_rows_nn(Ctx& c, const std::vector<GemmProb>& probs) {
         const int cl = p.n <= 8 ? 0 : p.n <= 16 ? 1 : 2;
         const int nb = (int)ceil_div(p.m, 256);
         map_append(map[cl], (int)ds[cl].size(), nb);
-        ds[cl].push_back({p.A, p.B, p.C, p.m, p.n, p.k, p.lda, p.ldb, p.ldc, p.alpha, p.beta, blocks[cl], 0, 0, 0});
+        ds[cl].push_back({p.A, p.B, p.C, p.m, p.n, p.k, p.lda, p.ldb, p.ldc, p.alpha, p.beta, blocks[cl], 0, 0, 0,
+                          p.Cin});
         blocks[cl] += nb;
         bytes += 8.0 * ((double)p.m * p.k + (double)p.k * p.n + (double)p.m * p.n * (p.beta != 0.0 ? 2 : 1));
     }
@@ -578,6 +581,8 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
     for (const auto& p : probs) {
         if (p.m <= 0 || p.n <= 0) continue;
         fl += 2.0 * p.m * p.n * (double)p.k;
+        KMH_CHECK(!p.Cin || p.Cin == p.C || rows_nn(ta, tb, p), KMH_E_BAD_ARG,
+                  "gemm: an out-of-place beta term needs n <= 32, k <= 64, no transposes");
         if (tall_tn(ta, tb, p)) tall.push_back(p);
         else if (rows_nn(ta, tb, p)) rows.push_back(p);
         else if (wide_nn(ta, tb, p)) wide.push_back(p);

@@ -257,6 +257,8 @@ static void launch_tile_qr(Ctx& c, const std::vector<SqrDesc>& ds) {
 struct TsqrPanel {
     double* A; int m; int nb; int lda;
     double* R; int ldr;
+    const double* src;   // input panel (null: A itself), leading dimension lds
+    int lds;
 };
 
 // Thin QR of m x nb panels (nb <= QC, m >= nb) by TSQR: leaf tiles of <= 128
@@ -299,7 +301,8 @@ static void tsqr(Ctx& c, const std::vector<TsqrPanel>& ps) {
             const auto& P = ps[t.panel];
             const bool single = cur[t.panel].size() == 1;
             double* a = P.A + t.row0;
-            ds.push_back({a, P.lda, 1, t.rows, 0, a, P.lda,
+            const double* in = P.src ? P.src + t.row0 : a;
+            ds.push_back({in, P.src ? P.lds : P.lda, 1, t.rows, 0, a, P.lda,
                           single ? P.R : RT + (size_t)t.node * RS, single ? P.ldr : QC, P.nb, 0});
         }
         launch_tile_qr(c, ds);
@@ -402,22 +405,34 @@ void qr_panels(Ctx& c, const std::vector<QrPanel>& ps) {
             if (p.r > c0 && c0 > 0) tsz += (size_t)c0 * std::min(QC, p.r - c0);
         double* T2 = tsz ? static_cast<double*>(c.scratch(tsz * sizeof(double))) : nullptr;
         size_t toff = 0;
+        std::vector<Copy2D> pre;
         for (const auto& p : ps) {
             if (p.r <= c0) continue;
             const int nb = std::min(QC, p.r - c0);
             double* Aj = p.A + (size_t)c0 * p.lda;
             double* Rj = p.R + (size_t)c0 * p.ldr;
+            // this column block's input: the source panel on first touch
+            const double* sj = p.src ? p.src + (size_t)c0 * p.lds : nullptr;
+            if (sj && c0 > 64) {   // beyond the out-of-place update's k range: copy first
+                pre.push_back({sj, p.lds, Aj, p.lda, p.m, nb, 1.0});
+                sj = nullptr;
+            }
             if (c0 > 0) {
                 double* t2 = T2 + toff;
                 toff += (size_t)c0 * nb;
-                p1.push_back({c0, nb, p.m, p.A, p.lda, Aj, p.lda, Rj, p.ldr, 1.0, 0.0, 0});
-                u1.push_back({p.m, nb, c0, p.A, p.lda, Rj, p.ldr, Aj, p.lda, -1.0, 1.0, 0});
+                p1.push_back({c0, nb, p.m, p.A, p.lda, sj ? sj : Aj, sj ? p.lds : p.lda, Rj, p.ldr, 1.0, 0.0, 0});
+                GemmProb g1{p.m, nb, c0, p.A, p.lda, Rj, p.ldr, Aj, p.lda, -1.0, 1.0, 0};
+                g1.Cin = sj;
+                u1.push_back(g1);
                 p2.push_back({c0, nb, p.m, p.A, p.lda, Aj, p.lda, t2, c0, 1.0, 0.0, 0});
                 u2.push_back({p.m, nb, c0, p.A, p.lda, t2, c0, Aj, p.lda, -1.0, 1.0, 0});
                 acc.push_back({t2, c0, Rj, p.ldr, c0, nb, 1.0, 1.0});
+                tp.push_back({Aj, p.m, nb, p.lda, Rj + c0, p.ldr, nullptr, 0});
+            } else {
+                tp.push_back({Aj, p.m, nb, p.lda, Rj + c0, p.ldr, sj, p.lds});
             }
-            tp.push_back({Aj, p.m, nb, p.lda, Rj + c0, p.ldr});
         }
+        copy2d(c, pre);
         if (!p1.empty()) {
             gemm(c, true, false, p1);
             gemm(c, false, false, u1);
