@@ -662,98 +662,95 @@ struct LdiagDesc {
     int chunk0, pad;
 };
 
+// lower triangle of a 64 x 64 block packed by columns: column l holds rows l..63
+__device__ __forceinline__ int tri_col(int l) { return (l * (2 * PB + 1 - l)) >> 1; }
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_trsm_diag(const LdiagDesc* __restrict__ ds, const int* __restrict__ map) {
-    __shared__ double Ls[PB][PB + 1];   // Ls[i][l] = L(i, l), identity padding
-    __shared__ double Xs[PB][PB + 1];   // Xs[i][c]: tile row i, right-hand c
+    __shared__ double Lt[PB * (PB + 1) / 2];   // L's lower triangle, identity beyond nb
+    __shared__ double Xp[16][PB + 1];          // the solved rows of the current group, published to the others
     const LdiagDesc d = ds[map[blockIdx.x]];
-    const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
+    const int tid = threadIdx.x, c = tid & 63;
+    const int g = __builtin_amdgcn_readfirstlane(tid >> 6);   // row group of this wave
     const int c0 = ((int)blockIdx.x - d.chunk0) * PB;
     const int nc = min(PB, d.ncols - c0);
+    const int nb = __builtin_amdgcn_readfirstlane(d.nb);
     for (int e = tid; e < PB * PB; e += 256) {
         const int i = e & 63, l = e >> 6;
+        if (i < l) continue;
         double v;
-        if (i < d.nb && l < d.nb) v = (l <= i) ? d.L[i + (size_t)l * d.ldl] : 0.0;
+        if (i < nb && l < nb) v = d.L[i + (size_t)l * d.ldl];
         else v = (i == l) ? 1.0 : 0.0;
-        Ls[i][l] = v;
+        Lt[tri_col(l) + i - l] = v;
     }
+    // L(i, l) for i >= l (all lanes read the same element)
+    auto Lv = [&](int i, int l) { return Lt[tri_col(l) + i - l]; };
     double x[16];
-    if (MODE == 2) {
+    // MODE 2: right-hand c is row c0 + c of B (ncols x nb); else column c0 + c
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int row = 16 * g + i;
-            x[i] = (c < nc && row < d.nb) ? d.B[(size_t)(c0 + c) + (size_t)row * d.ldb] : 0.0;
-        }
-        __syncthreads();
-    } else {
-        for (int e = tid; e < PB * PB; e += 256) {
-            const int i = e & 63, cc = e >> 6;
-            Xs[i][cc] = (i < d.nb && cc < nc) ? d.B[i + (size_t)(c0 + cc) * d.ldb] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = Xs[16 * g + i][c];
+    for (int i = 0; i < 16; ++i) {
+        const int row = 16 * g + i;
+        const bool ok = c < nc && row < nb;
+        x[i] = ok ? (MODE == 2 ? d.B[(size_t)(c0 + c) + (size_t)row * d.ldb] : d.B[row + (size_t)(c0 + c) * d.ldb])
+                  : 0.0;
     }
-    if (MODE != 1) {
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            if (g == p) {
-#pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    double s = x[i];
-#pragma unroll
-                    for (int l = 0; l < i; ++l) s -= Ls[16 * p + i][16 * p + l] * x[l];
-                    x[i] = s / Ls[16 * p + i][16 * p + i];
-                    Xs[16 * p + i][c] = x[i];
-                }
-            }
-            __syncthreads();
-            if (g > p) {
+    __syncthreads();
+    const int ng = (nb + 15) >> 4;   // row groups holding rows of L
+    const int rg = min(16, nb - 16 * g);   // rows of this wave's group (<= 0: none)
+#pragma unroll 1
+    for (int q = 0; q < ng; ++q) {
+        const int p = MODE == 1 ? ng - 1 - q : q;   // group solved in this round
+        const int rp = min(16, nb - 16 * p);
+        if (g == p) {
+            const int b = 16 * p;   // diagonal block of group p
+            if (MODE != 1) {
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
-                    double s = x[i];
+                    if (i < rp) {
+                        double s = x[i];
 #pragma unroll
-                    for (int l = 0; l < 16; ++l) s -= Ls[16 * g + i][16 * p + l] * Xs[16 * p + l][c];
-                    x[i] = s;
+                        for (int l = 0; l < i; ++l) s -= Lv(b + i, b + l) * x[l];
+                        x[i] = s / Lv(b + i, b + i);
+                    }
                 }
-            }
-        }
-    } else {
-#pragma unroll
-        for (int p = 3; p >= 0; --p) {
-            if (g == p) {
+            } else {
 #pragma unroll
                 for (int i = 15; i >= 0; --i) {
-                    double s = x[i];
+                    if (i < rp) {
+                        double s = x[i];
 #pragma unroll
-                    for (int l = i + 1; l < 16; ++l) s -= Ls[16 * p + l][16 * p + i] * x[l];
-                    x[i] = s / Ls[16 * p + i][16 * p + i];
-                    Xs[16 * p + i][c] = x[i];
+                        for (int l = i + 1; l < 16; ++l)
+                            if (l < rp) s -= Lv(b + l, b + i) * x[l];
+                        x[i] = s / Lv(b + i, b + i);
+                    }
                 }
             }
-            __syncthreads();
-            if (g < p) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) Xp[i][c] = x[i];
+        }
+        __syncthreads();
+        if ((MODE != 1 ? g > p : g < p) && rg > 0) {
+            // rows of group g against the solved group p: L(16g + i, 16p + l)
+            // (MODE 1: L^T, i.e. L(16p + l, 16g + i))
+#pragma unroll
+            for (int l = 0; l < 16; ++l) {
+                if (l >= rp) break;
+                const double xl = Xp[l][c];
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
-                    double s = x[i];
-#pragma unroll
-                    for (int l = 0; l < 16; ++l) s -= Ls[16 * p + l][16 * g + i] * Xs[16 * p + l][c];
-                    x[i] = s;
+                    const double lv = MODE != 1 ? Lv(16 * g + i, 16 * p + l) : Lv(16 * p + l, 16 * g + i);
+                    if (i < rg) x[i] -= lv * xl;
                 }
             }
         }
+        __syncthreads();   // Xp is rewritten in the next round
     }
-    if (MODE == 2) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int row = 16 * g + i;
-            if (c < nc && row < d.nb) d.B[(size_t)(c0 + c) + (size_t)row * d.ldb] = x[i];
-        }
-    } else {
-        // every row group has published its solution in Xs before the last barrier
-        for (int e = tid; e < PB * PB; e += 256) {
-            const int i = e & 63, cc = e >> 6;
-            if (i < d.nb && cc < nc) d.B[i + (size_t)(c0 + cc) * d.ldb] = Xs[i][cc];
+    for (int i = 0; i < 16; ++i) {
+        const int row = 16 * g + i;
+        if (c < nc && row < nb) {
+            if (MODE == 2) d.B[(size_t)(c0 + c) + (size_t)row * d.ldb] = x[i];
+            else d.B[row + (size_t)(c0 + c) * d.ldb] = x[i];
         }
     }
 }
