@@ -101,7 +101,9 @@ typedef struct kmh_hodlr kmh_hodlr;
 const char* kmh_last_error(void);
 int kmh_device_count(int* count);
 
-/* ---- context: device, stream, HBM budget for handles (0 = unlimited) ---- */
+/* ---- context: device, stream, HBM budget for handles (0 = unlimited) ----
+ * HODLR handles and servers keep their context alive: destroying a context
+ * that still has them only marks it; it is torn down with the last one.   */
 int kmh_ctx_create(int device, uint64_t hbm_budget_bytes, kmh_ctx** out);
 int kmh_ctx_destroy(kmh_ctx* ctx);
 int kmh_ctx_synchronize(kmh_ctx* ctx);

@@ -7,6 +7,12 @@
 
 struct kmh_hodlr {
     std::unique_ptr<kmh::Hodlr> h;
+    kmh_ctx* owner = nullptr;
+    explicit kmh_hodlr(kmh_ctx* c) : owner(c) { kmh_ctx_hold(c); }
+    ~kmh_hodlr() {
+        h.reset();   // buffers back to the context's pool first
+        kmh_ctx_release(owner);
+    }
 };
 
 using kmh::Error;
@@ -77,18 +83,9 @@ int kmh_ctx_create(int device, uint64_t hbm_budget_bytes, kmh_ctx** out) {
 int kmh_ctx_destroy(kmh_ctx* ctx) {
     KMH_TRY({
         if (!ctx) return KMH_OK;
-        kmh::Ctx& c = ctx->c;
-        c.set_device();
-        c.release_pool();
-        kmh::forget_queued(&c);
-        for (auto e : c.ev_pool) (void)hipEventDestroy(e);
-        for (auto& f : c.seg_fence)
-            if (f.ev) (void)hipEventDestroy(f.ev);
-        if (c.arena) (void)hipFree(c.arena);
-        if (c.staging) (void)hipHostFree(c.staging);
-        if (c.d_info) (void)hipFree(c.d_info);
-        (void)hipStreamDestroy(c.stream);
-        delete ctx;
+        KMH_CHECK(!ctx->closing, KMH_E_STATE, "context already destroyed");
+        ctx->closing = true;
+        if (ctx->holders == 0) kmh::ctx_teardown(ctx);
     })
 }
 
@@ -158,7 +155,7 @@ static int build_common(kmh_ctx* ctx, const double* pts, bool dev, int64_t n, in
                         const kmh_build_opts* opts, kmh_hodlr** out) {
     KMH_TRY({
         KMH_CHECK(ctx && kernel && opts && out, KMH_E_BAD_ARG, "null argument");
-        std::unique_ptr<kmh_hodlr> h(new kmh_hodlr());
+        std::unique_ptr<kmh_hodlr> h(new kmh_hodlr(ctx));
         h->h.reset(kmh::hodlr_build(ctx->c, pts, dev, n, d, *kernel, *opts));
         *out = h.release();
     })
@@ -312,7 +309,7 @@ int kmh_hodlr_build_batch(kmh_ctx* ctx, int32_t count, const kmh_build_req* reqs
             status[i] = st[i].code;
             out[i] = nullptr;
             if (hs[i]) {
-                out[i] = new kmh_hodlr();
+                out[i] = new kmh_hodlr(ctx);
                 out[i]->h.reset(hs[i]);
             } else if (first == KMH_OK) {
                 first = st[i].code;

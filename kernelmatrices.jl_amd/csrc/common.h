@@ -182,7 +182,18 @@ inline void map_append(std::vector<int>& map, int problem, int64_t count) {
 
 }  // namespace kmh
 
-// Opaque C-ABI context handle.
+// Opaque C-ABI context handle.  HODLR handles and servers hold their
+// context: kmh_ctx_destroy with holders left only marks it closing, and the
+// last holder's destroy completes the teardown (kmh_ctx_release).
 struct kmh_ctx {
     kmh::Ctx c;
+    int holders = 0;
+    bool closing = false;
 };
+namespace kmh {
+void ctx_teardown(kmh_ctx* ctx) noexcept;   // frees the context's device state and ctx itself
+}
+inline void kmh_ctx_hold(kmh_ctx* ctx) { ++ctx->holders; }
+inline void kmh_ctx_release(kmh_ctx* ctx) noexcept {
+    if (--ctx->holders == 0 && ctx->closing) kmh::ctx_teardown(ctx);
+}

@@ -37,7 +37,10 @@ void DevBuf::alloc(Ctx* c, size_t nbytes) {
 
 void DevBuf::release() {
     if (p) {
-        ctx->scratch_free(p);
+        try {
+            ctx->scratch_free(p);
+        } catch (...) {   // a block unknown to the pool is left alone
+        }
         ctx->in_use -= bytes;
     }
     p = nullptr; bytes = 0; ctx = nullptr;
@@ -229,6 +232,21 @@ void Ctx::release_pool() noexcept {
     free_blocks.clear();
     live.clear();
     pool_bytes = 0;
+}
+
+void ctx_teardown(kmh_ctx* ctx) noexcept {
+    Ctx& c = ctx->c;
+    (void)hipSetDevice(c.device);
+    c.release_pool();
+    forget_queued(&c);
+    for (auto e : c.ev_pool) (void)hipEventDestroy(e);
+    for (auto& f : c.seg_fence)
+        if (f.ev) (void)hipEventDestroy(f.ev);
+    if (c.arena) (void)hipFree(c.arena);
+    if (c.staging) (void)hipHostFree(c.staging);
+    if (c.d_info) (void)hipFree(c.d_info);
+    (void)hipStreamDestroy(c.stream);
+    delete ctx;
 }
 
 }  // namespace kmh

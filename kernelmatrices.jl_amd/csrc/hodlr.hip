@@ -548,7 +548,8 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
     double* Rb = static_cast<double*>(c.scratch(rsz * sizeof(double)));
     std::vector<QrPanel> qp;
     std::vector<GemmProb> bm;
-    std::vector<DiagOp> ident_x, ident_g, plus1, minus1;
+    std::vector<DiagOp> ident_x, ident_g, plus1, minus1, eye_s, eye_g;
+    std::vector<GemmProb> schur, g21;
     std::vector<Copy2D> lowleft;
     std::vector<SqMat> cm;
     std::vector<double*> cm_out;
@@ -585,28 +586,47 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
             bm.push_back({r, r, r, R1, r, R2, r, Bm, r, 1.0, 0.0, 0});
             double* Xb = X + H.coff_blk[lv][k];
             double* Gb = G + H.coff_blk[lv][k];
-            ident_x.push_back({Xb, 2 * r, 2 * r, 1.0});
-            ident_g.push_back({Gb, 2 * r, 2 * r, 1.0});
-            plus1.push_back({Gb, 2 * r, 2 * r, 1.0});
-            minus1.push_back({Xb, 2 * r, 2 * r, -1.0});
-            lowleft.push_back({Bm, r, Xb + r, 2 * r, r, r, 1.0});
-            cm.push_back({Xb, 2 * r, 2 * r, H.info.as<int>()});
+            // C = [I, Bm; Bm^T, I] (2r x 2r, Bm = R1 R2^T) has the Cholesky
+            // factor [I, 0; Bm^T, L22] with L22 = chol(S), S = I - Bm^T Bm (the
+            // first r columns of a blocked Cholesky of C are exact), so only
+            // S is factored:
+            //   X = chol(C) - I    = [0, 0; Bm^T, L22 - I]
+            //   G = I - chol(C)^-1 = [0, 0; L22^-1 Bm^T, I - L22^-1]
+            const int l2 = 2 * r;
+            double* X21 = Xb + r;
+            double* X22 = Xb + r + (size_t)r * l2;
+            double* G21 = Gb + r;
+            double* G22 = Gb + r + (size_t)r * l2;
+            ident_x.push_back({Xb, l2, l2, 0.0});            // X = 0, then the blocks below
+            ident_g.push_back({Gb, l2, l2, 0.0});
+            lowleft.push_back({Bm, r, X21, l2, r, r, 1.0});   // X21 = Bm^T
+            eye_s.push_back({X22, r, l2, 1.0});              // S = I ...
+            schur.push_back({r, r, r, Bm, r, Bm, r, X22, l2, -1.0, 1.0, 0});   // ... - Bm^T Bm
+            cm.push_back({X22, r, l2, H.info.as<int>()});
             cm_out.push_back(H.parts + pb + (int)k);
-            ginv.push_back({Xb, 2 * r, 2 * r, Gb, 2 * r, 2 * r});
-            neg.push_back({Gb, 2 * r, Gb, 2 * r, 2 * r, 2 * r, 0.0, -1.0});
+            eye_g.push_back({G22, r, l2, 1.0});
+            ginv.push_back({X22, r, l2, G22, r, l2});        // G22 = L22^-1
+            g21.push_back({r, r, r, G22, l2, X21, l2, G21, l2, 1.0, 0.0, 0});
+            neg.push_back({G22, l2, G22, l2, r, r, 0.0, -1.0});
+            plus1.push_back({G22, r, l2, 1.0});
+            minus1.push_back({X22, r, l2, -1.0});
         }
     }
     qr_panels(c, qp);
-    gemm(c, false, true, bm);                    // R1 R2^T
-    set_identity(c, ident_x);
-    copy2d(c, lowleft, true);                    // C = [I, *; (R1R2^T)^T, I] (lower part)
-    potrf(c, cm);                                // chol(C)
+    gemm(c, false, true, bm);                    // Bm = R1 R2^T
+    set_identity(c, ident_x);                    // X = 0
+    set_identity(c, ident_g);                    // G = 0
+    copy2d(c, lowleft, true);                    // X21 = Bm^T
+    set_identity(c, eye_s);
+    gemm(c, true, false, schur);                 // S = I - Bm^T Bm
+    potrf(c, cm);                                // L22 = chol(S): logdet C = logdet S
     log_diag_sums(c, cm, cm_out);
-    set_identity(c, ident_g);
-    trsm_left(c, false, ginv);                   // G = chol(C)^{-1}
+    set_identity(c, eye_g);
+    trsm_left(c, false, ginv);                   // G22 = L22^-1
+    gemm(c, false, false, g21);                  // G21 = L22^-1 Bm^T
     axpby2d(c, neg);
-    add_diag(c, plus1);                          // G = I - chol(C)^{-1}
-    add_diag(c, minus1);                         // X = chol(C) - I
+    add_diag(c, plus1);                          // G22 = I - L22^-1
+    add_diag(c, minus1);                         // X22 = L22 - I
     // Woodbury update of every higher level (symfact.jl:34-37)
     std::vector<Rhs> wr;
     std::vector<Hodlr*> nys;

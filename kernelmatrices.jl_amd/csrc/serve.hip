@@ -275,6 +275,14 @@ int server_step(Server& S) {
 // ---------------------------------------------------------------------------
 struct kmh_server {
     kmh::Server s;
+    kmh_ctx* owner = nullptr;
+    explicit kmh_server(kmh_ctx* c) : owner(c) { kmh_ctx_hold(c); }
+    ~kmh_server() {
+        s.all.clear();   // request handles back to the context's pool first
+        s.inflight.clear();
+        s.waiting.clear();
+        kmh_ctx_release(owner);
+    }
 };
 
 #define SRV_TRY(...) KMH_API_TRY(__VA_ARGS__)
@@ -284,7 +292,7 @@ extern "C" {
 int kmh_server_create(kmh_ctx* ctx, int32_t max_inflight, uint64_t hbm_budget_bytes, kmh_server** out) {
     SRV_TRY({
         KMH_CHECK(ctx && out && max_inflight >= 1, KMH_E_BAD_ARG, "invalid server arguments");
-        auto* s = new kmh_server();
+        auto* s = new kmh_server(ctx);
         s->s.ctx = &ctx->c;
         s->s.max_inflight = max_inflight;
         s->s.budget = hbm_budget_bytes;

@@ -2,8 +2,11 @@
 the GPU against the oracle's results for the same inputs, committed in
 tests/golden/fullsize_n65536.json by tests/golden/make_fullsize.py (the
 oracle needs minutes at this size).  Bars: per-block ranks equal, logdet and
-negloglik within 1e-10 relative, the solve K^-1 y within 1e-8 relative
-(norm, dot with y, sampled entries), and the normwise backward error of the
+negloglik within 1e-10 relative, the solve K^-1 y within 1e-8 relative in
+norm and in its dot with y, sampled entries within 1e-7 of the largest (a
+forward error: with the 1e-6 nugget cond(K) is about 4e9 here, so two
+backward-stable solvers may differ by up to cond(K) u ~ 4e-7 entrywise), and
+the normwise backward error of the
 solve through the factorized matvec W W^T, ||W W^T s - y|| / ||s||, below
 1e-12 (K has unit diagonal, so ||K|| >= 1; with the 1e-6 nugget ||s|| is
 about 1e5 ||y||, so a bar relative to ||y|| would test the conditioning,
@@ -52,6 +55,6 @@ def test_config2_full_size(km, golden, name):
     assert abs(np.linalg.norm(s) - g["solve_norm"]) <= 1e-8 * g["solve_norm"]
     assert abs(np.dot(s, y) - g["solve_dot_y"]) <= 1e-8 * abs(g["solve_dot_y"])
     ref = np.asarray(g["solve_every_512"])
-    assert np.abs(s[::512] - ref).max() <= 1e-8 * np.abs(ref).max()
+    assert np.abs(s[::512] - ref).max() <= 1e-7 * np.abs(ref).max()
     r = HK.matvec(s) - y
     assert np.linalg.norm(r) <= 1e-12 * np.linalg.norm(s)

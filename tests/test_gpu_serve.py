@@ -131,3 +131,36 @@ def test_serve_admission_budget(gpu_lib):
     # a request larger than the whole budget fails alone, without blocking
     res2, _ = _serve(lib, L, ctx, probs[:2], 8, budget=1000)
     assert all(r[1] == L.KMH_E_BUDGET for r in res2.values())
+
+
+def test_context_destroyed_before_its_handles(gpu_lib):
+    """A context destroyed while a HODLR handle and a server still hold it is
+    torn down only when the last of them is destroyed; the handle stays
+    usable until then."""
+    import kmhodlr as km
+    from kmhodlr import _lib as L
+    lib = km.load()
+    ctx = C.c_void_p()
+    assert lib.kmh_ctx_create(0, 0, C.byref(ctx)) == 0
+    rng = np.random.default_rng(7)
+    n = 1500
+    x = rng.random((n, 2))
+    kern = L.kmh_kernel()
+    kern.family = L.KMH_MATERN32
+    kern.params[0], kern.params[1], kern.nugget = 1.0, 0.1, 1e-6
+    opts = L.kmh_build_opts()
+    opts.method = L.KMH_NYSTROM
+    opts.maxrank = 40
+    opts.level_kind = L.KMH_FIXED_LEVEL
+    opts.level = 2
+    h = C.c_void_p()
+    assert lib.kmh_hodlr_build(ctx, np.ascontiguousarray(x).ctypes.data, n, 2, C.byref(kern), C.byref(opts),
+                               C.byref(h)) == 0, lib.last_error()
+    srv = C.c_void_p()
+    assert lib.kmh_server_create(ctx, 4, 0, C.byref(srv)) == 0
+    assert lib.kmh_ctx_destroy(ctx) == 0          # deferred: two holders left
+    assert lib.kmh_hodlr_factorize(h) == 0, lib.last_error()
+    ld = C.c_double()
+    assert lib.kmh_hodlr_logdet(h, C.byref(ld)) == 0 and np.isfinite(ld.value)
+    assert lib.kmh_server_destroy(srv) == 0
+    assert lib.kmh_hodlr_destroy(h) == 0          # last holder: the context goes with it
