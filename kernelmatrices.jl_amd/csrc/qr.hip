@@ -89,16 +89,18 @@ template <int N> using ic = std::integral_constant<int, N>;
 // indices are static.  The steps run in phases of 8 whose body covers only
 // the W columns that can still be non-zero (W = NC, NC - 8, ...).  Pivot-row
 // values come from lane j by readlane, column sums from the wave-local
-// reduction; the Householder vector of step j is parked in column j of the Q
-// output (each lane its own rows) for the backward accumulation of Q, which
-// shifts right one column per step (phases of growing W).  Lanes k < nc - j
-// write row j of R straight to the R output.  Two waves per SIMD: the tile
+// reduction, and the products w_k of a step reach every lane through LDS.
+// The Householder vector of step j is parked in column j of the Q output
+// (each lane its own rows) for the backward accumulation of Q, which shifts
+// right one column per step (phases of growing W).  Lanes k < nc - j
+// write row j of R straight to the R output.  Three waves per SIMD: the tile
 // (2 x NC doubles per lane) stays in registers.
 template <int NC>
-__global__ __launch_bounds__(64, 2) void k_tile_qr(const SqrDesc* __restrict__ ds) {
+__global__ __launch_bounds__(64, 3) void k_tile_qr(const SqrDesc* __restrict__ ds) {
     __shared__ double pw[64 * 17];
     __shared__ double rowj[NC];
     __shared__ double taus[NC];
+    __shared__ double wsh[NC];   // w_k of the current step, read back as broadcasts
     const SqrDesc d = ds[blockIdx.x];
     const int nc = d.nc, mt = d.nstack * d.nbs;
     const int lane = threadIdx.x;
@@ -148,13 +150,16 @@ __global__ __launch_bounds__(64, 2) void k_tile_qr(const SqrDesc* __restrict__ d
             const double v1 = scal * a1[0];
             const double wl = rk + scal * sk;   // v^T a_k for this lane's column
             if (tau != 0.0) {
+                if (lane < W) wsh[lane] = wl;
+                __builtin_amdgcn_wave_barrier();
                 const double t0 = tau * v0, t1 = tau * v1;
 #pragma unroll
                 for (int k = 1; k < W; ++k) {
-                    const double wk = qr_bcast(wl, k);
+                    const double wk = wsh[k];
                     a0[k] = __fma_rn(-t0, wk, a0[k]);
                     a1[k] = __fma_rn(-t1, wk, a1[k]);
                 }
+                __builtin_amdgcn_wave_barrier();
             }
             if (r0 < mt) q0[(size_t)j * d.ldq] = v0;
             if (r1 < mt) q1[(size_t)j * d.ldq] = v1;
@@ -210,13 +215,16 @@ __global__ __launch_bounds__(64, 2) void k_tile_qr(const SqrDesc* __restrict__ d
             const double tau = taus[j];
             const double dk = wave_colsum<NC, W>(v0, a0, v1, a1, pw, lane);
             if (tau != 0.0) {
+                if (lane < W) wsh[lane] = dk;
+                __builtin_amdgcn_wave_barrier();
                 const double t0 = tau * v0, t1 = tau * v1;
 #pragma unroll
                 for (int k = 1; k < W; ++k) {
-                    const double w = qr_bcast(dk, k);
+                    const double w = wsh[k];
                     a0[k] -= t0 * w;
                     a1[k] -= t1 * w;
                 }
+                __builtin_amdgcn_wave_barrier();
             }
             a0[0] = (r0 > j) ? -tau * v0 : (r0 == j ? 1.0 - tau : 0.0);
             a1[0] = -tau * v1;
