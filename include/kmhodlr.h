@@ -264,6 +264,29 @@ int kmh_stoch_hessian(kmh_hodlr* h, const kmh_kernel* d1, int32_t p, const kmh_k
  * midpoint after a stable sort along the coordinate of largest spread.      */
 int kmh_kd_order(const double* pts, int64_t n, int32_t d, int32_t leafsize, int64_t* perm);
 
+/* ---- HODLR.RKernelHODLR(K, tol, maxrank, lvl)  constructor.jl:130-161 ------
+ * Recursive HODLR of K(x1, x2) (x1 and x2 may differ, also in size) for
+ * matvecs: rows and columns split at floor(len/2), ACA (opts->tol,
+ * opts->maxrank) of both off-diagonal blocks of every node, dense diagonal
+ * blocks at the last level; level 0 is the dense matrix (:141).  opts:
+ * level_kind/level as for kmh_hodlr_build (LogLevel(k) -> FixedLevel(
+ * floor(log2(min(n1, n2))) - k), :133-136); the level is clamped to
+ * [0, floor(log2(min(n1, n2)))]; opts->method is ignored.  pts1 n1 x d,
+ * pts2 n2 x d row-major (host), d = 2.                                    */
+typedef struct kmh_rhodlr kmh_rhodlr;
+int kmh_rhodlr_build(kmh_ctx* ctx, const double* pts1, int64_t n1, const double* pts2, int64_t n2, int32_t d,
+                     const kmh_kernel* kernel, const kmh_build_opts* opts, kmh_rhodlr** out);
+/* levels and the matrix size (any output may be null)                        */
+int kmh_rhodlr_levels(kmh_rhodlr* h, int32_t* levels, int64_t* n1, int64_t* n2);
+/* ACA ranks, depth by depth from the root, nodes left to right, (K12, K21)
+ * per node: 2 (2^levels - 1) entries                                          */
+int kmh_rhodlr_ranks(kmh_rhodlr* h, int32_t* ranks);
+/* R * x  baseoverloads.jl:299-303: x n2 x nrhs, y n1 x nrhs column-major;
+ * host buffers, or device buffers stream-ordered on the context (_dev)      */
+int kmh_rhodlr_matvec(kmh_rhodlr* h, const double* x, double* y, int64_t nrhs);
+int kmh_rhodlr_matvec_dev(kmh_rhodlr* h, const double* x, double* y, int64_t nrhs);
+int kmh_rhodlr_destroy(kmh_rhodlr* h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,12 +19,14 @@ struct EvalDesc {
 void eval_blocks(Ctx& c, const kmh_kernel& kern, std::vector<EvalDesc> ds);   // sets every k
 void eval_blocks(Ctx& c, std::vector<EvalDesc> ds);                           // per-descriptor k
 
-// One off-diagonal block for ACA: rows a..a+nro (second child), cols
-// c..c+ncl (first child).  U (nro x cap) and V (ncl x cap) share the leading
-// dimension ld.  Outputs: rank, status (0 ok, 1 start-row error, 2 capacity
-// reached before convergence).
+// One off-diagonal block for ACA: rows a..a+nro of the row points, cols
+// c..c+ncl of the column points (a symmetric HODLR passes its points for
+// both).  U (nro x cap) and V (ncl x cap) share the leading dimension ld.
+// Outputs: rank, status (0 ok, 1 start-row error, 2 capacity reached before
+// convergence).
 struct AcaDesc {
-    const double* px; const double* py;   // the request's points (SoA)
+    const double* px; const double* py;   // row points (SoA)
+    const double* qx; const double* qy;   // column points (SoA)
     int a, nro, c, ncl;
     double* U;
     double* V;

@@ -468,7 +468,7 @@ AcaRun::AcaRun(Ctx& c, const std::vector<AcaDesc>& ds) : m_(new Impl(c)) {
         AcaState& s = m.h[b];
         std::memset(&s, 0, sizeof(s));
         s.rx = d.px + d.a; s.ry = d.py + d.a;
-        s.cx = d.px + d.c; s.cy = d.py + d.c;
+        s.cx = d.qx + d.c; s.cy = d.qy + d.c;
         s.U = d.U; s.V = d.V;
         s.nro = d.nro; s.ncl = d.ncl; s.ld = d.ld; s.maxsz = d.maxsz; s.cap = d.cap;
         s.rch = (int)ceil_div(d.nro, ACH);

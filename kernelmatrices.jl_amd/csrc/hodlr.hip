@@ -184,8 +184,8 @@ static void build_aca_batch(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<S
             const int nro = (int)(b.b1 - b.a0), ncl = (int)(b.d1 - b.c0);
             const int h = std::min(nro, ncl);
             const int ms = H.opts.maxrank == 0 ? h : std::min(H.opts.maxrank, h);
-            ds.push_back({px, px + H.n, (int)b.a0, nro, (int)b.c0, ncl, T + b.a0, T + b.c0, (int)H.n, ms,
-                          v.cap, 0, d_rank + bi, d_stat + bi, H.opts.tol, to_kdev(H.kern)});
+            ds.push_back({px, px + H.n, px, px + H.n, (int)b.a0, nro, (int)b.c0, ncl, T + b.a0, T + b.c0, (int)H.n,
+                          ms, v.cap, 0, d_rank + bi, d_stat + bi, H.opts.tol, to_kdev(H.kern)});
             ++bi;
         }
     }

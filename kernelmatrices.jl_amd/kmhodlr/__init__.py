@@ -6,7 +6,8 @@ libkmhodlr.so (HIP, gfx950) through its C ABI.
 from . import _lib
 from ._lib import load, LibraryMissing
 from .api import (Context, ErrorException, FactorHODLR, FixedLevel, HBMBudgetExceeded, Kernel,
-                  KernelHODLR, KernelMatrix, LogLevel, PosDefException, cauchy_kernel, default_context,
+                  KernelHODLR, KernelMatrix, LogLevel, PosDefException, RKernelHODLR, cauchy_kernel,
+                  default_context,
                   full, kd_order, maxlikdata, ldiv, logdet, matern12, matern32, negloglik, profile_negloglik,
                   rational_kernel, stoch_fisher, stoch_gradient, stoch_hessian, stoch_profile_gradient,
                   symmetricfactorize)
@@ -18,6 +19,7 @@ from . import maxlik
 class HODLR:
     """Namespace mirroring `KernelMatrices.HODLR`."""
     KernelHODLR = KernelHODLR
+    RKernelHODLR = RKernelHODLR
     FixedLevel = FixedLevel
     LogLevel = LogLevel
     symmetricfactorize = staticmethod(symmetricfactorize)

@@ -108,6 +108,13 @@ SIGNATURES = {
     "kmh_server_pending": [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_uint64)],
     "kmh_server_result": [_P, C.c_int64, C.POINTER(kmh_serve_result)],
     "kmh_kd_order": [_P, _I64, C.c_int32, C.c_int32, _P],
+    "kmh_rhodlr_build": [_P, _P, _I64, _P, _I64, C.c_int32, C.POINTER(kmh_kernel), C.POINTER(kmh_build_opts),
+                         C.POINTER(C.c_void_p)],
+    "kmh_rhodlr_levels": [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
+    "kmh_rhodlr_ranks": [_P, _P],
+    "kmh_rhodlr_matvec": [_P, _P, _P, _I64],
+    "kmh_rhodlr_matvec_dev": [_P, _P, _P, _I64],
+    "kmh_rhodlr_destroy": [_P],
 }
 
 # include/kmhodlr_testing.h

@@ -372,6 +372,74 @@ class KernelHODLR:
             pass
 
 
+class RKernelHODLR:
+    """HODLR.RKernelHODLR(K, tol, maxrank=0, lvl=LogLevel(7))
+    (constructor.jl:130-161): recursive HODLR of K(x1, x2) for matvecs; x1
+    and x2 may differ, also in size.  Rows and columns split at floor(len/2);
+    both off-diagonal blocks of every node are ACA-compressed, the last
+    level's diagonal blocks are dense.  Level 0 keeps the whole matrix dense
+    (the reference returns full(K) there, :141)."""
+
+    def __init__(self, K, tol, maxrank=0, lvl=None, ctx=None):
+        lvl = lvl if lvl is not None else LogLevel(7)
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        self.K = K
+        o = L.kmh_build_opts()
+        o.method = L.KMH_ACA
+        o.level_kind = lvl.kind
+        o.level = lvl.lv
+        o.maxrank = int(maxrank)
+        o.tol = float(tol)
+        o.precision = L.KMH_FP64
+        kd = K.kernel.desc(K.parms)
+        p1, p2 = _f64(K.x1), _f64(K.x2)
+        p = C.c_void_p()
+        _check(self.lib, self.lib.kmh_rhodlr_build(self.ctx.ptr, p1.ctypes.data, p1.shape[0], p2.ctypes.data,
+                                                   p2.shape[0], p1.shape[1], C.byref(kd), C.byref(o), C.byref(p)))
+        self.ptr = p
+        lv, n1, n2 = C.c_int32(), C.c_int64(), C.c_int64()
+        _check(self.lib, self.lib.kmh_rhodlr_levels(self.ptr, C.byref(lv), C.byref(n1), C.byref(n2)))
+        self.levels, self.shape = lv.value, (n1.value, n2.value)
+
+    def size(self, j=None):
+        return self.shape if j is None else self.shape[j - 1]
+
+    def ranks(self):
+        """[(rank K12, rank K21) per node] per depth, root first."""
+        nb = 2 ** self.levels - 1
+        r = (C.c_int32 * max(1, 2 * nb))()
+        _check(self.lib, self.lib.kmh_rhodlr_ranks(self.ptr, r))
+        out, q = [], 0
+        for d in range(self.levels):
+            out.append([(r[q + 2 * t], r[q + 2 * t + 1]) for t in range(2 ** d)])
+            q += 2 ** (d + 1)
+        return out
+
+    def mul(self, x):
+        """R * x (baseoverloads.jl:299-303) for a vector or an n2 x k matrix."""
+        xb, k, vec = _as_cols(x, self.shape[1])
+        y = np.zeros((self.shape[0],) if vec else (self.shape[0], k), order="F")
+        _check(self.lib, self.lib.kmh_rhodlr_matvec(self.ptr, xb.ctypes.data, y.ctypes.data, k))
+        return y
+
+    __matmul__ = mul
+
+    def __mul__(self, x):
+        return self.mul(x)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.lib.kmh_rhodlr_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class FactorHODLR:
     """HK.W with K = W W^T: W*x, W\\x and their adjoints."""
 

@@ -818,3 +818,80 @@ def stoch_hessian(K, HK, y, d1kernels, d2kernels, vecs, exact=False):
                 o = o - HK.solve(DKs[j].matvec(HK.solve(DKs[k].matvec(HKsd))))
                 out[j, k] -= 0.5 * float(np.dot(y, o))
     return np.triu(out) + np.triu(out, 1).T
+
+
+# ---------------------------------------------------------------------------
+# RKernelHODLR: recursive HODLR for (possibly non-symmetric) matvecs
+# (constructor.jl:130-161, structstypes.jl:22-27, baseoverloads.jl:285-305)
+# ---------------------------------------------------------------------------
+
+class UVt:
+    """UVt (structstypes.jl): block = U V^T; U*(V'v) (baseoverloads.jl:289)."""
+
+    def __init__(self, U, V):
+        self.U, self.V = U, V
+
+    def mul(self, v):
+        return self.U @ (self.V.T @ v)
+
+
+class RKernelHODLR:
+    """constructor.jl:130-161.  A11/A22 are dense matrices at the last level,
+    else RKernelHODLR; A12 = ACA(K12), A21 = ACA(K21).  Rows and columns
+    split at floor(len/2) (the first half is the smaller one)."""
+
+    def __init__(self, A11, A22, A12, A21):
+        self.A11, self.A22, self.A12, self.A21 = A11, A22, A12, A21
+
+    @property
+    def shape(self):
+        s11 = self.A11.shape
+        s22 = self.A22.shape
+        return (s11[0] + s22[0], s11[1] + s22[1])
+
+    def mul(self, v):
+        """baseoverloads.jl:299-303."""
+        n11 = self.A11.shape[1]
+        v1, v2 = v[:n11], v[n11:]
+        m11 = self.A11 @ v1 if isinstance(self.A11, np.ndarray) else self.A11.mul(v1)
+        m22 = self.A22 @ v2 if isinstance(self.A22, np.ndarray) else self.A22.mul(v2)
+        return np.concatenate([m11 + self.A12.mul(v2), self.A21.mul(v1) + m22])
+
+    def ranks(self):
+        """[(rank K12, rank K21) per node] per depth, root first, nodes left to
+        right (the order the library reports)."""
+        out = []
+        level = [self]
+        while level and isinstance(level[0], RKernelHODLR):
+            out.append([(b.A12.U.shape[1], b.A21.U.shape[1]) for b in level])
+            nxt = []
+            for b in level:
+                if isinstance(b.A11, RKernelHODLR):
+                    nxt += [b.A11, b.A22]
+            level = nxt
+        return out
+
+
+def rkernelhodlr(K, tol, maxrank=0, lvl=None):
+    """RKernelHODLR(K, tol, maxrank, lvl) (constructor.jl:130-161): returns a
+    dense matrix when the level is 0 (:141)."""
+    if lvl is None:
+        lvl = LogLevel(7)
+    if isinstance(lvl, LogLevel):   # :133-136
+        lv = int(math.floor(math.log2(min(K.shape)))) - lvl.lv
+        return rkernelhodlr(K, tol, maxrank, FixedLevel(lv))
+    if lvl.lv == 0:                 # :141
+        return K.full()
+    len1, len2 = K.shape
+    mpt1, mpt2 = len1 // 2, len2 // 2
+    K11 = K.sub(0, mpt1, 0, mpt2)
+    K22 = K.sub(mpt1, len1, mpt2, len2)
+    K12 = K.sub(0, mpt1, mpt2, len2)
+    K21 = K.sub(mpt1, len1, 0, mpt2)
+    A12 = UVt(*aca(K12, tol, maxrank))
+    A21 = UVt(*aca(K21, tol, maxrank))
+    if lvl.lv == 1:
+        return RKernelHODLR(K11.full(), K22.full(), A12, A21)
+    A11 = rkernelhodlr(K11, tol, maxrank, FixedLevel(lvl.lv - 1))
+    A22 = rkernelhodlr(K22, tol, maxrank, FixedLevel(lvl.lv - 1))
+    return RKernelHODLR(A11, A22, A12, A21)
