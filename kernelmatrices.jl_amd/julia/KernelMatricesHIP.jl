@@ -311,6 +311,41 @@ module HODLR
                 P, length(pts), d, leafsize, perm))
     return perm .+ 1
   end
+
+  # HODLR.RKernelHODLR(K, tol, maxrank=0, lvl=LogLevel(7))  constructor.jl:130-161:
+  # recursive HODLR of K(x1, x2) for matvecs (x1 and x2 may differ).
+  mutable struct RKernelHODLR
+    ptr::Ptr{Cvoid}
+    n1::Int
+    n2::Int
+  end
+
+  function RKernelHODLR(K::KernelMatrix, tol::Float64, maxrank::Int64=0, lvl::HierLevel=LogLevel(7);
+                        ctx=default_context())
+    kd = Ref(descriptor(K.kernel, K.parms))
+    op = Ref(kmh_build_opts(0, kind(lvl), lvl.lv, maxrank, tol, 0, (0, 0, 0)))
+    P1, P2 = points_matrix(K.x1), points_matrix(K.x2)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:kmh_rhodlr_build, libkmhodlr), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Int32, Ptr{Cvoid}, Ptr{Cvoid},
+                 Ptr{Ptr{Cvoid}}),
+                ctx.ptr, P1, length(K.x1), P2, length(K.x2), 2, kd, op, h))
+    R = RKernelHODLR(h[], length(K.x1), length(K.x2))
+    finalizer(x -> ccall((:kmh_rhodlr_destroy, libkmhodlr), Cint, (Ptr{Cvoid},), x.ptr), R)
+    return R
+  end
+
+  Base.size(R::RKernelHODLR) = (R.n1, R.n2)
+  Base.size(R::RKernelHODLR, j::Int64) = size(R)[j]
+
+  # baseoverloads.jl:299-303
+  function Base.:*(R::RKernelHODLR, V::VecOrMat{Float64})
+    size(V, 1) == R.n2 || throw(DimensionMismatch("matvec with a vector of the wrong length"))
+    out = V isa Vector ? Vector{Float64}(undef, R.n1) : Matrix{Float64}(undef, R.n1, size(V, 2))
+    check(ccall((:kmh_rhodlr_matvec, libkmhodlr), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64),
+                R.ptr, V, out, size(V, 2)))
+    return out
+  end
 end # module HODLR
 
 end # module KernelMatricesHIP
