@@ -11,10 +11,14 @@ parameters (sigma ~ U[0.5, 2], rho ~ U[0.05, 0.2]), inputs resident in HBM.
 
 Legs: `plain` runs the requests one after another through the per-request
 C ABI; `served` keeps `--inflight` requests in the server, which advances
-all of them stage by stage in shared grouped launches.  Both legs see the
-same requests; their logdets are compared.  The JSON line carries the
+all of them stage by stage in shared grouped launches, spread over the
+server's lanes (HIP streams stepped on parallel host threads).  Both legs
+see the same requests; their logdets are compared.  `roofline` runs one
+served batch on a single lane with every k_gemm_nn_wide launch timed.  The JSON line carries the
 roofline of the grouped FP64 MFMA GEMM (the dominant kernel; HIP events on
-the library's stream) and a CPU baseline (the numpy oracle, one BLAS thread).
+the library's stream, on every PROF_STRIDE-th launch of that kernel inside
+the timed region: a timed launch idles the queue for a few microseconds) and
+a CPU baseline (the numpy oracle, one BLAS thread).
 """
 import argparse
 import ctypes as C
@@ -31,6 +35,7 @@ for p in (ROOT, os.path.join(ROOT, "kernelmatrices.jl_amd")):
 
 FP64_MFMA_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBPS = 8000.0
+PROF_STRIDE = 4   # time every 4th k_gemm_nn_wide launch of a timed region
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "k_gemm_nn_wide_served_n65536_traffic.md")
 
@@ -139,7 +144,8 @@ def main():
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--inflight", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--legs", default="plain,served,plain_grad,served_grad")
+    ap.add_argument("--legs", default="plain,served,plain_grad,served_grad,roofline")
+    ap.add_argument("--lanes", type=int, default=0, help="server lanes (0: the library's default)")
     ap.add_argument("--grad-vecs", type=int, default=32, help="SAA vectors of the gradient legs")
     ap.add_argument("--gemm-shapes", action="store_true",
                     help="time the grouped GEMM alone on the skinny shapes of the factorization "
@@ -237,12 +243,29 @@ def main():
         return 0.5 * ld.value + 0.5 * quad, g
 
     legs = args.legs.split(",")
+    check(lib.kmh_ctx_profile_config(ctx.ptr, 1 << L.KMH_PROF_NN_WIDE, PROF_STRIDE))
+
+    def prof_begin():
+        check(lib.kmh_ctx_profile_begin(ctx.ptr))
+
+    def prof_end():
+        """k_gemm_nn_wide totals of the sampled launches since prof_begin"""
+        fl, ms, cnt = C.c_double(), C.c_double(), C.c_int64()
+        check(lib.kmh_ctx_profile_end(ctx.ptr, C.byref(fl), C.byref(ms), C.byref(cnt)))
+        wb, wms, wl = C.c_double(), C.c_double(), C.c_int64()
+        check(lib.kmh_ctx_profile_query(ctx.ptr, L.KMH_PROF_NN_WIDE, C.byref(wb), C.byref(wms), C.byref(wl)))
+        k = max(1, wl.value)
+        return {"nn_wide_bytes_per_launch": wb.value / k, "nn_wide_us_per_launch": 1e3 * wms.value / k,
+                "nn_wide_gbps": wb.value / (wms.value * 1e-3) / 1e9 if wms.value else 0.0,
+                "nn_wide_tflops": fl.value / (ms.value * 1e-3) / 1e12 if ms.value else 0.0,
+                "nn_wide_timed_launches": wl.value, "nn_wide_sample_stride": PROF_STRIDE}
+
     plain_ld = {}
     if "plain" in legs:
         for i in range(args.warmup):
             plain_eval(params[i])
         torch.cuda.synchronize()
-        check(lib.kmh_ctx_profile_begin(ctx.ptr))
+        prof_begin()
         lat = []
         t0 = time.perf_counter()
         for i in range(args.warmup, total):
@@ -252,12 +275,8 @@ def main():
             plain_ld[i] = ld
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-        gfl, gms, gl = C.c_double(), C.c_double(), C.c_int64()
-        check(lib.kmh_ctx_profile_end(ctx.ptr, C.byref(gfl), C.byref(gms), C.byref(gl)))
-        out["plain"] = {"evals_per_s": args.steps / el, "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99),
-                        "gemm_tflops": gfl.value / (gms.value * 1e-3) / 1e12 if gms.value else 0.0,
-                        "gemm_flops_per_eval": gfl.value / args.steps, "gemm_ms_per_eval": gms.value / args.steps,
-                        "gemm_launches_per_eval": gl.value / args.steps}
+        out["plain"] = {"evals_per_s": args.steps / el, "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99)}
+        out["plain"].update(prof_end())
 
     plain_grad = {}
     if "plain_grad" in legs:
@@ -275,9 +294,11 @@ def main():
                              "saa_vectors": args.grad_vecs}
 
     # ---- served: continuous batching -----------------------------------
-    if "served" in legs or "served_grad" in legs:
+    if "served" in legs or "served_grad" in legs or "roofline" in legs:
         srv = C.c_void_p()
         check(lib.kmh_server_create(ctx.ptr, args.inflight, 0, C.byref(srv)))
+        if args.lanes:
+            check(lib.kmh_server_set_lanes(srv, args.lanes))
         reqs = []
 
         grad_mode = [False]
@@ -309,7 +330,7 @@ def main():
             live = {}
             res = {}
             if profile:
-                check(lib.kmh_ctx_profile_begin(ctx.ptr))
+                prof_begin()
             t0 = time.perf_counter()
             while pending or live:
                 while pending and len(live) < args.inflight:
@@ -328,14 +349,7 @@ def main():
                             raise RuntimeError("request failed: %s" % lib.last_error())
                         res[live.pop(rid)] = (o.nll, o.logdet, o.latency_us * 1e-3, [o.grad[0], o.grad[1]])
             el = time.perf_counter() - t0
-            prof = None
-            if profile:
-                gfl, gms, gl = C.c_double(), C.c_double(), C.c_int64()
-                check(lib.kmh_ctx_profile_end(ctx.ptr, C.byref(gfl), C.byref(gms), C.byref(gl)))
-                wb, wms, wl = C.c_double(), C.c_double(), C.c_int64()
-                check(lib.kmh_ctx_profile_query(ctx.ptr, L.KMH_PROF_NN_WIDE, C.byref(wb), C.byref(wms), C.byref(wl)))
-                prof = (gfl.value, gms.value, gl.value, wb.value, wms.value, wl.value)
-            return res, el, prof
+            return res, el, (prof_end() if profile else None)
 
         if "served" in legs:
             run(range(args.warmup), False)
@@ -343,13 +357,8 @@ def main():
             res, el, prof = run(range(args.warmup, total), True)
             lat = [v[2] for v in res.values()]
             out["served"] = {"evals_per_s": args.steps / el, "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99),
-                             "gemm_tflops": prof[0] / (prof[1] * 1e-3) / 1e12 if prof[1] else 0.0,
-                             "gemm_flops_per_eval": prof[0] / args.steps, "gemm_ms_per_eval": prof[1] / args.steps,
-                             "gemm_launches_total": prof[2], "inflight": args.inflight,
-                             "nn_wide_bytes_per_launch": prof[3] / max(1, prof[5]),
-                             "nn_wide_us_per_launch": 1e3 * prof[4] / max(1, prof[5]),
-                             "nn_wide_gbps": prof[3] / (prof[4] * 1e-3) / 1e9 if prof[4] else 0.0,
-                             "nn_wide_launches": prof[5]}
+                             "inflight": args.inflight}
+            out["served"].update(prof)
             if plain_ld:
                 errs = [abs(res[i][1] - plain_ld[i]) / abs(plain_ld[i]) for i in plain_ld]
                 out["served"]["max_rel_logdet_err_vs_plain"] = max(errs)
@@ -365,17 +374,40 @@ def main():
                 gerr = max(abs(res[i][3][j] - plain_grad[i][1][j]) / abs(plain_grad[i][1][j])
                            for i in plain_grad for j in range(2))
                 out["served_grad"]["max_rel_grad_err_vs_plain"] = gerr
+        if "roofline" in legs:
+            # the dominant kernel without another stream's kernels beside it:
+            # one served batch on one lane, every k_gemm_nn_wide launch timed
+            grad_mode[0] = False
+            check(lib.kmh_server_set_lanes(srv, 1))
+            check(lib.kmh_ctx_profile_config(ctx.ptr, 1 << L.KMH_PROF_NN_WIDE, 1))
+            k = min(args.inflight, total)
+            run(range(k), False)
+            torch.cuda.synchronize()
+            res, el, prof = run(range(k), True)
+            check(lib.kmh_ctx_profile_config(ctx.ptr, 1 << L.KMH_PROF_NN_WIDE, PROF_STRIDE))
+            lat = [v[2] for v in res.values()]
+            out["roofline_pass"] = {"requests": k, "lanes": 1, "evals_per_s": k / el, "p50_ms": pct(lat, 0.5),
+                                    "p99_ms": pct(lat, 0.99)}
+            out["roofline_pass"].update(prof)
         lib.kmh_server_destroy(srv)
 
-    head = out.get("served") or out.get("plain") or out.get("served_grad") or out.get("plain_grad")
+    head = (out.get("served") or out.get("plain") or out.get("served_grad") or out.get("plain_grad")
+            or out.get("roofline_pass"))
 
     def roofline(h):
         """Dominant kernel of the served leg: k_gemm_nn_wide (HBM-bound skinny
         FP64 GEMM).  achieved = its algorithmic bytes per launch / its average
-        launch time, both measured live with HIP events on the library's
-        stream; traffic = measured HBM bytes per launch (rocprofv3 FETCH_SIZE
-        + WRITE_SIZE of the same command, calibrated on a copy kernel) when
-        the committed summary holds it."""
+        launch time, both measured live with HIP events on the dispatches of
+        the library's stream: in the roofline pass (one served batch on one
+        lane, every launch timed) when it ran, else every PROF_STRIDE-th
+        launch of the headline leg, where the other lane's kernels share the
+        device.  traffic = measured HBM bytes per launch (rocprofv3
+        FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel) when the
+        committed summary holds it."""
+        where = "headline leg, lane 0, every %d-th launch" % PROF_STRIDE
+        if "roofline_pass" in out:
+            h = out["roofline_pass"]
+            where = "roofline pass: one served batch of %d requests on one lane, every launch" % h["requests"]
         if "nn_wide_gbps" not in h:
             return {"bound": "hbm", "kernel": "k_gemm_nn_wide", "achieved": None, "peak": HBM_PEAK_GBPS,
                     "unit": "GB/s", "frac": None, "traffic": None}
@@ -384,8 +416,9 @@ def main():
                 "frac": round(h["nn_wide_gbps"] / HBM_PEAK_GBPS, 4),
                 "algorithmic_bytes_per_launch": round(h["nn_wide_bytes_per_launch"]),
                 "us_per_launch": round(h["nn_wide_us_per_launch"], 2),
+                "timed_launches": h["nn_wide_timed_launches"], "measured_in": where,
                 "traffic": measured_traffic("k_gemm_nn_wide"),
-                "gemm_all_tflops": round(h["gemm_tflops"], 3), "fp64_mfma_peak_tflops": FP64_MFMA_PEAK_TFLOPS,
+                "tflops": round(h["nn_wide_tflops"], 3), "fp64_mfma_peak_tflops": FP64_MFMA_PEAK_TFLOPS,
                 "factor_flops_structured_per_eval": ff}
     L_ = info.get("levels", args.level)
     nblocks = [2 ** (L_ - l - 1) for l in range(L_)]
@@ -409,7 +442,8 @@ def main():
         "config": {"workload": "n=65536 Matern-3/2 (sigma~U[0.5,2], rho~U[0.05,0.2], nugget 1e-6) "
                                "Nystrom r=72 FixedLevel(8) negloglik requests, %s form"
                                % ("served+batched FP64" if "served" in out else "plain FP64"),
-                   "n": n, "leaf": m, "nystrom_rank": 72, "levels": L_, "inflight": args.inflight},
+                   "n": n, "leaf": m, "nystrom_rank": 72, "levels": L_, "inflight": args.inflight,
+                   "server_lanes": args.lanes or (2 if args.inflight >= 2 else 1)},
         "legs": {k: {kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in d.items()}
                  for k, d in out.items()},
         "roofline": roofline(head),

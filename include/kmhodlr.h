@@ -109,20 +109,28 @@ int kmh_ctx_destroy(kmh_ctx* ctx);
 int kmh_ctx_synchronize(kmh_ctx* ctx);
 int kmh_ctx_bytes_in_use(kmh_ctx* ctx, uint64_t* out);
 /* Measurement hooks (no counterpart in the reference): between begin and end
- * the launches below are bracketed by HIP events on the context's stream.
- * end returns the KMH_PROF_GEMM totals (every grouped FP64 GEMM call: the
- * algorithmic flops 2mnk, summed duration in ms, call count); query returns
- * any tag's totals of the last begin..end window:
- *   KMH_PROF_GEMM     grouped GEMM calls, amount = flops
+ * every FP64 GEMM kernel launch carries a start and a stop HIP event on its
+ * own dispatch (hipExtLaunchKernel: no packets between kernels), so a span
+ * is one launch and its duration excludes queue gaps.  end returns the
+ * KMH_PROF_GEMM totals (algorithmic flops 2mnk, summed kernel duration in ms,
+ * launch count); query returns any tag's totals of the last begin..end window:
+ *   KMH_PROF_GEMM     every GEMM kernel launch (split-k reductions included,
+ *                     with 0 flops), amount = flops
  *   KMH_PROF_NN_WIDE  k_gemm_nn_wide launches, amount = algorithmic HBM bytes
  *                     (A, B and C written, plus C read when beta != 0)
  *   KMH_PROF_TN_TALL  k_gemm_tn_tall launches, amount = bytes of A, B, C
- *   KMH_PROF_NN_ROWS  k_gemm_nn_rows launches (one span per grouped call),
- *                     amount = bytes as for KMH_PROF_NN_WIDE               */
+ *   KMH_PROF_NN_ROWS  k_gemm_nn_rows launches, amount = bytes as for
+ *                     KMH_PROF_NN_WIDE                                      */
 enum { KMH_PROF_GEMM = 0, KMH_PROF_NN_WIDE = 1, KMH_PROF_TN_TALL = 2, KMH_PROF_NN_ROWS = 3, KMH_PROF_TAGS = 4 };
 int kmh_ctx_profile_begin(kmh_ctx* ctx);
 int kmh_ctx_profile_end(kmh_ctx* ctx, double* gemm_flops, double* gemm_ms, int64_t* gemm_launches);
 int kmh_ctx_profile_query(kmh_ctx* ctx, int32_t tag, double* amount, double* ms, int64_t* spans);
+/* Which launches the hooks time (default: every tag, every launch): the tags
+ * whose bit (1 << tag) is set in tag_mask, each at every stride-th launch of
+ * that tag, counted from begin.  A timed launch leaves a few microseconds of
+ * idle around it, so a throughput run times a sample.  The KMH_PROF_GEMM
+ * totals of end cover the timed launches only.  Not inside begin..end. */
+int kmh_ctx_profile_config(kmh_ctx* ctx, uint32_t tag_mask, int32_t stride);
 
 /* ---- HODLR.KernelHODLR(K, ep, maxrank, lvl; nystrom)  constructor.jl:6-58 ----
  * K = KernelMatrix(pts, pts, params, kernel)  src/structstypes.jl:17-23      */
@@ -192,8 +200,16 @@ int kmh_hodlr_solve_batch_dev(kmh_ctx* ctx, int32_t count, kmh_hodlr* const* hs,
  * (0 = unlimited) while fewer than max_inflight run, then advances every
  * in-flight request by one stage (build, leaf stage, one tree level, finish)
  * with the stages of all requests in shared grouped launches.  Inputs are
- * copied at submit; results are kept until read with kmh_server_result.    */
+ * copied at submit; results are kept until read with kmh_server_result.
+ * The in-flight requests are spread over lanes (a new request joins the lane
+ * with the fewest): lane 0 runs on the server's context, every other lane on
+ * a private context (own HIP stream and scratch pool, no HBM budget of its
+ * own: bound a multi-lane server with hbm_budget_bytes), and a step advances
+ * the lanes on parallel host threads so that their work overlaps on the
+ * device.  Default: 2 lanes when the context has no HBM budget and
+ * max_inflight >= 2, else 1.  Results do not depend on the lane count.    */
 #define KMH_SERVE_MAX_PARAMS 4
+#define KMH_SERVE_MAX_LANES 4
 typedef struct kmh_server kmh_server;
 typedef struct kmh_serve_req {
     const double* pts;        /* n x 2 row-major (host, or device if on_device) */
@@ -220,6 +236,9 @@ typedef struct kmh_serve_result {
     double  latency_us;       /* submit -> completion                       */
 } kmh_serve_result;
 int kmh_server_create(kmh_ctx* ctx, int32_t max_inflight, uint64_t hbm_budget_bytes, kmh_server** out);
+/* 1..KMH_SERVE_MAX_LANES lanes; KMH_E_STATE while requests are queued or
+ * running (no reference counterpart)                                        */
+int kmh_server_set_lanes(kmh_server* srv, int32_t lanes);
 int kmh_server_destroy(kmh_server* srv);
 int kmh_server_submit(kmh_server* srv, const kmh_serve_req* req, int64_t* id);
 int kmh_server_step(kmh_server* srv, int32_t* completed);

@@ -42,6 +42,31 @@ static void with_device_rhs(kmh::Hodlr& H, const double* x, double* y, int64_t n
     c.sync();
 }
 
+namespace kmh {
+kmh_ctx* ctx_create(int device, uint64_t budget) {
+    int n = 0;
+    KMH_HIP(hipGetDeviceCount(&n));
+    KMH_CHECK(device >= 0 && device < n, KMH_E_BAD_ARG, "device index out of range");
+    std::unique_ptr<kmh_ctx> c(new kmh_ctx());
+    c->c.device = device;
+    c->c.budget = budget;
+    KMH_HIP(hipSetDevice(device));
+    KMH_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+    try {
+        c->c.arena_size = size_t(64) << 20;
+        KMH_HIP(hipMalloc(&c->c.arena, c->c.arena_size));
+        KMH_HIP(hipHostMalloc(&c->c.staging, c->c.arena_size, hipHostMallocDefault));
+        KMH_HIP(hipMalloc(&c->c.d_info, 256));
+        KMH_HIP(hipMemset(c->c.d_info, 0, 256));
+    } catch (...) {
+        ctx_teardown(c.release());
+        throw;
+    }
+    (void)hipGetLastError();
+    return c.release();
+}
+}  // namespace kmh
+
 extern "C" {
 
 const char* kmh_last_error(void) { return kmh::last_error_cstr(); }
@@ -59,24 +84,11 @@ int kmh_device_count(int* count) {
     })
 }
 
+
 int kmh_ctx_create(int device, uint64_t hbm_budget_bytes, kmh_ctx** out) {
     KMH_TRY({
         KMH_CHECK(out, KMH_E_BAD_ARG, "null output");
-        int n = 0;
-        KMH_HIP(hipGetDeviceCount(&n));
-        KMH_CHECK(device >= 0 && device < n, KMH_E_BAD_ARG, "device index out of range");
-        std::unique_ptr<kmh_ctx> c(new kmh_ctx());
-        c->c.device = device;
-        c->c.budget = hbm_budget_bytes;
-        KMH_HIP(hipSetDevice(device));
-        KMH_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
-        c->c.arena_size = size_t(64) << 20;
-        KMH_HIP(hipMalloc(&c->c.arena, c->c.arena_size));
-        KMH_HIP(hipHostMalloc(&c->c.staging, c->c.arena_size, hipHostMallocDefault));
-        KMH_HIP(hipMalloc(&c->c.d_info, 256));
-        KMH_HIP(hipMemset(c->c.d_info, 0, 256));
-        (void)hipGetLastError();
-        *out = c.release();
+        *out = kmh::ctx_create(device, hbm_budget_bytes);
     })
 }
 
@@ -114,6 +126,18 @@ int kmh_ctx_profile_begin(kmh_ctx* ctx) {
         c.ev_used = 0;
         c.spans.clear();
         for (auto& t : c.prof_sum) t[0] = t[1] = t[2] = 0.0;
+        for (auto& k : c.prof_seen) k = 0;
+    })
+}
+
+int kmh_ctx_profile_config(kmh_ctx* ctx, uint32_t tag_mask, int32_t stride) {
+    KMH_TRY({
+        KMH_CHECK(ctx, KMH_E_BAD_ARG, "null context");
+        KMH_CHECK(stride >= 1, KMH_E_BAD_ARG, "profile stride must be >= 1");
+        KMH_CHECK(tag_mask != 0 && tag_mask < (1u << KMH_PROF_TAGS), KMH_E_BAD_ARG, "profile tag mask out of range");
+        KMH_CHECK(!ctx->c.prof, KMH_E_STATE, "profile_config inside a begin..end window");
+        ctx->c.prof_mask = tag_mask;
+        ctx->c.prof_stride = stride;
     })
 }
 
@@ -127,9 +151,14 @@ int kmh_ctx_profile_end(kmh_ctx* ctx, double* gemm_flops, double* gemm_ms, int64
         for (const auto& sp : c.spans) {
             float t = 0.f;
             KMH_HIP(hipEventElapsedTime(&t, sp.a, sp.b));
-            c.prof_sum[sp.tag][0] += sp.amount;
-            c.prof_sum[sp.tag][1] += t;
-            c.prof_sum[sp.tag][2] += 1.0;
+            c.prof_sum[KMH_PROF_GEMM][0] += sp.flops;
+            c.prof_sum[KMH_PROF_GEMM][1] += t;
+            c.prof_sum[KMH_PROF_GEMM][2] += 1.0;
+            if (sp.tag != KMH_PROF_GEMM) {
+                c.prof_sum[sp.tag][0] += sp.amount;
+                c.prof_sum[sp.tag][1] += t;
+                c.prof_sum[sp.tag][2] += 1.0;
+            }
         }
         *gemm_flops = c.prof_sum[KMH_PROF_GEMM][0];
         *gemm_ms = c.prof_sum[KMH_PROF_GEMM][1];

@@ -1,8 +1,10 @@
 // Shared host-side infrastructure of libkmhodlr: errors, the context (device,
 // stream, descriptor arena, launch queue, scratch pool), device buffers.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <functional>
@@ -79,25 +81,32 @@ struct Ctx {
     // Launch queue: kernels are recorded and issued, in order and after the
     // descriptor copy, at the next flush.  Every direct operation on the
     // stream goes through q(), which flushes first, so stream order is the
-    // program order.
+    // program order.  After a synchronization the stream is idle, so the
+    // queue is flushed early, at 1, 2, 4, ... pending launches (up to
+    // FLUSH_MAX): the device starts on a stage while the host still plans
+    // the rest of it.
     std::vector<std::function<void()>> pending;
+    static constexpr size_t FLUSH_MAX = 64;
+    size_t flush_at = 1;
     // Device flag for Cholesky failures (non-positive pivots).
     int* d_info = nullptr;
-    // Profiling spans (measurement hooks): HIP events recorded in stream
-    // order around a launch or group of launches, with a tag and the
-    // algorithmic amount (flops or bytes) of the work between them.
+    // Profiling spans (measurement hooks): one GEMM kernel launch each, timed
+    // by a start and a stop event that ride on the dispatch itself, with a
+    // tag, the algorithmic amount (bytes) for that tag and the launch's flops.
     struct ProfSpan {
         int tag;
         double amount;
+        double flops;
         hipEvent_t a, b;
     };
     bool prof = false;
+    uint32_t prof_mask = ~0u;   // tags whose launches are timed
+    int prof_stride = 1;        // time every prof_stride-th launch of a tag
+    uint64_t prof_seen[KMH_PROF_TAGS] = {};
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<ProfSpan> spans;
     double prof_sum[KMH_PROF_TAGS][3] = {};   // per tag: amount, ms, spans
-    size_t prof_begin_span(int tag, double amount);
-    void prof_end_span(size_t i);
     hipEvent_t prof_event();
     // Scratch pool: device chunks carved first-fit on the host.  One in-order
     // stream runs every kernel of the context, so a block freed here may be
@@ -117,6 +126,10 @@ struct Ctx {
     template <class F> void launch(F&& f) {
         pending.emplace_back(std::forward<F>(f));
         if (!queued) note_queued();
+        if (pending.size() >= flush_at) {
+            flush();
+            flush_at = std::min(flush_at * 2, FLUSH_MAX);
+        }
     }
     // Queue kernel<<<g, b, shm>>>(args...): the arguments are converted to the
     // kernel's parameter types now and held by value until the launch.
@@ -130,19 +143,43 @@ struct Ctx {
             }, args);
         });
     }
+    // launch_kernel, timed while profiling is on (see ProfSpan): the events
+    // go into hipExtLaunchKernel, so timing adds nothing between kernels.
+    template <class... P, class... A>
+    void launch_timed(int tag, double amount, double flops, void (*kernel)(P...), dim3 g, dim3 b, unsigned shm,
+                      A&&... a) {
+        if (!prof || !((prof_mask >> tag) & 1u) || prof_seen[tag]++ % (uint64_t)prof_stride != 0) {
+            launch_kernel(kernel, g, b, shm, std::forward<A>(a)...);
+            return;
+        }
+        static_assert(sizeof...(P) == sizeof...(A) && sizeof...(P) > 0, "kernel argument count");
+        hipEvent_t e0 = prof_event(), e1 = prof_event();
+        spans.push_back({tag, amount, flops, e0, e1});
+        launch([kernel, g, b, shm, s = stream, e0, e1, args = std::tuple<P...>(static_cast<P>(a)...)]() mutable {
+            std::apply([&](auto&... x) {
+                void* argv[] = {static_cast<void*>(&x)...};
+                KMH_HIP(hipExtLaunchKernel(reinterpret_cast<const void*>(kernel), g, b, argv, shm, s, e0, e1, 0));
+            }, args);
+        });
+    }
     void note_queued();
     void flush();
     hipStream_t q() { flush(); return stream; }
     void* scratch(size_t bytes);
     void scratch_free(void* p);
-    void sync() { flush(); KMH_HIP(hipStreamSynchronize(stream)); }
+    void sync() {
+        flush();
+        KMH_HIP(hipStreamSynchronize(stream));
+        flush_at = 1;
+    }
     // flush + synchronize without throwing (destructors, teardown)
     void drain() noexcept;
     void release_pool() noexcept;
 };
 
 // Issue the launches every context queued in this thread (end of a C-ABI
-// call), or drop them (a call that failed part-way; nothing it queued runs).
+// call), or drop them (a call that failed part-way: what it queued and did
+// not issue yet does not run).
 void flush_queued();
 void discard_queued() noexcept;
 void forget_queued(Ctx* c) noexcept;
@@ -170,6 +207,9 @@ void forget_queued(Ctx* c) noexcept;
 
 // Record a kernel launch on the context's queue (see Ctx::launch).
 #define KMH_LAUNCH(ctx, kernel, grid, block, ...) (ctx).launch_kernel(kernel, grid, block, 0u, __VA_ARGS__)
+// The same, timed under the profiling hooks with a tag, bytes and flops.
+#define KMH_LAUNCH_TIMED(ctx, tag, amount, flops, kernel, grid, block, ...) \
+    (ctx).launch_timed(tag, amount, flops, kernel, grid, block, 0u, __VA_ARGS__)
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
@@ -191,7 +231,8 @@ struct kmh_ctx {
     bool closing = false;
 };
 namespace kmh {
-void ctx_teardown(kmh_ctx* ctx) noexcept;   // frees the context's device state and ctx itself
+kmh_ctx* ctx_create(int device, uint64_t budget);   // a new context (stream, arena, flags)
+void ctx_teardown(kmh_ctx* ctx) noexcept;           // frees the context's device state and ctx itself
 }
 inline void kmh_ctx_hold(kmh_ctx* ctx) { ++ctx->holders; }
 inline void kmh_ctx_release(kmh_ctx* ctx) noexcept {

@@ -150,6 +150,7 @@ void Ctx::drain() noexcept {
         pending.clear();
     }
     (void)hipStreamSynchronize(stream);
+    flush_at = 1;
 }
 
 hipEvent_t Ctx::prof_event() {
@@ -159,18 +160,6 @@ hipEvent_t Ctx::prof_event() {
         ev_pool.push_back(e);
     }
     return ev_pool[ev_used++];
-}
-
-size_t Ctx::prof_begin_span(int tag, double amount) {
-    hipEvent_t a = prof_event(), b = prof_event();
-    spans.push_back({tag, amount, a, b});
-    launch([a, s = stream]() { KMH_HIP(hipEventRecord(a, s)); });
-    return spans.size() - 1;
-}
-
-void Ctx::prof_end_span(size_t i) {
-    hipEvent_t b = spans[i].b;
-    launch([b, s = stream]() { KMH_HIP(hipEventRecord(b, s)); });
 }
 
 void* Ctx::scratch(size_t bytes) {

@@ -406,6 +406,20 @@ static bool wide_nn(bool ta, bool tb, const GemmProb& p) {
     return !ta && !tb && !p.lower && p.n <= WN_MAX && (p.k <= 512 || p.m >= 64 * 64);
 }
 
+// Algorithmic flops (2mnk) and HBM bytes (A and B read once, C written, and
+// read too when beta != 0) of a launch's descriptor list, for the timed spans.
+static double desc_flops(const std::vector<GemmDesc>& ds) {
+    double f = 0.0;
+    for (const auto& d : ds) f += 2.0 * d.m * d.n * (double)d.k;
+    return f;
+}
+static double desc_bytes(const std::vector<GemmDesc>& ds) {
+    double b = 0.0;
+    for (const auto& d : ds)
+        b += 8.0 * ((double)d.k * d.m + (double)d.k * d.n + (double)d.m * d.n * (d.beta != 0.0 ? 2 : 1));
+    return b;
+}
+
 static void gemm_generic(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
     std::vector<int> split(probs.size(), 1), kchunk(probs.size(), 0);
     size_t wsz = 0;
@@ -465,15 +479,17 @@ static void gemm_generic(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& 
         GemmDesc* dd = c.upload_vec(ds);
         const int* dm = c.upload_vec(map);
         const dim3 g(tiles);
-        if (!ta && !tb) KMH_LAUNCH(c, (k_gemm<false, false>), g, dim3(256), dd, dm);
-        else if (!ta && tb) KMH_LAUNCH(c, (k_gemm<false, true>), g, dim3(256), dd, dm);
-        else if (ta && !tb) KMH_LAUNCH(c, (k_gemm<true, false>), g, dim3(256), dd, dm);
-        else KMH_LAUNCH(c, (k_gemm<true, true>), g, dim3(256), dd, dm);
+        const double fl = c.prof ? desc_flops(ds) : 0.0;
+        const int T = KMH_PROF_GEMM;
+        if (!ta && !tb) KMH_LAUNCH_TIMED(c, T, fl, fl, (k_gemm<false, false>), g, dim3(256), dd, dm);
+        else if (!ta && tb) KMH_LAUNCH_TIMED(c, T, fl, fl, (k_gemm<false, true>), g, dim3(256), dd, dm);
+        else if (ta && !tb) KMH_LAUNCH_TIMED(c, T, fl, fl, (k_gemm<true, false>), g, dim3(256), dd, dm);
+        else KMH_LAUNCH_TIMED(c, T, fl, fl, (k_gemm<true, true>), g, dim3(256), dd, dm);
     }
     if (!rs.empty()) {
         RedDesc* dr = c.upload_vec(rs);
         const int* drm = c.upload_vec(rmap);
-        KMH_LAUNCH(c, k_splitk_reduce, dim3((unsigned)rblk), dim3(256), dr, drm);
+        KMH_LAUNCH_TIMED(c, KMH_PROF_GEMM, 0.0, 0.0, k_splitk_reduce, dim3((unsigned)rblk), dim3(256), dr, drm);
     }
     if (W) c.scratch_free(W);
 }
@@ -508,19 +524,12 @@ static void gemm_tall_tn(Ctx& c, const std::vector<GemmProb>& probs) {
         rblk += nb;
     }
     GemmDesc* dd = c.upload_vec(ds);
-    size_t span = 0;
-    if (c.prof) {
-        double bytes = 0.0;
-        for (const auto& d : ds)
-            bytes += 8.0 * ((double)d.k * d.m + (double)d.k * d.n + (double)d.m * d.n * (d.beta != 0.0 ? 2 : 1));
-        span = c.prof_begin_span(KMH_PROF_TN_TALL, bytes);
-    }
-    KMH_LAUNCH(c, k_gemm_tn_tall, dim3((unsigned)ds.size()), dim3(256), dd);
-    if (c.prof) c.prof_end_span(span);
+    KMH_LAUNCH_TIMED(c, KMH_PROF_TN_TALL, c.prof ? desc_bytes(ds) : 0.0, c.prof ? desc_flops(ds) : 0.0,
+                     k_gemm_tn_tall, dim3((unsigned)ds.size()), dim3(256), dd);
     if (!rs.empty()) {
         RedDesc* dr = c.upload_vec(rs);
         const int* drm = c.upload_vec(rmap);
-        KMH_LAUNCH(c, k_splitk_reduce, dim3((unsigned)rblk), dim3(256), dr, drm);
+        KMH_LAUNCH_TIMED(c, KMH_PROF_GEMM, 0.0, 0.0, k_splitk_reduce, dim3((unsigned)rblk), dim3(256), dr, drm);
     }
     if (W) c.scratch_free(W);
 }
@@ -537,15 +546,8 @@ static void gemm_wide_nn(Ctx& c, const std::vector<GemmProb>& probs) {
     }
     GemmDesc* dd = c.upload_vec(ds);
     const int* dm = c.upload_vec(map);
-    size_t span = 0;
-    if (c.prof) {
-        double bytes = 0.0;
-        for (const auto& p : probs)
-            bytes += 8.0 * ((double)p.m * p.k + (double)p.k * p.n + (double)p.m * p.n * (p.beta != 0.0 ? 2 : 1));
-        span = c.prof_begin_span(KMH_PROF_NN_WIDE, bytes);
-    }
-    KMH_LAUNCH(c, k_gemm_nn_wide, dim3(tiles), dim3(256), dd, dm);
-    if (c.prof) c.prof_end_span(span);
+    KMH_LAUNCH_TIMED(c, KMH_PROF_NN_WIDE, c.prof ? desc_bytes(ds) : 0.0, c.prof ? desc_flops(ds) : 0.0,
+                     k_gemm_nn_wide, dim3(tiles), dim3(256), dd, dm);
 }
 
 static void gemm_rows_nn(Ctx& c, const std::vector<GemmProb>& probs) {
@@ -553,7 +555,6 @@ static void gemm_rows_nn(Ctx& c, const std::vector<GemmProb>& probs) {
     std::vector<GemmDesc> ds[3];
     std::vector<int> map[3];
     int blocks[3] = {0, 0, 0};
-    double bytes = 0.0;
     for (const auto& p : probs) {
         const int cl = p.n <= 8 ? 0 : p.n <= 16 ? 1 : 2;
         const int nb = (int)ceil_div(p.m, 256);
@@ -561,26 +562,24 @@ static void gemm_rows_nn(Ctx& c, const std::vector<GemmProb>& probs) {
         ds[cl].push_back({p.A, p.B, p.C, p.m, p.n, p.k, p.lda, p.ldb, p.ldc, p.alpha, p.beta, blocks[cl], 0, 0, 0,
                           p.Cin});
         blocks[cl] += nb;
-        bytes += 8.0 * ((double)p.m * p.k + (double)p.k * p.n + (double)p.m * p.n * (p.beta != 0.0 ? 2 : 1));
     }
-    const size_t span = c.prof ? c.prof_begin_span(KMH_PROF_NN_ROWS, bytes) : 0;
     for (int cl = 0; cl < 3; ++cl) {
         if (ds[cl].empty()) continue;
         GemmDesc* dd = c.upload_vec(ds[cl]);
         const int* dm = c.upload_vec(map[cl]);
-        if (cl == 0) KMH_LAUNCH(c, k_gemm_nn_rows<8>, dim3(blocks[cl]), dim3(256), dd, dm);
-        else if (cl == 1) KMH_LAUNCH(c, k_gemm_nn_rows<16>, dim3(blocks[cl]), dim3(256), dd, dm);
-        else KMH_LAUNCH(c, k_gemm_nn_rows<32>, dim3(blocks[cl]), dim3(256), dd, dm);
+        const double by = c.prof ? desc_bytes(ds[cl]) : 0.0, fl = c.prof ? desc_flops(ds[cl]) : 0.0;
+        const int T = KMH_PROF_NN_ROWS;
+        const dim3 g(blocks[cl]);
+        if (cl == 0) KMH_LAUNCH_TIMED(c, T, by, fl, k_gemm_nn_rows<8>, g, dim3(256), dd, dm);
+        else if (cl == 1) KMH_LAUNCH_TIMED(c, T, by, fl, k_gemm_nn_rows<16>, g, dim3(256), dd, dm);
+        else KMH_LAUNCH_TIMED(c, T, by, fl, k_gemm_nn_rows<32>, g, dim3(256), dd, dm);
     }
-    if (c.prof) c.prof_end_span(span);
 }
 
 void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
     std::vector<GemmProb> gen, tall, wide, rows;
-    double fl = 0.0;
     for (const auto& p : probs) {
         if (p.m <= 0 || p.n <= 0) continue;
-        fl += 2.0 * p.m * p.n * (double)p.k;
         KMH_CHECK(!p.Cin || p.Cin == p.C || rows_nn(ta, tb, p), KMH_E_BAD_ARG,
                   "gemm: an out-of-place beta term needs n <= 32, k <= 64, no transposes");
         if (tall_tn(ta, tb, p)) tall.push_back(p);
@@ -589,12 +588,10 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
         else gen.push_back(p);
     }
     if (gen.empty() && tall.empty() && wide.empty() && rows.empty()) return;
-    const size_t span = c.prof ? c.prof_begin_span(KMH_PROF_GEMM, fl) : 0;
     if (!gen.empty()) gemm_generic(c, ta, tb, gen);
     if (!tall.empty()) gemm_tall_tn(c, tall);
     if (!wide.empty()) gemm_wide_nn(c, wide);
     if (!rows.empty()) gemm_rows_nn(c, rows);
-    if (c.prof) c.prof_end_span(span);
 }
 
 // ---------------------------------------------------------------------------

@@ -10,11 +10,21 @@
 // so requests that arrive while others are on the GPU join at the next step
 // and their work overlaps.  A request's result is the same alone and in the
 // mix: every reduction has a fixed order independent of the other requests.
+//
+// Lanes: the in-flight requests are spread over lanes, each with its own
+// context (HIP stream, descriptor arena, scratch pool); lane 0 is the
+// caller's context.  A step advances the lanes on parallel host threads, so
+// one lane's latency-bound phases (small tree levels, host synchronizations,
+// launch planning) overlap another lane's bandwidth-bound launches on the
+// device.  Lanes change which launches a request shares, never its
+// arithmetic.
 #include <algorithm>
 #include <chrono>
 #include <deque>
+#include <exception>
 #include <map>
 #include <memory>
+#include <thread>
 
 #include "hodlr.h"
 #include "la.h"
@@ -33,6 +43,7 @@ struct Req {
     int64_t id = 0;
     int stage = ST_WAIT;
     int lv = 0;
+    int lane = 0;
     std::vector<double> pts, y, saa;
     const double* dpts = nullptr;   // device-resident inputs (not owned)
     const double* dy = nullptr;
@@ -50,8 +61,15 @@ struct Req {
     std::vector<double> grad;
 };
 
-struct Server {
+struct Lane {
+    kmh_ctx* own = nullptr;   // private context (lanes after the first)
     Ctx* ctx = nullptr;
+    std::vector<Req*> inflight;
+};
+
+struct Server {
+    Ctx* ctx = nullptr;       // the caller's context (lane 0)
+    std::vector<Lane> lanes;
     int max_inflight = 1;
     uint64_t budget = 0;
     uint64_t reserved = 0;
@@ -81,8 +99,7 @@ uint64_t request_footprint(int64_t n, const kmh_build_opts& o, int nv) {
     return b;
 }
 
-static void finish_batch(Server& S, std::vector<Req*>& rs) {
-    Ctx& c = *S.ctx;
+static void finish_batch(Ctx& c, std::vector<Req*>& rs) {
     std::vector<Hodlr*> hs;
     for (Req* q : rs) hs.push_back(q->H.get());
     std::vector<Status> st;
@@ -179,35 +196,13 @@ static void finish_batch(Server& S, std::vector<Req*>& rs) {
     for (Req* q : rs) q->stage = ST_DONE;
 }
 
-int server_step(Server& S) {
-    Ctx& c = *S.ctx;
+// One stage of every in-flight request of a lane, on the lane's context.
+static void lane_step(Lane& ln) {
+    Ctx& c = *ln.ctx;
     c.set_device();
-    // admission by footprint and slots
-    std::vector<Req*> fresh;
-    while (!S.waiting.empty() && (int)S.inflight.size() < S.max_inflight) {
-        Req* q = S.waiting.front();
-        if (S.budget && S.reserved + q->footprint > S.budget) {
-            if (S.inflight.empty() && fresh.empty()) {
-                // can never fit: fail it instead of blocking the queue
-                S.waiting.pop_front();
-                q->st.code = KMH_E_BUDGET;
-                q->st.msg = "request footprint exceeds the HBM budget";
-                q->stage = ST_DONE;
-                q->t_done = now_us();
-                continue;
-            }
-            break;
-        }
-        S.waiting.pop_front();
-        S.reserved += q->footprint;
-        q->t_admit = now_us();
-        q->stage = ST_BUILD;
-        S.inflight.push_back(q);
-        fresh.push_back(q);
-    }
-    // group in-flight requests by stage (each advances one stage this step)
+    // group the lane's requests by stage (each advances one stage this step)
     std::vector<Req*> build, leaf, level, fin;
-    for (Req* q : S.inflight) {
+    for (Req* q : ln.inflight) {
         switch (q->stage) {
             case ST_BUILD: build.push_back(q); break;
             case ST_LEAF: leaf.push_back(q); break;
@@ -250,20 +245,86 @@ int server_step(Server& S) {
             }
         }
     }
-    if (!fin.empty()) finish_batch(S, fin);
-    // retire finished requests
-    int done = 0;
-    std::vector<Req*> keep;
-    for (Req* q : S.inflight) {
-        if (q->stage == ST_DONE) {
-            q->t_done = now_us();
-            q->H.reset();
-            S.reserved -= q->footprint;
-            ++done;
-        } else {
-            keep.push_back(q);
+    if (!fin.empty()) finish_batch(c, fin);
+}
+
+int server_step(Server& S) {
+    S.ctx->set_device();
+    // admission by footprint and slots; a new request joins the lane with
+    // the fewest in flight
+    while (!S.waiting.empty() && (int)S.inflight.size() < S.max_inflight) {
+        Req* q = S.waiting.front();
+        if (S.budget && S.reserved + q->footprint > S.budget) {
+            if (S.inflight.empty()) {
+                // can never fit: fail it instead of blocking the queue
+                S.waiting.pop_front();
+                q->st.code = KMH_E_BUDGET;
+                q->st.msg = "request footprint exceeds the HBM budget";
+                q->stage = ST_DONE;
+                q->t_done = now_us();
+                continue;
+            }
+            break;
+        }
+        S.waiting.pop_front();
+        S.reserved += q->footprint;
+        q->t_admit = now_us();
+        q->stage = ST_BUILD;
+        size_t best = 0;
+        for (size_t l = 1; l < S.lanes.size(); ++l)
+            if (S.lanes[l].inflight.size() < S.lanes[best].inflight.size()) best = l;
+        q->lane = (int)best;
+        S.lanes[best].inflight.push_back(q);
+        S.inflight.push_back(q);
+    }
+    // advance the lanes: the first active one on this thread, the others on
+    // their own threads (each flushes its own launch queue)
+    std::vector<Lane*> act;
+    for (auto& ln : S.lanes)
+        if (!ln.inflight.empty()) act.push_back(&ln);
+    std::vector<std::exception_ptr> err(act.size());
+    std::vector<std::thread> th;
+    for (size_t i = 1; i < act.size(); ++i)
+        th.emplace_back([&, i]() {
+            try {
+                lane_step(*act[i]);
+                flush_queued();
+            } catch (...) {
+                discard_queued();
+                err[i] = std::current_exception();
+            }
+        });
+    if (!act.empty()) {
+        try {
+            lane_step(*act[0]);
+            flush_queued();
+        } catch (...) {
+            discard_queued();
+            err[0] = std::current_exception();
         }
     }
+    for (auto& t : th) t.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+    // retire finished requests
+    int done = 0;
+    for (auto& ln : S.lanes) {
+        std::vector<Req*> keep;
+        for (Req* q : ln.inflight) {
+            if (q->stage == ST_DONE) {
+                q->t_done = now_us();
+                q->H.reset();
+                S.reserved -= q->footprint;
+                ++done;
+            } else {
+                keep.push_back(q);
+            }
+        }
+        ln.inflight.swap(keep);
+    }
+    std::vector<Req*> keep;
+    for (Req* q : S.inflight)
+        if (q->stage != ST_DONE) keep.push_back(q);
     S.inflight.swap(keep);
     return done;
 }
@@ -278,10 +339,28 @@ struct kmh_server {
     kmh_ctx* owner = nullptr;
     explicit kmh_server(kmh_ctx* c) : owner(c) { kmh_ctx_hold(c); }
     ~kmh_server() {
-        s.all.clear();   // request handles back to the context's pool first
+        s.all.clear();   // request handles back to their lanes' pools first
         s.inflight.clear();
         s.waiting.clear();
+        drop_lanes();
         kmh_ctx_release(owner);
+    }
+    void drop_lanes() noexcept {
+        for (auto& ln : s.lanes)
+            if (ln.own) kmh::ctx_teardown(ln.own);
+        s.lanes.clear();
+    }
+    // lane 0 runs on the caller's context, the others on private ones
+    void set_lanes(int n) {
+        drop_lanes();
+        s.lanes.resize(1);
+        s.lanes[0].ctx = s.ctx;
+        for (int l = 1; l < n; ++l) {
+            kmh::Lane ln;
+            ln.own = kmh::ctx_create(s.ctx->device, 0);
+            ln.ctx = &ln.own->c;
+            s.lanes.push_back(ln);
+        }
     }
 };
 
@@ -292,18 +371,30 @@ extern "C" {
 int kmh_server_create(kmh_ctx* ctx, int32_t max_inflight, uint64_t hbm_budget_bytes, kmh_server** out) {
     SRV_TRY({
         KMH_CHECK(ctx && out && max_inflight >= 1, KMH_E_BAD_ARG, "invalid server arguments");
-        auto* s = new kmh_server(ctx);
+        std::unique_ptr<kmh_server> s(new kmh_server(ctx));
         s->s.ctx = &ctx->c;
         s->s.max_inflight = max_inflight;
         s->s.budget = hbm_budget_bytes;
-        *out = s;
+        s->set_lanes(ctx->c.budget == 0 && max_inflight >= 2 ? 2 : 1);
+        *out = s.release();
+    })
+}
+
+int kmh_server_set_lanes(kmh_server* srv, int32_t lanes) {
+    SRV_TRY({
+        KMH_CHECK(srv, KMH_E_BAD_ARG, "null server");
+        KMH_CHECK(lanes >= 1 && lanes <= KMH_SERVE_MAX_LANES, KMH_E_BAD_ARG, "lane count out of range");
+        KMH_CHECK(srv->s.inflight.empty() && srv->s.waiting.empty(), KMH_E_STATE,
+                  "lanes can change only while no request is queued or running");
+        srv->s.ctx->set_device();
+        srv->set_lanes(lanes);
     })
 }
 
 int kmh_server_destroy(kmh_server* srv) {
     SRV_TRY({
         if (srv) {
-            srv->s.ctx->sync();
+            for (auto& ln : srv->s.lanes) ln.ctx->sync();
             delete srv;
         }
     })

@@ -76,6 +76,7 @@ SIGNATURES = {
     "kmh_ctx_profile_begin": [_P],
     "kmh_ctx_profile_end": [_P, _D, _D, C.POINTER(C.c_int64)],
     "kmh_ctx_profile_query": [_P, C.c_int32, _D, _D, C.POINTER(C.c_int64)],
+    "kmh_ctx_profile_config": [_P, C.c_uint32, C.c_int32],
     "kmh_hodlr_build": [_P, _P, _I64, C.c_int32, C.POINTER(kmh_kernel), C.POINTER(kmh_build_opts), C.POINTER(_P)],
     "kmh_hodlr_build_dev": [_P, _P, _I64, C.c_int32, C.POINTER(kmh_kernel), C.POINTER(kmh_build_opts), C.POINTER(_P)],
     "kmh_hodlr_destroy": [_P],
@@ -102,6 +103,7 @@ SIGNATURES = {
     "kmh_stoch_hessian": [_P, C.POINTER(kmh_kernel), C.c_int32, C.POINTER(kmh_kernel), _P, _P, _I64, C.c_int32,
                           _P],
     "kmh_server_create": [_P, C.c_int32, C.c_uint64, C.POINTER(_P)],
+    "kmh_server_set_lanes": [_P, C.c_int32],
     "kmh_server_destroy": [_P],
     "kmh_server_submit": [_P, C.POINTER(kmh_serve_req), C.POINTER(C.c_int64)],
     "kmh_server_step": [_P, C.POINTER(C.c_int32)],

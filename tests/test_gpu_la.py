@@ -150,3 +150,43 @@ def test_descriptor_arena_ring(ctx):
     rc = lib.kmh_test_arena(ctx.ptr, 3000, 12345, C.byref(bad))
     assert rc == 0, lib.last_error()
     assert bad.value == 0
+
+
+def test_profile_hooks_time_sampled_launches(gpu_lib):
+    """kmh_ctx_profile_config: only the masked tag is timed, at every
+    stride-th launch; the GEMM totals then cover those launches.  The wide NN
+    shape (4097 x 144, k = 72) routes to k_gemm_nn_wide."""
+    import kmhodlr
+    from kmhodlr import _lib as L
+    ctx = kmhodlr.Context(0)
+    lib = ctx.lib
+    rng = np.random.default_rng(7)
+    m, n, k = 4097, 144, 72
+    A, B = rng.standard_normal((m, k)), rng.standard_normal((k, n))
+    byt = 8.0 * (m * k + k * n + m * n)
+    fl = 2.0 * m * n * k
+
+    def window(calls):
+        assert lib.kmh_ctx_profile_begin(ctx.ptr) == 0
+        for _ in range(calls):
+            out = _gemm(ctx, 0, 0, A, B, np.zeros((m, n)))
+        gfl, gms, gl = C.c_double(), C.c_double(), C.c_int64()
+        assert lib.kmh_ctx_profile_end(ctx.ptr, C.byref(gfl), C.byref(gms), C.byref(gl)) == 0
+        wb, wms, wl = C.c_double(), C.c_double(), C.c_int64()
+        assert lib.kmh_ctx_profile_query(ctx.ptr, L.KMH_PROF_NN_WIDE, C.byref(wb), C.byref(wms), C.byref(wl)) == 0
+        np.testing.assert_allclose(out, A @ B, rtol=1e-12, atol=1e-12 * np.abs(A @ B).max())
+        return gfl.value, gms.value, gl.value, wb.value, wms.value, wl.value
+
+    gfl, gms, gl, wb, wms, wl = window(5)          # defaults: every launch of every tag
+    assert gl == 5 and wl == 5 and gfl == 5 * fl and wb == 5 * byt and wms > 0 and gms >= wms
+    assert lib.kmh_ctx_profile_config(ctx.ptr, 1 << L.KMH_PROF_NN_WIDE, 2) == 0
+    gfl, gms, gl, wb, wms, wl = window(5)          # launches 0, 2, 4
+    assert wl == 3 and gl == 3 and wb == 3 * byt and gfl == 3 * fl and wms > 0
+    assert lib.kmh_ctx_profile_config(ctx.ptr, 1 << L.KMH_PROF_TN_TALL, 1) == 0
+    assert window(2)[5] == 0                       # NN_WIDE not in the mask
+    assert lib.kmh_ctx_profile_config(ctx.ptr, 0, 1) == L.KMH_E_BAD_ARG
+    assert lib.kmh_ctx_profile_config(ctx.ptr, 1, 0) == L.KMH_E_BAD_ARG
+    assert lib.kmh_ctx_profile_begin(ctx.ptr) == 0
+    assert lib.kmh_ctx_profile_config(ctx.ptr, 1, 1) == L.KMH_E_STATE
+    d = C.c_double()
+    assert lib.kmh_ctx_profile_end(ctx.ptr, C.byref(d), C.byref(d), C.byref(C.c_int64())) == 0

@@ -49,9 +49,11 @@ def _problems(n_list, seed=0, nv=8):
     return out
 
 
-def _serve(lib, L, ctx, probs, max_inflight, budget=0, order=None):
+def _serve(lib, L, ctx, probs, max_inflight, budget=0, order=None, lanes=None):
     srv = C.c_void_p()
     assert lib.kmh_server_create(ctx.ptr, max_inflight, budget, C.byref(srv)) == 0
+    if lanes is not None:
+        assert lib.kmh_server_set_lanes(srv, lanes) == 0, lib.last_error()
     ids = {}
     keep = []
     order = order or list(range(len(probs)))
@@ -94,6 +96,49 @@ def test_serve_alone_equals_mixed(gpu_lib):
         assert alone[i][2] == mixed[i][2] and alone[i][3] == mixed[i][3]
         assert alone[i][4] == mixed[i][4]
         print("request", i, "nll", alone[i][2], "grad", alone[i][4])
+
+
+def test_serve_lanes_do_not_change_results(gpu_lib):
+    """The same requests through 1, 2 and 3 lanes (private contexts stepped
+    on parallel host threads): bitwise the same nll, logdet and gradient."""
+    import kmhodlr as km
+    from kmhodlr import _lib as L
+    lib = km.load()
+    ctx = km.Context(0)
+    probs = _problems([1500, 2600, 900, 3100, 2048, 1200], seed=11)
+    ref, _ = _serve(lib, L, ctx, probs, 6, lanes=1)
+    for lanes in (2, 3):
+        res, peak = _serve(lib, L, ctx, probs, 6, order=[5, 2, 0, 4, 1, 3], lanes=lanes)
+        assert peak > 1
+        for i in range(len(probs)):
+            assert res[i][0] == 0 and res[i][1] == 0, lib.last_error()
+            assert res[i][2] == ref[i][2] and res[i][3] == ref[i][3] and res[i][4] == ref[i][4]
+
+
+def test_serve_set_lanes_errors(gpu_lib):
+    import kmhodlr as km
+    from kmhodlr import _lib as L
+    lib = km.load()
+    ctx = km.Context(0)
+    srv = C.c_void_p()
+    assert lib.kmh_server_create(ctx.ptr, 4, 0, C.byref(srv)) == 0
+    assert lib.kmh_server_set_lanes(srv, 0) == L.KMH_E_BAD_ARG
+    assert lib.kmh_server_set_lanes(srv, 5) == L.KMH_E_BAD_ARG
+    assert lib.kmh_server_set_lanes(srv, 4) == 0
+    (x, y, saa, params, level), = _problems([1024], seed=2)
+    r = _mkreq(L, x, y, saa, params, level, 40, grad=False)
+    rid = C.c_int64()
+    assert lib.kmh_server_submit(srv, C.byref(r), C.byref(rid)) == 0
+    assert lib.kmh_server_set_lanes(srv, 1) == L.KMH_E_STATE     # a request is queued
+    done = C.c_int32()
+    for _ in range(100):
+        assert lib.kmh_server_step(srv, C.byref(done)) == 0, lib.last_error()
+        if done.value:
+            break
+    out = L.kmh_serve_result()
+    assert lib.kmh_server_result(srv, rid.value, C.byref(out)) == 0 and out.status == 0
+    assert lib.kmh_server_set_lanes(srv, 1) == 0
+    assert lib.kmh_server_destroy(srv) == 0
 
 
 def test_serve_matches_direct_path(gpu_lib):
