@@ -30,6 +30,11 @@ int kmh_test_qr(kmh_ctx* ctx, int32_t m, int32_t r, double* A, int32_t lda, doub
  * back by a kernel that checks its contents; *bad = uploads whose kernel saw
  * other data (0 when the ring's fences hold) */
 int kmh_test_arena(kmh_ctx* ctx, int32_t count, int32_t words, int64_t* bad);
+/* the timed launches of the open profiling window (kmh_ctx_profile_begin),
+ * in launch order: tag, algorithmic amount, flops, kernel ms, workgroups;
+ * up to cap entries, *count = all of them                                */
+int kmh_test_profile_spans(kmh_ctx* ctx, int32_t cap, int32_t* tags, double* amounts, double* flops, double* ms,
+                           int32_t* grids, int32_t* count);
 
 #ifdef __cplusplus
 }

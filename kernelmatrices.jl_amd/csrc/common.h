@@ -98,6 +98,7 @@ struct Ctx {
         double amount;
         double flops;
         hipEvent_t a, b;
+        unsigned grid;   // workgroups of the launch
     };
     bool prof = false;
     uint32_t prof_mask = ~0u;   // tags whose launches are timed
@@ -154,7 +155,7 @@ struct Ctx {
         }
         static_assert(sizeof...(P) == sizeof...(A) && sizeof...(P) > 0, "kernel argument count");
         hipEvent_t e0 = prof_event(), e1 = prof_event();
-        spans.push_back({tag, amount, flops, e0, e1});
+        spans.push_back({tag, amount, flops, e0, e1, g.x});
         launch([kernel, g, b, shm, s = stream, e0, e1, args = std::tuple<P...>(static_cast<P>(a)...)]() mutable {
             std::apply([&](auto&... x) {
                 void* argv[] = {static_cast<void*>(&x)...};

@@ -13,7 +13,7 @@ namespace kmh {
 // above the diagonal (symmetric rank-k updates of a lower triangle).
 // Cin (optional, same leading dimension as C): the beta term reads Cin
 // instead of C (out-of-place update); only for untransposed products with
-// n <= 32 and k <= 64.
+// n <= 32 and k <= 64 (k <= 128 for n <= 16, k <= 256 for n <= 8).
 struct GemmProb {
     int m, n, k;
     const double* A; int lda;

@@ -182,46 +182,57 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(const RedDesc* __restrict
 // workgroup holds the whole (16-padded) output for a slab of TT_KS rows of k,
 // so A and B are read once; slabs of one problem are summed by the split-k
 // reduction in slab order.  The 16x16 output blocks are dealt round-robin to
-// the four waves.
+// the four waves.  Two staging shapes: KC = 16 rows of k by up to 80 columns
+// per step, and for outputs of at most 32 x 32 KC = 64 rows by 32 columns,
+// so narrow products keep four times the loads in flight.  Both issue the
+// MFMAs in the same k order, so the shape class never changes a result.
 constexpr int TT_MAX = 80;
 constexpr int TT_KS = 256;
-constexpr int TT_KC = 16;
+constexpr int TT_NARROW = 32;
 
+template <int KC, int MW>
 __global__ __launch_bounds__(256, 2) void k_gemm_tn_tall(const GemmDesc* __restrict__ ds) {
-    __shared__ double As[TT_KC][TT_MAX + 1];
-    __shared__ double Bs[TT_KC][TT_MAX + 1];
+    constexpr int NG = 256 / KC;                      // column groups of the staging
+    constexpr int CPT = (MW + NG - 1) / NG;           // columns per thread and operand
+    constexpr int NBLK = ((MW + 15) / 16) * ((MW + 15) / 16);
+    constexpr int T = (NBLK + 3) / 4;                 // output blocks per wave
+    __shared__ double As[KC][MW + 1];
+    __shared__ double Bs[KC][MW + 1];
     const GemmDesc d = ds[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int NB = (d.n + 15) >> 4, nblk = ((d.m + 15) >> 4) * NB;
-    d4 acc[7];
+    d4 acc[T];
 #pragma unroll
-    for (int t = 0; t < 7; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
-    // staging: thread tid moves (k = tid & 15, column (tid >> 4) + 16 q)
-    const int skk = tid & 15, scol = tid >> 4;
-    double ra[5], rb[5];
+    for (int t = 0; t < T; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
+    // staging: thread tid moves (k = tid % KC, columns tid / KC + NG q)
+    const int skk = tid % KC, scol = tid / KC;
+    double ra[CPT], rb[CPT];
     auto fetch = [&](int k0) {
         const int gk = k0 + skk;
 #pragma unroll
-        for (int q = 0; q < 5; ++q) {
-            const int col = scol + 16 * q;
+        for (int q = 0; q < CPT; ++q) {
+            const int col = scol + NG * q;
             ra[q] = (col < d.m && gk < d.k) ? d.A[gk + (size_t)col * d.lda] : 0.0;
             rb[q] = (col < d.n && gk < d.k) ? d.B[gk + (size_t)col * d.ldb] : 0.0;
         }
     };
     fetch(0);
-    for (int k0 = 0; k0 < d.k; k0 += TT_KC) {
+    for (int k0 = 0; k0 < d.k; k0 += KC) {
 #pragma unroll
-        for (int q = 0; q < 5; ++q) {
-            As[skk][scol + 16 * q] = ra[q];
-            Bs[skk][scol + 16 * q] = rb[q];
+        for (int q = 0; q < CPT; ++q) {
+            const int col = scol + NG * q;
+            if (col < MW) {
+                As[skk][col] = ra[q];
+                Bs[skk][col] = rb[q];
+            }
         }
         __syncthreads();
-        if (k0 + TT_KC < d.k) fetch(k0 + TT_KC);
+        if (k0 + KC < d.k) fetch(k0 + KC);
 #pragma unroll
-        for (int ks = 0; ks < TT_KC / 4; ++ks) {
+        for (int ks = 0; ks < KC / 4; ++ks) {
             const int kk = ks * 4 + (lane >> 4);
 #pragma unroll
-            for (int t = 0; t < 7; ++t) {
+            for (int t = 0; t < T; ++t) {
                 const int b = wave + 4 * t;
                 if (b < nblk) {
                     const int mb = b / NB, nb = b - mb * NB;
@@ -233,9 +244,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn_tall(const GemmDesc* __restr
         }
         __syncthreads();
     }
-    double cv[7][4];
+    double cv[T][4];
 #pragma unroll
-    for (int t = 0; t < 7; ++t) {
+    for (int t = 0; t < T; ++t) {
         const int b = wave + 4 * t;
         const int mb = b / NB, nb = b - mb * NB;
 #pragma unroll
@@ -245,7 +256,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn_tall(const GemmDesc* __restr
         }
     }
 #pragma unroll
-    for (int t = 0; t < 7; ++t) {
+    for (int t = 0; t < T; ++t) {
         const int b = wave + 4 * t;
         const int mb = b / NB, nb = b - mb * NB;
 #pragma unroll
@@ -343,24 +354,28 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn_wide(const GemmDesc* __restr
     }
 }
 
-// Narrow NN products C = alpha A B + beta C with n <= 32 and k <= 64 (the
-// panel QR updates X -= Q R and the TSQR assembly Q <- Q C): one thread per
-// row of C, B in LDS (broadcast reads), the row of A streamed once with
-// coalesced loads and n accumulators in registers.  The multiply-adds run
+// Narrow NN products C = alpha A B + beta C with n <= 32 and k <= 2048 / NMAX
+// (NMAX = 8, 16, 32: the panel QR updates X -= Q R, the TSQR assembly
+// Q <- Q C, the one-column Woodbury updates of a solve): one thread per row
+// of C, B in LDS (broadcast reads), the row of A streamed once with
+// coalesced loads and n accumulators in registers.  ROWS threads (128 or
+// 256) per workgroup, 128 when every problem of the launch has <= 128 rows.  The multiply-adds run
 // in k order for each element.  A thread reads its whole row of A before it
 // writes its row of C, so C may alias A.
-constexpr int NR_K = 64;
+// k limit of the row kernel per width class: B (k x NMAX) fills 16 KB of LDS
+template <int NMAX> constexpr int nr_kmax() { return 2048 / NMAX; }
 
-template <int NMAX>
-__global__ __launch_bounds__(256) void k_gemm_nn_rows(const GemmDesc* __restrict__ ds, const int* __restrict__ map) {
-    __shared__ double Bs[NR_K * NMAX];
+template <int NMAX, int ROWS>
+__global__ __launch_bounds__(ROWS) void k_gemm_nn_rows(const GemmDesc* __restrict__ ds, const int* __restrict__ map) {
+    constexpr int KMAX = nr_kmax<NMAX>();
+    __shared__ double Bs[KMAX * NMAX];
     const GemmDesc d = ds[map[blockIdx.x]];
-    for (int e = threadIdx.x; e < NR_K * NMAX; e += 256) {
+    for (int e = threadIdx.x; e < KMAX * NMAX; e += ROWS) {
         const int kk = e / NMAX, j = e - kk * NMAX;
         Bs[e] = (kk < d.k && j < d.n) ? d.B[kk + (size_t)j * d.ldb] : 0.0;
     }
     __syncthreads();
-    const int row = (blockIdx.x - d.tile0) * 256 + threadIdx.x;
+    const int row = (blockIdx.x - d.tile0) * ROWS + threadIdx.x;
     if (row >= d.m) return;
     double acc[NMAX];
 #pragma unroll
@@ -400,7 +415,8 @@ static bool tall_tn(bool ta, bool tb, const GemmProb& p) {
     return ta && !tb && !p.lower && p.m <= TT_MAX && p.n <= TT_MAX;
 }
 static bool rows_nn(bool ta, bool tb, const GemmProb& p) {
-    return !ta && !tb && !p.lower && p.n <= 32 && p.k <= NR_K;
+    if (ta || tb || p.lower || p.n > 32) return false;
+    return p.k <= (p.n <= 8 ? nr_kmax<8>() : p.n <= 16 ? nr_kmax<16>() : nr_kmax<32>());
 }
 static bool wide_nn(bool ta, bool tb, const GemmProb& p) {
     return !ta && !tb && !p.lower && p.n <= WN_MAX && (p.k <= 512 || p.m >= 64 * 64);
@@ -523,9 +539,19 @@ static void gemm_tall_tn(Ctx& c, const std::vector<GemmProb>& probs) {
         woff += (size_t)s * p.m * p.n;
         rblk += nb;
     }
-    GemmDesc* dd = c.upload_vec(ds);
-    KMH_LAUNCH_TIMED(c, KMH_PROF_TN_TALL, c.prof ? desc_bytes(ds) : 0.0, c.prof ? desc_flops(ds) : 0.0,
-                     k_gemm_tn_tall, dim3((unsigned)ds.size()), dim3(256), dd);
+    // one launch per staging shape (by the problem's own m and n)
+    std::vector<GemmDesc> dn, dw;
+    for (const auto& d : ds) (d.m <= TT_NARROW && d.n <= TT_NARROW ? dn : dw).push_back(d);
+    if (!dn.empty()) {
+        GemmDesc* dd = c.upload_vec(dn);
+        KMH_LAUNCH_TIMED(c, KMH_PROF_TN_TALL, c.prof ? desc_bytes(dn) : 0.0, c.prof ? desc_flops(dn) : 0.0,
+                         (k_gemm_tn_tall<64, TT_NARROW>), dim3((unsigned)dn.size()), dim3(256), dd);
+    }
+    if (!dw.empty()) {
+        GemmDesc* dd = c.upload_vec(dw);
+        KMH_LAUNCH_TIMED(c, KMH_PROF_TN_TALL, c.prof ? desc_bytes(dw) : 0.0, c.prof ? desc_flops(dw) : 0.0,
+                         (k_gemm_tn_tall<16, TT_MAX>), dim3((unsigned)dw.size()), dim3(256), dd);
+    }
     if (!rs.empty()) {
         RedDesc* dr = c.upload_vec(rs);
         const int* drm = c.upload_vec(rmap);
@@ -551,28 +577,39 @@ static void gemm_wide_nn(Ctx& c, const std::vector<GemmProb>& probs) {
 }
 
 static void gemm_rows_nn(Ctx& c, const std::vector<GemmProb>& probs) {
-    // one launch per width class (8, 16, 32)
-    std::vector<GemmDesc> ds[3];
-    std::vector<int> map[3];
-    int blocks[3] = {0, 0, 0};
-    for (const auto& p : probs) {
-        const int cl = p.n <= 8 ? 0 : p.n <= 16 ? 1 : 2;
-        const int nb = (int)ceil_div(p.m, 256);
-        map_append(map[cl], (int)ds[cl].size(), nb);
-        ds[cl].push_back({p.A, p.B, p.C, p.m, p.n, p.k, p.lda, p.ldb, p.ldc, p.alpha, p.beta, blocks[cl], 0, 0, 0,
-                          p.Cin});
-        blocks[cl] += nb;
-    }
+    // one launch per width class (8, 16, 32); 128-row workgroups when every
+    // problem of the class has at most 128 rows (TSQR tiles), else 256
+    std::vector<const GemmProb*> cls[3];
+    for (const auto& p : probs) cls[p.n <= 8 ? 0 : p.n <= 16 ? 1 : 2].push_back(&p);
     for (int cl = 0; cl < 3; ++cl) {
-        if (ds[cl].empty()) continue;
-        GemmDesc* dd = c.upload_vec(ds[cl]);
-        const int* dm = c.upload_vec(map[cl]);
-        const double by = c.prof ? desc_bytes(ds[cl]) : 0.0, fl = c.prof ? desc_flops(ds[cl]) : 0.0;
+        if (cls[cl].empty()) continue;
+        int mmax = 0;
+        for (const GemmProb* p : cls[cl]) mmax = std::max(mmax, p->m);
+        const int rows = mmax <= 128 ? 128 : 256;
+        std::vector<GemmDesc> ds;
+        std::vector<int> map;
+        int blocks = 0;
+        for (const GemmProb* pp : cls[cl]) {
+            const GemmProb& p = *pp;
+            const int nb = (int)ceil_div(p.m, rows);
+            map_append(map, (int)ds.size(), nb);
+            ds.push_back({p.A, p.B, p.C, p.m, p.n, p.k, p.lda, p.ldb, p.ldc, p.alpha, p.beta, blocks, 0, 0, 0, p.Cin});
+            blocks += nb;
+        }
+        GemmDesc* dd = c.upload_vec(ds);
+        const int* dm = c.upload_vec(map);
+        const double by = c.prof ? desc_bytes(ds) : 0.0, fl = c.prof ? desc_flops(ds) : 0.0;
         const int T = KMH_PROF_NN_ROWS;
-        const dim3 g(blocks[cl]);
-        if (cl == 0) KMH_LAUNCH_TIMED(c, T, by, fl, k_gemm_nn_rows<8>, g, dim3(256), dd, dm);
-        else if (cl == 1) KMH_LAUNCH_TIMED(c, T, by, fl, k_gemm_nn_rows<16>, g, dim3(256), dd, dm);
-        else KMH_LAUNCH_TIMED(c, T, by, fl, k_gemm_nn_rows<32>, g, dim3(256), dd, dm);
+        const dim3 g(blocks), b(rows);
+        if (rows == 128) {
+            if (cl == 0) KMH_LAUNCH_TIMED(c, T, by, fl, (k_gemm_nn_rows<8, 128>), g, b, dd, dm);
+            else if (cl == 1) KMH_LAUNCH_TIMED(c, T, by, fl, (k_gemm_nn_rows<16, 128>), g, b, dd, dm);
+            else KMH_LAUNCH_TIMED(c, T, by, fl, (k_gemm_nn_rows<32, 128>), g, b, dd, dm);
+        } else {
+            if (cl == 0) KMH_LAUNCH_TIMED(c, T, by, fl, (k_gemm_nn_rows<8, 256>), g, b, dd, dm);
+            else if (cl == 1) KMH_LAUNCH_TIMED(c, T, by, fl, (k_gemm_nn_rows<16, 256>), g, b, dd, dm);
+            else KMH_LAUNCH_TIMED(c, T, by, fl, (k_gemm_nn_rows<32, 256>), g, b, dd, dm);
+        }
     }
 }
 
@@ -581,7 +618,7 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
     for (const auto& p : probs) {
         if (p.m <= 0 || p.n <= 0) continue;
         KMH_CHECK(!p.Cin || p.Cin == p.C || rows_nn(ta, tb, p), KMH_E_BAD_ARG,
-                  "gemm: an out-of-place beta term needs n <= 32, k <= 64, no transposes");
+                  "gemm: an out-of-place beta term needs n <= 32, k <= 2048 / (8, 16 or 32), no transposes");
         if (tall_tn(ta, tb, p)) tall.push_back(p);
         else if (rows_nn(ta, tb, p)) rows.push_back(p);
         else if (wide_nn(ta, tb, p)) wide.push_back(p);

@@ -113,4 +113,27 @@ int kmh_test_arena(kmh_ctx* ctx, int32_t count, int32_t words, int64_t* bad) {
     })
 }
 
+int kmh_test_profile_spans(kmh_ctx* ctx, int32_t cap, int32_t* tags, double* amounts, double* flops, double* ms,
+                           int32_t* grids, int32_t* count) {
+    TRY_({
+        KMH_CHECK(ctx && count && cap >= 0 && (cap == 0 || (tags && amounts && flops && ms && grids)),
+                  KMH_E_BAD_ARG, "null argument");
+        kmh::Ctx& c = ctx->c;
+        KMH_CHECK(c.prof, KMH_E_STATE, "no open profiling window");
+        c.set_device();
+        c.sync();
+        *count = (int32_t)c.spans.size();
+        for (int32_t i = 0; i < cap && i < (int32_t)c.spans.size(); ++i) {
+            const auto& sp = c.spans[(size_t)i];
+            float t = 0.f;
+            KMH_HIP(hipEventElapsedTime(&t, sp.a, sp.b));
+            tags[i] = sp.tag;
+            amounts[i] = sp.amount;
+            flops[i] = sp.flops;
+            ms[i] = t;
+            grids[i] = (int32_t)sp.grid;
+        }
+    })
+}
+
 }  // extern "C"

@@ -35,8 +35,11 @@ def _gemm(ctx, ta, tb, A, B, Cm, alpha=1.0, beta=0.0, lower=0):
                                    (72, 72, 256), (72, 72, 257), (80, 80, 4097), (32, 8, 32768),
                                    (80, 1, 513), (1, 80, 255), (81, 80, 300),
                                    (4097, 144, 72), (8192, 8, 64), (4096, 16, 600), (63, 32, 17),
-                                   # row-per-thread narrow A B shapes
-                                   (300, 32, 64), (5000, 16, 33), (77, 8, 8), (257, 9, 1)])
+                                   # row-per-thread narrow A B shapes (k up to 2048 / 8, 16, 32;
+                                   # 128-row workgroups when m <= 128)
+                                   (300, 32, 64), (5000, 16, 33), (77, 8, 8), (257, 9, 1),
+                                   (1000, 1, 72), (700, 8, 256), (300, 16, 128), (129, 12, 100),
+                                   (128, 32, 32), (100, 5, 200), (128, 8, 257)])
 def test_gemm_matches_numpy(ctx, ta, tb, m, n, k):
     print("gemm", ta, tb, m, n, k, flush=True)
     rng = np.random.default_rng(m * 1000 + n * 10 + k)
@@ -184,6 +187,20 @@ def test_profile_hooks_time_sampled_launches(gpu_lib):
     assert wl == 3 and gl == 3 and wb == 3 * byt and gfl == 3 * fl and wms > 0
     assert lib.kmh_ctx_profile_config(ctx.ptr, 1 << L.KMH_PROF_TN_TALL, 1) == 0
     assert window(2)[5] == 0                       # NN_WIDE not in the mask
+    assert lib.kmh_ctx_profile_config(ctx.ptr, 0xF, 1) == 0
+    assert lib.kmh_ctx_profile_begin(ctx.ptr) == 0   # per-launch spans (testing hook)
+    _gemm(ctx, 0, 0, A, B, np.zeros((m, n)))
+    _gemm(ctx, 0, 0, A, B, np.zeros((m, n)))
+    cap = 4
+    tg, gr, cnt = (C.c_int32 * cap)(), (C.c_int32 * cap)(), C.c_int32()
+    am, fp, ms = (C.c_double * cap)(), (C.c_double * cap)(), (C.c_double * cap)()
+    assert lib.kmh_test_profile_spans(ctx.ptr, cap, tg, am, fp, ms, gr, C.byref(cnt)) == 0
+    assert cnt.value == 2
+    for i in range(2):
+        assert tg[i] == L.KMH_PROF_NN_WIDE and am[i] == byt and fp[i] == fl and ms[i] > 0 and gr[i] == (m + 63) // 64
+    d = C.c_double()
+    assert lib.kmh_ctx_profile_end(ctx.ptr, C.byref(d), C.byref(d), C.byref(C.c_int64())) == 0
+    assert lib.kmh_test_profile_spans(ctx.ptr, cap, tg, am, fp, ms, gr, C.byref(cnt)) == L.KMH_E_STATE
     assert lib.kmh_ctx_profile_config(ctx.ptr, 0, 1) == L.KMH_E_BAD_ARG
     assert lib.kmh_ctx_profile_config(ctx.ptr, 1, 0) == L.KMH_E_BAD_ARG
     assert lib.kmh_ctx_profile_begin(ctx.ptr) == 0
