@@ -6,8 +6,11 @@ the library, y ~ N(0,1); Matern nu=3/2 with a 1e-6 nugget; FixedLevel(8) (256 le
 256 points); Nystrom rank 72 (the served objective of the reference,
 nll_objective, always uses Nystrom).  A request is one log-likelihood
 evaluation, negloglik (maxlikfunctions.jl:2-9): HODLR build +
-symmetricfactorize! + logdet + one solve, FP64 on the GPU, at its own kernel
+symmetricfactorize! + logdet + y' K^-1 y, FP64 on the GPU, at its own kernel
 parameters (sigma ~ U[0.5, 2], rho ~ U[0.05, 0.2]), inputs resident in HBM.
+Both legs take y' K^-1 y as |W^-1 y|^2 with the symmetric factor K = W W^T
+(one W^-1 application instead of the full solve); the gradient legs apply
+W^-T on top where the gradient needs K^-1 y.
 
 Legs: `plain` runs the requests one after another through the per-request
 C ABI; `served` keeps `--inflight` requests in the server, which advances
@@ -208,6 +211,13 @@ def main():
     out = {}
 
     # ---- plain: one request at a time ----------------------------------
+    def quad_form(h):
+        """y' K^-1 y = |W^-1 y|^2 with the symmetric factor K = W W^T, the
+        same arithmetic as the server's"""
+        check(lib.kmh_hodlr_factor_apply_dev(h, L.KMH_WINV, y_d.data_ptr(), sol_d.data_ptr(), 1))
+        check(lib.kmh_ctx_synchronize(ctx.ptr))
+        return float(torch.dot(sol_d, sol_d))
+
     def plain_eval(p):
         h = C.c_void_p()
         kern = kernel_of(p)
@@ -215,9 +225,7 @@ def main():
         check(lib.kmh_hodlr_factorize(h))
         ld = C.c_double()
         check(lib.kmh_hodlr_logdet(h, C.byref(ld)))
-        check(lib.kmh_hodlr_solve_dev(h, y_d.data_ptr(), sol_d.data_ptr(), 1))
-        check(lib.kmh_ctx_synchronize(ctx.ptr))
-        quad = float(torch.dot(y_d, sol_d))
+        quad = quad_form(h)
         if not info:
             ii = L.kmh_info()
             check(lib.kmh_hodlr_info(h, C.byref(ii)))
@@ -233,9 +241,7 @@ def main():
         check(lib.kmh_hodlr_factorize(h))
         ld = C.c_double()
         check(lib.kmh_hodlr_logdet(h, C.byref(ld)))
-        check(lib.kmh_hodlr_solve_dev(h, y_d.data_ptr(), sol_d.data_ptr(), 1))
-        check(lib.kmh_ctx_synchronize(ctx.ptr))
-        quad = float(torch.dot(y_d, sol_d))
+        quad = quad_form(h)
         dks = (L.kmh_kernel * 2)(kernel_of(p, L.KMH_D1), kernel_of(p, L.KMH_D2))
         g = np.zeros(2)
         check(lib.kmh_stoch_gradient_dev(h, dks, 2, y_d.data_ptr(), saa_d.data_ptr(), saa.shape[1], g.ctypes.data))

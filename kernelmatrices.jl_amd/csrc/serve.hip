@@ -110,7 +110,10 @@ static void finish_batch(Ctx& c, std::vector<Req*>& rs) {
         if (st[i].code != KMH_OK) rs[i]->st = st[i];
         else { ok.push_back(hs[i]); okr.push_back(rs[i]); }
     }
-    // negloglik = 0.5 logdet + 0.5 y' K^{-1} y  (maxlikfunctions.jl:2-9)
+    // negloglik = 0.5 logdet + 0.5 y' K^{-1} y  (maxlikfunctions.jl:2-9),
+    // with y' K^{-1} y = |W^{-1} y|^2 for the symmetric factor K = W W^T:
+    // one application of W^{-1} per request, and W^{-T} on top only for the
+    // requests whose gradient needs K^{-1} y
     if (!ok.empty()) {
         size_t tot = 0;
         for (Hodlr* h : ok) tot += 2 * (size_t)h->n;
@@ -143,15 +146,20 @@ static void finish_batch(Ctx& c, std::vector<Req*>& rs) {
         }
         copy2d(c, cp);
         copy2d(c, cp2);
-        hodlr_solve_batch(c, ok, T, k);
+        hodlr_factor_apply_batch(c, KMH_WINV, ok, T, k);   // z = W^{-1} y
         std::vector<DotJob> dj;
-        off = 0;
-        for (size_t i = 0; i < ok.size(); ++i) {
-            const int64_t n = ok[i]->n;
-            dj.push_back({buf + off, buf + off + n, n, dots + i});
-            off += 2 * n;
-        }
+        for (size_t i = 0; i < ok.size(); ++i) dj.push_back({T[i], T[i], ok[i]->n, dots + i});
         dots_batch(c, dj);
+        std::vector<Hodlr*> gh;
+        std::vector<double*> gT;
+        std::vector<int> gk;
+        for (size_t i = 0; i < ok.size(); ++i)
+            if (!okr[i]->dks.empty()) {
+                gh.push_back(ok[i]);
+                gT.push_back(T[i]);
+                gk.push_back(1);
+            }
+        if (!gh.empty()) hodlr_factor_apply_batch(c, KMH_WTINV, gh, gT, gk);   // K^{-1} y = W^{-T} z
         std::vector<double> hd(ok.size());
         KMH_HIP(hipMemcpyAsync(hd.data(), dots, ok.size() * sizeof(double), hipMemcpyDeviceToHost, c.q()));
         c.sync();
