@@ -17,11 +17,12 @@ C ABI; `served` keeps `--inflight` requests in the server, which advances
 all of them stage by stage in shared grouped launches, spread over the
 server's lanes (HIP streams stepped on parallel host threads).  Both legs
 see the same requests; their logdets are compared.  `roofline` runs one
-served batch on a single lane with every k_gemm_nn_wide launch timed.  The JSON line carries the
-roofline of the grouped FP64 MFMA GEMM (the dominant kernel; HIP events on
-the library's stream, on every PROF_STRIDE-th launch of that kernel inside
-the timed region: a timed launch idles the queue for a few microseconds) and
-a CPU baseline (the numpy oracle, one BLAS thread).
+served batch on a single lane with every launch of the two timed kernels
+(k_tile_qr<32>, k_gemm_nn_wide) timed by HIP events on their dispatches; the
+headline legs time every PROF_STRIDE-th launch (a timed launch idles the
+queue for a few microseconds).  The JSON line carries the roofline of the
+one with more kernel time (the dominant kernel) with the other beside it,
+and a CPU baseline (the numpy oracle, one BLAS thread).
 """
 import argparse
 import ctypes as C
@@ -249,22 +250,27 @@ def main():
         return 0.5 * ld.value + 0.5 * quad, g
 
     legs = args.legs.split(",")
-    check(lib.kmh_ctx_profile_config(ctx.ptr, 1 << L.KMH_PROF_NN_WIDE, PROF_STRIDE))
+    PROF_MASK = (1 << L.KMH_PROF_NN_WIDE) | (1 << L.KMH_PROF_TILE_QR)
+    check(lib.kmh_ctx_profile_config(ctx.ptr, PROF_MASK, PROF_STRIDE))
 
     def prof_begin():
         check(lib.kmh_ctx_profile_begin(ctx.ptr))
 
     def prof_end():
-        """k_gemm_nn_wide totals of the sampled launches since prof_begin"""
+        """k_gemm_nn_wide and k_tile_qr<32> totals of the timed launches
+        since prof_begin"""
         fl, ms, cnt = C.c_double(), C.c_double(), C.c_int64()
         check(lib.kmh_ctx_profile_end(ctx.ptr, C.byref(fl), C.byref(ms), C.byref(cnt)))
-        wb, wms, wl = C.c_double(), C.c_double(), C.c_int64()
-        check(lib.kmh_ctx_profile_query(ctx.ptr, L.KMH_PROF_NN_WIDE, C.byref(wb), C.byref(wms), C.byref(wl)))
-        k = max(1, wl.value)
-        return {"nn_wide_bytes_per_launch": wb.value / k, "nn_wide_us_per_launch": 1e3 * wms.value / k,
-                "nn_wide_gbps": wb.value / (wms.value * 1e-3) / 1e9 if wms.value else 0.0,
-                "nn_wide_tflops": fl.value / (ms.value * 1e-3) / 1e12 if ms.value else 0.0,
-                "nn_wide_timed_launches": wl.value, "nn_wide_sample_stride": PROF_STRIDE}
+        out_ = {}
+        for tag, key in ((L.KMH_PROF_NN_WIDE, "nn_wide"), (L.KMH_PROF_TILE_QR, "tile_qr")):
+            b, t, nl = C.c_double(), C.c_double(), C.c_int64()
+            check(lib.kmh_ctx_profile_query(ctx.ptr, tag, C.byref(b), C.byref(t), C.byref(nl)))
+            k = max(1, nl.value)
+            out_.update({key + "_bytes_per_launch": b.value / k, key + "_us_per_launch": 1e3 * t.value / k,
+                         key + "_ms": t.value, key + "_timed_launches": nl.value,
+                         key + "_gbps": b.value / (t.value * 1e-3) / 1e9 if t.value else 0.0})
+        out_["nn_wide_tflops"] = fl.value / (ms.value * 1e-3) / 1e12 if ms.value else 0.0
+        return out_
 
     plain_ld = {}
     if "plain" in legs:
@@ -381,16 +387,16 @@ def main():
                            for i in plain_grad for j in range(2))
                 out["served_grad"]["max_rel_grad_err_vs_plain"] = gerr
         if "roofline" in legs:
-            # the dominant kernel without another stream's kernels beside it:
-            # one served batch on one lane, every k_gemm_nn_wide launch timed
+            # the timed kernels without another stream's kernels beside them:
+            # one served batch on one lane, every launch timed
             grad_mode[0] = False
             check(lib.kmh_server_set_lanes(srv, 1))
-            check(lib.kmh_ctx_profile_config(ctx.ptr, 1 << L.KMH_PROF_NN_WIDE, 1))
+            check(lib.kmh_ctx_profile_config(ctx.ptr, PROF_MASK, 1))
             k = min(args.inflight, total)
             run(range(k), False)
             torch.cuda.synchronize()
             res, el, prof = run(range(k), True)
-            check(lib.kmh_ctx_profile_config(ctx.ptr, 1 << L.KMH_PROF_NN_WIDE, PROF_STRIDE))
+            check(lib.kmh_ctx_profile_config(ctx.ptr, PROF_MASK, PROF_STRIDE))
             lat = [v[2] for v in res.values()]
             out["roofline_pass"] = {"requests": k, "lanes": 1, "evals_per_s": k / el, "p50_ms": pct(lat, 0.5),
                                     "p99_ms": pct(lat, 0.99)}
@@ -400,32 +406,54 @@ def main():
     head = (out.get("served") or out.get("plain") or out.get("served_grad") or out.get("plain_grad")
             or out.get("roofline_pass"))
 
+    KERNELS = {
+        "tile_qr": ("k_tile_qr<32>",
+                    "Householder QR + explicit Q of <= 128 x 32 tiles (TSQR leaves and inner nodes), one wave "
+                    "per tile; bytes = tiles read, Q and R written; bound by LDS traffic and dependent latency, "
+                    "so far below its HBM roofline"),
+        "nn_wide": ("k_gemm_nn_wide", "skinny FP64 GEMM on v_mfma_f64_16x16x4f64 (Woodbury updates, leaf-solve "
+                                      "updates, Nystrom product); bytes = A, B, C written, C read when beta != 0"),
+    }
+
     def roofline(h):
-        """Dominant kernel of the served leg: k_gemm_nn_wide (HBM-bound skinny
-        FP64 GEMM).  achieved = its algorithmic bytes per launch / its average
-        launch time, both measured live with HIP events on the dispatches of
-        the library's stream: in the roofline pass (one served batch on one
-        lane, every launch timed) when it ran, else every PROF_STRIDE-th
-        launch of the headline leg, where the other lane's kernels share the
-        device.  traffic = measured HBM bytes per launch (rocprofv3
-        FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel) when the
-        committed summary holds it."""
+        """Roofline of the dominant kernel: of the two kernels timed live
+        (k_tile_qr<32>, k_gemm_nn_wide), the one with more kernel time in the
+        roofline pass (one served batch on one lane, every launch timed with
+        HIP event markers around it), or else in the headline leg (every
+        PROF_STRIDE-th launch, the other lane's kernels beside it).  Both are
+        HBM-bound in the ideal (a 32-column panel QR does about 8 flops per
+        byte, under the 78.6 TFLOP/s / 8 TB/s balance of MI355X), so the
+        bound is the 8 TB/s HBM peak.  achieved = algorithmic bytes per
+        launch / average launch time; traffic = measured HBM bytes per launch
+        (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel) from
+        the committed summary.  The other kernel's figures ride along."""
         where = "headline leg, lane 0, every %d-th launch" % PROF_STRIDE
         if "roofline_pass" in out:
             h = out["roofline_pass"]
             where = "roofline pass: one served batch of %d requests on one lane, every launch" % h["requests"]
-        if "nn_wide_gbps" not in h:
-            return {"bound": "hbm", "kernel": "k_gemm_nn_wide", "achieved": None, "peak": HBM_PEAK_GBPS,
-                    "unit": "GB/s", "frac": None, "traffic": None}
-        return {"bound": "hbm", "kernel": "k_gemm_nn_wide (skinny FP64 GEMM, v_mfma_f64_16x16x4f64)",
-                "achieved": round(h["nn_wide_gbps"], 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(h["nn_wide_gbps"] / HBM_PEAK_GBPS, 4),
-                "algorithmic_bytes_per_launch": round(h["nn_wide_bytes_per_launch"]),
-                "us_per_launch": round(h["nn_wide_us_per_launch"], 2),
-                "timed_launches": h["nn_wide_timed_launches"], "measured_in": where,
-                "traffic": measured_traffic("k_gemm_nn_wide"),
-                "tflops": round(h["nn_wide_tflops"], 3), "fp64_mfma_peak_tflops": FP64_MFMA_PEAK_TFLOPS,
-                "factor_flops_structured_per_eval": ff}
+        keys = [k for k in ("tile_qr", "nn_wide") if h.get(k + "_ms")]
+        if not keys:
+            return {"bound": "hbm", "kernel": None, "achieved": None, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                    "frac": None, "traffic": None}
+
+        def one(k):
+            name, what = KERNELS[k]
+            return {"bound": "hbm", "kernel": name, "what": what,
+                    "achieved": round(h[k + "_gbps"], 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                    "frac": round(h[k + "_gbps"] / HBM_PEAK_GBPS, 4),
+                    "algorithmic_bytes_per_launch": round(h[k + "_bytes_per_launch"]),
+                    "us_per_launch": round(h[k + "_us_per_launch"], 2), "kernel_ms": round(h[k + "_ms"], 3),
+                    "timed_launches": h[k + "_timed_launches"], "measured_in": where,
+                    "traffic": measured_traffic(name)}
+        keys.sort(key=lambda k: -h[k + "_ms"])
+        r = one(keys[0])
+        if len(keys) > 1:
+            r["other_timed_kernel"] = one(keys[1])
+        r["gemm_tflops_nn_wide"] = round(h.get("nn_wide_tflops", 0.0), 3)
+        r["fp64_mfma_peak_tflops"] = FP64_MFMA_PEAK_TFLOPS
+        r["factor_flops_structured_per_eval"] = ff
+        return r
+
     L_ = info.get("levels", args.level)
     nblocks = [2 ** (L_ - l - 1) for l in range(L_)]
     m = n // (2 ** L_)

@@ -109,26 +109,38 @@ int kmh_ctx_destroy(kmh_ctx* ctx);
 int kmh_ctx_synchronize(kmh_ctx* ctx);
 int kmh_ctx_bytes_in_use(kmh_ctx* ctx, uint64_t* out);
 /* Measurement hooks (no counterpart in the reference): between begin and end
- * every FP64 GEMM kernel launch carries a start and a stop HIP event on its
- * own dispatch (hipExtLaunchKernel: no packets between kernels), so a span
- * is one launch and its duration excludes queue gaps.  end returns the
- * KMH_PROF_GEMM totals (algorithmic flops 2mnk, summed kernel duration in ms,
- * launch count); query returns any tag's totals of the last begin..end window:
+ * every FP64 GEMM kernel launch and every 32-column tile QR launch is
+ * bracketed by HIP event markers on the context's stream, so a span is one
+ * launch (its duration plus the launch latency; each marker pair idles the
+ * queue for several microseconds, see kmh_ctx_profile_config).  end returns
+ * the KMH_PROF_GEMM totals (algorithmic
+ * flops 2mnk, summed kernel duration in ms, launch count) of the timed GEMM
+ * launches; query returns any tag's totals of the last begin..end window:
  *   KMH_PROF_GEMM     every GEMM kernel launch (split-k reductions included,
  *                     with 0 flops), amount = flops
  *   KMH_PROF_NN_WIDE  k_gemm_nn_wide launches, amount = algorithmic HBM bytes
  *                     (A, B and C written, plus C read when beta != 0)
  *   KMH_PROF_TN_TALL  k_gemm_tn_tall launches, amount = bytes of A, B, C
  *   KMH_PROF_NN_ROWS  k_gemm_nn_rows launches, amount = bytes as for
- *                     KMH_PROF_NN_WIDE                                      */
-enum { KMH_PROF_GEMM = 0, KMH_PROF_NN_WIDE = 1, KMH_PROF_TN_TALL = 2, KMH_PROF_NN_ROWS = 3, KMH_PROF_TAGS = 4 };
+ *                     KMH_PROF_NN_WIDE
+ *   KMH_PROF_TILE_QR  k_tile_qr<32> launches (Householder QR + explicit Q of
+ *                     <= 128 x 32 tiles), amount = bytes of the tiles read
+ *                     and of Q and R written                                 */
+enum {
+    KMH_PROF_GEMM = 0,
+    KMH_PROF_NN_WIDE = 1,
+    KMH_PROF_TN_TALL = 2,
+    KMH_PROF_NN_ROWS = 3,
+    KMH_PROF_TILE_QR = 4,
+    KMH_PROF_TAGS = 5
+};
 int kmh_ctx_profile_begin(kmh_ctx* ctx);
 int kmh_ctx_profile_end(kmh_ctx* ctx, double* gemm_flops, double* gemm_ms, int64_t* gemm_launches);
 int kmh_ctx_profile_query(kmh_ctx* ctx, int32_t tag, double* amount, double* ms, int64_t* spans);
 /* Which launches the hooks time (default: every tag, every launch): the tags
  * whose bit (1 << tag) is set in tag_mask, each at every stride-th launch of
- * that tag, counted from begin.  A timed launch leaves a few microseconds of
- * idle around it, so a throughput run times a sample.  The KMH_PROF_GEMM
+ * that tag, counted from begin.  A timed launch leaves several microseconds
+ * of idle around it, so a throughput run times a sample.  The KMH_PROF_GEMM
  * totals of end cover the timed launches only.  Not inside begin..end. */
 int kmh_ctx_profile_config(kmh_ctx* ctx, uint32_t tag_mask, int32_t stride);
 

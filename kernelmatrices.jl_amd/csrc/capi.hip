@@ -151,9 +151,11 @@ int kmh_ctx_profile_end(kmh_ctx* ctx, double* gemm_flops, double* gemm_ms, int64
         for (const auto& sp : c.spans) {
             float t = 0.f;
             KMH_HIP(hipEventElapsedTime(&t, sp.a, sp.b));
-            c.prof_sum[KMH_PROF_GEMM][0] += sp.flops;
-            c.prof_sum[KMH_PROF_GEMM][1] += t;
-            c.prof_sum[KMH_PROF_GEMM][2] += 1.0;
+            if (sp.flops >= 0.0) {
+                c.prof_sum[KMH_PROF_GEMM][0] += sp.flops;
+                c.prof_sum[KMH_PROF_GEMM][1] += t;
+                c.prof_sum[KMH_PROF_GEMM][2] += 1.0;
+            }
             if (sp.tag != KMH_PROF_GEMM) {
                 c.prof_sum[sp.tag][0] += sp.amount;
                 c.prof_sum[sp.tag][1] += t;

@@ -1,7 +1,6 @@
 // Shared host-side infrastructure of libkmhodlr: errors, the context (device,
 // stream, descriptor arena, launch queue, scratch pool), device buffers.
 #pragma once
-#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -90,9 +89,11 @@ struct Ctx {
     size_t flush_at = 1;
     // Device flag for Cholesky failures (non-positive pivots).
     int* d_info = nullptr;
-    // Profiling spans (measurement hooks): one GEMM kernel launch each, timed
-    // by a start and a stop event that ride on the dispatch itself, with a
-    // tag, the algorithmic amount (bytes) for that tag and the launch's flops.
+    // Profiling spans (measurement hooks): one kernel launch each, timed by
+    // event markers recorded around it on the stream, with a tag,
+    // the algorithmic amount (bytes) for that tag and the launch's flops
+    // (GEMM launches; negative for the others, which stay out of the GEMM
+    // totals).
     struct ProfSpan {
         int tag;
         double amount;
@@ -144,8 +145,12 @@ struct Ctx {
             }, args);
         });
     }
-    // launch_kernel, timed while profiling is on (see ProfSpan): the events
-    // go into hipExtLaunchKernel, so timing adds nothing between kernels.
+    // launch_kernel, timed while profiling is on (see ProfSpan): event
+    // markers on the stream right before and right after the launch, so the
+    // span starts when the previous work has finished (a start event carried
+    // on the dispatch itself would be taken when the dispatch is processed,
+    // possibly before its predecessor ends).  A marker pair idles the queue
+    // for several microseconds: time a sample (prof_mask, prof_stride).
     template <class... P, class... A>
     void launch_timed(int tag, double amount, double flops, void (*kernel)(P...), dim3 g, dim3 b, unsigned shm,
                       A&&... a) {
@@ -159,7 +164,9 @@ struct Ctx {
         launch([kernel, g, b, shm, s = stream, e0, e1, args = std::tuple<P...>(static_cast<P>(a)...)]() mutable {
             std::apply([&](auto&... x) {
                 void* argv[] = {static_cast<void*>(&x)...};
-                KMH_HIP(hipExtLaunchKernel(reinterpret_cast<const void*>(kernel), g, b, argv, shm, s, e0, e1, 0));
+                KMH_HIP(hipEventRecord(e0, s));
+                KMH_HIP(hipLaunchKernel(reinterpret_cast<const void*>(kernel), g, b, argv, shm, s));
+                KMH_HIP(hipEventRecord(e1, s));
             }, args);
         });
     }

@@ -258,7 +258,16 @@ static void launch_tile_qr(Ctx& c, const std::vector<SqrDesc>& ds) {
         const dim3 g((unsigned)cls[k].size());
         if (k == 0) KMH_LAUNCH(c, k_tile_qr<8>, g, dim3(64), dd);
         else if (k == 1) KMH_LAUNCH(c, k_tile_qr<16>, g, dim3(64), dd);
-        else KMH_LAUNCH(c, k_tile_qr<32>, g, dim3(64), dd);
+        else {
+            // algorithmic bytes: the tile read, Q and R written
+            double by = 0.0;
+            if (c.prof)
+                for (const auto& d : cls[k]) {
+                    const double mt = (double)d.nstack * d.nbs;
+                    by += 8.0 * (2.0 * mt * d.nc + (double)d.nc * d.nc);
+                }
+            KMH_LAUNCH_TIMED(c, KMH_PROF_TILE_QR, by, -1.0, k_tile_qr<32>, g, dim3(64), dd);
+        }
     }
 }
 

@@ -21,7 +21,7 @@ KMH_ACA, KMH_NYSTROM = 0, 1
 KMH_FIXED_LEVEL, KMH_LOG_LEVEL = 0, 1
 KMH_FP64 = 0
 KMH_W, KMH_WT, KMH_WINV, KMH_WTINV = 0, 1, 2, 3
-KMH_PROF_GEMM, KMH_PROF_NN_WIDE, KMH_PROF_TN_TALL, KMH_PROF_NN_ROWS = 0, 1, 2, 3
+KMH_PROF_GEMM, KMH_PROF_NN_WIDE, KMH_PROF_TN_TALL, KMH_PROF_NN_ROWS, KMH_PROF_TILE_QR = 0, 1, 2, 3, 4
 
 
 class kmh_kernel(C.Structure):

@@ -207,3 +207,31 @@ def test_profile_hooks_time_sampled_launches(gpu_lib):
     assert lib.kmh_ctx_profile_config(ctx.ptr, 1, 1) == L.KMH_E_STATE
     d = C.c_double()
     assert lib.kmh_ctx_profile_end(ctx.ptr, C.byref(d), C.byref(d), C.byref(C.c_int64())) == 0
+
+
+def test_profile_hooks_time_tile_qr(gpu_lib):
+    """KMH_PROF_TILE_QR: the 32-column tile QR launches of a panel QR are
+    timed with their algorithmic bytes (tiles read, Q and R written) and stay
+    out of the GEMM totals.  A 300 x 32 panel: three leaf tiles (100 rows
+    each) and one inner node stacking their R factors."""
+    import kmhodlr
+    from kmhodlr import _lib as L
+    ctx = kmhodlr.Context(0)
+    lib = ctx.lib
+    m, r = 300, 32
+    A0 = np.random.default_rng(3).standard_normal((m, r))
+    A = np.asfortranarray(A0.copy())
+    R = np.zeros((r, r), order="F")
+    assert lib.kmh_ctx_profile_config(ctx.ptr, 1 << L.KMH_PROF_TILE_QR, 1) == 0
+    assert lib.kmh_ctx_profile_begin(ctx.ptr) == 0
+    assert lib.kmh_test_qr(ctx.ptr, m, r, A.ctypes.data, m, R.ctypes.data, r) == 0, lib.last_error()
+    gfl, gms, gl = C.c_double(), C.c_double(), C.c_int64()
+    assert lib.kmh_ctx_profile_end(ctx.ptr, C.byref(gfl), C.byref(gms), C.byref(gl)) == 0
+    b, ms, n = C.c_double(), C.c_double(), C.c_int64()
+    assert lib.kmh_ctx_profile_query(ctx.ptr, L.KMH_PROF_TILE_QR, C.byref(b), C.byref(ms), C.byref(n)) == 0
+    assert gl.value == 0 and gfl.value == 0.0           # the mask times the tile QR only
+    assert n.value == 2 and ms.value > 0                # the leaf launch and the inner level
+    leaves = 3 * 8.0 * (2 * 100 * r + r * r)
+    inner = 8.0 * (2 * 3 * r * r + r * r)
+    assert b.value == leaves + inner
+    assert np.abs(A @ R - A0).max() <= 1e-13 * np.abs(A0).max()
