@@ -18,7 +18,7 @@ all of them stage by stage in shared grouped launches, spread over the
 server's lanes (HIP streams stepped on parallel host threads).  Both legs
 see the same requests; their logdets are compared.  `roofline` runs one
 served batch on a single lane with every launch of the two timed kernels
-(k_tile_qr<32>, k_gemm_nn_wide) timed by HIP events on their dispatches; the
+(k_tile_qr<24>, k_gemm_nn_wide) timed by HIP events on their dispatches; the
 headline legs time every PROF_STRIDE-th launch (a timed launch idles the
 queue for a few microseconds).  The JSON line carries the roofline of the
 one with more kernel time (the dominant kernel) with the other beside it,
@@ -257,7 +257,7 @@ def main():
         check(lib.kmh_ctx_profile_begin(ctx.ptr))
 
     def prof_end():
-        """k_gemm_nn_wide and k_tile_qr<32> totals of the timed launches
+        """k_gemm_nn_wide and k_tile_qr<24> totals of the timed launches
         since prof_begin"""
         fl, ms, cnt = C.c_double(), C.c_double(), C.c_int64()
         check(lib.kmh_ctx_profile_end(ctx.ptr, C.byref(fl), C.byref(ms), C.byref(cnt)))
@@ -407,8 +407,8 @@ def main():
             or out.get("roofline_pass"))
 
     KERNELS = {
-        "tile_qr": ("k_tile_qr<32>",
-                    "Householder QR + explicit Q of <= 128 x 32 tiles (TSQR leaves and inner nodes), one wave "
+        "tile_qr": ("k_tile_qr<24>",
+                    "Householder QR + explicit Q of <= 128 x 24 tiles (TSQR leaves and inner nodes), one wave "
                     "per tile; bytes = tiles read, Q and R written; bound by LDS traffic and dependent latency, "
                     "so far below its HBM roofline"),
         "nn_wide": ("k_gemm_nn_wide", "skinny FP64 GEMM on v_mfma_f64_16x16x4f64 (Woodbury updates, leaf-solve "
@@ -417,11 +417,11 @@ def main():
 
     def roofline(h):
         """Roofline of the dominant kernel: of the two kernels timed live
-        (k_tile_qr<32>, k_gemm_nn_wide), the one with more kernel time in the
+        (k_tile_qr<24>, k_gemm_nn_wide), the one with more kernel time in the
         roofline pass (one served batch on one lane, every launch timed with
         HIP event markers around it), or else in the headline leg (every
         PROF_STRIDE-th launch, the other lane's kernels beside it).  Both are
-        HBM-bound in the ideal (a 32-column panel QR does about 8 flops per
+        HBM-bound in the ideal (a 24-column panel QR does about 6 flops per
         byte, under the 78.6 TFLOP/s / 8 TB/s balance of MI355X), so the
         bound is the 8 TB/s HBM peak.  achieved = algorithmic bytes per
         launch / average launch time; traffic = measured HBM bytes per launch

@@ -109,7 +109,7 @@ int kmh_ctx_destroy(kmh_ctx* ctx);
 int kmh_ctx_synchronize(kmh_ctx* ctx);
 int kmh_ctx_bytes_in_use(kmh_ctx* ctx, uint64_t* out);
 /* Measurement hooks (no counterpart in the reference): between begin and end
- * every FP64 GEMM kernel launch and every 32-column tile QR launch is
+ * every FP64 GEMM kernel launch and every 24-column tile QR launch is
  * bracketed by HIP event markers on the context's stream, so a span is one
  * launch (its duration plus the launch latency; each marker pair idles the
  * queue for several microseconds, see kmh_ctx_profile_config).  end returns
@@ -123,8 +123,8 @@ int kmh_ctx_bytes_in_use(kmh_ctx* ctx, uint64_t* out);
  *   KMH_PROF_TN_TALL  k_gemm_tn_tall launches, amount = bytes of A, B, C
  *   KMH_PROF_NN_ROWS  k_gemm_nn_rows launches, amount = bytes as for
  *                     KMH_PROF_NN_WIDE
- *   KMH_PROF_TILE_QR  k_tile_qr<32> launches (Householder QR + explicit Q of
- *                     <= 128 x 32 tiles), amount = bytes of the tiles read
+ *   KMH_PROF_TILE_QR  k_tile_qr<24> launches (Householder QR + explicit Q of
+ *                     <= 128 x 24 tiles), amount = bytes of the tiles read
  *                     and of Q and R written                                 */
 enum {
     KMH_PROF_GEMM = 0,

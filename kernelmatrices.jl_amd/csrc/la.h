@@ -45,7 +45,7 @@ void trsm_left(Ctx& c, bool trans, const std::vector<TrsmProb>& probs);
 
 // Thin QR of each m x r panel (m >= r): A <- Q (orthonormal columns),
 // R (r x r upper, leading dimension ldr) written in full (lower part zero).
-// Block classical Gram-Schmidt with reorthogonalisation across 32-column
+// Block classical Gram-Schmidt with reorthogonalisation across 24-column
 // blocks, TSQR (single-wave Householder tiles, fan-in-4 tree) inside.  With
 // src set, the panel is read from src (leading dimension lds) and only Q is
 // written to A.

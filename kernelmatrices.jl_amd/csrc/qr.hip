@@ -1,7 +1,9 @@
 // Thin QR of batched tall panels on gfx950 (see la.h): Householder QR of
-// <=128 x 32 tiles, one wave per tile with the rows in registers, combined by
-// a TSQR tree of fan-in 4; columns beyond 32 by block Gram-Schmidt with one
-// reorthogonalisation pass whose projections run on the MFMA GEMM.
+// <=128 x 24 tiles, one wave per tile with the rows in registers, combined by
+// a TSQR tree of fan-in 4; columns beyond 24 by block Gram-Schmidt with one
+// reorthogonalisation pass whose projections run on the MFMA GEMM.  (The
+// tile cost grows with the square of its width: three 24-column blocks of a
+// 72-column panel cost less than 32 + 32 + 8.)
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
@@ -14,7 +16,8 @@ namespace kmh {
 // Householder QR of small tiles (TSQR building block)
 // ---------------------------------------------------------------------------
 constexpr int QT = 128;   // max rows of a tile: one wave, two rows per lane
-constexpr int QC = 32;    // max columns of a tile (wider panels: block Gram-Schmidt)
+constexpr int QC = 24;    // max columns of a tile (wider panels: block Gram-Schmidt;
+                          // the Nystrom rank 72 splits into three blocks)
 constexpr int QF = 4;     // TSQR fan-in: QF stacked QC x QC factors fill one tile
 
 // A tile is nstack blocks of nbs rows stacked vertically; block i starts at
@@ -96,7 +99,7 @@ template <int N> using ic = std::integral_constant<int, N>;
 // write row j of R straight to the R output.  Three waves per SIMD: the tile
 // (2 x NC doubles per lane) stays in registers.
 template <int NC>
-__global__ __launch_bounds__(64, 3) void k_tile_qr(const SqrDesc* __restrict__ ds) {
+__global__ __launch_bounds__(64, NC <= 24 ? 4 : 3) void k_tile_qr(const SqrDesc* __restrict__ ds) {
     __shared__ double pw[64 * 17];
     __shared__ double rowj[NC];
     __shared__ double taus[NC];
@@ -244,13 +247,13 @@ __global__ __launch_bounds__(64, 3) void k_tile_qr(const SqrDesc* __restrict__ d
     }
 }
 
-// One launch per column class (8, 16, 32) of the descriptors.
+// One launch per column class (8, 16, QC = 24) of the descriptors.
 static void launch_tile_qr(Ctx& c, const std::vector<SqrDesc>& ds) {
     std::vector<SqrDesc> cls[3];
     for (const auto& d : ds) {
         KMH_CHECK(d.nc <= QC && d.nstack * d.nbs <= QT && d.nstack * d.nbs >= d.nc, KMH_E_BAD_ARG,
                   "tile QR beyond its size limits");
-        cls[d.nc <= 8 ? 0 : d.nc <= 16 ? 1 : 2].push_back(d);
+        cls[d.nc <= 8 ? 0 : d.nc <= 16 ? 1 : 2].push_back(d);   // NC = 8, 16, 24
     }
     for (int k = 0; k < 3; ++k) {
         if (cls[k].empty()) continue;
@@ -266,7 +269,7 @@ static void launch_tile_qr(Ctx& c, const std::vector<SqrDesc>& ds) {
                     const double mt = (double)d.nstack * d.nbs;
                     by += 8.0 * (2.0 * mt * d.nc + (double)d.nc * d.nc);
                 }
-            KMH_LAUNCH_TIMED(c, KMH_PROF_TILE_QR, by, -1.0, k_tile_qr<32>, g, dim3(64), dd);
+            KMH_LAUNCH_TIMED(c, KMH_PROF_TILE_QR, by, -1.0, k_tile_qr<QC>, g, dim3(64), dd);
         }
     }
 }

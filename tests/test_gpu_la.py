@@ -210,15 +210,15 @@ def test_profile_hooks_time_sampled_launches(gpu_lib):
 
 
 def test_profile_hooks_time_tile_qr(gpu_lib):
-    """KMH_PROF_TILE_QR: the 32-column tile QR launches of a panel QR are
+    """KMH_PROF_TILE_QR: the 24-column tile QR launches of a panel QR are
     timed with their algorithmic bytes (tiles read, Q and R written) and stay
-    out of the GEMM totals.  A 300 x 32 panel: three leaf tiles (100 rows
+    out of the GEMM totals.  A 300 x 24 panel: three leaf tiles (100 rows
     each) and one inner node stacking their R factors."""
     import kmhodlr
     from kmhodlr import _lib as L
     ctx = kmhodlr.Context(0)
     lib = ctx.lib
-    m, r = 300, 32
+    m, r = 300, 24
     A0 = np.random.default_rng(3).standard_normal((m, r))
     A = np.asfortranarray(A0.copy())
     R = np.zeros((r, r), order="F")
