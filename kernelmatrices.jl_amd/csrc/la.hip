@@ -44,6 +44,11 @@ constexpr int GK = 32;
 // 16x16x4 form instead holds D(4g + r, j).)
 __device__ __forceinline__ int mfma_d_row(int g, int r) { return 4 * r + g; }
 
+// Streamed operands of the skinny products are touched once per launch:
+// non-temporal loads and stores keep them from displacing reused data.
+__device__ __forceinline__ double ld_stream(const double* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st_stream(double v, double* p) { __builtin_nontemporal_store(v, p); }
+
 // Operand staging: each thread moves GT*GK/256 elements of A and of B per
 // k-chunk.  Element e of thread tid maps to (row-or-col index, k index) so
 // that consecutive threads read consecutive addresses of the column-major
@@ -212,8 +217,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn_tall(const GemmDesc* __restr
 #pragma unroll
         for (int q = 0; q < CPT; ++q) {
             const int col = scol + NG * q;
-            ra[q] = (col < d.m && gk < d.k) ? d.A[gk + (size_t)col * d.lda] : 0.0;
-            rb[q] = (col < d.n && gk < d.k) ? d.B[gk + (size_t)col * d.ldb] : 0.0;
+            ra[q] = (col < d.m && gk < d.k) ? ld_stream(d.A + gk + (size_t)col * d.lda) : 0.0;
+            rb[q] = (col < d.n && gk < d.k) ? ld_stream(d.B + gk + (size_t)col * d.ldb) : 0.0;
         }
     };
     fetch(0);
@@ -306,7 +311,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn_wide(const GemmDesc* __restr
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) {
             const int ka = k0 + ks * 4 + (lane >> 4);
-            ra[ks] = (arow_ok && ka < d.k) ? ap[(size_t)ka * d.lda] : 0.0;
+            ra[ks] = (arow_ok && ka < d.k) ? ld_stream(ap + (size_t)ka * d.lda) : 0.0;
         }
     };
     fetch(0);
@@ -337,7 +342,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn_wide(const GemmDesc* __restr
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int col = (h + b) * 16 + mfma_d_row(lane >> 4, r);
-                cv[b][r] = (h + b < NB && d.beta != 0.0 && arow_ok && col < d.n) ? d.C[arow + (size_t)col * d.ldc]
+                cv[b][r] = (h + b < NB && d.beta != 0.0 && arow_ok && col < d.n) ? ld_stream(d.C + arow + (size_t)col * d.ldc)
                                                                                    : 0.0;
             }
 #pragma unroll
@@ -348,7 +353,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn_wide(const GemmDesc* __restr
                 if (h + b < NB && arow_ok && col < d.n) {
                     double v = d.alpha * acc[h + b][r];
                     if (d.beta != 0.0) v += d.beta * cv[b][r];
-                    d.C[arow + (size_t)col * d.ldc] = v;
+                    st_stream(v, d.C + arow + (size_t)col * d.ldc);
                 }
             }
     }
@@ -384,7 +389,7 @@ __global__ __launch_bounds__(ROWS) void k_gemm_nn_rows(const GemmDesc* __restric
     for (int k0 = 0; k0 < d.k; k0 += 8) {
         double a[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) a[q] = (k0 + q < d.k) ? ap[(size_t)(k0 + q) * d.lda] : 0.0;
+        for (int q = 0; q < 8; ++q) a[q] = (k0 + q < d.k) ? ld_stream(ap + (size_t)(k0 + q) * d.lda) : 0.0;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const double* b = Bs + (k0 + q) * NMAX;
@@ -397,14 +402,14 @@ __global__ __launch_bounds__(ROWS) void k_gemm_nn_rows(const GemmDesc* __restric
         const double* ci = (d.Cin ? d.Cin : d.C) + row;
         double cv[NMAX];
 #pragma unroll
-        for (int j = 0; j < NMAX; ++j) cv[j] = j < d.n ? ci[(size_t)j * d.ldc] : 0.0;
+        for (int j = 0; j < NMAX; ++j) cv[j] = j < d.n ? ld_stream(ci + (size_t)j * d.ldc) : 0.0;
 #pragma unroll
         for (int j = 0; j < NMAX; ++j)
-            if (j < d.n) cp[(size_t)j * d.ldc] = d.alpha * acc[j] + d.beta * cv[j];
+            if (j < d.n) st_stream(d.alpha * acc[j] + d.beta * cv[j], cp + (size_t)j * d.ldc);
     } else {
 #pragma unroll
         for (int j = 0; j < NMAX; ++j)
-            if (j < d.n) cp[(size_t)j * d.ldc] = d.alpha * acc[j];
+            if (j < d.n) st_stream(d.alpha * acc[j], cp + (size_t)j * d.ldc);
     }
 }
 
