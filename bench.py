@@ -18,7 +18,7 @@ all of them stage by stage in shared grouped launches, spread over the
 server's lanes (HIP streams stepped on parallel host threads).  Both legs
 see the same requests; their logdets are compared.  `roofline` runs one
 served batch on a single lane with every launch of the two timed kernels
-(k_tile_qr<24>, k_gemm_nn_wide) timed by HIP events on their dispatches; the
+(k_tile_qr<24>, k_gemm_nn_wide) timed by HIP event markers around them; the
 headline legs time every PROF_STRIDE-th launch (a timed launch idles the
 queue for a few microseconds).  The JSON line carries the roofline of the
 one with more kernel time (the dominant kernel) with the other beside it,
