@@ -209,3 +209,33 @@ def test_context_destroyed_before_its_handles(gpu_lib):
     assert lib.kmh_hodlr_logdet(h, C.byref(ld)) == 0 and np.isfinite(ld.value)
     assert lib.kmh_server_destroy(srv) == 0
     assert lib.kmh_hodlr_destroy(h) == 0          # last holder: the context goes with it
+
+
+def test_server_destroyed_with_requests_in_flight(gpu_lib):
+    """A server with three lanes destroyed after two steps (requests built,
+    factorization under way): its private lane contexts go after the
+    requests' handles have been returned to their pools, and the caller's
+    context stays usable for another server."""
+    import kmhodlr as km
+    from kmhodlr import _lib as L
+    lib = km.load()
+    ctx = km.Context(0)
+    probs = _problems([1500, 2600, 900, 3100, 2048], seed=13)
+    srv = C.c_void_p()
+    assert lib.kmh_server_create(ctx.ptr, 8, 0, C.byref(srv)) == 0
+    assert lib.kmh_server_set_lanes(srv, 3) == 0
+    keep = []
+    for x, y, saa, params, level in probs:
+        r = _mkreq(L, x, y, saa, params, level, 40)
+        keep.append(r)
+        rid = C.c_int64()
+        assert lib.kmh_server_submit(srv, C.byref(r), C.byref(rid)) == 0, lib.last_error()
+    done = C.c_int32()
+    for _ in range(2):
+        assert lib.kmh_server_step(srv, C.byref(done)) == 0, lib.last_error()
+    w, f, rb = C.c_int32(), C.c_int32(), C.c_uint64()
+    assert lib.kmh_server_pending(srv, C.byref(w), C.byref(f), C.byref(rb)) == 0
+    assert f.value == len(probs) and done.value == 0
+    assert lib.kmh_server_destroy(srv) == 0
+    res, _ = _serve(lib, L, ctx, probs[:2], 4)
+    assert all(r[0] == 0 and r[1] == 0 for r in res.values())
