@@ -239,3 +239,43 @@ def test_server_destroyed_with_requests_in_flight(gpu_lib):
     assert lib.kmh_server_destroy(srv) == 0
     res, _ = _serve(lib, L, ctx, probs[:2], 4)
     assert all(r[0] == 0 and r[1] == 0 for r in res.values())
+
+
+def test_serve_failed_request_leaves_the_others(gpu_lib):
+    """A request whose kernel matrix is not positive definite (nugget -5)
+    fails with KMH_E_NOT_PD; the requests beside it, in its lane and in the
+    other lane, finish with the same results as alone."""
+    import kmhodlr as km
+    from kmhodlr import _lib as L
+    lib = km.load()
+    ctx = km.Context(0)
+    probs = _problems([1500, 2600, 900, 3100], seed=17)
+    alone, _ = _serve(lib, L, ctx, probs, 1, lanes=1)
+    srv = C.c_void_p()
+    assert lib.kmh_server_create(ctx.ptr, 8, 0, C.byref(srv)) == 0
+    assert lib.kmh_server_set_lanes(srv, 2) == 0
+    keep, ids = [], []
+    for i, (x, y, saa, params, level) in enumerate(probs + [probs[1]]):
+        r = _mkreq(L, x, y, saa, params, level, 40)
+        if i == len(probs):
+            r.kernel.nugget = -5.0
+        keep.append(r)
+        rid = C.c_int64()
+        assert lib.kmh_server_submit(srv, C.byref(r), C.byref(rid)) == 0, lib.last_error()
+        ids.append(rid.value)
+    done = C.c_int32()
+    for _ in range(100):
+        assert lib.kmh_server_step(srv, C.byref(done)) == 0, lib.last_error()
+        w, f, rb = C.c_int32(), C.c_int32(), C.c_uint64()
+        lib.kmh_server_pending(srv, C.byref(w), C.byref(f), C.byref(rb))
+        if w.value == 0 and f.value == 0:
+            break
+    for i, rid in enumerate(ids):
+        out = L.kmh_serve_result()
+        rc = lib.kmh_server_result(srv, rid, C.byref(out))
+        if i == len(probs):
+            assert rc == 0 and out.status == L.KMH_E_NOT_PD
+        else:
+            assert rc == 0 and out.status == 0, lib.last_error()
+            assert out.nll == alone[i][2] and out.logdet == alone[i][3]
+    assert lib.kmh_server_destroy(srv) == 0
