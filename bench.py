@@ -41,7 +41,7 @@ FP64_MFMA_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBPS = 8000.0
 PROF_STRIDE = 4   # time every 4th k_gemm_nn_wide launch of a timed region
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                            "k_gemm_nn_wide_served_n65536_traffic.md")
+                            "tile_qr_nn_wide_served_n65536_traffic.md")
 
 
 def measured_traffic(kernel):
@@ -408,9 +408,9 @@ def main():
 
     KERNELS = {
         "tile_qr": ("k_tile_qr<24>",
-                    "Householder QR + explicit Q of <= 128 x 24 tiles (TSQR leaves and inner nodes), one wave "
-                    "per tile; bytes = tiles read, Q and R written; bound by LDS traffic and dependent latency, "
-                    "so far below its HBM roofline"),
+                    "Householder QR of <= 128 x 24 tiles with Q left in compact-WY form (TSQR leaves and "
+                    "inner nodes), one wave per tile; bytes = tiles read, Householder vectors and R written; "
+                    "bound by LDS traffic and dependent latency, so far below its HBM roofline"),
         "nn_wide": ("k_gemm_nn_wide", "skinny FP64 GEMM on v_mfma_f64_16x16x4f64 (Woodbury updates, leaf-solve "
                                       "updates, Nystrom product); bytes = A, B, C written, C read when beta != 0"),
     }

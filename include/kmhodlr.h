@@ -123,9 +123,10 @@ int kmh_ctx_bytes_in_use(kmh_ctx* ctx, uint64_t* out);
  *   KMH_PROF_TN_TALL  k_gemm_tn_tall launches, amount = bytes of A, B, C
  *   KMH_PROF_NN_ROWS  k_gemm_nn_rows launches, amount = bytes as for
  *                     KMH_PROF_NN_WIDE
- *   KMH_PROF_TILE_QR  k_tile_qr<24> launches (Householder QR + explicit Q of
- *                     <= 128 x 24 tiles), amount = bytes of the tiles read
- *                     and of Q and R written                                 */
+ *   KMH_PROF_TILE_QR  k_tile_qr<24> launches (Householder QR of <= 128 x 24
+ *                     tiles, Q left in compact-WY form), amount = bytes of
+ *                     the tiles read and of the Householder vectors and R
+ *                     written                                                */
 enum {
     KMH_PROF_GEMM = 0,
     KMH_PROF_NN_WIDE = 1,

@@ -277,12 +277,12 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn_tall(const GemmDesc* __restr
 }
 
 // Wide NN products C = alpha A B + beta C with n <= 160 (panel updates
-// X -= Q R, the Woodbury apply, the TSQR Q assembly): a workgroup owns 64
+// X -= Q R, the Woodbury apply): a workgroup owns 64
 // rows and every output column, so each row of A is read once; wave w takes
 // rows 16w..16w+15.  A is loaded straight from memory (16 consecutive rows
 // per k index), B through LDS in 16-deep chunks.  A wave reads all of its
-// own rows of A before it writes the same rows of C, so C may alias A (the
-// in-place TSQR assembly Q <- Q C with n = k).
+// own rows of A before it writes the same rows of C, so C may alias A (an
+// in-place product A <- A B with n = k).
 constexpr int WN_MAX = 160;
 constexpr int WN_KC = 16;
 
@@ -360,8 +360,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn_wide(const GemmDesc* __restr
 }
 
 // Narrow NN products C = alpha A B + beta C with n <= 32 and k <= 2048 / NMAX
-// (NMAX = 8, 16, 32: the panel QR updates X -= Q R, the TSQR assembly
-// Q <- Q C, the one-column Woodbury updates of a solve): one thread per row
+// (NMAX = 8, 16, 32: the panel QR updates X -= Q R, the one-column
+// Woodbury updates of a solve): one thread per row
 // of C, B in LDS (broadcast reads), the row of A streamed once with
 // coalesced loads and n accumulators in registers.  ROWS threads (128 or
 // 256) per workgroup, 128 when every problem of the launch has <= 128 rows.  The multiply-adds run

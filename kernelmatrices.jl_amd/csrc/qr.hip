@@ -1,7 +1,9 @@
 // Thin QR of batched tall panels on gfx950 (see la.h): Householder QR of
 // <=128 x 24 tiles, one wave per tile with the rows in registers, combined by
-// a TSQR tree of fan-in 4; columns beyond 24 by block Gram-Schmidt with one
-// reorthogonalisation pass whose projections run on the MFMA GEMM.  (The
+// a TSQR tree of fan-in 4, with Q kept in compact-WY form (vectors V and
+// coupling Z) until one fused pass per tree level forms Q times the
+// coefficient block from above; columns beyond 24 by block Gram-Schmidt with
+// one reorthogonalisation pass whose projections run on the MFMA GEMM.  (The
 // tile cost grows with the square of its width: three 24-column blocks of a
 // 72-column panel cost less than 32 + 32 + 8.)
 #include <algorithm>
@@ -27,6 +29,7 @@ struct SqrDesc {
     const double* src; int ld; int nstack; int nbs; long long slot;
     double* qout; int ldq;
     double* rout; int ldr;
+    double* zout;     // compact-WY coupling Z, ld QC
     int nc, pad;
 };
 
@@ -37,19 +40,14 @@ __device__ __forceinline__ double qr_bcast(double v, int lane) {
     return __hiloint2double(hi, lo);
 }
 
-// One round of column sums over the wave's 128 rows of p(row, k) =
-// c(row) a(row, k) for the H columns cb..cb+H-1: lane l ends with the sum of
-// column cb + l % H.  Lane l holds rows l and l + 64 (c0, a0 and c1, a1).
-// The lanes write their per-lane products (two rows already added) to LDS,
+// Sums over the wave of per-lane values p[k], k < H: lane l ends with the
+// sum of p[l % H] over all 64 lanes.  The lanes write their values to LDS,
 // lane l sums a segment of column l % H, and the segments meet by
 // xor-shuffles.  LDS operations of one wave complete in issue order, so no
 // barrier is needed.  Fixed summation order.  LDS: 64 x (H + 1) doubles.
-template <int NC, int H, int CB>
-__device__ __forceinline__ double colsum_round(double c0, const double (&a0)[NC], double c1, const double (&a1)[NC],
-                                               double* pw, int lane) {
+template <int H>
+__device__ __forceinline__ double lds_colsum_read(double* pw, int lane) {
     constexpr int SEGS = 64 / H, RPS = 64 / SEGS;
-#pragma unroll
-    for (int k = 0; k < H; ++k) pw[lane * (H + 1) + k] = c0 * a0[CB + k] + c1 * a1[CB + k];
     __builtin_amdgcn_wave_barrier();
     const int k = lane % H, seg = lane / H;
     double s0 = 0.0, s1 = 0.0;
@@ -63,6 +61,23 @@ __device__ __forceinline__ double colsum_round(double c0, const double (&a0)[NC]
     for (int m = H; m < 64; m <<= 1) s += __shfl_xor(s, m);
     __builtin_amdgcn_wave_barrier();
     return s;
+}
+template <int H>
+__device__ __forceinline__ double lds_colsum(const double (&p)[H], double* pw, int lane) {
+#pragma unroll
+    for (int k = 0; k < H; ++k) pw[lane * (H + 1) + k] = p[k];
+    return lds_colsum_read<H>(pw, lane);
+}
+
+// One round of column sums over the wave's 128 rows of p(row, k) =
+// c(row) a(row, k) for the H columns cb..cb+H-1: lane l ends with the sum of
+// column cb + l % H.  Lane l holds rows l and l + 64 (c0, a0 and c1, a1).
+template <int NC, int H, int CB>
+__device__ __forceinline__ double colsum_round(double c0, const double (&a0)[NC], double c1, const double (&a1)[NC],
+                                               double* pw, int lane) {
+#pragma unroll
+    for (int k = 0; k < H; ++k) pw[lane * (H + 1) + k] = c0 * a0[CB + k] + c1 * a1[CB + k];
+    return lds_colsum_read<H>(pw, lane);
 }
 
 // Column sums of the first W columns (W a multiple of 8): lane l < W ends
@@ -85,24 +100,36 @@ __device__ __forceinline__ double wave_colsum(double c0, const double (&a0)[NC],
 
 template <int N> using ic = std::integral_constant<int, N>;
 
-// Householder QR (LAPACK dgeqr2 + dorg2r arithmetic) of a tile of <= 128
-// rows and nc <= NC columns by one wave: lane l holds rows l and l + 64 in
-// registers.  The column loop has a fixed body: after step j the registers
-// shift left by one, so the pivot column is always a[0] and all register
-// indices are static.  The steps run in phases of 8 whose body covers only
-// the W columns that can still be non-zero (W = NC, NC - 8, ...).  Pivot-row
-// values come from lane j by readlane, column sums from the wave-local
-// reduction, and the products w_k of a step reach every lane through LDS.
-// The Householder vector of step j is parked in column j of the Q output
-// (each lane its own rows) for the backward accumulation of Q, which shifts
-// right one column per step (phases of growing W).  Lanes k < nc - j
-// write row j of R straight to the R output.  Three waves per SIMD: the tile
-// (2 x NC doubles per lane) stays in registers.
+// Householder QR (LAPACK dgeqr2 arithmetic) of a tile of <= 128 rows and
+// nc <= NC columns by one wave: lane l holds rows l and l + 64 in the
+// register slots a0[NC], a1[NC].  The outputs are R, the Householder vectors
+// V (explicit: zero above row j, 1 at row j, below it scal times the pivot
+// column) and the compact-WY coupling Z = diag(tau) + striu(V^T V) (ld QC),
+// from which k_wy_apply forms Q C: with T the triangular factor of
+// Q = H_0 ... H_{nc-1} = I - V T V^T, T^-1 = diag(1/tau) + striu(V^T V)
+// (Schreiber-Van Loan's recurrence for T, inverted column by column).
+//
+// The steps run in phases of 8 whose body covers only the W = NC - 8p slots
+// [0, W) that can still hold live columns.  Before step j = 8p + s:
+//   [0, W - s)   columns j .. (the pivot in slot 0; zero beyond nc)
+//   [W - s, W)   v_{8p} .. v_{j-1}, this phase's vectors
+//   [W, NC)      the vectors of earlier phases, v_{8p'+q} in NC - 8(p'+1) + q
+// and after the step the slots of the window shift left by one, v_j
+// entering slot W - 1.  Each step reduces the window against the pivot
+// column.  For a column slot k, w_k = r_k + scal s_k (pivot-row entry plus
+// scal times the sum over the rows below the pivot) is v^T a_k, the update
+// coefficient; for a vector slot the same expression is v_i^T v_j, the Z
+// entry of two vectors of one phase, and the update leaves the slot alone.
+// The Z entries of vectors of different phases come after the last step
+// from the same wave-wide reduction over pair products of the slots.
+// Pivot-row values come from lane j through LDS (rowj), column sums from the
+// wave-local reduction, and w reaches every lane through LDS (wsh).  Lanes
+// k < nc - j write row j of R straight to the R output.
 template <int NC>
 __global__ __launch_bounds__(64, NC <= 24 ? 4 : 3) void k_tile_qr(const SqrDesc* __restrict__ ds) {
+    static_assert(NC % 8 == 0 && NC <= QC, "tile width");
     __shared__ double pw[64 * 17];
     __shared__ double rowj[NC];
-    __shared__ double taus[NC];
     __shared__ double wsh[NC];   // w_k of the current step, read back as broadcasts
     const SqrDesc d = ds[blockIdx.x];
     const int nc = d.nc, mt = d.nstack * d.nbs;
@@ -126,11 +153,12 @@ __global__ __launch_bounds__(64, NC <= 24 ? 4 : 3) void k_tile_qr(const SqrDesc*
     double* q0 = d.qout + (r0 < mt ? r0 : 0);
     double* q1 = d.qout + (r1 < mt ? r1 : 0);
 
-    // factorization steps j0 <= j < j1 with W live columns
+    // factorization steps j0 <= j < j1 with window W
     auto fwd = [&](auto wc, int j0, int j1) {
         constexpr int W = decltype(wc)::value;
 #pragma unroll 1
         for (int j = j0; j < j1; ++j) {
+            const int vb = W - (j - j0);   // first vector slot of this phase
             // S_k = sum over rows > j of a(row, 0) a(row, k)   (rows of a1 are >= 64 > j)
             const double h0 = r0 > j ? a0[0] : 0.0;
             if (r0 == j) {
@@ -151,9 +179,9 @@ __global__ __launch_bounds__(64, NC <= 24 ? 4 : 3) void k_tile_qr(const SqrDesc*
             }
             const double v0 = (r0 > j) ? scal * a0[0] : (r0 == j ? 1.0 : 0.0);
             const double v1 = scal * a1[0];
-            const double wl = rk + scal * sk;   // v^T a_k for this lane's column
+            const double wl = rk + scal * sk;   // v^T a_k, or v_i^T v_j in a vector slot
             if (tau != 0.0) {
-                if (lane < W) wsh[lane] = wl;
+                if (lane < W) wsh[lane] = lane < vb ? wl : 0.0;
                 __builtin_amdgcn_wave_barrier();
                 const double t0 = tau * v0, t1 = tau * v1;
 #pragma unroll
@@ -172,78 +200,80 @@ __global__ __launch_bounds__(64, NC <= 24 ? 4 : 3) void k_tile_qr(const SqrDesc*
                 const double rv = lane == 0 ? beta : (tau != 0.0 ? __fma_rn(-tau, wl, rk) : rk);
                 d.rout[j + (size_t)(j + lane) * d.ldr] = rv;
             }
-            if (r0 == j) taus[j] = tau;
+            // column j of Z: tau on the diagonal, v_i^T v_j of this phase's vectors above it
+            double* zc = d.zout + (size_t)j * QC;
+            if (lane == 0) zc[j] = tau;
+            else if (lane >= vb && lane < W) zc[j0 + lane - vb] = wl;
 #pragma unroll
             for (int k = 0; k < W - 1; ++k) {
                 a0[k] = a0[k + 1];
                 a1[k] = a1[k + 1];
             }
-            a0[W - 1] = 0.0;
-            a1[W - 1] = 0.0;
+            a0[W - 1] = v0;
+            a1[W - 1] = v1;
         }
+    };
+    // rotate the window [0, W) left by one slot
+    auto rot1 = [&](auto wc) {
+        constexpr int W = decltype(wc)::value;
+        const double t0 = a0[0], t1 = a1[0];
+#pragma unroll
+        for (int k = 0; k < W - 1; ++k) {
+            a0[k] = a0[k + 1];
+            a1[k] = a1[k + 1];
+        }
+        a0[W - 1] = t0;
+        a1[W - 1] = t1;
+    };
+    // Z entries v_i^T v_j, i in phase PI < PJ the phase of j: 16 pairs
+    // (i = 8 PI + Q0 + k / 8, j = 8 PJ + k % 8) per reduction round
+    auto cross = [&](auto pic, auto pjc, auto qc) {
+        constexpr int PI = decltype(pic)::value, PJ = decltype(pjc)::value, Q0 = decltype(qc)::value;
+        double pr[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int si = NC - 8 * (PI + 1) + Q0 + k / 8, sj = NC - 8 * (PJ + 1) + k % 8;
+            pr[k] = a0[si] * a0[sj] + a1[si] * a1[sj];
+        }
+        const double g = lds_colsum<16>(pr, pw, lane);
+        const int i = 8 * PI + Q0 + (lane & 15) / 8, j = 8 * PJ + (lane & 7);
+        if (lane < 16 && j < nc) d.zout[i + (size_t)j * QC] = g;
     };
     fwd(ic<NC>{}, 0, min(nc, 8));
     if constexpr (NC > 8) fwd(ic<NC - 8>{}, 8, min(nc, 16));
     if constexpr (NC > 16) fwd(ic<NC - 16>{}, 16, min(nc, 24));
-    if constexpr (NC > 24) fwd(ic<NC - 24>{}, 24, min(nc, 32));
+    if constexpr (NC > 8) {
+        // a short last phase leaves its vectors at the top of its window:
+        // rotate them down to slots NC - 8(p + 1) + q
+        const int lastp = (nc - 1) / 8, nrot = 8 * (lastp + 1) - nc;
+#pragma unroll 1
+        for (int t = 0; t < nrot; ++t) {
+            if (lastp == 1) rot1(ic<NC - 8>{});
+            if constexpr (NC > 16)
+                if (lastp == 2) rot1(ic<NC - 16>{});
+        }
+        if (nc > 8) {
+            cross(ic<0>{}, ic<1>{}, ic<0>{});
+            cross(ic<0>{}, ic<1>{}, ic<2>{});
+            cross(ic<0>{}, ic<1>{}, ic<4>{});
+            cross(ic<0>{}, ic<1>{}, ic<6>{});
+        }
+        if constexpr (NC > 16)
+            if (nc > 16) {
+                cross(ic<0>{}, ic<2>{}, ic<0>{});
+                cross(ic<0>{}, ic<2>{}, ic<2>{});
+                cross(ic<0>{}, ic<2>{}, ic<4>{});
+                cross(ic<0>{}, ic<2>{}, ic<6>{});
+                cross(ic<1>{}, ic<2>{}, ic<0>{});
+                cross(ic<1>{}, ic<2>{}, ic<2>{});
+                cross(ic<1>{}, ic<2>{}, ic<4>{});
+                cross(ic<1>{}, ic<2>{}, ic<6>{});
+            }
+    }
     __builtin_amdgcn_wave_barrier();
     for (int e = lane; e < nc * nc; e += 64) {   // zero strict lower triangle of R
         const int i = e % nc, cc = e / nc;
         if (i > cc) d.rout[i + (size_t)cc * d.ldr] = 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < NC; ++k) {
-        a0[k] = 0.0;
-        a1[k] = 0.0;
-    }
-    double n0 = (nc > 0 && r0 < mt) ? q0[(size_t)(nc - 1) * d.ldq] : 0.0;
-    double n1 = (nc > 0 && r1 < mt) ? q1[(size_t)(nc - 1) * d.ldq] : 0.0;
-    // accumulation steps jhi >= j >= jlo with W live columns
-    auto bwd = [&](auto wc, int jhi, int jlo) {
-        constexpr int W = decltype(wc)::value;
-#pragma unroll 1
-        for (int j = jhi; j >= jlo; --j) {
-            const double v0 = n0, v1 = n1;
-            if (j > 0) {
-                n0 = (r0 < mt) ? q0[(size_t)(j - 1) * d.ldq] : 0.0;
-                n1 = (r1 < mt) ? q1[(size_t)(j - 1) * d.ldq] : 0.0;
-            }
-#pragma unroll
-            for (int k = W - 1; k > 0; --k) {
-                a0[k] = a0[k - 1];
-                a1[k] = a1[k - 1];
-            }
-            a0[0] = 0.0;
-            a1[0] = 0.0;
-            const double tau = taus[j];
-            const double dk = wave_colsum<NC, W>(v0, a0, v1, a1, pw, lane);
-            if (tau != 0.0) {
-                if (lane < W) wsh[lane] = dk;
-                __builtin_amdgcn_wave_barrier();
-                const double t0 = tau * v0, t1 = tau * v1;
-#pragma unroll
-                for (int k = 1; k < W; ++k) {
-                    const double w = wsh[k];
-                    a0[k] -= t0 * w;
-                    a1[k] -= t1 * w;
-                }
-                __builtin_amdgcn_wave_barrier();
-            }
-            a0[0] = (r0 > j) ? -tau * v0 : (r0 == j ? 1.0 - tau : 0.0);
-            a1[0] = -tau * v1;
-        }
-    };
-    // step j touches columns j..nc-1: W >= NC - j
-    bwd(ic<8>{}, nc - 1, NC - 8);
-    if constexpr (NC > 8) bwd(ic<16>{}, min(nc - 1, NC - 9), NC - 16);
-    if constexpr (NC > 16) bwd(ic<24>{}, min(nc - 1, NC - 17), NC - 24);
-    if constexpr (NC > 24) bwd(ic<32>{}, min(nc - 1, NC - 25), NC - 32);
-#pragma unroll
-    for (int k = 0; k < NC; ++k) {
-        if (k < nc) {
-            if (r0 < mt) q0[(size_t)k * d.ldq] = a0[k];
-            if (r1 < mt) q1[(size_t)k * d.ldq] = a1[k];
-        }
     }
 }
 
@@ -262,7 +292,7 @@ static void launch_tile_qr(Ctx& c, const std::vector<SqrDesc>& ds) {
         if (k == 0) KMH_LAUNCH(c, k_tile_qr<8>, g, dim3(64), dd);
         else if (k == 1) KMH_LAUNCH(c, k_tile_qr<16>, g, dim3(64), dd);
         else {
-            // algorithmic bytes: the tile read, Q and R written
+            // algorithmic bytes: the tile read, V and R written
             double by = 0.0;
             if (c.prof)
                 for (const auto& d : cls[k]) {
@@ -274,6 +304,180 @@ static void launch_tile_qr(Ctx& c, const std::vector<SqrDesc>& ds) {
     }
 }
 
+// out = Q C = [C; 0] - V Y with Y = T V1^T C (V1 the top nb rows of V) for
+// the Householder vectors V (mt x nb) and coupling Z of k_tile_qr; C null
+// means C = I (the explicit Q).  out may be V itself.
+struct WyDesc {
+    const double* V; int ldv; int mt;
+    const double* Z;          // ld QC
+    const double* C; int ldc; int nb;
+    double* out; int ldo; int pad;
+};
+
+typedef double qd4 __attribute__((ext_vector_type(4)));
+
+// One 128-thread workgroup per tile.  V1, C and Z go to LDS; X = V1^T C
+// (MFMA); Y solves T^-1 Y = X, i.e. y_i = tau_i (x_i - sum_{k>i}
+// z_ik y_k), one column per thread with the right-hand side in registers
+// (bottom row first; a zero tau gives a zero row, H_i = I).  Then each wave
+// forms 16-row blocks of out^T = Cpad^T - Y^T V^T on v_mfma_f64_16x16x4f64
+// (first operand -Y^T from LDS, second V^T streamed from memory, the
+// accumulator starting from Cpad^T), so that a store instruction writes
+// 16-row column segments of out.  A wave reads a row block of V before it
+// writes that block, and V1 is in LDS before any block is written, so out
+// may alias V.  (Up to 128 rows: eight 16-row blocks, four per wave.)
+__global__ __launch_bounds__(128, 4) void k_wy_apply(const WyDesc* __restrict__ ds) {
+    constexpr int S = QC + 1;
+    constexpr int KS = QC / 4;   // k steps of 4
+    __shared__ double Vs[QC * S];
+    __shared__ double Cs[QC * S];
+    __shared__ double Zs[QC * S];
+    __shared__ double Ys[QC * QC];
+    const WyDesc d = ds[blockIdx.x];
+    const int nb = d.nb, t = threadIdx.x;
+    const int wave = t >> 6, lane = t & 63, li = lane & 15, lk = lane >> 4;
+    // wave w takes row blocks w, w + 2, w + 4, w + 6 in two passes of two;
+    // the first pass's V^T fragments V(row, 4 ks + lk) load before the
+    // prologue so that their latency overlaps it
+    double vf[2][KS];
+    auto load_rows = [&](int pass) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int row = (wave + 4 * pass + 2 * h) * 16 + li;
+            const bool rok = row < d.mt;
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const int k = 4 * ks + lk;
+                const double x = __builtin_nontemporal_load(d.V + (rok ? row : 0) + (size_t)(k < nb ? k : 0) * d.ldv);
+                vf[h][ks] = (rok && k < nb) ? x : 0.0;
+            }
+        }
+    };
+    load_rows(0);
+    {
+        // V1, C and Z, zero outside nb x nb: all loads in flight before the LDS writes
+        constexpr int NE = (QC * QC + 127) / 128;
+        double lv[NE], lc[NE], lz[NE];
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+            const int e = t + 128 * q, i = e % QC, k = e / QC;
+            const bool in = e < QC * QC && i < nb && k < nb;
+            const size_t ii = in ? i : 0, kk = in ? k : 0;
+            lv[q] = in ? d.V[ii + kk * d.ldv] : 0.0;
+            lc[q] = in ? (d.C ? d.C[ii + kk * d.ldc] : (i == k ? 1.0 : 0.0)) : 0.0;
+            lz[q] = (in && i <= k) ? d.Z[ii + kk * QC] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+            const int e = t + 128 * q, i = e % QC, k = e / QC;
+            if (e < QC * QC) {
+                Vs[i * S + k] = lv[q];
+                Cs[i * S + k] = lc[q];
+                Zs[i * S + k] = lz[q];
+            }
+        }
+    }
+    __syncthreads();
+    {
+        // X = V1^T C on v_mfma_f64_16x16x4f64: wave w takes the 16-row block
+        // w of X (rows a = 16 w + x), both 16-column blocks; V1^T and C are
+        // zero outside nb x nb, and rows / columns beyond QC are not stored
+        qd4 xa[2] = {qd4{0.0, 0.0, 0.0, 0.0}, qd4{0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const int k = 4 * ks + lk, a = 16 * wave + li;
+            const double va = a < QC ? Vs[k * S + a] : 0.0;     // V1^T(a, k)
+#pragma unroll
+            for (int yb = 0; yb < 2; ++yb) {
+                const int b = 16 * yb + li;
+                const double cb = b < QC ? Cs[k * S + b] : 0.0;   // C(k, b)
+                xa[yb] = __builtin_amdgcn_mfma_f64_16x16x4f64(va, cb, xa[yb], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int yb = 0; yb < 2; ++yb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int a = 16 * wave + 4 * r + lk, b = 16 * yb + li;
+                if (a < QC && b < QC) Ys[a * QC + b] = xa[yb][r];
+            }
+    }
+    __syncthreads();
+    if (t < nb) {
+        double x[QC];
+#pragma unroll
+        for (int i = 0; i < QC; ++i) x[i] = Ys[i * QC + t];
+#pragma unroll
+        for (int i = QC - 1; i >= 0; --i) {
+            if (i < nb) {
+                const double y = Zs[i * S + i] * x[i];
+                x[i] = y;
+#pragma unroll
+                for (int r = 0; r < i; ++r) x[r] = __fma_rn(-Zs[r * S + i], y, x[r]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < QC; ++i)
+            if (i < nb) Ys[i * QC + t] = x[i];
+    }
+    __syncthreads();
+    double yf[2][KS];   // -Y(4 ks + lk, 16 cb + li)
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const int k = 4 * ks + lk, col = 16 * cb + li;
+            yf[cb][ks] = (k < nb && col < nb) ? -Ys[k * QC + col] : 0.0;
+        }
+#pragma unroll 1
+    for (int pass = 0; pass < 2; ++pass) {
+        if ((wave + 4 * pass) * 16 >= d.mt) break;
+        if (pass > 0) load_rows(pass);
+        qd4 acc[2][2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int row = (wave + 4 * pass + 2 * h) * 16 + li;
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = 16 * cb + 4 * r + lk;
+                    acc[h][cb][r] = (row < nb && col < nb) ? Cs[row * S + col] : 0.0;
+                }
+        }
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            if (4 * ks < nb) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    acc[h][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(yf[0][ks], vf[h][ks], acc[h][0], 0, 0, 0);
+                    acc[h][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(yf[1][ks], vf[h][ks], acc[h][1], 0, 0, 0);
+                }
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int row = (wave + 4 * pass + 2 * h) * 16 + li;
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = 16 * cb + 4 * r + lk;
+                    if (row < d.mt && col < nb)
+                        __builtin_nontemporal_store(acc[h][cb][r], d.out + row + (size_t)col * d.ldo);
+                }
+        }
+    }
+}
+
+static void wy_apply(Ctx& c, const std::vector<WyDesc>& ds) {
+    if (ds.empty()) return;
+    for (const auto& d : ds)
+        KMH_CHECK(d.nb <= QC && d.mt <= QT && d.mt >= d.nb, KMH_E_BAD_ARG, "WY apply beyond its size limits");
+    WyDesc* dd = c.upload_vec(ds);
+    KMH_LAUNCH(c, k_wy_apply, dim3((unsigned)ds.size()), dim3(128), dd);
+}
+
 struct TsqrPanel {
     double* A; int m; int nb; int lda;
     double* R; int ldr;
@@ -283,8 +487,10 @@ struct TsqrPanel {
 
 // Thin QR of m x nb panels (nb <= QC, m >= nb) by TSQR: leaf tiles of <= 128
 // rows, then levels of inner nodes each stacking up to QF children's R
-// factors; the leaf Q factors are finally multiplied by the product of the
-// inner Q blocks on the path to the root.
+// factors.  Every tile QR leaves its Householder vectors and coupling Z;
+// the down pass then forms each node's Q times its parent's coefficient
+// block (the root's is I) level by level, in place, and finally each leaf
+// tile's Q times its coefficient block, in place in the panel.
 static void tsqr(Ctx& c, const std::vector<TsqrPanel>& ps) {
     if (ps.empty()) return;
     struct Tile { int panel; int row0; int rows; int node; };
@@ -309,12 +515,12 @@ static void tsqr(Ctx& c, const std::vector<TsqrPanel>& ps) {
         while (k > 1) { const int up = (int)ceil_div(k, QF); ninternal += up; k = up; }
     }
     const int total_nodes = nnodes + ninternal;
-    const size_t RS = (size_t)QC * QC;          // R slot, ld QC
-    const size_t QS = (size_t)QF * QC * QC;     // Q slot of an inner node, (QF QC) x QC, ld QF QC
+    const size_t RS = (size_t)QC * QC;          // R and Z slots, ld QC
+    const size_t QS = (size_t)QF * QC * QC;     // V slot of an inner node, (QF QC) x QC, ld QF QC
     double* RT = static_cast<double*>(c.scratch(std::max<size_t>(1, total_nodes) * RS * sizeof(double)));
     double* QTb = static_cast<double*>(c.scratch(std::max(1, ninternal) * QS * sizeof(double)));
-    double* CS = static_cast<double*>(c.scratch(std::max<size_t>(1, total_nodes) * RS * sizeof(double)));
-    // leaf tile QRs (Q in place)
+    double* ZT = static_cast<double*>(c.scratch(std::max<size_t>(1, total_nodes) * RS * sizeof(double)));
+    // leaf tile QRs (V in place)
     {
         std::vector<SqrDesc> ds;
         for (const auto& t : tiles) {
@@ -323,7 +529,8 @@ static void tsqr(Ctx& c, const std::vector<TsqrPanel>& ps) {
             double* a = P.A + t.row0;
             const double* in = P.src ? P.src + t.row0 : a;
             ds.push_back({in, P.src ? P.lds : P.lda, 1, t.rows, 0, a, P.lda,
-                          single ? P.R : RT + (size_t)t.node * RS, single ? P.ldr : QC, P.nb, 0});
+                          single ? P.R : RT + (size_t)t.node * RS, single ? P.ldr : QC,
+                          ZT + (size_t)t.node * RS, P.nb, 0});
         }
         launch_tile_qr(c, ds);
     }
@@ -347,14 +554,11 @@ static void tsqr(Ctx& c, const std::vector<TsqrPanel>& ps) {
                 const int node = nnodes + next_internal;
                 const int qi = next_internal++;
                 inner.push_back({(int)p, node, L[i], cnt, level});
-                if (cnt == 1) {
-                    // a lone child passes through: its R is the node's R
-                    ds.push_back({RT + (size_t)L[i] * RS, QC, 1, nb, 0, QTb + (size_t)qi * QS, QF * QC,
-                                  final_level ? ps[p].R : RT + (size_t)node * RS, final_level ? ps[p].ldr : QC, nb, 0});
-                } else {
-                    ds.push_back({RT + (size_t)L[i] * RS, QC, cnt, nb, (long long)RS, QTb + (size_t)qi * QS, QF * QC,
-                                  final_level ? ps[p].R : RT + (size_t)node * RS, final_level ? ps[p].ldr : QC, nb, 0});
-                }
+                // children's R factors stacked (a lone child passes through)
+                ds.push_back({RT + (size_t)L[i] * RS, QC, cnt, nb, cnt == 1 ? 0 : (long long)RS,
+                              QTb + (size_t)qi * QS, QF * QC,
+                              final_level ? ps[p].R : RT + (size_t)node * RS, final_level ? ps[p].ldr : QC,
+                              ZT + (size_t)node * RS, nb, 0});
                 nxt.push_back(node);
                 if (final_level) root[p] = node;
             }
@@ -364,44 +568,37 @@ static void tsqr(Ctx& c, const std::vector<TsqrPanel>& ps) {
         launch_tile_qr(c, ds);
         ++level;
     }
-    // down pass: coefficient of each node (pointer + ld); the root's is I
+    // down pass: each inner node's Q times its coefficient block (the
+    // root's is I), in place in its V slot; child h's coefficient block is
+    // rows h nb .. h nb + nb - 1 of that product
     std::vector<const double*> cptr(total_nodes, nullptr);
     std::vector<int> cld(total_nodes, 0);
-    std::vector<char> is_ident(total_nodes, 0);
-    for (size_t p = 0; p < ps.size(); ++p) is_ident[root[p]] = 1;
     for (int lv = level - 1; lv >= 0; --lv) {
-        std::vector<GemmProb> gp;
+        std::vector<WyDesc> wd;
         for (const auto& nd : inner) {
             if (nd.level != lv) continue;
             const int nb = ps[nd.panel].nb;
-            const double* Q = QTb + (size_t)(nd.node - nnodes) * QS;
+            double* Q = QTb + (size_t)(nd.node - nnodes) * QS;
+            wd.push_back({Q, QF * QC, nd.count * nb, ZT + (size_t)nd.node * RS, cptr[nd.node], cld[nd.node], nb,
+                          Q, QF * QC, 0});
             for (int h = 0; h < nd.count; ++h) {
-                const int child = nd.first + h;
-                if (is_ident[nd.node]) {
-                    cptr[child] = Q + (size_t)h * nb;
-                    cld[child] = QF * QC;
-                } else {
-                    double* dst = CS + (size_t)child * RS;
-                    gp.push_back({nb, nb, nb, Q + (size_t)h * nb, QF * QC, cptr[nd.node], cld[nd.node], dst, QC,
-                                  1.0, 0.0, 0});
-                    cptr[child] = dst;
-                    cld[child] = QC;
-                }
+                cptr[nd.first + h] = Q + (size_t)h * nb;
+                cld[nd.first + h] = QF * QC;
             }
         }
-        gemm(c, false, false, gp);
+        wy_apply(c, wd);
     }
-    // tiles: A_tile <- Q_tile * C_tile (in place; n = nb <= QC keeps each
-    // output row block inside one workgroup, which reads it before writing)
-    std::vector<GemmProb> gp;
-    for (const auto& t : tiles) {
-        if (is_ident[t.node]) continue;
-        const auto& P = ps[t.panel];
-        double* a = P.A + t.row0;
-        gp.push_back({t.rows, P.nb, P.nb, a, P.lda, cptr[t.node], cld[t.node], a, P.lda, 1.0, 0.0, 0});
+    // leaf tiles: A_tile <- Q_tile C_tile in place
+    {
+        std::vector<WyDesc> wd;
+        for (const auto& t : tiles) {
+            const auto& P = ps[t.panel];
+            double* a = P.A + t.row0;
+            wd.push_back({a, P.lda, t.rows, ZT + (size_t)t.node * RS, cptr[t.node], cld[t.node], P.nb, a, P.lda, 0});
+        }
+        wy_apply(c, wd);
     }
-    gemm(c, false, false, gp);
-    c.scratch_free(CS);
+    c.scratch_free(ZT);
     c.scratch_free(QTb);
     c.scratch_free(RT);
 }

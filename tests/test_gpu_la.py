@@ -122,6 +122,26 @@ def test_qr(ctx, m, r):
     assert np.all(np.tril(R, -1) == 0.0)
 
 
+def test_qr_tile_widths(ctx):
+    """Block widths on both sides of every 8-column phase boundary of the
+    tile QR (a short last phase rotates its Householder vectors into place
+    before the cross-phase V^T V entries), on one tile (Q = E - V T V1^T
+    straight from the root) and on a three-tile tree (inner node, lone
+    pass-through nodes of a 5-tile tree)."""
+    rng = np.random.default_rng(11)
+    for m in (100, 300, 600):
+        for r in (1, 2, 7, 8, 9, 15, 16, 17, 23, 24):
+            print("qr widths", m, r, flush=True)
+            A0 = rng.standard_normal((m, r)) * np.logspace(0, -6, r)
+            A = np.asfortranarray(A0.copy())
+            R = np.zeros((r, r), order="F")
+            rc = ctx.lib.kmh_test_qr(ctx.ptr, m, r, A.ctypes.data, m, R.ctypes.data, r)
+            assert rc == 0, ctx.lib.last_error()
+            assert np.abs(A.T @ A - np.eye(r)).max() < 1e-12, (m, r)
+            assert np.abs(A @ R - A0).max() <= 1e-13 * np.abs(A0).max(), (m, r)
+            assert np.all(np.tril(R, -1) == 0.0), (m, r)
+
+
 def test_qr_rank_deficient(ctx):
     """Zero and repeated columns inside a 64-column block (Householder
     tiles) and numerically dependent columns in a later block still give
@@ -211,9 +231,9 @@ def test_profile_hooks_time_sampled_launches(gpu_lib):
 
 def test_profile_hooks_time_tile_qr(gpu_lib):
     """KMH_PROF_TILE_QR: the 24-column tile QR launches of a panel QR are
-    timed with their algorithmic bytes (tiles read, Q and R written) and stay
-    out of the GEMM totals.  A 300 x 24 panel: three leaf tiles (100 rows
-    each) and one inner node stacking their R factors."""
+    timed with their algorithmic bytes (tiles read, Householder vectors and R
+    written) and stay out of the GEMM totals.  A 300 x 24 panel: three leaf
+    tiles (100 rows each) and one inner node stacking their R factors."""
     import kmhodlr
     from kmhodlr import _lib as L
     ctx = kmhodlr.Context(0)
