@@ -124,9 +124,11 @@ template <int N> using ic = std::integral_constant<int, N>;
 // from the same wave-wide reduction over pair products of the slots.
 // Pivot-row values come from lane j through LDS (rowj), column sums from the
 // wave-local reduction, and w reaches every lane through LDS (wsh).  Lanes
-// k < nc - j write row j of R straight to the R output.
+// k < nc - j write row j of R straight to the R output.  The 24-column tile
+// runs three waves per SIMD: its step loop spills at the 128 registers of
+// four, which costs more than the fourth wave gains.
 template <int NC>
-__global__ __launch_bounds__(64, NC <= 24 ? 4 : 3) void k_tile_qr(const SqrDesc* __restrict__ ds) {
+__global__ __launch_bounds__(64, NC <= 16 ? 4 : 3) void k_tile_qr(const SqrDesc* __restrict__ ds) {
     static_assert(NC % 8 == 0 && NC <= QC, "tile width");
     __shared__ double pw[64 * 17];
     __shared__ double rowj[NC];
