@@ -267,3 +267,61 @@ def test_second_derivative_blocks_vs_finite_difference(j, k, code):
         leaf[a0:b1, c0:d1] = True
     assert np.abs((ref - fd)[leaf]).max() <= 1e-6 * scale
     assert np.abs((ref - fd)[~leaf]).max() > 1e-2 * np.abs(fd[~leaf]).max()
+
+
+@pytest.mark.parametrize("j,k,code", [(0, 1, kk.D12), (1, 0, kk.D12), (1, 1, kk.D22)])
+def test_reference_second_derivative_blocks_are_exact_plus_stated_delta(j, k, code):
+    """Pins the reference-form block products SDBlock_mul / SDBlock_mul_t
+    (HODLR/src/utils.jl:110-155) through the exact form (pinned by finite
+    differences above): block by block, reference - exact equals the
+    difference read off the reference's term list, written out here as
+    separate dense products.  With A = K(P,P)^-1, K1 = K(rows,P),
+    K2 = K(P,cols), a subscript j or k the parameter derivative and
+    S_j, S_k, S_jk the derivatives of K(P,P):
+      SDBlock_mul - exact =
+          2 K1k A Sj A K2 - 2 K1 A Sk A Sj A K2 + 2 K1 A Sjk A K2
+        - 2 K1 A Sj A Sk A K2 + K1 A Sj A (K2j + K2k)
+        + K1k A (K2 - K2j) - K1 A Sk A (K2 - K2j)
+      SDBlock_mul_t - exact^T = ( (K1k - K1j) A K2k + 2 K1k A Sj A K2
+        - 2 K1 A Sk A Sj A K2 + 2 K1 A Sjk A K2 - 2 K1 A Sj A Sk A K2
+        + 2 K1 A Sj A K2k )^T
+    (sign flips of the d_k(-K1 A Sj A K2) terms, K2 in place of K2j in
+    two terms of SDBlock_mul, K1k in place of K1j and the flipped K2k term
+    in SDBlock_mul_t).  A dropped term, a wrong sign or a swapped operand in
+    the restatement of either reference function breaks one of the two."""
+    n = 128
+    K, rng = make(n, params=(1.2, 0.25), seed=5)
+    HK = H.KernelHODLR(K, 0.0, 12, H.FixedLevel(2), nystrom=True)
+    d1 = [K.kernel.derivative(kk.D1), K.kernel.derivative(kk.D2)]
+    DKj = H.DerivativeHODLR(K, d1[j], HK)
+    DKk = H.DerivativeHODLR(K, d1[k], HK)
+    D2 = H.SecondDerivative(K, K.kernel.derivative(code), HK)
+    lm = K.x1[H.nystrom_landmarks(n, HK.mrnk)]
+    Sfull = H.kernel_matrix(K.kernel, lm, lm)
+    A = np.linalg.inv(Sfull)
+    Sj, Sk, Sjk = DKj.Sj, DKk.Sj, D2.Sjk
+    nchk = 0
+    for lev in range(len(DKj.B)):
+        for b, (a0, b1, c0, d1_) in enumerate(DKj.nonleafindices[lev]):
+            Bj, Bk, Bjk = DKj.B[lev][b], DKk.B[lev][b], D2.B[lev][b]
+            K1, K2 = Bj.K1p, Bj.Kp2
+            K1j, K1k, K2j, K2k = Bj.K1pd, Bk.K1pd, Bj.Kp2d, Bk.Kp2d
+            args = (Bj, Bk, Bjk)
+            mats = (DKj.S, Sj, Sk, Sjk)
+            Mref = np.stack([H.sdblock_mul(*args, e, *mats) for e in np.eye(d1_ - c0)], axis=1)
+            Nref = np.stack([H.sdblock_mul_t(*args, e, *mats) for e in np.eye(b1 - a0)], axis=1)
+            E = H.sdblock_full_exact(*args, *mats)
+            delta = (2 * K1k @ A @ Sj @ A @ K2 - 2 * K1 @ A @ Sk @ A @ Sj @ A @ K2
+                     + 2 * K1 @ A @ Sjk @ A @ K2 - 2 * K1 @ A @ Sj @ A @ Sk @ A @ K2
+                     + K1 @ A @ Sj @ A @ (K2j + K2k)
+                     + K1k @ A @ (K2 - K2j) - K1 @ A @ Sk @ A @ (K2 - K2j))
+            delta_t = ((K1k - K1j) @ A @ K2k + 2 * K1k @ A @ Sj @ A @ K2
+                       - 2 * K1 @ A @ Sk @ A @ Sj @ A @ K2 + 2 * K1 @ A @ Sjk @ A @ K2
+                       - 2 * K1 @ A @ Sj @ A @ Sk @ A @ K2 + 2 * K1 @ A @ Sj @ A @ K2k).T
+            scale = max(np.abs(Mref).max(), np.abs(E).max(), np.abs(delta).max())
+            assert np.abs(delta).max() > 1e-3 * scale          # the forms do differ
+            assert np.abs((Mref - E) - delta).max() <= 1e-9 * scale
+            scale_t = max(np.abs(Nref).max(), np.abs(E).max(), np.abs(delta_t).max())
+            assert np.abs((Nref - E.T) - delta_t).max() <= 1e-9 * scale_t
+            nchk += 1
+    assert nchk == 3
