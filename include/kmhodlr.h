@@ -155,6 +155,8 @@ int kmh_hodlr_build_dev(kmh_ctx* ctx, const double* pts_dev, int64_t n, int32_t 
                         kmh_hodlr** out);
 int kmh_hodlr_destroy(kmh_hodlr* h);
 int kmh_hodlr_info(kmh_hodlr* h, kmh_info* out);
+/* wait for the handle's queued device work (its context's stream)          */
+int kmh_hodlr_synchronize(kmh_hodlr* h);
 
 /* leaf boundaries: nleaves+1 offsets (indices.jl:29-37)                      */
 int kmh_hodlr_leaf_offsets(kmh_hodlr* h, int64_t* offsets);
@@ -231,7 +233,9 @@ typedef struct kmh_serve_req {
     int32_t ndk;              /* number of derivative kernels (0: no gradient) */
     int32_t on_device;        /* pts, y and saa are device pointers the caller
                                  keeps alive until the request completes      */
-    int32_t reserved;
+    int32_t keep_factor;      /* 1: keep the factorized HODLR resident after
+                                 completion for follow-up solves
+                                 (kmh_server_factor) until kmh_server_release */
     kmh_kernel kernel;
     kmh_build_opts opts;
     const double* y;          /* data vector, n                             */
@@ -247,6 +251,9 @@ typedef struct kmh_serve_result {
     double  grad[KMH_SERVE_MAX_PARAMS];
     double  queue_us;         /* submit -> admission                        */
     double  latency_us;       /* submit -> completion                       */
+    int32_t resident;         /* 1: the factor is resident (keep_factor)    */
+    int32_t reserved;
+    int64_t factor_bytes;     /* device bytes the resident factor holds     */
 } kmh_serve_result;
 int kmh_server_create(kmh_ctx* ctx, int32_t max_inflight, uint64_t hbm_budget_bytes, kmh_server** out);
 /* 1..KMH_SERVE_MAX_LANES lanes; KMH_E_STATE while requests are queued or
@@ -254,10 +261,32 @@ int kmh_server_create(kmh_ctx* ctx, int32_t max_inflight, uint64_t hbm_budget_by
 int kmh_server_set_lanes(kmh_server* srv, int32_t lanes);
 int kmh_server_destroy(kmh_server* srv);
 int kmh_server_submit(kmh_server* srv, const kmh_serve_req* req, int64_t* id);
+/* A failure inside a step (device allocation, budget, HIP error) fails every
+ * request of the lane it happened on: their results carry the status and
+ * message, their device memory and reservations are released, and the step
+ * returns KMH_OK; other lanes and later steps proceed.  ACA requests are
+ * admitted on an estimate of their ranks; after the build the reservation
+ * follows the actual ranks, and a request that then exceeds the budget fails
+ * alone with KMH_E_BUDGET.  completed counts finished requests, failed ones
+ * included.                                                                 */
 int kmh_server_step(kmh_server* srv, int32_t* completed);
 int kmh_server_pending(kmh_server* srv, int32_t* waiting, int32_t* inflight, uint64_t* reserved_bytes);
-/* KMH_E_STATE while the request is still queued or running                 */
+/* KMH_E_STATE while the request is still queued or running.  A request
+ * without a resident factor is forgotten once its result is read; one with
+ * a resident factor stays until kmh_server_release.                        */
 int kmh_server_result(kmh_server* srv, int64_t id, kmh_serve_result* out);
+/* The resident factor of a completed keep_factor request, as a HODLR handle
+ * lent by the server: the factorized HK after nll_objective, for follow-up
+ * HK \ b, logdet and W applies (HODLR/src/baseoverloads.jl:95-209) with the
+ * kmh_hodlr_* entry points.  The handle lives on the context of the lane
+ * that served the request (the server's context for lane 0): its host-pointer
+ * calls are synchronous, its _dev calls are ordered on that lane's stream
+ * (kmh_hodlr_synchronize waits for them).  Valid until kmh_server_release
+ * or kmh_server_destroy; kmh_hodlr_destroy refuses it.  Its bytes stay
+ * reserved against the server's HBM budget.  (No reference counterpart.) */
+int kmh_server_factor(kmh_server* srv, int64_t id, kmh_hodlr** out);
+/* Forget a completed request: frees its resident factor and reservation.  */
+int kmh_server_release(kmh_server* srv, int64_t id);
 
 /* ---- HODLR.stoch_gradient(K, HK, y, dfuns, vecs)  stochasticgradient.jl:30-45
  * dks: p derivative kernels; y: n; saa: n x nv column-major; grad: p        */

@@ -36,6 +36,16 @@ int kmh_test_arena(kmh_ctx* ctx, int32_t count, int32_t words, int64_t* bad);
 int kmh_test_profile_spans(kmh_ctx* ctx, int32_t cap, int32_t* tags, double* amounts, double* flops, double* ms,
                            int32_t* grids, int32_t* count);
 
+/* failure injection: the context's next `after` scratch allocations succeed
+ * and the one after fails with KMH_E_OOM on the host (no device call);
+ * after < 0 turns it off                                                    */
+int kmh_test_fail_scratch(kmh_ctx* ctx, int64_t after);
+/* bytes of the context's scratch pool handed out and not returned          */
+int kmh_test_scratch_live(kmh_ctx* ctx, uint64_t* bytes);
+/* replace the descriptor arena by one of `bytes` (after draining the
+ * stream), so that the next large uploads make it grow                     */
+int kmh_test_arena_shrink(kmh_ctx* ctx, uint64_t bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,16 +5,6 @@
 #include "hodlr.h"
 #include "la.h"
 
-struct kmh_hodlr {
-    std::unique_ptr<kmh::Hodlr> h;
-    kmh_ctx* owner = nullptr;
-    explicit kmh_hodlr(kmh_ctx* c) : owner(c) { kmh_ctx_hold(c); }
-    ~kmh_hodlr() {
-        h.reset();   // buffers back to the context's pool first
-        kmh_ctx_release(owner);
-    }
-};
-
 using kmh::Error;
 
 #define KMH_TRY(...) KMH_API_TRY(__VA_ARGS__)
@@ -205,8 +195,17 @@ int kmh_hodlr_build_dev(kmh_ctx* ctx, const double* pts_dev, int64_t n, int32_t 
 int kmh_hodlr_destroy(kmh_hodlr* h) {
     KMH_TRY({
         if (!h) return KMH_OK;
+        KMH_CHECK(!h->borrowed, KMH_E_STATE,
+                  "this handle is a served request's resident factor: release it with kmh_server_release");
         if (h->h) h->h->ctx->set_device();
         delete h;
+    })
+}
+
+int kmh_hodlr_synchronize(kmh_hodlr* h) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        H.ctx->sync();
     })
 }
 

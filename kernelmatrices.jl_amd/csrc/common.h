@@ -115,8 +115,26 @@ struct Ctx {
     // handed to a later kernel at once.
     std::vector<std::pair<char*, size_t>> chunks;
     std::map<char*, std::pair<size_t, int>> free_blocks;   // start -> (bytes, chunk)
-    std::unordered_map<void*, std::pair<size_t, int>> live;
+    struct LiveBlock {
+        size_t bytes;
+        int chunk;
+        uint64_t seq;   // allocation number (scratch_sweep)
+    };
+    std::unordered_map<void*, LiveBlock> live;
+    uint64_t alloc_seq = 0;
     size_t pool_bytes = 0;
+    // Arenas replaced by a grow: a caller may still hold descriptor
+    // addresses in them for a launch it has not queued yet, so they are
+    // freed with the context, not at the grow.
+    std::vector<char*> retired_arenas;
+    // Test hook: scratch() fails (KMH_E_OOM, no device call) once this many
+    // more blocks have been handed out; negative = never.
+    int64_t fail_scratch_after = -1;
+    size_t live_bytes() const {
+        size_t b = 0;
+        for (const auto& kv : live) b += kv.second.bytes;
+        return b;
+    }
 
     void set_device() const { KMH_HIP(hipSetDevice(device)); }
     // Copy `bytes` of host data into the arena; returns the device address.
@@ -175,6 +193,10 @@ struct Ctx {
     hipStream_t q() { flush(); return stream; }
     void* scratch(size_t bytes);
     void scratch_free(void* p);
+    // Return every block handed out since allocation number `since` (after a
+    // failed stage: its stage-local blocks; the handles it built are freed
+    // by their owners first).
+    void scratch_sweep(uint64_t since);
     void sync() {
         flush();
         KMH_HIP(hipStreamSynchronize(stream));

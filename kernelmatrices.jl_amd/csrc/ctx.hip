@@ -55,11 +55,13 @@ void* Ctx::upload(const void* host, size_t bytes) {
     const size_t need = (bytes + 255) & ~size_t(255);
     const size_t seg = arena_size / ARENA_SEGS;
     if (need > seg) {
-        // grow (rare): drain the stream, reallocate both sides
+        // grow (rare): drain the stream, reallocate both sides.  The old
+        // arena stays allocated until teardown: descriptors uploaded into it
+        // by the caller for a launch not queued yet are still read from it.
         sync();
         size_t ns = std::max<size_t>(arena_size * 2, need * ARENA_SEGS * 2);
         ns = (ns + 255 * ARENA_SEGS) / (256 * ARENA_SEGS) * (256 * ARENA_SEGS);
-        if (arena) KMH_HIP(hipFree(arena));
+        if (arena) retired_arenas.push_back(arena);
         if (staging) KMH_HIP(hipHostFree(staging));
         arena = nullptr;
         staging = nullptr;
@@ -163,6 +165,8 @@ hipEvent_t Ctx::prof_event() {
 }
 
 void* Ctx::scratch(size_t bytes) {
+    if (fail_scratch_after >= 0 && fail_scratch_after-- == 0)
+        throw Error(KMH_E_OOM, "scratch allocation of " + std::to_string(bytes) + " bytes failed (injected)");
     bytes = std::max<size_t>(256, (bytes + 255) & ~size_t(255));
     auto it = free_blocks.begin();
     for (; it != free_blocks.end(); ++it)
@@ -186,7 +190,7 @@ void* Ctx::scratch(size_t bytes) {
     const int chunk = it->second.second;
     free_blocks.erase(it);
     if (avail > bytes) free_blocks.emplace(p + bytes, std::make_pair(avail - bytes, chunk));
-    live.emplace(p, std::make_pair(bytes, chunk));
+    live.emplace(p, LiveBlock{bytes, chunk, alloc_seq++});
     return p;
 }
 
@@ -195,8 +199,8 @@ void Ctx::scratch_free(void* vp) {
     auto lt = live.find(vp);
     KMH_CHECK(lt != live.end(), KMH_E_BAD_ARG, "scratch_free of an unknown block");
     char* p = static_cast<char*>(vp);
-    size_t bytes = lt->second.first;
-    const int chunk = lt->second.second;
+    size_t bytes = lt->second.bytes;
+    const int chunk = lt->second.chunk;
     live.erase(lt);
     // coalesce with the free neighbours of the same chunk
     auto nx = free_blocks.lower_bound(p);
@@ -212,6 +216,13 @@ void Ctx::scratch_free(void* vp) {
         }
     }
     free_blocks.emplace(p, std::make_pair(bytes, chunk));
+}
+
+void Ctx::scratch_sweep(uint64_t since) {
+    std::vector<void*> stale;
+    for (const auto& kv : live)
+        if (kv.second.seq >= since) stale.push_back(kv.first);
+    for (void* p : stale) scratch_free(p);
 }
 
 void Ctx::release_pool() noexcept {
@@ -232,6 +243,7 @@ void ctx_teardown(kmh_ctx* ctx) noexcept {
     for (auto& f : c.seg_fence)
         if (f.ev) (void)hipEventDestroy(f.ev);
     if (c.arena) (void)hipFree(c.arena);
+    for (char* a : c.retired_arenas) (void)hipFree(a);
     if (c.staging) (void)hipHostFree(c.staging);
     if (c.d_info) (void)hipFree(c.d_info);
     (void)hipStreamDestroy(c.stream);

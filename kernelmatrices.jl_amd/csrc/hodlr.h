@@ -2,6 +2,7 @@
 // factor.  Every stage runs over a list of handles, so several independent
 // problems (of different sizes, kernels and ranks) share each grouped launch.
 #pragma once
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -44,15 +45,36 @@ struct Hodlr {
 
     // factor
     bool factorized = false;
+    bool factor_started = false;                 // between factor_begin_batch and factor_end_batch
     std::vector<std::vector<int64_t>> coff_blk;  // offset of block (2r x 2r) in X / G
     DevBuf X;                                    // chol(C) - I per block
     DevBuf G;                                    // I - chol(C)^{-1} per block
     double logdet = 0.0;
 
-    // transient state of a factorization in progress
+    // transient state of a factorization in progress (scratch blocks of
+    // ctx's pool, released by factor_end_batch, by a new factorization or
+    // with the handle)
     double* AB = nullptr;                        // Nystrom cascade panels (n x 2r)
     double* parts = nullptr;                     // log-diagonal partial sums
     int nparts = 0;
+
+    Hodlr() = default;
+    Hodlr(const Hodlr&) = delete;
+    Hodlr& operator=(const Hodlr&) = delete;
+    ~Hodlr() { release_transient(); }
+    void release_transient() noexcept {
+        if (!ctx) return;
+        for (double** p : {&AB, &parts}) {
+            if (*p) {
+                try {
+                    ctx->scratch_free(*p);
+                } catch (...) {
+                }
+            }
+            *p = nullptr;
+        }
+        nparts = 0;
+    }
 
     bool nystrom() const { return opts.method == KMH_NYSTROM; }
     // U (rows a0..b1) and V (rows c0..d1) of block k at level l before
@@ -146,3 +168,17 @@ void hodlr_stoch_gradient_batch(Ctx& c, const std::vector<GradJob>& jobs);
 void check_kernel(const kmh_kernel& k, bool allow_deriv);
 
 }  // namespace kmh
+
+// Opaque C-ABI HODLR handle: the matrix and the context it lives on (held
+// until the handle is destroyed).  A handle lent out by a server for a
+// resident factor (kmh_server_factor) is owned by the server.
+struct kmh_hodlr {
+    std::unique_ptr<kmh::Hodlr> h;
+    kmh_ctx* owner = nullptr;
+    bool borrowed = false;
+    explicit kmh_hodlr(kmh_ctx* c) : owner(c) { kmh_ctx_hold(c); }
+    ~kmh_hodlr() {
+        h.reset();   // buffers back to the context's pool first
+        kmh_ctx_release(owner);
+    }
+};

@@ -663,6 +663,8 @@ void factor_begin_batch(Ctx& c, const std::vector<Hodlr*>& hs) {
     std::vector<Rhs> lt;
     for (Hodlr* h : hs) {
         Hodlr& H = *h;
+        H.release_transient();
+        H.factor_started = true;
         const int64_t n = H.n;
         const int L = H.L, nl = H.nleaves();
         H.nparts = nl + H.nblocks();
@@ -720,10 +722,8 @@ void factor_end_batch(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<Status>
     std::vector<int*> failed;
     for (size_t j = 0; j < hs.size(); ++j) {
         Hodlr& H = *hs[j];
-        if (H.AB) c.scratch_free(H.AB);
-        c.scratch_free(H.parts);
-        H.AB = nullptr;
-        H.parts = nullptr;
+        H.release_transient();
+        H.factor_started = false;
         if (st[j].code != KMH_OK) {
             failed.push_back(H.info.as<int>());
         } else {
@@ -743,6 +743,12 @@ void hodlr_factorize_batch(Ctx& c, const std::vector<Hodlr*>& hs_in, std::vector
         Hodlr* h = hs_in[i];
         if (!h) { st[i].code = KMH_E_BAD_ARG; st[i].msg = "null HODLR handle"; continue; }
         if (h->factorized) { st[i].code = KMH_E_STATE; st[i].msg = "The matrix is already factorized."; continue; }
+        if (h->factor_started) {
+            // the panels were transformed in place by the part that ran
+            st[i].code = KMH_E_STATE;
+            st[i].msg = "A previous factorization of this matrix failed part-way; build it again.";
+            continue;
+        }
         hs.push_back(h);
         pos.push_back(i);
     }

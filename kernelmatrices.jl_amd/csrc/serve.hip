@@ -22,8 +22,10 @@
 #include <chrono>
 #include <deque>
 #include <exception>
+#include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <thread>
 
 #include "hodlr.h"
@@ -59,10 +61,14 @@ struct Req {
     Status st;
     double nll = 0, logdet = 0;
     std::vector<double> grad;
+    bool keep = false;               // keep the factor resident after completion
+    bool read = false;               // result read (kmh_server_result)
+    std::unique_ptr<kmh_hodlr> handle;   // the resident factor, lent out by kmh_server_factor
 };
 
 struct Lane {
     kmh_ctx* own = nullptr;   // private context (lanes after the first)
+    kmh_ctx* kctx = nullptr;  // the C-ABI context of the lane (the caller's for lane 0)
     Ctx* ctx = nullptr;
     std::vector<Req*> inflight;
 };
@@ -81,21 +87,63 @@ struct Server {
 
 // Device bytes a request holds while in flight (FP64): points, leaves,
 // Nystrom panels, level slabs, per-block factors and the transient cascade
-// panels; ACA panels are bounded by maxrank (or the leaf size).
-uint64_t request_footprint(int64_t n, const kmh_build_opts& o, int nv) {
+// panels.  ACA ranks are not known before the build: the estimate takes the
+// largest rank the build's first panels hold (256, or maxrank, or the block
+// size if smaller) on every level, and counts the build's transient panels
+// (one n x capacity panel per level beside the compacted level slabs).
+// After the build the reservation is set from the actual ranks
+// (resident_bytes + the factorization's transients), and a request that no
+// longer fits the budget then fails alone.
+static int levels_of(int64_t n, const kmh_build_opts& o) {
     const int fl = [&] { int l = -1; int64_t m = n; while (m) { m >>= 1; ++l; } return l; }();
-    int L = o.level_kind == KMH_FIXED_LEVEL ? std::min(std::max(0, o.level), fl)
-                                            : std::min(std::max(0, fl - o.level), fl);
+    return o.level_kind == KMH_FIXED_LEVEL ? std::min(std::max(0, o.level), fl)
+                                           : std::min(std::max(0, fl - o.level), fl);
+}
+
+uint64_t request_footprint(int64_t n, const kmh_build_opts& o, int nv) {
+    const int L = levels_of(n, o);
     const int64_t nleaf = int64_t(1) << L;
     const int64_t m = (n + nleaf - 1) / nleaf;
-    int r = o.maxrank;
-    if (o.method == KMH_ACA && r == 0) r = (int)std::min<int64_t>(m, 512);
-    uint64_t b = 16 * (uint64_t)n + 8 * (uint64_t)n * m;
-    b += 8 * (uint64_t)n * r * (uint64_t)L;                       // level slabs
-    b += 8 * (uint64_t)(nleaf - 1) * 2 * (uint64_t)(4 * r * r);   // X and G
-    if (o.method == KMH_NYSTROM) b += 8 * (uint64_t)n * r * 4 + 16 * (uint64_t)r * r;
+    uint64_t r = (uint64_t)o.maxrank;
+    uint64_t slabs = 0, xg = 0;
+    if (o.method == KMH_ACA) {
+        for (int l = 0; l < L; ++l) {
+            const uint64_t h = (uint64_t)m << l;   // block size at level l (1-based l+1)
+            uint64_t rl = std::min<uint64_t>(h, o.maxrank > 0 ? (uint64_t)o.maxrank : h);
+            rl = std::min<uint64_t>(rl, 256);
+            slabs += 8 * (uint64_t)n * rl * 2;                   // slab + the build's panel
+            xg += 8 * (uint64_t)(nleaf >> (l + 1)) * 2 * (4 * rl * rl);
+            r = std::max(r, rl);
+        }
+    } else {
+        slabs = 8 * (uint64_t)n * r * (uint64_t)L;
+        xg = 8 * (uint64_t)(nleaf - 1) * 2 * (4 * r * r);
+    }
+    uint64_t b = 16 * (uint64_t)n + 8 * (uint64_t)n * m + slabs + xg;
+    if (o.method == KMH_NYSTROM) b += 8 * (uint64_t)n * r * 4 + 16 * r * r;
     b += 8 * (uint64_t)n * (2 + nv) * 2;                          // right-hand sides
-    if (nv > 0) b += 8 * (uint64_t)n * (m + (uint64_t)r + 2);     // derivative leaves and panel
+    if (nv > 0) b += 8 * (uint64_t)n * (m + r + 2);               // derivative leaves and panel
+    return b;
+}
+
+// Device bytes of a request from its built handle: the handle's buffers,
+// the factorization's transients (cascade panels, per-block factors not
+// allocated yet) and the right-hand sides.
+static uint64_t built_footprint(const Hodlr& H, int nv) {
+    uint64_t xg = 0, rmax = 0;
+    for (const auto& lv : H.ranks)
+        for (int r : lv) {
+            xg += 8 * 2 * (4 * (uint64_t)r * r);
+            rmax = std::max<uint64_t>(rmax, (uint64_t)r);
+        }
+    const uint64_t n = (uint64_t)H.n;
+    uint64_t b = H.device_bytes() + xg + 8 * n * (2 + nv) * 2;
+    if (H.nystrom()) b += 8 * n * (uint64_t)H.Rtot + 8 * n * 2 * rmax;   // level slabs + cascade panels
+    if (nv > 0) {
+        int64_t m = 0;
+        for (int i = 0; i < H.nleaves(); ++i) m = std::max<int64_t>(m, H.leaf_size(i));
+        b += 8 * n * ((uint64_t)m + rmax + 2);
+    }
     return b;
 }
 
@@ -205,7 +253,8 @@ static void finish_batch(Ctx& c, std::vector<Req*>& rs) {
 }
 
 // One stage of every in-flight request of a lane, on the lane's context.
-static void lane_step(Lane& ln) {
+// adjust: called (on the lane's thread) with the requests just built.
+static void lane_step(Lane& ln, const std::function<void(std::vector<Req*>&)>& adjust) {
     Ctx& c = *ln.ctx;
     c.set_device();
     // group the lane's requests by stage (each advances one stage this step)
@@ -252,6 +301,7 @@ static void lane_step(Lane& ln) {
                 build[i]->stage = ST_DONE;
             }
         }
+        if (adjust) adjust(build);
     }
     if (!fin.empty()) finish_batch(c, fin);
 }
@@ -291,11 +341,34 @@ int server_step(Server& S) {
     for (auto& ln : S.lanes)
         if (!ln.inflight.empty()) act.push_back(&ln);
     std::vector<std::exception_ptr> err(act.size());
+    // after the build the reservation follows the actual ranks; a request
+    // that no longer fits the budget fails alone (lanes run in parallel:
+    // the server's counters are updated under a lock)
+    std::mutex mu;
+    auto adjust = [&](std::vector<Req*>& built) {
+        std::lock_guard<std::mutex> lk(mu);
+        for (Req* q : built) {
+            if (!q->H) continue;
+            const uint64_t actual = built_footprint(*q->H, q->nv);
+            if (actual > q->footprint && S.budget && S.reserved - q->footprint + actual > S.budget) {
+                q->st.code = KMH_E_BUDGET;
+                q->st.msg = "request footprint exceeds the HBM budget after compression (" +
+                            std::to_string(actual) + " bytes)";
+                q->stage = ST_DONE;
+                q->H.reset();
+                continue;
+            }
+            S.reserved = S.reserved - q->footprint + actual;
+            q->footprint = actual;
+        }
+    };
+    std::vector<uint64_t> mark(act.size());
+    for (size_t i = 0; i < act.size(); ++i) mark[i] = act[i]->ctx->alloc_seq;
     std::vector<std::thread> th;
     for (size_t i = 1; i < act.size(); ++i)
         th.emplace_back([&, i]() {
             try {
-                lane_step(*act[i]);
+                lane_step(*act[i], adjust);
                 flush_queued();
             } catch (...) {
                 discard_queued();
@@ -304,7 +377,7 @@ int server_step(Server& S) {
         });
     if (!act.empty()) {
         try {
-            lane_step(*act[0]);
+            lane_step(*act[0], adjust);
             flush_queued();
         } catch (...) {
             discard_queued();
@@ -312,8 +385,35 @@ int server_step(Server& S) {
         }
     }
     for (auto& t : th) t.join();
-    for (auto& e : err)
-        if (e) std::rethrow_exception(e);
+    // A lane whose step failed (allocation failure, budget, HIP error) leaves
+    // its requests part-way through a stage whose in-place updates are not
+    // repeatable: every request of that lane fails with the error, and the
+    // other lanes and later steps go on.
+    for (size_t i = 0; i < act.size(); ++i) {
+        if (!err[i]) continue;
+        Status fs;
+        try {
+            std::rethrow_exception(err[i]);
+        } catch (const Error& e) {
+            fs.code = e.code;
+            fs.msg = e.what();
+        } catch (const std::bad_alloc&) {
+            fs.code = KMH_E_OOM;
+            fs.msg = "host allocation failed";
+        } catch (const std::exception& e) {
+            fs.code = KMH_E_DEVICE;
+            fs.msg = e.what();
+        }
+        Ctx& lc = *act[i]->ctx;
+        lc.drain();
+        for (Req* q : act[i]->inflight) {
+            q->H.reset();
+            if (q->stage == ST_DONE && q->st.code != KMH_OK) continue;
+            q->st = fs;
+            q->stage = ST_DONE;
+        }
+        lc.scratch_sweep(mark[i]);   // stage-local blocks of the failed step
+    }
     // retire finished requests
     int done = 0;
     for (auto& ln : S.lanes) {
@@ -321,8 +421,19 @@ int server_step(Server& S) {
         for (Req* q : ln.inflight) {
             if (q->stage == ST_DONE) {
                 q->t_done = now_us();
-                q->H.reset();
-                S.reserved -= q->footprint;
+                if (q->keep && q->st.code == KMH_OK && q->H) {
+                    // resident factor: the reservation becomes the factor's bytes
+                    const uint64_t res = q->H->device_bytes();
+                    S.reserved = S.reserved - q->footprint + res;
+                    q->footprint = res;
+                    q->handle.reset(new kmh_hodlr(ln.kctx));
+                    q->handle->borrowed = true;
+                    q->handle->h = std::move(q->H);
+                } else {
+                    q->H.reset();
+                    S.reserved -= q->footprint;
+                    q->footprint = 0;
+                }
                 ++done;
             } else {
                 keep.push_back(q);
@@ -355,7 +466,10 @@ struct kmh_server {
     }
     void drop_lanes() noexcept {
         for (auto& ln : s.lanes)
-            if (ln.own) kmh::ctx_teardown(ln.own);
+            if (ln.own) {
+                ln.own->closing = true;   // torn down with its last resident factor
+                if (ln.own->holders == 0) kmh::ctx_teardown(ln.own);
+            }
         s.lanes.clear();
     }
     // lane 0 runs on the caller's context, the others on private ones
@@ -363,9 +477,11 @@ struct kmh_server {
         drop_lanes();
         s.lanes.resize(1);
         s.lanes[0].ctx = s.ctx;
+        s.lanes[0].kctx = owner;
         for (int l = 1; l < n; ++l) {
             kmh::Lane ln;
             ln.own = kmh::ctx_create(s.ctx->device, 0);
+            ln.kctx = ln.own;
             ln.ctx = &ln.own->c;
             s.lanes.push_back(ln);
         }
@@ -394,6 +510,8 @@ int kmh_server_set_lanes(kmh_server* srv, int32_t lanes) {
         KMH_CHECK(lanes >= 1 && lanes <= KMH_SERVE_MAX_LANES, KMH_E_BAD_ARG, "lane count out of range");
         KMH_CHECK(srv->s.inflight.empty() && srv->s.waiting.empty(), KMH_E_STATE,
                   "lanes can change only while no request is queued or running");
+        for (const auto& kv : srv->s.all)
+            KMH_CHECK(!kv.second->handle, KMH_E_STATE, "lanes can change only while no factor is resident");
         srv->s.ctx->set_device();
         srv->set_lanes(lanes);
     })
@@ -427,6 +545,7 @@ int kmh_server_submit(kmh_server* srv, const kmh_serve_req* rq, int64_t* id) {
         }
         q->kern = rq->kernel;
         q->opts = rq->opts;
+        q->keep = rq->keep_factor != 0;
         q->nv = rq->ndk ? (int)rq->nv : 0;
         if (rq->ndk) {
             if (rq->on_device) q->dsaa = rq->saa;
@@ -475,8 +594,38 @@ int kmh_server_result(kmh_server* srv, int64_t id, kmh_serve_result* out) {
         for (int j = 0; j < KMH_SERVE_MAX_PARAMS; ++j) out->grad[j] = j < (int)q.grad.size() ? q.grad[j] : 0.0;
         out->queue_us = q.t_admit > 0 ? q.t_admit - q.t_submit : 0.0;
         out->latency_us = q.t_done - q.t_submit;
+        out->resident = q.handle ? 1 : 0;
+        out->factor_bytes = q.handle ? (int64_t)q.footprint : 0;
         if (q.st.code != KMH_OK) kmh::set_last_error(q.st.msg);
-        srv->s.all.erase(it);
+        q.read = true;
+        if (!q.handle) srv->s.all.erase(it);
+    })
+}
+
+int kmh_server_factor(kmh_server* srv, int64_t id, kmh_hodlr** out) {
+    SRV_TRY({
+        KMH_CHECK(srv && out, KMH_E_BAD_ARG, "null argument");
+        auto it = srv->s.all.find(id);
+        KMH_CHECK(it != srv->s.all.end(), KMH_E_BAD_ARG, "unknown request id");
+        kmh::Req& q = *it->second;
+        KMH_CHECK(q.stage == kmh::ST_DONE, KMH_E_STATE, "request not finished");
+        KMH_CHECK(q.handle, KMH_E_STATE, "the request keeps no resident factor");
+        *out = q.handle.get();
+    })
+}
+
+int kmh_server_release(kmh_server* srv, int64_t id) {
+    SRV_TRY({
+        KMH_CHECK(srv, KMH_E_BAD_ARG, "null server");
+        auto it = srv->s.all.find(id);
+        KMH_CHECK(it != srv->s.all.end(), KMH_E_BAD_ARG, "unknown request id");
+        kmh::Req& q = *it->second;
+        KMH_CHECK(q.stage == kmh::ST_DONE, KMH_E_STATE, "request not finished");
+        if (q.handle) {
+            q.handle->owner->c.set_device();
+            srv->s.reserved -= q.footprint;
+        }
+        srv->s.all.erase(it);   // the factor's buffers go back to its lane's pool
     })
 }
 

@@ -136,4 +136,41 @@ int kmh_test_profile_spans(kmh_ctx* ctx, int32_t cap, int32_t* tags, double* amo
     })
 }
 
+int kmh_test_fail_scratch(kmh_ctx* ctx, int64_t after) {
+    TRY_({
+        KMH_CHECK(ctx, KMH_E_BAD_ARG, "null context");
+        ctx->c.fail_scratch_after = after < 0 ? -1 : after;
+    })
+}
+
+int kmh_test_scratch_live(kmh_ctx* ctx, uint64_t* bytes) {
+    TRY_({
+        KMH_CHECK(ctx && bytes, KMH_E_BAD_ARG, "null argument");
+        *bytes = (uint64_t)ctx->c.live_bytes();
+    })
+}
+
+int kmh_test_arena_shrink(kmh_ctx* ctx, uint64_t bytes) {
+    TRY_({
+        KMH_CHECK(ctx, KMH_E_BAD_ARG, "null context");
+        kmh::Ctx& c = ctx->c;
+        const size_t segb = 256 * kmh::Ctx::ARENA_SEGS;
+        KMH_CHECK(bytes >= segb && bytes % segb == 0, KMH_E_BAD_ARG, "arena size must be a multiple of 1 KiB");
+        c.set_device();
+        c.sync();
+        char* na = nullptr;
+        char* ns = nullptr;
+        KMH_HIP(hipMalloc(&na, bytes));
+        KMH_HIP(hipHostMalloc(&ns, bytes, hipHostMallocDefault));
+        if (c.arena) c.retired_arenas.push_back(c.arena);
+        if (c.staging) KMH_HIP(hipHostFree(c.staging));
+        c.arena = na;
+        c.staging = ns;
+        c.arena_size = bytes;
+        c.arena_off = c.dirty_lo = c.dirty_hi = 0;
+        c.arena_seg = 0;
+        for (auto& f : c.seg_fence) f.pending = false;
+    })
+}
+
 }  // extern "C"

@@ -45,7 +45,7 @@ KMH_SERVE_MAX_PARAMS = 4
 
 class kmh_serve_req(C.Structure):
     _fields_ = [("pts", C.c_void_p), ("n", C.c_int64), ("d", C.c_int32), ("ndk", C.c_int32),
-                ("on_device", C.c_int32), ("reserved", C.c_int32),
+                ("on_device", C.c_int32), ("keep_factor", C.c_int32),
                 ("kernel", kmh_kernel), ("opts", kmh_build_opts), ("y", C.c_void_p), ("saa", C.c_void_p),
                 ("nv", C.c_int64), ("dks", kmh_kernel * KMH_SERVE_MAX_PARAMS)]
 
@@ -53,7 +53,8 @@ class kmh_serve_req(C.Structure):
 class kmh_serve_result(C.Structure):
     _fields_ = [("status", C.c_int32), ("ngrad", C.c_int32), ("nll", C.c_double), ("logdet", C.c_double),
                 ("grad", C.c_double * KMH_SERVE_MAX_PARAMS), ("queue_us", C.c_double),
-                ("latency_us", C.c_double)]
+                ("latency_us", C.c_double), ("resident", C.c_int32), ("reserved", C.c_int32),
+                ("factor_bytes", C.c_int64)]
 
 
 class kmh_info(C.Structure):
@@ -109,6 +110,9 @@ SIGNATURES = {
     "kmh_server_step": [_P, C.POINTER(C.c_int32)],
     "kmh_server_pending": [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_uint64)],
     "kmh_server_result": [_P, C.c_int64, C.POINTER(kmh_serve_result)],
+    "kmh_server_factor": [_P, C.c_int64, C.POINTER(_P)],
+    "kmh_server_release": [_P, C.c_int64],
+    "kmh_hodlr_synchronize": [_P],
     "kmh_kd_order": [_P, _I64, C.c_int32, C.c_int32, _P],
     "kmh_rhodlr_build": [_P, _P, _I64, _P, _I64, C.c_int32, C.POINTER(kmh_kernel), C.POINTER(kmh_build_opts),
                          C.POINTER(C.c_void_p)],
@@ -129,6 +133,9 @@ TEST_SIGNATURES = {
     "kmh_test_arena": [_P, C.c_int32, C.c_int32, C.POINTER(C.c_int64)],
     "kmh_test_profile_spans": [_P, C.c_int32, C.POINTER(C.c_int32), _D, _D, _D, C.POINTER(C.c_int32),
                                C.POINTER(C.c_int32)],
+    "kmh_test_fail_scratch": [_P, C.c_int64],
+    "kmh_test_scratch_live": [_P, C.POINTER(C.c_uint64)],
+    "kmh_test_arena_shrink": [_P, C.c_uint64],
 }
 
 

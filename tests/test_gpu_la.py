@@ -255,3 +255,46 @@ def test_profile_hooks_time_tile_qr(gpu_lib):
     inner = 8.0 * (2 * 3 * r * r + r * r)
     assert b.value == leaves + inner
     assert np.abs(A @ R - A0).max() <= 1e-13 * np.abs(A0).max()
+
+
+def test_arena_growth_between_uploads_of_one_launch(gpu_lib):
+    """Descriptor-arena growth in the middle of a launch's uploads (arena
+    shrunk through a test hook, so the HODLR stages' descriptor lists make it
+    grow at different points): the arena replaced by a grow stays valid for
+    the launch that uploaded into it, and build + factorize + solve give
+    bitwise the results of a context with the default arena."""
+    import ctypes as C
+    import kmhodlr as km
+    from kmhodlr import _lib as L
+    lib = km.load()
+    rng = np.random.default_rng(31)
+    n = 3000
+    x = np.ascontiguousarray(rng.random((n, 2)))
+    b = rng.standard_normal(n)
+    kern = L.kmh_kernel()
+    kern.family = L.KMH_MATERN32
+    kern.params[0], kern.params[1], kern.nugget = 1.0, 0.1, 1e-6
+    opts = L.kmh_build_opts()
+    opts.method = L.KMH_NYSTROM
+    opts.maxrank = 40
+    opts.level_kind = L.KMH_FIXED_LEVEL
+    opts.level = 4
+
+    def run(shrink):
+        ctx = km.Context(0)
+        if shrink:
+            assert lib.kmh_test_arena_shrink(ctx.ptr, shrink) == 0, lib.last_error()
+        h = C.c_void_p()
+        assert lib.kmh_hodlr_build(ctx.ptr, x.ctypes.data, n, 2, C.byref(kern), C.byref(opts), C.byref(h)) == 0
+        assert lib.kmh_hodlr_factorize(h) == 0, lib.last_error()
+        out = np.zeros(n)
+        assert lib.kmh_hodlr_solve(h, b.ctypes.data, out.ctypes.data, 1) == 0
+        ld = C.c_double()
+        assert lib.kmh_hodlr_logdet(h, C.byref(ld)) == 0
+        assert lib.kmh_hodlr_destroy(h) == 0
+        return out, ld.value
+
+    ref = run(0)
+    for shrink in (1024, 2048, 4096, 8192, 16384):
+        got = run(shrink)
+        assert np.array_equal(got[0], ref[0]) and got[1] == ref[1], shrink

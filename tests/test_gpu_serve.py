@@ -221,6 +221,7 @@ def test_server_destroyed_with_requests_in_flight(gpu_lib):
     lib = km.load()
     ctx = km.Context(0)
     probs = _problems([1500, 2600, 900, 3100, 2048], seed=13)
+    live0 = _live(lib, ctx)
     srv = C.c_void_p()
     assert lib.kmh_server_create(ctx.ptr, 8, 0, C.byref(srv)) == 0
     assert lib.kmh_server_set_lanes(srv, 3) == 0
@@ -237,6 +238,7 @@ def test_server_destroyed_with_requests_in_flight(gpu_lib):
     assert lib.kmh_server_pending(srv, C.byref(w), C.byref(f), C.byref(rb)) == 0
     assert f.value == len(probs) and done.value == 0
     assert lib.kmh_server_destroy(srv) == 0
+    assert _live(lib, ctx) == live0       # the transient factorization blocks went with the handles
     res, _ = _serve(lib, L, ctx, probs[:2], 4)
     assert all(r[0] == 0 and r[1] == 0 for r in res.values())
 
@@ -279,3 +281,210 @@ def test_serve_failed_request_leaves_the_others(gpu_lib):
             assert rc == 0 and out.status == 0, lib.last_error()
             assert out.nll == alone[i][2] and out.logdet == alone[i][3]
     assert lib.kmh_server_destroy(srv) == 0
+
+
+def _run_all(lib, srv, limit=200):
+    done = C.c_int32()
+    for _ in range(limit):
+        assert lib.kmh_server_step(srv, C.byref(done)) == 0, lib.last_error()
+        w, f, rb = C.c_int32(), C.c_int32(), C.c_uint64()
+        lib.kmh_server_pending(srv, C.byref(w), C.byref(f), C.byref(rb))
+        if w.value == 0 and f.value == 0:
+            return rb.value
+    raise AssertionError("server did not drain")
+
+
+def _live(lib, ctx):
+    b = C.c_uint64()
+    assert lib.kmh_test_scratch_live(ctx.ptr, C.byref(b)) == 0
+    return b.value
+
+
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_resident_factor_follow_up_solve(gpu_lib, lanes):
+    """keep_factor: a served request's factor stays resident after its
+    nll_objective; a follow-up HK \\ b on it (kmh_server_factor +
+    kmh_hodlr_solve, HODLR/src/baseoverloads.jl:181-192) equals the direct
+    path's build + symmetricfactorize! + solve bitwise, and so does logdet.
+    The factor's bytes stay reserved until kmh_server_release."""
+    import kmhodlr as km
+    from kmhodlr import _lib as L
+    lib = km.load()
+    ctx = km.Context(0)
+    probs = _problems([3000, 1800, 2500], seed=21)
+    srv = C.c_void_p()
+    assert lib.kmh_server_create(ctx.ptr, 8, 0, C.byref(srv)) == 0
+    assert lib.kmh_server_set_lanes(srv, lanes) == 0
+    keep, ids = [], []
+    for i, (x, y, saa, params, level) in enumerate(probs):
+        r = _mkreq(L, x, y, saa, params, level, 40, grad=(i != 1))
+        r.keep_factor = 1
+        keep.append(r)
+        rid = C.c_int64()
+        assert lib.kmh_server_submit(srv, C.byref(r), C.byref(rid)) == 0, lib.last_error()
+        ids.append(rid.value)
+    reserved = _run_all(lib, srv)
+    assert reserved > 0
+    rng = np.random.default_rng(5)
+    total = 0
+    for i, rid in enumerate(ids):
+        x, y, saa, params, level = probs[i]
+        n = x.shape[0]
+        out = L.kmh_serve_result()
+        assert lib.kmh_server_result(srv, rid, C.byref(out)) == 0 and out.status == 0, lib.last_error()
+        assert out.resident == 1 and out.factor_bytes > 0
+        total += out.factor_bytes
+        h = C.c_void_p()
+        assert lib.kmh_server_factor(srv, rid, C.byref(h)) == 0, lib.last_error()
+        b = np.asfortranarray(rng.standard_normal((n, 2)))
+        xs = np.zeros_like(b)
+        assert lib.kmh_hodlr_solve(h, b.ctypes.data, xs.ctypes.data, 2) == 0, lib.last_error()
+        ld = C.c_double()
+        assert lib.kmh_hodlr_logdet(h, C.byref(ld)) == 0 and ld.value == out.logdet
+        # the direct path on the same inputs
+        hd = C.c_void_p()
+        assert lib.kmh_hodlr_build(ctx.ptr, x.ctypes.data, n, 2, C.byref(keep[i].kernel), C.byref(keep[i].opts),
+                                   C.byref(hd)) == 0, lib.last_error()
+        assert lib.kmh_hodlr_factorize(hd) == 0, lib.last_error()
+        xd = np.zeros_like(b)
+        assert lib.kmh_hodlr_solve(hd, b.ctypes.data, xd.ctypes.data, 2) == 0
+        ldd = C.c_double()
+        assert lib.kmh_hodlr_logdet(hd, C.byref(ldd)) == 0
+        assert np.array_equal(xs, xd) and ldd.value == ld.value
+        assert lib.kmh_hodlr_destroy(hd) == 0
+        assert lib.kmh_hodlr_destroy(h) == L.KMH_E_STATE        # lent by the server
+    assert total == reserved
+    for rid in ids:
+        assert lib.kmh_server_release(srv, rid) == 0, lib.last_error()
+    w, f, rb = C.c_int32(), C.c_int32(), C.c_uint64()
+    assert lib.kmh_server_pending(srv, C.byref(w), C.byref(f), C.byref(rb)) == 0 and rb.value == 0
+    h = C.c_void_p()
+    assert lib.kmh_server_factor(srv, ids[0], C.byref(h)) == L.KMH_E_BAD_ARG
+    assert lib.kmh_server_destroy(srv) == 0
+
+
+def test_resident_factor_survives_server_destroy_order(gpu_lib):
+    """A server destroyed while factors are resident frees them with it (its
+    private lane contexts go after them); the caller's context's scratch
+    returns to its value before the server."""
+    import kmhodlr as km
+    from kmhodlr import _lib as L
+    lib = km.load()
+    ctx = km.Context(0)
+    probs = _problems([1500, 2200, 1800], seed=27)
+    live0 = _live(lib, ctx)
+    srv = C.c_void_p()
+    assert lib.kmh_server_create(ctx.ptr, 8, 0, C.byref(srv)) == 0
+    assert lib.kmh_server_set_lanes(srv, 3) == 0
+    keep = []
+    for x, y, saa, params, level in probs:
+        r = _mkreq(L, x, y, saa, params, level, 40, grad=False)
+        r.keep_factor = 1
+        keep.append(r)
+        rid = C.c_int64()
+        assert lib.kmh_server_submit(srv, C.byref(r), C.byref(rid)) == 0
+    _run_all(lib, srv)
+    assert lib.kmh_server_set_lanes(srv, 1) == L.KMH_E_STATE    # factors resident
+    assert lib.kmh_server_destroy(srv) == 0
+    assert _live(lib, ctx) == live0
+
+
+def test_serve_step_failure_fails_its_lane_only(gpu_lib):
+    """An allocation failure inside a step (injected on the host, no device
+    fault) fails every request of that lane with KMH_E_OOM; their device
+    memory, the failed stage's blocks and their reservations are released,
+    the step returns KMH_OK, and new requests afterwards get the same
+    results as before."""
+    import kmhodlr as km
+    from kmhodlr import _lib as L
+    lib = km.load()
+    ctx = km.Context(0)
+    probs = _problems([1500, 2600, 900], seed=23)
+    alone, _ = _serve(lib, L, ctx, probs, 1, lanes=1)
+    live0 = _live(lib, ctx)
+    for fail_after in (0, 3, 9):       # in the build, the leaf stage, a level
+        srv = C.c_void_p()
+        assert lib.kmh_server_create(ctx.ptr, 8, 0, C.byref(srv)) == 0
+        assert lib.kmh_server_set_lanes(srv, 1) == 0
+        keep, ids = [], []
+        for x, y, saa, params, level in probs:
+            r = _mkreq(L, x, y, saa, params, level, 40)
+            keep.append(r)
+            rid = C.c_int64()
+            assert lib.kmh_server_submit(srv, C.byref(r), C.byref(rid)) == 0
+            ids.append(rid.value)
+        done = C.c_int32()
+        if fail_after > 0:
+            assert lib.kmh_server_step(srv, C.byref(done)) == 0, lib.last_error()   # build
+        assert lib.kmh_test_fail_scratch(ctx.ptr, fail_after) == 0
+        assert lib.kmh_server_step(srv, C.byref(done)) == 0, lib.last_error()
+        assert lib.kmh_test_fail_scratch(ctx.ptr, -1) == 0
+        if fail_after > 0:
+            # a stage shared by the lane's requests: all of them fail
+            assert done.value == len(probs)
+            w, f, rb = C.c_int32(), C.c_int32(), C.c_uint64()
+            assert lib.kmh_server_pending(srv, C.byref(w), C.byref(f), C.byref(rb)) == 0
+            assert f.value == 0 and rb.value == 0
+        else:
+            # the build reports per request: the one whose allocation failed
+            # fails alone, the others go on
+            assert done.value == 1
+            assert _run_all(lib, srv) == 0
+        for i, rid in enumerate(ids):
+            out = L.kmh_serve_result()
+            assert lib.kmh_server_result(srv, rid, C.byref(out)) == 0
+            if fail_after > 0 or i == 0:
+                assert out.status == L.KMH_E_OOM
+            else:
+                assert out.status == 0 and out.nll == alone[i][2], lib.last_error()
+        assert _live(lib, ctx) == live0
+        # the same server serves the next requests
+        ids2 = []
+        for r in keep:
+            rid = C.c_int64()
+            assert lib.kmh_server_submit(srv, C.byref(r), C.byref(rid)) == 0
+            ids2.append(rid.value)
+        _run_all(lib, srv)
+        for i, rid in enumerate(ids2):
+            out = L.kmh_serve_result()
+            assert lib.kmh_server_result(srv, rid, C.byref(out)) == 0 and out.status == 0, lib.last_error()
+            assert out.nll == alone[i][2] and out.logdet == alone[i][3]
+            assert list(out.grad[:out.ngrad]) == alone[i][4]
+        assert lib.kmh_server_destroy(srv) == 0
+        assert _live(lib, ctx) == live0
+
+
+def test_serve_aca_request_under_budget(gpu_lib):
+    """ACA requests are admitted on a rank estimate; after the build the
+    reservation is the actual footprint.  Under a budget that fits the
+    estimate but not a much larger actual, the request fails alone with
+    KMH_E_BUDGET; under a roomy budget it succeeds and the reservation
+    returns to zero."""
+    import kmhodlr as km
+    from kmhodlr import _lib as L
+    lib = km.load()
+    ctx = km.Context(0)
+    (x, y, saa, params, level), = _problems([2048], seed=29)
+    r = _mkreq(L, x, y, saa, params, level, 0, grad=False)
+    r.opts.method = L.KMH_ACA
+    r.opts.tol = 1e-10
+    for budget, ok in ((1 << 34, True), (1 << 20, False)):
+        srv = C.c_void_p()
+        assert lib.kmh_server_create(ctx.ptr, 4, budget, C.byref(srv)) == 0
+        rid = C.c_int64()
+        assert lib.kmh_server_submit(srv, C.byref(r), C.byref(rid)) == 0
+        assert _run_all(lib, srv) == 0
+        out = L.kmh_serve_result()
+        assert lib.kmh_server_result(srv, rid.value, C.byref(out)) == 0
+        assert (out.status == 0) == ok, lib.last_error()
+        if not ok:
+            assert out.status == L.KMH_E_BUDGET
+        else:
+            res = out
+        assert lib.kmh_server_destroy(srv) == 0
+    out = res
+    # the direct path's nll on the same ACA matrix
+    K = km.KernelMatrix(x, x, params, km.matern32(1e-6))
+    HK = km.KernelHODLR(K, 1e-10, 0, km.FixedLevel(level), nystrom=False)
+    km.symmetricfactorize(HK)
+    assert abs(out.nll - km.negloglik(HK, y)) <= 1e-10 * abs(out.nll)
