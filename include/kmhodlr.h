@@ -185,6 +185,15 @@ int kmh_hodlr_factor_apply_dev(kmh_hodlr* h, int32_t op, const double* x, double
 int kmh_hodlr_solve(kmh_hodlr* h, const double* b, double* x, int64_t nrhs);
 int kmh_hodlr_solve_dev(kmh_hodlr* h, const double* b, double* x, int64_t nrhs);
 
+/* The same solve without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * stream capture, serving loops, harnesses that forbid hipMalloc).  Every
+ * temporary lives in one fixed 32 MiB device array per GPU, held until the
+ * process ends; a call whose temporaries could exceed it (large nrhs or
+ * ranks) fails with KMH_E_BAD_ARG before any work is queued.  Calls that
+ * overlap on one GPU (different contexts or threads) race on that array.  */
+int kmh_hodlr_solve_dev_noalloc(kmh_hodlr* h, const double* b, double* x, int64_t nrhs);
+
 /* ---- logdet(HK)  HODLR/src/baseoverloads.jl:194-209 ---- */
 int kmh_hodlr_logdet(kmh_hodlr* h, double* out);
 

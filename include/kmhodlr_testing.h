@@ -42,6 +42,8 @@ int kmh_test_profile_spans(kmh_ctx* ctx, int32_t cap, int32_t* tags, double* amo
 int kmh_test_fail_scratch(kmh_ctx* ctx, int64_t after);
 /* bytes of the context's scratch pool handed out and not returned          */
 int kmh_test_scratch_live(kmh_ctx* ctx, uint64_t* bytes);
+/* number of blocks the context's scratch pool has handed out so far        */
+int kmh_test_pool_allocs(kmh_ctx* ctx, uint64_t* count);
 /* replace the descriptor arena by one of `bytes` (after draining the
  * stream), so that the next large uploads make it grow                     */
 int kmh_test_arena_shrink(kmh_ctx* ctx, uint64_t bytes);

@@ -130,6 +130,18 @@ struct Ctx {
     // Test hook: scratch() fails (KMH_E_OOM, no device call) once this many
     // more blocks have been handed out; negative = never.
     int64_t fail_scratch_after = -1;
+    // Fixed scratch region (the allocation-free entry points): while set,
+    // scratch() hands out blocks of [fixed_base, fixed_base + fixed_size) as
+    // a stack (a block is reused once it and every block above it are
+    // freed), never touches the pool, and fails when the region is full;
+    // upload() fails instead of growing the arena.
+    char* fixed_base = nullptr;
+    size_t fixed_size = 0;
+    struct FixedBlock {
+        size_t off, bytes;
+        bool freed;
+    };
+    std::vector<FixedBlock> fixed_stack;
     size_t live_bytes() const {
         size_t b = 0;
         for (const auto& kv : live) b += kv.second.bytes;

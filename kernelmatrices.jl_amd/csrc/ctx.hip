@@ -55,6 +55,8 @@ void* Ctx::upload(const void* host, size_t bytes) {
     const size_t need = (bytes + 255) & ~size_t(255);
     const size_t seg = arena_size / ARENA_SEGS;
     if (need > seg) {
+        KMH_CHECK(!fixed_base, KMH_E_BAD_ARG,
+                  "descriptor upload of " + std::to_string(bytes) + " bytes needs a larger arena (allocation-free call)");
         // grow (rare): drain the stream, reallocate both sides.  The old
         // arena stays allocated until teardown: descriptors uploaded into it
         // by the caller for a launch not queued yet are still read from it.
@@ -168,6 +170,13 @@ void* Ctx::scratch(size_t bytes) {
     if (fail_scratch_after >= 0 && fail_scratch_after-- == 0)
         throw Error(KMH_E_OOM, "scratch allocation of " + std::to_string(bytes) + " bytes failed (injected)");
     bytes = std::max<size_t>(256, (bytes + 255) & ~size_t(255));
+    if (fixed_base) {
+        const size_t off = fixed_stack.empty() ? 0 : fixed_stack.back().off + fixed_stack.back().bytes;
+        KMH_CHECK(off + bytes <= fixed_size, KMH_E_BAD_ARG,
+                  "the fixed scratch of " + std::to_string(fixed_size) + " bytes is too small for this call");
+        fixed_stack.push_back({off, bytes, false});
+        return fixed_base + off;
+    }
     auto it = free_blocks.begin();
     for (; it != free_blocks.end(); ++it)
         if (it->second.first >= bytes) break;
@@ -196,6 +205,13 @@ void* Ctx::scratch(size_t bytes) {
 
 void Ctx::scratch_free(void* vp) {
     if (!vp) return;
+    char* cp = static_cast<char*>(vp);
+    if (fixed_base && cp >= fixed_base && cp < fixed_base + fixed_size) {
+        for (auto& b : fixed_stack)
+            if (fixed_base + b.off == cp) b.freed = true;
+        while (!fixed_stack.empty() && fixed_stack.back().freed) fixed_stack.pop_back();
+        return;
+    }
     auto lt = live.find(vp);
     KMH_CHECK(lt != live.end(), KMH_E_BAD_ARG, "scratch_free of an unknown block");
     char* p = static_cast<char*>(vp);

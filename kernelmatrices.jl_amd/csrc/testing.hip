@@ -150,6 +150,13 @@ int kmh_test_scratch_live(kmh_ctx* ctx, uint64_t* bytes) {
     })
 }
 
+int kmh_test_pool_allocs(kmh_ctx* ctx, uint64_t* count) {
+    TRY_({
+        KMH_CHECK(ctx && count, KMH_E_BAD_ARG, "null argument");
+        *count = ctx->c.alloc_seq;
+    })
+}
+
 int kmh_test_arena_shrink(kmh_ctx* ctx, uint64_t bytes) {
     TRY_({
         KMH_CHECK(ctx, KMH_E_BAD_ARG, "null context");

@@ -93,6 +93,7 @@ SIGNATURES = {
     "kmh_hodlr_factor_apply_dev": [_P, C.c_int32, _P, _P, _I64],
     "kmh_hodlr_solve": [_P, _P, _P, _I64],
     "kmh_hodlr_solve_dev": [_P, _P, _P, _I64],
+    "kmh_hodlr_solve_dev_noalloc": [_P, _P, _P, _I64],
     "kmh_hodlr_logdet": [_P, _D],
     "kmh_stoch_gradient": [_P, C.POINTER(kmh_kernel), C.c_int32, _P, _P, _I64, _P],
     "kmh_stoch_gradient_dev": [_P, C.POINTER(kmh_kernel), C.c_int32, _P, _P, _I64, _P],
@@ -135,6 +136,7 @@ TEST_SIGNATURES = {
                                C.POINTER(C.c_int32)],
     "kmh_test_fail_scratch": [_P, C.c_int64],
     "kmh_test_scratch_live": [_P, C.POINTER(C.c_uint64)],
+    "kmh_test_pool_allocs": [_P, C.POINTER(C.c_uint64)],
     "kmh_test_arena_shrink": [_P, C.c_uint64],
 }
 
