@@ -68,6 +68,13 @@ typedef struct kmh_kernel {
 #define KMH_FIXED_LEVEL 0   /* structstypes.jl:30-32 */
 #define KMH_LOG_LEVEL   1   /* structstypes.jl:35-37 */
 #define KMH_FP64       0
+#define KMH_MIXED      1   /* FP32 off-diagonal panels, cascade panels and Q
+                              factors (FP64 MFMA accumulation); FP64 leaves,
+                              capacitance and logdet; solves refined to FP64
+                              against the FP64 HODLR matvec                  */
+#define KMH_MIXED_Q    2   /* as KMH_MIXED with the Nystrom cascade panels in
+                              FP64: only the panel QR and its Q factors are
+                              FP32 (ACA panels are FP32 in both modes)       */
 
 typedef struct kmh_build_opts {
     int32_t method;      /* KMH_ACA or KMH_NYSTROM (kwarg nystrom)            */
@@ -75,7 +82,7 @@ typedef struct kmh_build_opts {
     int32_t level;       /* lv of FixedLevel(lv) / LogLevel(lv)               */
     int32_t maxrank;     /* maxrank (0 = no limit for ACA; Nystrom rank)      */
     double  tol;         /* ep: ACA relative tolerance                        */
-    int32_t precision;   /* KMH_FP64                                          */
+    int32_t precision;   /* KMH_FP64, KMH_MIXED or KMH_MIXED_Q                */
     int32_t reserved[3];
 } kmh_build_opts;
 
@@ -193,6 +200,12 @@ int kmh_hodlr_solve_dev(kmh_hodlr* h, const double* b, double* x, int64_t nrhs);
  * ranks) fails with KMH_E_BAD_ARG before any work is queued.  Calls that
  * overlap on one GPU (different contexts or threads) race on that array.  */
 int kmh_hodlr_solve_dev_noalloc(kmh_hodlr* h, const double* b, double* x, int64_t nrhs);
+
+/* Iterative refinement of the last solve of a KMH_MIXED handle (no
+ * reference counterpart): corrections made and the final largest relative
+ * residual |b - K x| / |b| over the right-hand sides, K the FP64 HODLR
+ * operator (leaves as L L^T, off-diagonal U V^T); 0 and 0 for FP64 handles */
+int kmh_hodlr_refinement(kmh_hodlr* h, int32_t* iters, double* relres);
 
 /* ---- logdet(HK)  HODLR/src/baseoverloads.jl:194-209 ---- */
 int kmh_hodlr_logdet(kmh_hodlr* h, double* out);

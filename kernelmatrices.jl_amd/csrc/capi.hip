@@ -316,6 +316,15 @@ int kmh_hodlr_solve_dev(kmh_hodlr* h, const double* b, double* x, int64_t nrhs) 
     })
 }
 
+int kmh_hodlr_refinement(kmh_hodlr* h, int32_t* iters, double* relres) {
+    KMH_TRY({
+        kmh::Hodlr& H = H_of(h);
+        KMH_CHECK(iters && relres, KMH_E_BAD_ARG, "null output");
+        *iters = H.ir_iters;
+        *relres = H.ir_relres;
+    })
+}
+
 int kmh_hodlr_logdet(kmh_hodlr* h, double* out) {
     KMH_TRY({
         kmh::Hodlr& H = H_of(h);

@@ -34,6 +34,8 @@ struct Hodlr {
     DevBuf leaves;                               // leaf i at leaf_mat_off[i], ld = size
     std::vector<int64_t> leaf_mat_off;
     DevBuf P;                                    // n x Rtot, ld n: ACA U/V, then Q per level
+                                                 // (FP32 elements in the mixed form)
+    DevBuf UV;                                   // mixed-form ACA: U/V kept beside the Q slabs (FP32)
 
     // Nystrom state (kept for the derivative HODLR)
     int nys_r = 0;
@@ -50,6 +52,9 @@ struct Hodlr {
     DevBuf X;                                    // chol(C) - I per block
     DevBuf G;                                    // I - chol(C)^{-1} per block
     double logdet = 0.0;
+    // iterative refinement of the last solve (mixed form)
+    int ir_iters = 0;
+    double ir_relres = 0.0;
 
     // transient state of a factorization in progress (scratch blocks of
     // ctx's pool, released by factor_end_batch, by a new factorization or
@@ -77,18 +82,33 @@ struct Hodlr {
     }
 
     bool nystrom() const { return opts.method == KMH_NYSTROM; }
-    // U (rows a0..b1) and V (rows c0..d1) of block k at level l before
-    // factorization, leading dimension n.  Nystrom blocks all share the two
-    // panels K(x,P) and K(x,P)K(P,P)^{-1} (src/factorizations.jl:86-94);
-    // ACA blocks live in P.
+    // mixed forms: the Q slabs (and ACA panels) stored in FP32; the Nystrom
+    // cascade panels too in KMH_MIXED
+    bool p32() const { return opts.precision != KMH_FP64; }
+    bool ab32() const { return opts.precision == KMH_MIXED; }
+    // element i of P (FP64 or FP32 storage) as a descriptor pointer
+    double* Pat(size_t i) const { return p32() ? reinterpret_cast<double*>(P.as<float>() + i) : P.as<double>() + i; }
+    // U (rows a0..b1) and V (rows c0..d1) of block k at level l, leading
+    // dimension n.  Nystrom blocks all share the two panels K(x,P) and
+    // K(x,P)K(P,P)^{-1} (src/factorizations.jl:86-94; FP64 in both forms);
+    // ACA blocks live in P before factorization (in UV after it, mixed form
+    // only), FP32 when uv32().
+    bool uv32() const { return p32() && !nystrom(); }
+    const double* uvbase(size_t i) const {
+        const DevBuf& s = (UV.p && (factorized || factor_started)) ? UV : P;
+        return p32() ? reinterpret_cast<const double*>(s.as<float>() + i) : s.as<double>() + i;
+    }
     const double* Uptr(int l, int k) const {
         const Block& b = blocks[l][k];
-        return nystrom() ? kxp.as<double>() + b.a0 : P.as<double>() + (size_t)coff[l] * n + b.a0;
+        return nystrom() ? kxp.as<double>() + b.a0 : uvbase((size_t)coff[l] * n + b.a0);
     }
     const double* Vptr(int l, int k) const {
         const Block& b = blocks[l][k];
-        return nystrom() ? ynys.as<double>() + b.c0 : P.as<double>() + (size_t)coff[l] * n + b.c0;
+        return nystrom() ? ynys.as<double>() + b.c0 : uvbase((size_t)coff[l] * n + b.c0);
     }
+    // U/V available for the operator's matvec (before factorization, or
+    // kept for the refinement of a mixed-form solve)
+    bool has_uv() const { return !factorized || nystrom() || UV.p; }
     int64_t leaf_size(int i) const { return leaf_off[i + 1] - leaf_off[i]; }
     int nleaves() const { return (int)leaf_off.size() - 1; }
     int nblocks() const {
@@ -98,8 +118,8 @@ struct Hodlr {
     }
     double* leaf_ptr(int i) const { return leaves.as<double>() + leaf_mat_off[i]; }
     size_t device_bytes() const {
-        return pts.bytes + info.bytes + leaves.bytes + P.bytes + lm.bytes + kpp.bytes + kxp.bytes + ynys.bytes +
-               X.bytes + G.bytes;
+        return pts.bytes + info.bytes + leaves.bytes + P.bytes + UV.bytes + lm.bytes + kpp.bytes + kxp.bytes +
+               ynys.bytes + X.bytes + G.bytes;
     }
 };
 

@@ -116,7 +116,8 @@ static void validate(const BuildReq& r) {
     const auto& o = r.opts;
     KMH_CHECK(o.method == KMH_ACA || o.method == KMH_NYSTROM, KMH_E_BAD_ARG, "unknown compression method");
     KMH_CHECK(o.level_kind == KMH_FIXED_LEVEL || o.level_kind == KMH_LOG_LEVEL, KMH_E_BAD_ARG, "unknown level kind");
-    KMH_CHECK(o.precision == KMH_FP64, KMH_E_BAD_ARG, "unsupported precision mode");
+    KMH_CHECK(o.precision == KMH_FP64 || o.precision == KMH_MIXED || o.precision == KMH_MIXED_Q, KMH_E_BAD_ARG,
+              "unsupported precision mode");
     KMH_CHECK(o.maxrank >= 0, KMH_E_BAD_ARG, "maxrank must be >= 0");
 }
 
@@ -249,17 +250,20 @@ static void build_aca_batch(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<S
         H.coff.assign(H.L + 1, 0);
         for (int l = 0; l < H.L; ++l) H.coff[l + 1] = H.coff[l] + H.rpad[l];
         H.Rtot = H.coff[H.L];
-        H.P.alloc(&c, (size_t)H.n * H.Rtot * sizeof(double));
+        H.P.alloc(&c, (size_t)H.n * H.Rtot * (H.p32() ? sizeof(float) : sizeof(double)));
         KMH_HIP(hipMemsetAsync(H.P.p, 0, H.P.bytes, c.q()));
-        double* P = H.P.as<double>();
         for (int l = 0; l < H.L; ++l) {
-            double* Pl = P + (size_t)H.coff[l] * H.n;
+            const size_t Pl = (size_t)H.coff[l] * H.n;
             const double* T = lvs[q0 + l].tmp.as<double>();
             for (size_t k = 0; k < H.blocks[l].size(); ++k) {
                 const auto& b = H.blocks[l][k];
                 const int r = H.ranks[l][k];
-                cp.push_back({T + b.a0, (int)H.n, Pl + b.a0, (int)H.n, (int)(b.b1 - b.a0), r, 1.0});
-                cp.push_back({T + b.c0, (int)H.n, Pl + b.c0, (int)H.n, (int)(b.d1 - b.c0), r, 1.0});
+                // the mixed form keeps the ACA panels in FP32
+                Copy2D u{T + b.a0, (int)H.n, H.Pat(Pl + b.a0), (int)H.n, (int)(b.b1 - b.a0), r, 1.0};
+                Copy2D v{T + b.c0, (int)H.n, H.Pat(Pl + b.c0), (int)H.n, (int)(b.d1 - b.c0), r, 1.0};
+                u.ed = v.ed = H.p32();
+                cp.push_back(u);
+                cp.push_back(v);
             }
         }
     }
@@ -406,8 +410,12 @@ struct Rhs {
     double* T;
     int64_t ldT;
     int ncols;
-    int lv = 0;   // level whose factors apply (woodbury_batch)
+    int lv = 0;     // level whose factors apply (woodbury_batch)
+    bool e32 = false;   // T stored in FP32 (the mixed form's panels)
 };
+
+// T advanced by e elements of its storage type
+static double* tadv(double* T, size_t e, bool f32) { return f32 ? f64p(f32p(T) + e) : T + e; }
 
 static void leaf_trsm_batch(Ctx& c, bool trans, const std::vector<Rhs>& rs) {
     std::vector<TrsmProb> pr;
@@ -415,7 +423,9 @@ static void leaf_trsm_batch(Ctx& c, bool trans, const std::vector<Rhs>& rs) {
         Hodlr& H = *x.H;
         for (int i = 0; i < H.nleaves(); ++i) {
             const int m = (int)H.leaf_size(i);
-            pr.push_back({H.leaf_ptr(i), m, m, x.T + H.leaf_off[i], x.ncols, (int)x.ldT});
+            TrsmProb t{H.leaf_ptr(i), m, m, tadv(x.T, H.leaf_off[i], x.e32), x.ncols, (int)x.ldT};
+            t.eb = x.e32;
+            pr.push_back(t);
         }
     }
     trsm_left(c, trans, pr);
@@ -455,23 +465,32 @@ static void woodbury_batch(Ctx& c, int mode, const std::vector<Rhs>& rs) {
         Hodlr& H = *x.H;
         const int lv = x.lv;
         const int64_t n = H.n;
-        const double* P = H.P.as<double>() + (size_t)H.coff[lv] * n;
+        const size_t P = (size_t)H.coff[lv] * n;
         const double* Mb = inv ? H.G.as<double>() : H.X.as<double>();
         const int nc = x.ncols;
+        const uint8_t q32 = H.p32(), t32 = x.e32;
         for (size_t k = 0; k < H.blocks[lv].size(); ++k) {
             const auto& b = H.blocks[lv][k];
             const int r = H.ranks[lv][k];
             const int h1 = (int)(b.d1 - b.c0), h2 = (int)(b.b1 - b.a0);
-            const double* Q1 = P + b.c0;
-            const double* Q2 = P + b.a0;
+            const double* Q1 = H.Pat(P + b.c0);
+            const double* Q2 = H.Pat(P + b.a0);
             double* s = S + off;
             double* s2 = S2 + off;
             off += (size_t)2 * r * nc;
-            tn.push_back({r, nc, h1, Q1, (int)n, x.T + b.c0, (int)x.ldT, s, 2 * r, 1.0, 0.0, 0});
-            tn.push_back({r, nc, h2, Q2, (int)n, x.T + b.a0, (int)x.ldT, s + r, 2 * r, 1.0, 0.0, 0});
+            GemmProb t1{r, nc, h1, Q1, (int)n, tadv(x.T, b.c0, t32), (int)x.ldT, s, 2 * r, 1.0, 0.0, 0};
+            GemmProb t2{r, nc, h2, Q2, (int)n, tadv(x.T, b.a0, t32), (int)x.ldT, s + r, 2 * r, 1.0, 0.0, 0};
+            t1.ea = t2.ea = q32;
+            t1.eb = t2.eb = t32;
+            tn.push_back(t1);
+            tn.push_back(t2);
             op.push_back({2 * r, nc, 2 * r, Mb + H.coff_blk[lv][k], 2 * r, s, 2 * r, s2, 2 * r, 1.0, 0.0, 0});
-            up.push_back({h1, nc, r, Q1, (int)n, s2, 2 * r, x.T + b.c0, (int)x.ldT, sg, 1.0, 0});
-            up.push_back({h2, nc, r, Q2, (int)n, s2 + r, 2 * r, x.T + b.a0, (int)x.ldT, sg, 1.0, 0});
+            GemmProb u1{h1, nc, r, Q1, (int)n, s2, 2 * r, tadv(x.T, b.c0, t32), (int)x.ldT, sg, 1.0, 0};
+            GemmProb u2{h2, nc, r, Q2, (int)n, s2 + r, 2 * r, tadv(x.T, b.a0, t32), (int)x.ldT, sg, 1.0, 0};
+            u1.ea = u2.ea = q32;
+            u1.ec = u2.ec = t32;
+            up.push_back(u1);
+            up.push_back(u2);
         }
     }
     gemm(c, true, false, tn);
@@ -504,8 +523,9 @@ static void woodbury_shared_basis(Ctx& c, const std::vector<Hodlr*>& hs, const s
         Hodlr& H = *hs[i];
         const int lv = lvs[i];
         const int64_t n = H.n;
-        const double* P = H.P.as<double>() + (size_t)H.coff[lv] * n;
+        const size_t P = (size_t)H.coff[lv] * n;
         const double* G = H.G.as<double>();
+        const bool e = H.ab32(), eq = H.p32();
         double* AB = H.AB;
         const double* rb = rbs[i];
         for (size_t k = 0; k < H.blocks[lv].size(); ++k) {
@@ -513,8 +533,8 @@ static void woodbury_shared_basis(Ctx& c, const std::vector<Hodlr*>& hs, const s
             const int r = H.ranks[lv][k];
             KMH_CHECK(r == H.nys_r, KMH_E_BAD_ARG, "shared-basis panels need the full Nystrom rank");
             const int h1 = (int)(b.d1 - b.c0), h2 = (int)(b.b1 - b.a0);
-            const double* Q1 = P + b.c0;
-            const double* Q2 = P + b.a0;
+            const double* Q1 = H.Pat(P + b.c0);
+            const double* Q2 = H.Pat(P + b.a0);
             const double* R1 = rb;
             const double* R2 = rb + (size_t)r * r;
             rb += (size_t)3 * r * r;
@@ -522,13 +542,22 @@ static void woodbury_shared_basis(Ctx& c, const std::vector<Hodlr*>& hs, const s
             double* s2 = S2 + off;
             off += (size_t)4 * r * r;
             const int l2 = 2 * r;
-            tn.push_back({r, r, h1, Q1, (int)n, AB + b.c0, (int)n, s, l2, 1.0, 0.0, 0});
-            tn.push_back({r, r, h2, Q2, (int)n, AB + (size_t)n * r + b.a0, (int)n, s + r + (size_t)r * l2, l2, 1.0, 0.0, 0});
+            GemmProb t1{r, r, h1, Q1, (int)n, tadv(AB, b.c0, e), (int)n, s, l2, 1.0, 0.0, 0};
+            GemmProb t2{r, r, h2, Q2, (int)n, tadv(AB, (size_t)n * r + b.a0, e), (int)n, s + r + (size_t)r * l2, l2,
+                        1.0, 0.0, 0};
+            t1.ea = t2.ea = eq;
+            t1.eb = t2.eb = e;
+            tn.push_back(t1);
+            tn.push_back(t2);
             rc.push_back({R1, r, s + (size_t)r * l2, l2, r, r, 1.0});
             rc.push_back({R2, r, s + r, l2, r, r, 1.0});
             op.push_back({l2, l2, l2, G + H.coff_blk[lv][k], l2, s, l2, s2, l2, 1.0, 0.0, 0});
-            up.push_back({h1, l2, r, Q1, (int)n, s2, l2, AB + b.c0, (int)n, -1.0, 1.0, 0});
-            up.push_back({h2, l2, r, Q2, (int)n, s2 + r, l2, AB + b.a0, (int)n, -1.0, 1.0, 0});
+            GemmProb u1{h1, l2, r, Q1, (int)n, s2, l2, tadv(AB, b.c0, e), (int)n, -1.0, 1.0, 0};
+            GemmProb u2{h2, l2, r, Q2, (int)n, s2 + r, l2, tadv(AB, b.a0, e), (int)n, -1.0, 1.0, 0};
+            u1.ea = u2.ea = eq;
+            u1.ec = u2.ec = e;
+            up.push_back(u1);
+            up.push_back(u2);
         }
     }
     gemm(c, true, false, tn);
@@ -547,6 +576,7 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
         for (int r : act[ai]->ranks[lvs[ai]]) rsz += (size_t)3 * r * r;
     double* Rb = static_cast<double*>(c.scratch(rsz * sizeof(double)));
     std::vector<QrPanel> qp;
+    std::vector<Copy2D> qcopy;
     std::vector<GemmProb> bm;
     std::vector<DiagOp> ident_x, ident_g, plus1, minus1, eye_s, eye_g;
     std::vector<GemmProb> schur, g21;
@@ -560,7 +590,8 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
         Hodlr& H = *act[ai];
         const int lv = lvs[ai];
         const int64_t n = H.n;
-        double* P = H.P.as<double>() + (size_t)H.coff[lv] * n;
+        const size_t P = (size_t)H.coff[lv] * n;
+        const bool e = H.p32(), eab = H.ab32();
         double* X = H.X.as<double>();
         double* G = H.G.as<double>();
         int pb = H.nleaves();
@@ -573,13 +604,21 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
             double* Bm = R2 + (size_t)r * r;
             off += (size_t)3 * r * r;
             // lrsymfact(V, U): V (first child) first (symfact.jl:31, HODLR/src/utils.jl:29-35)
-            QrPanel qv{P + b.c0, (int)(b.d1 - b.c0), r, (int)n, R1, r};
-            QrPanel qu{P + b.a0, (int)(b.b1 - b.a0), r, (int)n, R2, r};
-            if (H.AB) {   // V: B half of the first child's rows; U: A half of the second child's
-                qv.src = H.AB + (size_t)n * r + b.c0;
+            QrPanel qv{H.Pat(P + b.c0), (int)(b.d1 - b.c0), r, (int)n, R1, r};
+            QrPanel qu{H.Pat(P + b.a0), (int)(b.b1 - b.a0), r, (int)n, R2, r};
+            qv.e32 = qu.e32 = e;
+            if (H.AB && e == eab) {   // V: B half of the first child's rows; U: A half of the second child's
+                qv.src = tadv(H.AB, (size_t)n * r + b.c0, e);
                 qv.lds = (int)n;
-                qu.src = H.AB + b.a0;
+                qu.src = tadv(H.AB, b.a0, e);
                 qu.lds = (int)n;
+            } else if (H.AB) {
+                // KMH_MIXED_Q: the FP64 cascade panels rounded into the FP32 slab, QR in place
+                Copy2D cv{H.AB + (size_t)n * r + b.c0, (int)n, qv.A, (int)n, qv.m, r, 1.0};
+                Copy2D cu{H.AB + b.a0, (int)n, qu.A, (int)n, qu.m, r, 1.0};
+                cv.ed = cu.ed = 1;
+                qcopy.push_back(cv);
+                qcopy.push_back(cu);
             }
             qp.push_back(qv);
             qp.push_back(qu);
@@ -612,6 +651,7 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
             minus1.push_back({X22, r, l2, -1.0});
         }
     }
+    copy2d(c, qcopy);
     qr_panels(c, qp);
     gemm(c, false, true, bm);                    // Bm = R1 R2^T
     set_identity(c, ident_x);                    // X = 0
@@ -645,7 +685,7 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
             nys_lv.push_back(lv);
             nys_r.push_back(r0);
         } else {
-            wr.push_back({h, H.P.as<double>() + (size_t)H.coff[lv + 1] * H.n, H.n, (int)(H.Rtot - H.coff[lv + 1]), lv});
+            wr.push_back({h, H.Pat((size_t)H.coff[lv + 1] * H.n), H.n, (int)(H.Rtot - H.coff[lv + 1]), lv, H.p32()});
         }
     }
     woodbury_batch(c, W_INV, wr);
@@ -692,14 +732,25 @@ void factor_begin_batch(Ctx& c, const std::vector<Hodlr*>& hs) {
             // ranges of W_l^{-1}...W_1^{-1} L^{-1} [A B]: the cascade updates
             // these 2r columns instead of every higher level's panel (same W).
             const int r = H.nys_r;
-            H.P.alloc(&c, (size_t)n * H.Rtot * sizeof(double));
-            H.AB = static_cast<double*>(c.scratch((size_t)n * 2 * r * sizeof(double)));
-            abinit.push_back({H.kxp.as<double>(), (int)n, H.AB, (int)n, (int)n, r, 1.0});
-            abinit.push_back({H.ynys.as<double>(), (int)n, H.AB + (size_t)n * r, (int)n, (int)n, r, 1.0});
-            lt.push_back({h, H.AB, n, 2 * r});
+            const bool e = H.ab32();
+            H.P.alloc(&c, (size_t)n * H.Rtot * (H.p32() ? sizeof(float) : sizeof(double)));
+            H.AB = static_cast<double*>(c.scratch((size_t)n * 2 * r * (e ? sizeof(float) : sizeof(double))));
+            Copy2D a{H.kxp.as<double>(), (int)n, H.AB, (int)n, (int)n, r, 1.0};
+            Copy2D b{H.ynys.as<double>(), (int)n, tadv(H.AB, (size_t)n * r, e), (int)n, (int)n, r, 1.0};
+            a.ed = b.ed = e;
+            abinit.push_back(a);
+            abinit.push_back(b);
+            lt.push_back({h, H.AB, n, 2 * r, 0, e});
         } else {
+            if (H.p32()) {
+                // the mixed form keeps U/V beside the Q slabs for the refinement's matvec
+                H.UV.alloc(&c, H.P.bytes);
+                Copy2D u{H.P.as<double>(), (int)n, H.UV.as<double>(), (int)n, (int)n, (int)H.Rtot, 1.0};
+                u.es = u.ed = 1;
+                abinit.push_back(u);
+            }
             // leaf inverse on every level's U and V (symfact.jl:21)
-            lt.push_back({h, H.P.as<double>(), n, (int)H.Rtot});
+            lt.push_back({h, H.P.as<double>(), n, (int)H.Rtot, 0, H.p32()});
         }
     }
     // leaf Cholesky (symfact.jl:17)
@@ -777,8 +828,9 @@ void hodlr_factorize(Hodlr& H) {
 // solve and factor applies
 // ---------------------------------------------------------------------------
 
-void hodlr_solve_batch(Ctx& c, const std::vector<Hodlr*>& hs, const std::vector<double*>& T,
-                       const std::vector<int>& nrhs) {
+// W^-T W^-1 T in place: the factorization's own solve
+static void direct_solve_batch(Ctx& c, const std::vector<Hodlr*>& hs, const std::vector<double*>& T,
+                               const std::vector<int>& nrhs) {
     std::vector<Rhs> rs;
     int maxL = 0;
     for (size_t i = 0; i < hs.size(); ++i) {
@@ -801,6 +853,102 @@ void hodlr_solve_batch(Ctx& c, const std::vector<Hodlr*>& hs, const std::vector<
         woodbury_batch(c, W_INVT, a);
     }
     leaf_trsm_batch(c, true, rs);
+}
+
+static void operator_matvec(Hodlr& H, const double* x, double* y, int nrhs);
+
+// Iterative refinement of the mixed form's solves against the FP64 HODLR
+// operator: x <- x + (W W^T)^{-1} (b - K x) until the largest column's
+// relative residual is <= KMH_IR_TOL, stops improving (less than halved) or
+// KMH_IR_MAX corrections were made.  The factor W of the mixed form is an
+// FP32-accurate factorization of K, so each correction gains several digits.
+constexpr double KMH_IR_TOL = 1e-12;
+constexpr int KMH_IR_MAX = 8;
+
+static void refine_solve_batch(Ctx& c, const std::vector<Hodlr*>& hs, const std::vector<double*>& T,
+                               const std::vector<int>& nrhs) {
+    const size_t N = hs.size();
+    std::vector<double*> B0(N), R(N);
+    size_t ncol = 0;
+    std::vector<size_t> col0(N);
+    for (size_t i = 0; i < N; ++i) {
+        KMH_CHECK(hs[i]->has_uv(), KMH_E_STATE, "refinement needs the HODLR operator's panels");
+        const size_t bytes = (size_t)hs[i]->n * nrhs[i] * sizeof(double);
+        B0[i] = static_cast<double*>(c.scratch(bytes));
+        R[i] = static_cast<double*>(c.scratch(bytes));
+        col0[i] = ncol;
+        ncol += nrhs[i];
+    }
+    double* dots = static_cast<double*>(c.scratch(2 * ncol * sizeof(double)));
+    std::vector<Copy2D> keep;
+    for (size_t i = 0; i < N; ++i) keep.push_back({T[i], (int)hs[i]->n, B0[i], (int)hs[i]->n, (int)hs[i]->n, nrhs[i], 1.0});
+    copy2d(c, keep);
+    for (size_t i = 0; i < N; ++i)
+        column_dots(c, B0[i], (int)hs[i]->n, B0[i], (int)hs[i]->n, (int)hs[i]->n, nrhs[i], dots + ncol + col0[i]);
+    direct_solve_batch(c, hs, T, nrhs);
+    std::vector<char> active(N, 1);
+    std::vector<double> prev(N, INFINITY), h(2 * ncol);
+    for (size_t i = 0; i < N; ++i) hs[i]->ir_iters = 0;
+    for (int it = 0;; ++it) {
+        // R = b - K x for the active systems
+        std::vector<Axpby2D> res;
+        for (size_t i = 0; i < N; ++i) {
+            if (!active[i]) continue;
+            operator_matvec(*hs[i], T[i], R[i], nrhs[i]);
+            res.push_back({B0[i], (int)hs[i]->n, R[i], (int)hs[i]->n, (int)hs[i]->n, nrhs[i], -1.0, 1.0});
+        }
+        axpby2d(c, res);
+        for (size_t i = 0; i < N; ++i)
+            if (active[i]) column_dots(c, R[i], (int)hs[i]->n, R[i], (int)hs[i]->n, (int)hs[i]->n, nrhs[i], dots + col0[i]);
+        KMH_HIP(hipMemcpyAsync(h.data(), dots, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.q()));
+        c.sync();
+        std::vector<Hodlr*> ah;
+        std::vector<double*> aR;
+        std::vector<int> an;
+        std::vector<Axpby2D> upd;
+        for (size_t i = 0; i < N; ++i) {
+            if (!active[i]) continue;
+            double rel = 0.0;
+            for (int j = 0; j < nrhs[i]; ++j) {
+                const double bb = h[ncol + col0[i] + j];
+                rel = std::max(rel, bb > 0 ? std::sqrt(h[col0[i] + j] / bb) : 0.0);
+            }
+            hs[i]->ir_relres = rel;
+            if (rel <= KMH_IR_TOL || it >= KMH_IR_MAX || rel > 0.5 * prev[i]) {
+                active[i] = 0;
+                continue;
+            }
+            prev[i] = rel;
+            hs[i]->ir_iters = it + 1;
+            ah.push_back(hs[i]);
+            aR.push_back(R[i]);
+            an.push_back(nrhs[i]);
+            upd.push_back({R[i], (int)hs[i]->n, T[i], (int)hs[i]->n, (int)hs[i]->n, nrhs[i], 1.0, 1.0});
+        }
+        if (ah.empty()) break;
+        direct_solve_batch(c, ah, aR, an);
+        axpby2d(c, upd);
+    }
+    c.scratch_free(dots);
+    for (size_t i = N; i-- > 0;) {
+        c.scratch_free(R[i]);
+        c.scratch_free(B0[i]);
+    }
+}
+
+// K^{-1} T: the factorization's solve, refined to FP64 for the mixed form
+void hodlr_solve_batch(Ctx& c, const std::vector<Hodlr*>& hs, const std::vector<double*>& T,
+                       const std::vector<int>& nrhs) {
+    std::vector<Hodlr*> h64, h32;
+    std::vector<double*> t64, t32;
+    std::vector<int> n64, n32;
+    for (size_t i = 0; i < hs.size(); ++i) {
+        KMH_CHECK(hs[i]->factorized, KMH_E_NOT_FACTORIZED, "No solves without factorization.");
+        if (hs[i]->p32()) { h32.push_back(hs[i]); t32.push_back(T[i]); n32.push_back(nrhs[i]); }
+        else { h64.push_back(hs[i]); t64.push_back(T[i]); n64.push_back(nrhs[i]); }
+    }
+    if (!h64.empty()) direct_solve_batch(c, h64, t64, n64);
+    if (!h32.empty()) refine_solve_batch(c, h32, t32, n32);
 }
 
 void hodlr_factor_apply_batch(Ctx& c, int op, const std::vector<Hodlr*>& hs, const std::vector<double*>& T,
@@ -860,19 +1008,35 @@ void hodlr_solve(Hodlr& H, double* T, int nrhs) { hodlr_solve_batch(*H.ctx, {&H}
 void hodlr_matvec(Hodlr& H, const double* x, double* y, int nrhs) {
     Ctx& c = *H.ctx;
     const int64_t n = H.n;
-    if (H.factorized) {
+    if (H.factorized && !H.p32()) {
+        // W (W^T x) (HODLR/src/baseoverloads.jl:170-177)
         copy2d(c, {{x, (int)n, y, (int)n, (int)n, nrhs, 1.0}});
         hodlr_factor_apply(H, KMH_WT, y, nrhs);
         hodlr_factor_apply(H, KMH_W, y, nrhs);
         return;
     }
-    std::vector<GemmProb> g;
-    for (int i = 0; i < H.nleaves(); ++i) {
-        const int m = (int)H.leaf_size(i);
-        const int64_t s = H.leaf_off[i];
-        g.push_back({m, nrhs, m, H.leaf_ptr(i), m, x + s, (int)n, y + s, (int)n, 1.0, 0.0, 0});
+    operator_matvec(H, x, y, nrhs);
+}
+
+// y = K x from the HODLR matrix's FP64 pieces: the leaves (their Cholesky
+// factors L L^T once factorized) and the off-diagonal U V^T (the Nystrom
+// panels, or the ACA panels, FP32 in the mixed form), FP64 arithmetic
+static void operator_matvec(Hodlr& H, const double* x, double* y, int nrhs) {
+    Ctx& c = *H.ctx;
+    const int64_t n = H.n;
+    if (H.factorized || H.factor_started) {
+        copy2d(c, {{x, (int)n, y, (int)n, (int)n, nrhs, 1.0}});
+        leaf_mul(H, true, y, nrhs);
+        leaf_mul(H, false, y, nrhs);
+    } else {
+        std::vector<GemmProb> g;
+        for (int i = 0; i < H.nleaves(); ++i) {
+            const int m = (int)H.leaf_size(i);
+            const int64_t s = H.leaf_off[i];
+            g.push_back({m, nrhs, m, H.leaf_ptr(i), m, x + s, (int)n, y + s, (int)n, 1.0, 0.0, 0});
+        }
+        gemm(c, false, false, g);
     }
-    gemm(c, false, false, g);
     if (H.L == 0) return;
     size_t ssz = 0;
     for (int l = 0; l < H.L; ++l)
@@ -891,11 +1055,18 @@ void hodlr_matvec(Hodlr& H, const double* x, double* y, int nrhs) {
             double* s1 = S + off;
             double* s2 = s1 + (size_t)r * nrhs;
             off += (size_t)2 * r * nrhs;
-            tn.push_back({r, nrhs, h2, U, (int)n, x + b.a0, (int)n, s1, r, 1.0, 0.0, 0});
-            tn.push_back({r, nrhs, h1, V, (int)n, x + b.c0, (int)n, s2, r, 1.0, 0.0, 0});
+            const uint8_t e = H.uv32();
+            GemmProb t1{r, nrhs, h2, U, (int)n, x + b.a0, (int)n, s1, r, 1.0, 0.0, 0};
+            GemmProb t2{r, nrhs, h1, V, (int)n, x + b.c0, (int)n, s2, r, 1.0, 0.0, 0};
+            t1.ea = t2.ea = e;
+            tn.push_back(t1);
+            tn.push_back(t2);
             // HODLR/src/baseoverloads.jl:166-167
-            up[l].push_back({h1, nrhs, r, V, (int)n, s1, r, y + b.c0, (int)n, 1.0, 1.0, 0});
-            up[l].push_back({h2, nrhs, r, U, (int)n, s2, r, y + b.a0, (int)n, 1.0, 1.0, 0});
+            GemmProb u1{h1, nrhs, r, V, (int)n, s1, r, y + b.c0, (int)n, 1.0, 1.0, 0};
+            GemmProb u2{h2, nrhs, r, U, (int)n, s2, r, y + b.a0, (int)n, 1.0, 1.0, 0};
+            u1.ea = u2.ea = e;
+            up[l].push_back(u1);
+            up[l].push_back(u2);
         }
     }
     gemm(c, true, false, tn);
@@ -904,10 +1075,18 @@ void hodlr_matvec(Hodlr& H, const double* x, double* y, int nrhs) {
 }
 
 // copy an h x r column block (ld n on the device) to a dense host array
-static void fetch_cols(Ctx& c, const double* src, int64_t ld, int64_t h, int r, double* dst) {
+static void fetch_cols(Ctx& c, const double* src, int64_t ld, int64_t h, int r, double* dst, bool f32 = false) {
     if (h <= 0 || r <= 0) return;
-    KMH_HIP(hipMemcpy2DAsync(dst, h * sizeof(double), src, ld * sizeof(double), h * sizeof(double), r,
+    if (!f32) {
+        KMH_HIP(hipMemcpy2DAsync(dst, h * sizeof(double), src, ld * sizeof(double), h * sizeof(double), r,
+                                 hipMemcpyDeviceToHost, c.q()));
+        return;
+    }
+    std::vector<float> tmp((size_t)h * r);
+    KMH_HIP(hipMemcpy2DAsync(tmp.data(), h * sizeof(float), src, ld * sizeof(float), h * sizeof(float), r,
                              hipMemcpyDeviceToHost, c.q()));
+    c.sync();
+    for (size_t i = 0; i < tmp.size(); ++i) dst[i] = (double)tmp[i];
 }
 
 void hodlr_full_host(Hodlr& H, double* out) {
@@ -929,8 +1108,8 @@ void hodlr_full_host(Hodlr& H, double* out) {
             const int r = H.ranks[l][k];
             const int64_t h2 = b.b1 - b.a0, h1 = b.d1 - b.c0;
             std::vector<double> U((size_t)h2 * r), V((size_t)h1 * r);
-            fetch_cols(c, H.Uptr(l, (int)k), n, h2, r, U.data());
-            fetch_cols(c, H.Vptr(l, (int)k), n, h1, r, V.data());
+            fetch_cols(c, H.Uptr(l, (int)k), n, h2, r, U.data(), H.uv32());
+            fetch_cols(c, H.Vptr(l, (int)k), n, h1, r, V.data(), H.uv32());
             c.sync();
             for (int64_t j = 0; j < h1; ++j)
                 for (int64_t i = 0; i < h2; ++i) {
@@ -950,8 +1129,8 @@ void hodlr_block_uv_host(Hodlr& H, int level, int block, double* U, double* V) {
     Ctx& c = *H.ctx;
     const auto& b = H.blocks[level - 1][block];
     const int r = H.ranks[level - 1][block];
-    fetch_cols(c, H.Uptr(level - 1, block), H.n, b.b1 - b.a0, r, U);
-    fetch_cols(c, H.Vptr(level - 1, block), H.n, b.d1 - b.c0, r, V);
+    fetch_cols(c, H.Uptr(level - 1, block), H.n, b.b1 - b.a0, r, U, H.uv32());
+    fetch_cols(c, H.Vptr(level - 1, block), H.n, b.d1 - b.c0, r, V, H.uv32());
     c.sync();
 }
 

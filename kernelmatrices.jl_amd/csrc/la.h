@@ -14,6 +14,13 @@ namespace kmh {
 // Cin (optional, same leading dimension as C): the beta term reads Cin
 // instead of C (out-of-place update); only for untransposed products with
 // n <= 32 and k <= 64 (k <= 128 for n <= 16, k <= 256 for n <= 8).
+// Storage types: an operand flagged f32 (ea, eb, ec; Cin follows C) is
+// FP32 in memory behind the double pointer (reinterpret it with f32p / f64p
+// below); it is widened on load and the product accumulates in FP64 on the
+// FP64 MFMA (these products are HBM-bound, so FP32 storage halves their
+// bytes while FP64 accumulation costs nothing); an FP32 C is rounded on
+// store.  Supported (ea, eb, ec): all FP64, (0,1,1), (1,1,0), (1,0,1),
+// (1,0,0).
 struct GemmProb {
     int m, n, k;
     const double* A; int lda;
@@ -22,7 +29,13 @@ struct GemmProb {
     double alpha, beta;
     int lower;
     const double* Cin = nullptr;
+    uint8_t ea = 0, eb = 0, ec = 0;
 };
+// An FP32 array passed through a double-typed descriptor field.
+inline const double* f64p(const float* p) { return reinterpret_cast<const double*>(p); }
+inline double* f64p(float* p) { return reinterpret_cast<double*>(p); }
+inline float* f32p(double* p) { return reinterpret_cast<float*>(p); }
+inline const float* f32p(const double* p) { return reinterpret_cast<const float*>(p); }
 void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs);
 
 // A square matrix (or its leading n x n part) with leading dimension ld.
@@ -40,6 +53,7 @@ void potrf(Ctx& c, const std::vector<SqMat>& mats);
 struct TrsmProb {
     const double* L; int n; int ldl;
     double* B; int ncols; int ldb;
+    uint8_t eb = 0;   // B stored in FP32 (solved in FP64, rounded on store); left solves only
 };
 void trsm_left(Ctx& c, bool trans, const std::vector<TrsmProb>& probs);
 
@@ -49,11 +63,14 @@ void trsm_left(Ctx& c, bool trans, const std::vector<TrsmProb>& probs);
 // blocks, TSQR (single-wave Householder tiles, fan-in-4 tree) inside.  With
 // src set, the panel is read from src (leading dimension lds) and only Q is
 // written to A.
+// e32: the panel A (the Q output) and src are FP32 in memory (FP64
+// arithmetic, Q rounded on store); R stays FP64.
 struct QrPanel {
     double* A; int m; int r; int lda;
     double* R; int ldr;
     const double* src = nullptr;
     int lds = 0;
+    uint8_t e32 = 0;
 };
 void qr_panels(Ctx& c, const std::vector<QrPanel>& panels);
 
@@ -63,6 +80,7 @@ struct Copy2D {
     double* dst; int ldd;
     int m, n;
     double alpha;   // dst = alpha * src (transposed: dst(j,i) = alpha * src(i,j))
+    uint8_t es = 0, ed = 0;   // src / dst stored in FP32 (not with transpose)
 };
 void copy2d(Ctx& c, const std::vector<Copy2D>& probs, bool transpose = false);
 // dst(i,j) = a * dst(i,j) + b * src(i,j) for i < m, j < n

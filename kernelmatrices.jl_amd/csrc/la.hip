@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 
 #include "la.h"
 
@@ -48,6 +49,34 @@ __device__ __forceinline__ int mfma_d_row(int g, int r) { return 4 * r + g; }
 // non-temporal loads and stores keep them from displacing reused data.
 __device__ __forceinline__ double ld_stream(const double* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void st_stream(double v, double* p) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ double ld_stream(const float* p) { return (double)__builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st_stream(double v, float* p) { __builtin_nontemporal_store((float)v, p); }
+
+// Storage types of a GEMM launch: code TC = ea | eb << 1 | ec << 2, a set
+// bit meaning FP32 in memory (see GemmProb).
+template <int F> struct ElemOf { using T = double; };
+template <> struct ElemOf<1> { using T = float; };
+template <int TC> using EltA = typename ElemOf<TC & 1>::T;
+template <int TC> using EltB = typename ElemOf<(TC >> 1) & 1>::T;
+template <int TC> using EltC = typename ElemOf<(TC >> 2) & 1>::T;
+template <int N> using icst = std::integral_constant<int, N>;
+
+// call f(icst<TC>) for a supported storage code
+template <class F> void with_types(int tc, F&& f) {
+    switch (tc) {
+        case 0: f(icst<0>{}); break;
+        case 1: f(icst<1>{}); break;
+        case 2: f(icst<2>{}); break;
+        case 3: f(icst<3>{}); break;
+        case 5: f(icst<5>{}); break;
+        case 6: f(icst<6>{}); break;
+        default: throw Error(KMH_E_BAD_ARG, "gemm: unsupported combination of FP32 operands");
+    }
+}
+static int type_code(const GemmProb& p) { return (p.ea ? 1 : 0) | (p.eb ? 2 : 0) | (p.ec ? 4 : 0); }
+// p advanced by e elements of its storage type
+static const double* adv(const double* p, size_t e, bool f32) { return f32 ? f64p(f32p(p) + e) : p + e; }
+static double* adv(double* p, size_t e, bool f32) { return f32 ? f64p(f32p(p) + e) : p + e; }
 
 // Operand staging: each thread moves GT*GK/256 elements of A and of B per
 // k-chunk.  Element e of thread tid maps to (row-or-col index, k index) so
@@ -59,13 +88,16 @@ __device__ __forceinline__ void gemm_stage_idx(int tid, int q, int& i, int& kk) 
     else    { kk = tid & (GK - 1); i = tid / GK + (256 / GK) * q; }   // k contiguous in memory
 }
 
-template <bool TA, bool TB>
+template <bool TA, bool TB, int TC>
 __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ ds, const int* __restrict__ map) {
     constexpr int NQ = GT * GK / 256;
     __shared__ double As[GK][GT + 1];
     __shared__ double Bs[GK][GT + 1];
     const int t = blockIdx.x;
     const GemmDesc d = ds[map[t]];
+    const EltA<TC>* __restrict__ Ap = reinterpret_cast<const EltA<TC>*>(d.A);
+    const EltB<TC>* __restrict__ Bp = reinterpret_cast<const EltB<TC>*>(d.B);
+    EltC<TC>* Cp = reinterpret_cast<EltC<TC>*>(d.C);
     const int lt = t - d.tile0;
     const int row0 = (lt / d.tn) * GT, col0 = (lt % d.tn) * GT;
     if (d.lower && col0 > row0 + GT - 1) return;
@@ -86,7 +118,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ ds, c
             gemm_stage_idx<TA>(tid, q, i, kk);
             const int gi = row0 + i, gk = k0 + kk;
             double v = 0.0;
-            if (gi < d.m && gk < d.k) v = TA ? d.A[gk + (size_t)gi * d.lda] : d.A[gi + (size_t)gk * d.lda];
+            if (gi < d.m && gk < d.k) v = TA ? (double)Ap[gk + (size_t)gi * d.lda] : (double)Ap[gi + (size_t)gk * d.lda];
             ra[q] = v;
         }
 #pragma unroll
@@ -95,7 +127,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ ds, c
             gemm_stage_idx<!TB>(tid, q, j, kk);
             const int gj = col0 + j, gk = k0 + kk;
             double v = 0.0;
-            if (gj < d.n && gk < d.k) v = TB ? d.B[gj + (size_t)gk * d.ldb] : d.B[gk + (size_t)gj * d.ldb];
+            if (gj < d.n && gk < d.k) v = TB ? (double)Bp[gj + (size_t)gk * d.ldb] : (double)Bp[gk + (size_t)gj * d.ldb];
             rb[q] = v;
         }
     };
@@ -141,7 +173,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ ds, c
             for (int j = 0; j < 4; ++j) {
                 const int row = row0 + wr + mb * 16 + (lane & 15);
                 const int col = col0 + wc + nb * 16 + mfma_d_row(lane >> 4, j);
-                cv[mb][nb][j] = (d.beta != 0.0 && row < d.m && col < d.n) ? d.C[row + (size_t)col * d.ldc] : 0.0;
+                cv[mb][nb][j] = (d.beta != 0.0 && row < d.m && col < d.n) ? (double)Cp[row + (size_t)col * d.ldc] : 0.0;
             }
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
@@ -154,7 +186,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ ds, c
                 if (row < d.m && col < d.n) {
                     double v = d.alpha * acc[mb][nb][j];
                     if (d.beta != 0.0) v += d.beta * cv[mb][nb][j];
-                    d.C[row + (size_t)col * d.ldc] = v;
+                    Cp[row + (size_t)col * d.ldc] = (EltC<TC>)v;
                 }
             }
 }
@@ -168,6 +200,8 @@ struct RedDesc {
 };
 
 // elem0 is the problem's first workgroup; each workgroup sums 256 elements
+// (partials FP64; C of storage type EC)
+template <class EC>
 __global__ __launch_bounds__(256) void k_splitk_reduce(const RedDesc* __restrict__ ds, const int* __restrict__ map) {
     const RedDesc d = ds[map[blockIdx.x]];
     const long long loc = ((long long)blockIdx.x - d.elem0) * 256 + threadIdx.x;
@@ -176,10 +210,10 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(const RedDesc* __restrict
     const size_t mn = (size_t)d.m * d.n;
     double s = 0.0;
     for (int q = 0; q < d.s; ++q) s += d.W[q * mn + loc];
-    double* cp = d.C + i + (size_t)j * d.ldc;
+    EC* cp = reinterpret_cast<EC*>(d.C) + i + (size_t)j * d.ldc;
     double v = d.alpha * s;
-    if (d.beta != 0.0) v += d.beta * (*cp);
-    *cp = v;
+    if (d.beta != 0.0) v += d.beta * (double)(*cp);
+    *cp = (EC)v;
 }
 
 // Tall TN products C = alpha A^T B + beta C with m, n <= 80 and long k (the
@@ -195,7 +229,7 @@ constexpr int TT_MAX = 80;
 constexpr int TT_KS = 256;
 constexpr int TT_NARROW = 32;
 
-template <int KC, int MW>
+template <int KC, int MW, int TC>
 __global__ __launch_bounds__(256, 2) void k_gemm_tn_tall(const GemmDesc* __restrict__ ds) {
     constexpr int NG = 256 / KC;                      // column groups of the staging
     constexpr int CPT = (MW + NG - 1) / NG;           // columns per thread and operand
@@ -204,6 +238,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn_tall(const GemmDesc* __restr
     __shared__ double As[KC][MW + 1];
     __shared__ double Bs[KC][MW + 1];
     const GemmDesc d = ds[blockIdx.x];
+    const EltA<TC>* __restrict__ Ap = reinterpret_cast<const EltA<TC>*>(d.A);
+    const EltB<TC>* __restrict__ Bp = reinterpret_cast<const EltB<TC>*>(d.B);
+    EltC<TC>* Cp = reinterpret_cast<EltC<TC>*>(d.C);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int NB = (d.n + 15) >> 4, nblk = ((d.m + 15) >> 4) * NB;
     d4 acc[T];
@@ -217,8 +254,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn_tall(const GemmDesc* __restr
 #pragma unroll
         for (int q = 0; q < CPT; ++q) {
             const int col = scol + NG * q;
-            ra[q] = (col < d.m && gk < d.k) ? ld_stream(d.A + gk + (size_t)col * d.lda) : 0.0;
-            rb[q] = (col < d.n && gk < d.k) ? ld_stream(d.B + gk + (size_t)col * d.ldb) : 0.0;
+            ra[q] = (col < d.m && gk < d.k) ? ld_stream(Ap + gk + (size_t)col * d.lda) : 0.0;
+            rb[q] = (col < d.n && gk < d.k) ? ld_stream(Bp + gk + (size_t)col * d.ldb) : 0.0;
         }
     };
     fetch(0);
@@ -257,7 +294,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn_tall(const GemmDesc* __restr
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = mb * 16 + (lane & 15), col = nb * 16 + mfma_d_row(lane >> 4, r);
-            cv[t][r] = (b < nblk && d.beta != 0.0 && row < d.m && col < d.n) ? d.C[row + (size_t)col * d.ldc] : 0.0;
+            cv[t][r] = (b < nblk && d.beta != 0.0 && row < d.m && col < d.n) ? (double)Cp[row + (size_t)col * d.ldc] : 0.0;
         }
     }
 #pragma unroll
@@ -270,7 +307,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn_tall(const GemmDesc* __restr
             if (b < nblk && row < d.m && col < d.n) {
                 double v = d.alpha * acc[t][r];
                 if (d.beta != 0.0) v += d.beta * cv[t][r];
-                d.C[row + (size_t)col * d.ldc] = v;
+                Cp[row + (size_t)col * d.ldc] = (EltC<TC>)v;
             }
         }
     }
@@ -286,16 +323,19 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn_tall(const GemmDesc* __restr
 constexpr int WN_MAX = 160;
 constexpr int WN_KC = 16;
 
+template <int TC>
 __global__ __launch_bounds__(256, 2) void k_gemm_nn_wide(const GemmDesc* __restrict__ ds,
                                                         const int* __restrict__ map) {
     __shared__ double Bs[WN_KC][WN_MAX + 1];
     const int t = blockIdx.x;
     const GemmDesc d = ds[map[t]];
+    const EltB<TC>* __restrict__ Bp = reinterpret_cast<const EltB<TC>*>(d.B);
+    EltC<TC>* Cp = reinterpret_cast<EltC<TC>*>(d.C);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int NB = (d.n + 15) >> 4;
     const int arow = (t - d.tile0) * 64 + wave * 16 + (lane & 15);
     const bool arow_ok = arow < d.m;
-    const double* ap = d.A + (arow_ok ? arow : 0);
+    const EltA<TC>* ap = reinterpret_cast<const EltA<TC>*>(d.A) + (arow_ok ? arow : 0);
     d4 acc[10];
 #pragma unroll
     for (int b = 0; b < 10; ++b) acc[b] = d4{0.0, 0.0, 0.0, 0.0};
@@ -306,7 +346,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn_wide(const GemmDesc* __restr
 #pragma unroll
         for (int q = 0; q < 10; ++q) {
             const int col = scol + 16 * q;
-            rb[q] = (col < d.n && gk < d.k) ? d.B[gk + (size_t)col * d.ldb] : 0.0;
+            rb[q] = (col < d.n && gk < d.k) ? (double)Bp[gk + (size_t)col * d.ldb] : 0.0;
         }
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) {
@@ -342,7 +382,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn_wide(const GemmDesc* __restr
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int col = (h + b) * 16 + mfma_d_row(lane >> 4, r);
-                cv[b][r] = (h + b < NB && d.beta != 0.0 && arow_ok && col < d.n) ? ld_stream(d.C + arow + (size_t)col * d.ldc)
+                cv[b][r] = (h + b < NB && d.beta != 0.0 && arow_ok && col < d.n) ? ld_stream(Cp + arow + (size_t)col * d.ldc)
                                                                                    : 0.0;
             }
 #pragma unroll
@@ -353,7 +393,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn_wide(const GemmDesc* __restr
                 if (h + b < NB && arow_ok && col < d.n) {
                     double v = d.alpha * acc[h + b][r];
                     if (d.beta != 0.0) v += d.beta * cv[b][r];
-                    st_stream(v, d.C + arow + (size_t)col * d.ldc);
+                    st_stream(v, Cp + arow + (size_t)col * d.ldc);
                 }
             }
     }
@@ -370,14 +410,15 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn_wide(const GemmDesc* __restr
 // k limit of the row kernel per width class: B (k x NMAX) fills 16 KB of LDS
 template <int NMAX> constexpr int nr_kmax() { return 2048 / NMAX; }
 
-template <int NMAX, int ROWS>
+template <int NMAX, int ROWS, int TC>
 __global__ __launch_bounds__(ROWS) void k_gemm_nn_rows(const GemmDesc* __restrict__ ds, const int* __restrict__ map) {
     constexpr int KMAX = nr_kmax<NMAX>();
     __shared__ double Bs[KMAX * NMAX];
     const GemmDesc d = ds[map[blockIdx.x]];
+    const EltB<TC>* __restrict__ Bp = reinterpret_cast<const EltB<TC>*>(d.B);
     for (int e = threadIdx.x; e < KMAX * NMAX; e += ROWS) {
         const int kk = e / NMAX, j = e - kk * NMAX;
-        Bs[e] = (kk < d.k && j < d.n) ? d.B[kk + (size_t)j * d.ldb] : 0.0;
+        Bs[e] = (kk < d.k && j < d.n) ? (double)Bp[kk + (size_t)j * d.ldb] : 0.0;
     }
     __syncthreads();
     const int row = (blockIdx.x - d.tile0) * ROWS + threadIdx.x;
@@ -385,7 +426,7 @@ __global__ __launch_bounds__(ROWS) void k_gemm_nn_rows(const GemmDesc* __restric
     double acc[NMAX];
 #pragma unroll
     for (int j = 0; j < NMAX; ++j) acc[j] = 0.0;
-    const double* ap = d.A + row;
+    const EltA<TC>* ap = reinterpret_cast<const EltA<TC>*>(d.A) + row;
     for (int k0 = 0; k0 < d.k; k0 += 8) {
         double a[8];
 #pragma unroll
@@ -397,9 +438,9 @@ __global__ __launch_bounds__(ROWS) void k_gemm_nn_rows(const GemmDesc* __restric
             for (int j = 0; j < NMAX; ++j) acc[j] = __fma_rn(a[q], b[j], acc[j]);
         }
     }
-    double* cp = d.C + row;
+    EltC<TC>* cp = reinterpret_cast<EltC<TC>*>(d.C) + row;
     if (d.beta != 0.0) {
-        const double* ci = (d.Cin ? d.Cin : d.C) + row;
+        const EltC<TC>* ci = reinterpret_cast<const EltC<TC>*>(d.Cin ? d.Cin : d.C) + row;
         double cv[NMAX];
 #pragma unroll
         for (int j = 0; j < NMAX; ++j) cv[j] = j < d.n ? ld_stream(ci + (size_t)j * d.ldc) : 0.0;
@@ -434,14 +475,15 @@ static double desc_flops(const std::vector<GemmDesc>& ds) {
     for (const auto& d : ds) f += 2.0 * d.m * d.n * (double)d.k;
     return f;
 }
-static double desc_bytes(const std::vector<GemmDesc>& ds) {
+static double desc_bytes(const std::vector<GemmDesc>& ds, int tc = 0) {
+    const double sa = (tc & 1) ? 4.0 : 8.0, sb = (tc & 2) ? 4.0 : 8.0, sc = (tc & 4) ? 4.0 : 8.0;
     double b = 0.0;
     for (const auto& d : ds)
-        b += 8.0 * ((double)d.k * d.m + (double)d.k * d.n + (double)d.m * d.n * (d.beta != 0.0 ? 2 : 1));
+        b += sa * d.k * d.m + sb * (double)d.k * d.n + sc * (double)d.m * d.n * (d.beta != 0.0 ? 2 : 1);
     return b;
 }
 
-static void gemm_generic(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
+static void gemm_generic(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs, int tc) {
     std::vector<int> split(probs.size(), 1), kchunk(probs.size(), 0);
     size_t wsz = 0;
     for (size_t i = 0; i < probs.size(); ++i) {
@@ -458,6 +500,17 @@ static void gemm_generic(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& 
                 wsz += (size_t)s * p.m * p.n;
             }
         }
+    }
+    // an FP32 C cannot share a launch with FP64 split partials: such a call
+    // is split in two (the problem lists are independent)
+    bool split_any = false, whole_any = false;
+    for (size_t i = 0; i < probs.size(); ++i) (split[i] > 1 ? split_any : whole_any) = true;
+    if ((tc & 4) && split_any && whole_any) {
+        std::vector<GemmProb> a, b;
+        for (size_t i = 0; i < probs.size(); ++i) (split[i] > 1 ? a : b).push_back(probs[i]);
+        gemm_generic(c, ta, tb, a, tc);
+        gemm_generic(c, ta, tb, b, tc);
+        return;
     }
     double* W = wsz ? static_cast<double*>(c.scratch(wsz * sizeof(double))) : nullptr;
     std::vector<GemmDesc> ds;
@@ -482,8 +535,8 @@ static void gemm_generic(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& 
             for (int q = 0; q < s; ++q) {
                 const int k0 = q * kc;
                 const int kk = std::min(kc, p.k - k0);
-                const double* A = ta ? p.A + k0 : p.A + (size_t)k0 * p.lda;
-                const double* B = tb ? p.B + (size_t)k0 * p.ldb : p.B + k0;
+                const double* A = adv(p.A, ta ? (size_t)k0 : (size_t)k0 * p.lda, tc & 1);
+                const double* B = adv(p.B, tb ? (size_t)k0 * p.ldb : (size_t)k0, tc & 2);
                 map_append(map, (int)ds.size(), nt);
                 ds.push_back({A, B, W + woff + (size_t)q * p.m * p.n, p.m, p.n, kk, p.lda, p.ldb,
                               p.m, 1.0, 0.0, tiles, tn, 0, 0});
@@ -502,20 +555,29 @@ static void gemm_generic(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& 
         const dim3 g(tiles);
         const double fl = c.prof ? desc_flops(ds) : 0.0;
         const int T = KMH_PROF_GEMM;
-        if (!ta && !tb) KMH_LAUNCH_TIMED(c, T, fl, fl, (k_gemm<false, false>), g, dim3(256), dd, dm);
-        else if (!ta && tb) KMH_LAUNCH_TIMED(c, T, fl, fl, (k_gemm<false, true>), g, dim3(256), dd, dm);
-        else if (ta && !tb) KMH_LAUNCH_TIMED(c, T, fl, fl, (k_gemm<true, false>), g, dim3(256), dd, dm);
-        else KMH_LAUNCH_TIMED(c, T, fl, fl, (k_gemm<true, true>), g, dim3(256), dd, dm);
+        // split partials are FP64: their launches carry an FP64 C
+        const int tcw = split_any ? (tc & 3) : tc;
+        with_types(tcw, [&](auto k) {
+            constexpr int TC = decltype(k)::value;
+            if (!ta && !tb) KMH_LAUNCH_TIMED(c, T, fl, fl, (k_gemm<false, false, TC>), g, dim3(256), dd, dm);
+            else if (!ta && tb) KMH_LAUNCH_TIMED(c, T, fl, fl, (k_gemm<false, true, TC>), g, dim3(256), dd, dm);
+            else if (ta && !tb) KMH_LAUNCH_TIMED(c, T, fl, fl, (k_gemm<true, false, TC>), g, dim3(256), dd, dm);
+            else KMH_LAUNCH_TIMED(c, T, fl, fl, (k_gemm<true, true, TC>), g, dim3(256), dd, dm);
+        });
     }
     if (!rs.empty()) {
         RedDesc* dr = c.upload_vec(rs);
         const int* drm = c.upload_vec(rmap);
-        KMH_LAUNCH_TIMED(c, KMH_PROF_GEMM, 0.0, 0.0, k_splitk_reduce, dim3((unsigned)rblk), dim3(256), dr, drm);
+        if (tc & 4)
+            KMH_LAUNCH_TIMED(c, KMH_PROF_GEMM, 0.0, 0.0, k_splitk_reduce<float>, dim3((unsigned)rblk), dim3(256), dr, drm);
+        else
+            KMH_LAUNCH_TIMED(c, KMH_PROF_GEMM, 0.0, 0.0, k_splitk_reduce<double>, dim3((unsigned)rblk), dim3(256), dr,
+                             drm);
     }
     if (W) c.scratch_free(W);
 }
 
-static void gemm_tall_tn(Ctx& c, const std::vector<GemmProb>& probs) {
+static void gemm_tall_tn(Ctx& c, const std::vector<GemmProb>& probs, int tc) {
     size_t wsz = 0;
     for (const auto& p : probs) {
         const int s = (int)ceil_div(p.k, TT_KS);
@@ -535,7 +597,7 @@ static void gemm_tall_tn(Ctx& c, const std::vector<GemmProb>& probs) {
         }
         for (int q = 0; q < s; ++q) {
             const int k0 = q * TT_KS;
-            ds.push_back({p.A + k0, p.B + k0, W + woff + (size_t)q * p.m * p.n, p.m, p.n,
+            ds.push_back({adv(p.A, k0, tc & 1), adv(p.B, k0, tc & 2), W + woff + (size_t)q * p.m * p.n, p.m, p.n,
                           std::min(TT_KS, p.k - k0), p.lda, p.ldb, p.m, 1.0, 0.0, 0, 0, 0, 0});
         }
         const long long nb = ceil_div((long long)p.m * p.n, 256);
@@ -544,28 +606,44 @@ static void gemm_tall_tn(Ctx& c, const std::vector<GemmProb>& probs) {
         woff += (size_t)s * p.m * p.n;
         rblk += nb;
     }
-    // one launch per staging shape (by the problem's own m and n)
-    std::vector<GemmDesc> dn, dw;
-    for (const auto& d : ds) (d.m <= TT_NARROW && d.n <= TT_NARROW ? dn : dw).push_back(d);
-    if (!dn.empty()) {
-        GemmDesc* dd = c.upload_vec(dn);
-        KMH_LAUNCH_TIMED(c, KMH_PROF_TN_TALL, c.prof ? desc_bytes(dn) : 0.0, c.prof ? desc_flops(dn) : 0.0,
-                         (k_gemm_tn_tall<64, TT_NARROW>), dim3((unsigned)dn.size()), dim3(256), dd);
+    // one launch per staging shape (by the problem's own m and n) and per C
+    // type (split slabs write FP64 partials)
+    std::vector<GemmDesc> dn[2], dw[2];
+    for (const auto& d : ds) {
+        const int part = (d.C >= W && d.C < W + wsz) ? 1 : 0;
+        (d.m <= TT_NARROW && d.n <= TT_NARROW ? dn[part] : dw[part]).push_back(d);
     }
-    if (!dw.empty()) {
-        GemmDesc* dd = c.upload_vec(dw);
-        KMH_LAUNCH_TIMED(c, KMH_PROF_TN_TALL, c.prof ? desc_bytes(dw) : 0.0, c.prof ? desc_flops(dw) : 0.0,
-                         (k_gemm_tn_tall<16, TT_MAX>), dim3((unsigned)dw.size()), dim3(256), dd);
+    for (int part = 0; part < 2; ++part) {
+        const int tcl = part ? (tc & 3) : tc;
+        with_types(tcl, [&](auto k) {
+            constexpr int TC = decltype(k)::value;
+            if (!dn[part].empty()) {
+                GemmDesc* dd = c.upload_vec(dn[part]);
+                KMH_LAUNCH_TIMED(c, KMH_PROF_TN_TALL, c.prof ? desc_bytes(dn[part], TC) : 0.0,
+                                 c.prof ? desc_flops(dn[part]) : 0.0, (k_gemm_tn_tall<64, TT_NARROW, TC>),
+                                 dim3((unsigned)dn[part].size()), dim3(256), dd);
+            }
+            if (!dw[part].empty()) {
+                GemmDesc* dd = c.upload_vec(dw[part]);
+                KMH_LAUNCH_TIMED(c, KMH_PROF_TN_TALL, c.prof ? desc_bytes(dw[part], TC) : 0.0,
+                                 c.prof ? desc_flops(dw[part]) : 0.0, (k_gemm_tn_tall<16, TT_MAX, TC>),
+                                 dim3((unsigned)dw[part].size()), dim3(256), dd);
+            }
+        });
     }
     if (!rs.empty()) {
         RedDesc* dr = c.upload_vec(rs);
         const int* drm = c.upload_vec(rmap);
-        KMH_LAUNCH_TIMED(c, KMH_PROF_GEMM, 0.0, 0.0, k_splitk_reduce, dim3((unsigned)rblk), dim3(256), dr, drm);
+        if (tc & 4)
+            KMH_LAUNCH_TIMED(c, KMH_PROF_GEMM, 0.0, 0.0, k_splitk_reduce<float>, dim3((unsigned)rblk), dim3(256), dr, drm);
+        else
+            KMH_LAUNCH_TIMED(c, KMH_PROF_GEMM, 0.0, 0.0, k_splitk_reduce<double>, dim3((unsigned)rblk), dim3(256), dr,
+                             drm);
     }
     if (W) c.scratch_free(W);
 }
 
-static void gemm_wide_nn(Ctx& c, const std::vector<GemmProb>& probs) {
+static void gemm_wide_nn(Ctx& c, const std::vector<GemmProb>& probs, int tc) {
     std::vector<GemmDesc> ds;
     std::vector<int> map;
     int tiles = 0;
@@ -577,11 +655,14 @@ static void gemm_wide_nn(Ctx& c, const std::vector<GemmProb>& probs) {
     }
     GemmDesc* dd = c.upload_vec(ds);
     const int* dm = c.upload_vec(map);
-    KMH_LAUNCH_TIMED(c, KMH_PROF_NN_WIDE, c.prof ? desc_bytes(ds) : 0.0, c.prof ? desc_flops(ds) : 0.0,
-                     k_gemm_nn_wide, dim3(tiles), dim3(256), dd, dm);
+    with_types(tc, [&](auto k) {
+        constexpr int TC = decltype(k)::value;
+        KMH_LAUNCH_TIMED(c, KMH_PROF_NN_WIDE, c.prof ? desc_bytes(ds, TC) : 0.0, c.prof ? desc_flops(ds) : 0.0,
+                         k_gemm_nn_wide<TC>, dim3(tiles), dim3(256), dd, dm);
+    });
 }
 
-static void gemm_rows_nn(Ctx& c, const std::vector<GemmProb>& probs) {
+static void gemm_rows_nn(Ctx& c, const std::vector<GemmProb>& probs, int tc) {
     // one launch per width class (8, 16, 32); 128-row workgroups when every
     // problem of the class has at most 128 rows (TSQR tiles), else 256
     std::vector<const GemmProb*> cls[3];
@@ -603,37 +684,45 @@ static void gemm_rows_nn(Ctx& c, const std::vector<GemmProb>& probs) {
         }
         GemmDesc* dd = c.upload_vec(ds);
         const int* dm = c.upload_vec(map);
-        const double by = c.prof ? desc_bytes(ds) : 0.0, fl = c.prof ? desc_flops(ds) : 0.0;
+        const double by = c.prof ? desc_bytes(ds, tc) : 0.0, fl = c.prof ? desc_flops(ds) : 0.0;
         const int T = KMH_PROF_NN_ROWS;
         const dim3 g(blocks), b(rows);
-        if (rows == 128) {
-            if (cl == 0) KMH_LAUNCH_TIMED(c, T, by, fl, (k_gemm_nn_rows<8, 128>), g, b, dd, dm);
-            else if (cl == 1) KMH_LAUNCH_TIMED(c, T, by, fl, (k_gemm_nn_rows<16, 128>), g, b, dd, dm);
-            else KMH_LAUNCH_TIMED(c, T, by, fl, (k_gemm_nn_rows<32, 128>), g, b, dd, dm);
-        } else {
-            if (cl == 0) KMH_LAUNCH_TIMED(c, T, by, fl, (k_gemm_nn_rows<8, 256>), g, b, dd, dm);
-            else if (cl == 1) KMH_LAUNCH_TIMED(c, T, by, fl, (k_gemm_nn_rows<16, 256>), g, b, dd, dm);
-            else KMH_LAUNCH_TIMED(c, T, by, fl, (k_gemm_nn_rows<32, 256>), g, b, dd, dm);
-        }
+        with_types(tc, [&](auto k) {
+            constexpr int TC = decltype(k)::value;
+            if (rows == 128) {
+                if (cl == 0) KMH_LAUNCH_TIMED(c, T, by, fl, (k_gemm_nn_rows<8, 128, TC>), g, b, dd, dm);
+                else if (cl == 1) KMH_LAUNCH_TIMED(c, T, by, fl, (k_gemm_nn_rows<16, 128, TC>), g, b, dd, dm);
+                else KMH_LAUNCH_TIMED(c, T, by, fl, (k_gemm_nn_rows<32, 128, TC>), g, b, dd, dm);
+            } else {
+                if (cl == 0) KMH_LAUNCH_TIMED(c, T, by, fl, (k_gemm_nn_rows<8, 256, TC>), g, b, dd, dm);
+                else if (cl == 1) KMH_LAUNCH_TIMED(c, T, by, fl, (k_gemm_nn_rows<16, 256, TC>), g, b, dd, dm);
+                else KMH_LAUNCH_TIMED(c, T, by, fl, (k_gemm_nn_rows<32, 256, TC>), g, b, dd, dm);
+            }
+        });
     }
 }
 
 void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
-    std::vector<GemmProb> gen, tall, wide, rows;
-    for (const auto& p : probs) {
-        if (p.m <= 0 || p.n <= 0) continue;
-        KMH_CHECK(!p.Cin || p.Cin == p.C || rows_nn(ta, tb, p), KMH_E_BAD_ARG,
-                  "gemm: an out-of-place beta term needs n <= 32, k <= 2048 / (8, 16 or 32), no transposes");
-        if (tall_tn(ta, tb, p)) tall.push_back(p);
-        else if (rows_nn(ta, tb, p)) rows.push_back(p);
-        else if (wide_nn(ta, tb, p)) wide.push_back(p);
-        else gen.push_back(p);
+    // one pass per storage-type code present (problem lists are independent)
+    int seen = 0;
+    for (const auto& p : probs) seen |= 1 << type_code(p);
+    for (int tc = 0; tc < 8; ++tc) {
+        if (!((seen >> tc) & 1)) continue;
+        std::vector<GemmProb> gen, tall, wide, rows;
+        for (const auto& p : probs) {
+            if (p.m <= 0 || p.n <= 0 || type_code(p) != tc) continue;
+            KMH_CHECK(!p.Cin || p.Cin == p.C || rows_nn(ta, tb, p), KMH_E_BAD_ARG,
+                      "gemm: an out-of-place beta term needs n <= 32, k <= 2048 / (8, 16 or 32), no transposes");
+            if (tall_tn(ta, tb, p)) tall.push_back(p);
+            else if (rows_nn(ta, tb, p)) rows.push_back(p);
+            else if (wide_nn(ta, tb, p)) wide.push_back(p);
+            else gen.push_back(p);
+        }
+        if (!gen.empty()) gemm_generic(c, ta, tb, gen, tc);
+        if (!tall.empty()) gemm_tall_tn(c, tall, tc);
+        if (!wide.empty()) gemm_wide_nn(c, wide, tc);
+        if (!rows.empty()) gemm_rows_nn(c, rows, tc);
     }
-    if (gen.empty() && tall.empty() && wide.empty() && rows.empty()) return;
-    if (!gen.empty()) gemm_generic(c, ta, tb, gen);
-    if (!tall.empty()) gemm_tall_tn(c, tall);
-    if (!wide.empty()) gemm_wide_nn(c, wide);
-    if (!rows.empty()) gemm_rows_nn(c, rows);
 }
 
 // ---------------------------------------------------------------------------
@@ -704,11 +793,12 @@ struct LdiagDesc {
 // lower triangle of a 64 x 64 block packed by columns: column l holds rows l..63
 __device__ __forceinline__ int tri_col(int l) { return (l * (2 * PB + 1 - l)) >> 1; }
 
-template <int MODE>
+template <int MODE, class EB>
 __global__ __launch_bounds__(256) void k_trsm_diag(const LdiagDesc* __restrict__ ds, const int* __restrict__ map) {
     __shared__ double Lt[PB * (PB + 1) / 2];   // L's lower triangle, identity beyond nb
     __shared__ double Xp[16][PB + 1];          // the solved rows of the current group, published to the others
     const LdiagDesc d = ds[map[blockIdx.x]];
+    EB* Bp = reinterpret_cast<EB*>(d.B);
     const int tid = threadIdx.x, c = tid & 63;
     const int g = __builtin_amdgcn_readfirstlane(tid >> 6);   // row group of this wave
     const int c0 = ((int)blockIdx.x - d.chunk0) * PB;
@@ -730,7 +820,8 @@ __global__ __launch_bounds__(256) void k_trsm_diag(const LdiagDesc* __restrict__
     for (int i = 0; i < 16; ++i) {
         const int row = 16 * g + i;
         const bool ok = c < nc && row < nb;
-        x[i] = ok ? (MODE == 2 ? d.B[(size_t)(c0 + c) + (size_t)row * d.ldb] : d.B[row + (size_t)(c0 + c) * d.ldb])
+        x[i] = ok ? (MODE == 2 ? (double)Bp[(size_t)(c0 + c) + (size_t)row * d.ldb]
+                               : (double)Bp[row + (size_t)(c0 + c) * d.ldb])
                   : 0.0;
     }
     __syncthreads();
@@ -788,8 +879,8 @@ __global__ __launch_bounds__(256) void k_trsm_diag(const LdiagDesc* __restrict__
     for (int i = 0; i < 16; ++i) {
         const int row = 16 * g + i;
         if (c < nc && row < nb) {
-            if (MODE == 2) d.B[(size_t)(c0 + c) + (size_t)row * d.ldb] = x[i];
-            else d.B[row + (size_t)(c0 + c) * d.ldb] = x[i];
+            if (MODE == 2) Bp[(size_t)(c0 + c) + (size_t)row * d.ldb] = (EB)x[i];
+            else Bp[row + (size_t)(c0 + c) * d.ldb] = (EB)x[i];
         }
     }
 }
@@ -803,7 +894,7 @@ __global__ __launch_bounds__(256) void k_zero_upper(const SqMat* __restrict__ ms
     }
 }
 
-static void launch_trsm_diag(Ctx& c, int mode, const std::vector<LdiagDesc>& ds, int chunks) {
+static void launch_trsm_diag(Ctx& c, int mode, const std::vector<LdiagDesc>& ds, int chunks, bool b32 = false) {
     if (ds.empty()) return;
     std::vector<int> map;
     map.reserve(chunks);
@@ -811,12 +902,15 @@ static void launch_trsm_diag(Ctx& c, int mode, const std::vector<LdiagDesc>& ds,
         map_append(map, (int)i, (i + 1 < ds.size() ? ds[i + 1].chunk0 : chunks) - ds[i].chunk0);
     LdiagDesc* dd = c.upload_vec(ds);
     const int* dm = c.upload_vec(map);
-    if (mode == 0)
-        KMH_LAUNCH(c, k_trsm_diag<0>, dim3(chunks), dim3(256), dd, dm);
+    KMH_CHECK(!b32 || mode == 0, KMH_E_BAD_ARG, "FP32 right-hand sides only for left lower solves");
+    if (b32)
+        KMH_LAUNCH(c, (k_trsm_diag<0, float>), dim3(chunks), dim3(256), dd, dm);
+    else if (mode == 0)
+        KMH_LAUNCH(c, (k_trsm_diag<0, double>), dim3(chunks), dim3(256), dd, dm);
     else if (mode == 1)
-        KMH_LAUNCH(c, k_trsm_diag<1>, dim3(chunks), dim3(256), dd, dm);
+        KMH_LAUNCH(c, (k_trsm_diag<1, double>), dim3(chunks), dim3(256), dd, dm);
     else
-        KMH_LAUNCH(c, k_trsm_diag<2>, dim3(chunks), dim3(256), dd, dm);
+        KMH_LAUNCH(c, (k_trsm_diag<2, double>), dim3(chunks), dim3(256), dd, dm);
 }
 
 void potrf(Ctx& c, const std::vector<SqMat>& mats) {
@@ -853,6 +947,17 @@ void potrf(Ctx& c, const std::vector<SqMat>& mats) {
 }
 
 void trsm_left(Ctx& c, bool trans, const std::vector<TrsmProb>& probs) {
+    // FP32 right-hand sides (left lower solves only) in launches of their own
+    bool any32 = false, any64 = false;
+    for (const auto& p : probs) (p.eb ? any32 : any64) = true;
+    if (any32 && any64) {
+        std::vector<TrsmProb> a, b;
+        for (const auto& p : probs) (p.eb ? a : b).push_back(p);
+        trsm_left(c, trans, b);
+        trsm_left(c, trans, a);
+        return;
+    }
+    KMH_CHECK(!any32 || !trans, KMH_E_BAD_ARG, "FP32 right-hand sides only for left lower solves");
     int maxblk = 0;
     for (const auto& p : probs) maxblk = std::max<int>(maxblk, (int)ceil_div(p.n, PB));
     for (int s = 0; s < maxblk; ++s) {
@@ -865,19 +970,22 @@ void trsm_left(Ctx& c, bool trans, const std::vector<TrsmProb>& probs) {
             const int kb = trans ? nblk - 1 - s : s;
             const int k0 = kb * PB;
             const int nb = std::min(PB, p.n - k0);
-            diag.push_back({p.L + k0 + (size_t)k0 * p.ldl, nb, p.ldl, p.B + k0, p.ncols, p.ldb, chunks, 0});
+            diag.push_back({p.L + k0 + (size_t)k0 * p.ldl, nb, p.ldl, adv(p.B, k0, p.eb), p.ncols, p.ldb, chunks, 0});
             chunks += (int)ceil_div(p.ncols, PB);
             if (!trans) {
                 const int rest = p.n - k0 - nb;
-                if (rest > 0)
-                    upd.push_back({rest, p.ncols, nb, p.L + (k0 + nb) + (size_t)k0 * p.ldl, p.ldl,
-                                   p.B + k0, p.ldb, p.B + k0 + nb, p.ldb, -1.0, 1.0, 0});
+                if (rest > 0) {
+                    GemmProb g{rest, p.ncols, nb, p.L + (k0 + nb) + (size_t)k0 * p.ldl, p.ldl,
+                               adv(p.B, k0, p.eb), p.ldb, adv(p.B, k0 + nb, p.eb), p.ldb, -1.0, 1.0, 0};
+                    g.eb = g.ec = p.eb;
+                    upd.push_back(g);
+                }
             } else if (k0 > 0) {
                 upd.push_back({k0, p.ncols, nb, p.L + k0, p.ldl, p.B + k0, p.ldb, p.B, p.ldb, -1.0, 1.0, 0});
             }
         }
         if (diag.empty()) continue;
-        launch_trsm_diag(c, trans ? 1 : 0, diag, chunks);
+        launch_trsm_diag(c, trans ? 1 : 0, diag, chunks, any32);
         if (!upd.empty()) gemm(c, trans, false, upd);
     }
 }
@@ -901,31 +1009,49 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 // ---------------------------------------------------------------------------
 // elementwise helpers
 // ---------------------------------------------------------------------------
-template <bool TR>
+template <bool TR, class ES, class ED>
 __global__ __launch_bounds__(256) void k_copy2d(const Copy2D* __restrict__ ds) {
     const Copy2D d = ds[blockIdx.y];
+    const ES* src = reinterpret_cast<const ES*>(d.src);
+    ED* dst = reinterpret_cast<ED*>(d.dst);
     // 256-row segments of one column per step (uniform index arithmetic)
     const int rb = (d.m + 255) >> 8;
     const long long segs = (long long)rb * d.n;
     for (long long t = blockIdx.x; t < segs; t += gridDim.x) {
         const int j = (int)(t / rb), i = (int)(t - (long long)j * rb) * 256 + (int)threadIdx.x;
         if (i >= d.m) continue;
-        const double v = d.alpha * d.src[i + (size_t)j * d.lds];
-        if (TR) d.dst[j + (size_t)i * d.ldd] = v;
-        else d.dst[i + (size_t)j * d.ldd] = v;
+        const double v = d.alpha * (double)src[i + (size_t)j * d.lds];
+        if (TR) dst[j + (size_t)i * d.ldd] = (ED)v;
+        else dst[i + (size_t)j * d.ldd] = (ED)v;
     }
 }
 
 void copy2d(Ctx& c, const std::vector<Copy2D>& probs, bool transpose) {
     if (probs.empty()) return;
-    for (size_t off = 0; off < probs.size(); off += 65535) {
-        std::vector<Copy2D> part(probs.begin() + off, probs.begin() + std::min(probs.size(), off + 65535));
-        Copy2D* d = c.upload_vec(part);
-        long long mx = 0;
-        for (auto& p : part) mx = std::max<long long>(mx, (long long)p.m * p.n);
-        const unsigned gx = (unsigned)std::min<long long>(std::max<long long>(1, ceil_div(mx, 256)), 1024);
-        if (transpose) KMH_LAUNCH(c, (k_copy2d<true>), dim3(gx, (unsigned)part.size()), dim3(256), d);
-        else KMH_LAUNCH(c, (k_copy2d<false>), dim3(gx, (unsigned)part.size()), dim3(256), d);
+    // one launch group per storage-type pair
+    int seen = 0;
+    for (const auto& p : probs) {
+        KMH_CHECK(!transpose || (!p.es && !p.ed), KMH_E_BAD_ARG, "copy2d: no FP32 transposes");
+        seen |= 1 << ((p.es ? 1 : 0) | (p.ed ? 2 : 0));
+    }
+    for (int tc = 0; tc < 4; ++tc) {
+        if (!((seen >> tc) & 1)) continue;
+        std::vector<Copy2D> sel;
+        for (const auto& p : probs)
+            if (((p.es ? 1 : 0) | (p.ed ? 2 : 0)) == tc) sel.push_back(p);
+        for (size_t off = 0; off < sel.size(); off += 65535) {
+            std::vector<Copy2D> part(sel.begin() + off, sel.begin() + std::min(sel.size(), off + 65535));
+            Copy2D* d = c.upload_vec(part);
+            long long mx = 0;
+            for (auto& p : part) mx = std::max<long long>(mx, (long long)p.m * p.n);
+            const unsigned gx = (unsigned)std::min<long long>(std::max<long long>(1, ceil_div(mx, 256)), 1024);
+            const dim3 g(gx, (unsigned)part.size());
+            if (transpose) KMH_LAUNCH(c, (k_copy2d<true, double, double>), g, dim3(256), d);
+            else if (tc == 0) KMH_LAUNCH(c, (k_copy2d<false, double, double>), g, dim3(256), d);
+            else if (tc == 1) KMH_LAUNCH(c, (k_copy2d<false, float, double>), g, dim3(256), d);
+            else if (tc == 2) KMH_LAUNCH(c, (k_copy2d<false, double, float>), g, dim3(256), d);
+            else KMH_LAUNCH(c, (k_copy2d<false, float, float>), g, dim3(256), d);
+        }
     }
 }
 

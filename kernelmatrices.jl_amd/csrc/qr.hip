@@ -30,7 +30,8 @@ struct SqrDesc {
     double* qout; int ldq;
     double* rout; int ldr;
     double* zout;     // compact-WY coupling Z, ld QC
-    int nc, pad;
+    int nc;
+    int e32;          // src and qout stored in FP32 (R and Z stay FP64)
 };
 
 // value of v held by lane `lane` (uniform lane index)
@@ -127,7 +128,7 @@ template <int N> using ic = std::integral_constant<int, N>;
 // k < nc - j write row j of R straight to the R output.  The 24-column tile
 // runs three waves per SIMD: its step loop spills at the 128 registers of
 // four, which costs more than the fourth wave gains.
-template <int NC>
+template <int NC, class E>
 __global__ __launch_bounds__(64, NC <= 16 ? 4 : 3) void k_tile_qr(const SqrDesc* __restrict__ ds) {
     static_assert(NC % 8 == 0 && NC <= QC, "tile width");
     __shared__ double pw[64 * 17];
@@ -142,18 +143,18 @@ __global__ __launch_bounds__(64, NC <= 16 ? 4 : 3) void k_tile_qr(const SqrDesc*
         // branch-free loads: out-of-range rows and columns read a valid
         // element and are replaced by zero
         const bool in0 = r0 < mt, in1 = r1 < mt;
-        const double* s0 = d.src + (in0 ? (size_t)(r0 / d.nbs) * d.slot + (r0 % d.nbs) : 0);
-        const double* s1 = d.src + (in1 ? (size_t)(r1 / d.nbs) * d.slot + (r1 % d.nbs) : 0);
+        const E* s0 = reinterpret_cast<const E*>(d.src) + (in0 ? (size_t)(r0 / d.nbs) * d.slot + (r0 % d.nbs) : 0);
+        const E* s1 = reinterpret_cast<const E*>(d.src) + (in1 ? (size_t)(r1 / d.nbs) * d.slot + (r1 % d.nbs) : 0);
 #pragma unroll
         for (int k = 0; k < NC; ++k) {
             const size_t off = (size_t)(k < nc ? k : 0) * d.ld;
-            const double x0 = s0[off], x1 = s1[off];
+            const double x0 = (double)s0[off], x1 = (double)s1[off];
             a0[k] = (in0 && k < nc) ? x0 : 0.0;
             a1[k] = (in1 && k < nc) ? x1 : 0.0;
         }
     }
-    double* q0 = d.qout + (r0 < mt ? r0 : 0);
-    double* q1 = d.qout + (r1 < mt ? r1 : 0);
+    E* q0 = reinterpret_cast<E*>(d.qout) + (r0 < mt ? r0 : 0);
+    E* q1 = reinterpret_cast<E*>(d.qout) + (r1 < mt ? r1 : 0);
 
     // factorization steps j0 <= j < j1 with window W
     auto fwd = [&](auto wc, int j0, int j1) {
@@ -194,8 +195,8 @@ __global__ __launch_bounds__(64, NC <= 16 ? 4 : 3) void k_tile_qr(const SqrDesc*
                 }
                 __builtin_amdgcn_wave_barrier();
             }
-            if (r0 < mt) q0[(size_t)j * d.ldq] = v0;
-            if (r1 < mt) q1[(size_t)j * d.ldq] = v1;
+            if (r0 < mt) q0[(size_t)j * d.ldq] = (E)v0;
+            if (r1 < mt) q1[(size_t)j * d.ldq] = (E)v1;
             // row j of R: lane k holds column j + k (the pivot row's updated
             // entry, the same arithmetic as the pivot lane's update above)
             if (lane < nc - j) {
@@ -279,31 +280,38 @@ __global__ __launch_bounds__(64, NC <= 16 ? 4 : 3) void k_tile_qr(const SqrDesc*
     }
 }
 
-// One launch per column class (8, 16, QC = 24) of the descriptors.
-static void launch_tile_qr(Ctx& c, const std::vector<SqrDesc>& ds) {
-    std::vector<SqrDesc> cls[3];
-    for (const auto& d : ds) {
-        KMH_CHECK(d.nc <= QC && d.nstack * d.nbs <= QT && d.nstack * d.nbs >= d.nc, KMH_E_BAD_ARG,
-                  "tile QR beyond its size limits");
-        cls[d.nc <= 8 ? 0 : d.nc <= 16 ? 1 : 2].push_back(d);   // NC = 8, 16, 24
-    }
+// One launch per column class (8, 16, QC = 24) and storage type of the
+// descriptors.
+template <class E>
+static void launch_tile_qr_t(Ctx& c, std::vector<SqrDesc> (&cls)[3]) {
     for (int k = 0; k < 3; ++k) {
         if (cls[k].empty()) continue;
         SqrDesc* dd = c.upload_vec(cls[k]);
         const dim3 g((unsigned)cls[k].size());
-        if (k == 0) KMH_LAUNCH(c, k_tile_qr<8>, g, dim3(64), dd);
-        else if (k == 1) KMH_LAUNCH(c, k_tile_qr<16>, g, dim3(64), dd);
+        if (k == 0) KMH_LAUNCH(c, (k_tile_qr<8, E>), g, dim3(64), dd);
+        else if (k == 1) KMH_LAUNCH(c, (k_tile_qr<16, E>), g, dim3(64), dd);
         else {
             // algorithmic bytes: the tile read, V and R written
             double by = 0.0;
             if (c.prof)
                 for (const auto& d : cls[k]) {
                     const double mt = (double)d.nstack * d.nbs;
-                    by += 8.0 * (2.0 * mt * d.nc + (double)d.nc * d.nc);
+                    by += 2.0 * sizeof(E) * mt * d.nc + 8.0 * (double)d.nc * d.nc;
                 }
-            KMH_LAUNCH_TIMED(c, KMH_PROF_TILE_QR, by, -1.0, k_tile_qr<QC>, g, dim3(64), dd);
+            KMH_LAUNCH_TIMED(c, KMH_PROF_TILE_QR, by, -1.0, (k_tile_qr<QC, E>), g, dim3(64), dd);
         }
     }
+}
+
+static void launch_tile_qr(Ctx& c, const std::vector<SqrDesc>& ds) {
+    std::vector<SqrDesc> cls[2][3];
+    for (const auto& d : ds) {
+        KMH_CHECK(d.nc <= QC && d.nstack * d.nbs <= QT && d.nstack * d.nbs >= d.nc, KMH_E_BAD_ARG,
+                  "tile QR beyond its size limits");
+        cls[d.e32 ? 1 : 0][d.nc <= 8 ? 0 : d.nc <= 16 ? 1 : 2].push_back(d);   // NC = 8, 16, 24
+    }
+    launch_tile_qr_t<double>(c, cls[0]);
+    launch_tile_qr_t<float>(c, cls[1]);
 }
 
 // out = Q C = [C; 0] - V Y with Y = T V1^T C (V1 the top nb rows of V) for
@@ -313,7 +321,8 @@ struct WyDesc {
     const double* V; int ldv; int mt;
     const double* Z;          // ld QC
     const double* C; int ldc; int nb;
-    double* out; int ldo; int pad;
+    double* out; int ldo;
+    int e32;                  // V and out stored in FP32 (Z and C stay FP64)
 };
 
 typedef double qd4 __attribute__((ext_vector_type(4)));
@@ -328,6 +337,7 @@ typedef double qd4 __attribute__((ext_vector_type(4)));
 // 16-row column segments of out.  A wave reads a row block of V before it
 // writes that block, and V1 is in LDS before any block is written, so out
 // may alias V.  (Up to 128 rows: eight 16-row blocks, four per wave.)
+template <class E>
 __global__ __launch_bounds__(128, 4) void k_wy_apply(const WyDesc* __restrict__ ds) {
     constexpr int S = QC + 1;
     constexpr int KS = QC / 4;   // k steps of 4
@@ -350,7 +360,8 @@ __global__ __launch_bounds__(128, 4) void k_wy_apply(const WyDesc* __restrict__ 
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
                 const int k = 4 * ks + lk;
-                const double x = __builtin_nontemporal_load(d.V + (rok ? row : 0) + (size_t)(k < nb ? k : 0) * d.ldv);
+                const double x = (double)__builtin_nontemporal_load(reinterpret_cast<const E*>(d.V) + (rok ? row : 0) +
+                                                                    (size_t)(k < nb ? k : 0) * d.ldv);
                 vf[h][ks] = (rok && k < nb) ? x : 0.0;
             }
         }
@@ -365,7 +376,7 @@ __global__ __launch_bounds__(128, 4) void k_wy_apply(const WyDesc* __restrict__ 
             const int e = t + 128 * q, i = e % QC, k = e / QC;
             const bool in = e < QC * QC && i < nb && k < nb;
             const size_t ii = in ? i : 0, kk = in ? k : 0;
-            lv[q] = in ? d.V[ii + kk * d.ldv] : 0.0;
+            lv[q] = in ? (double)reinterpret_cast<const E*>(d.V)[ii + kk * d.ldv] : 0.0;
             lc[q] = in ? (d.C ? d.C[ii + kk * d.ldc] : (i == k ? 1.0 : 0.0)) : 0.0;
             lz[q] = (in && i <= k) ? d.Z[ii + kk * QC] : 0.0;
         }
@@ -466,7 +477,7 @@ __global__ __launch_bounds__(128, 4) void k_wy_apply(const WyDesc* __restrict__ 
                 for (int r = 0; r < 4; ++r) {
                     const int col = 16 * cb + 4 * r + lk;
                     if (row < d.mt && col < nb)
-                        __builtin_nontemporal_store(acc[h][cb][r], d.out + row + (size_t)col * d.ldo);
+                        __builtin_nontemporal_store((E)acc[h][cb][r], reinterpret_cast<E*>(d.out) + row + (size_t)col * d.ldo);
                 }
         }
     }
@@ -474,10 +485,19 @@ __global__ __launch_bounds__(128, 4) void k_wy_apply(const WyDesc* __restrict__ 
 
 static void wy_apply(Ctx& c, const std::vector<WyDesc>& ds) {
     if (ds.empty()) return;
-    for (const auto& d : ds)
+    std::vector<WyDesc> by[2];
+    for (const auto& d : ds) {
         KMH_CHECK(d.nb <= QC && d.mt <= QT && d.mt >= d.nb, KMH_E_BAD_ARG, "WY apply beyond its size limits");
-    WyDesc* dd = c.upload_vec(ds);
-    KMH_LAUNCH(c, k_wy_apply, dim3((unsigned)ds.size()), dim3(128), dd);
+        by[d.e32 ? 1 : 0].push_back(d);
+    }
+    if (!by[0].empty()) {
+        WyDesc* dd = c.upload_vec(by[0]);
+        KMH_LAUNCH(c, k_wy_apply<double>, dim3((unsigned)by[0].size()), dim3(128), dd);
+    }
+    if (!by[1].empty()) {
+        WyDesc* dd = c.upload_vec(by[1]);
+        KMH_LAUNCH(c, k_wy_apply<float>, dim3((unsigned)by[1].size()), dim3(128), dd);
+    }
 }
 
 struct TsqrPanel {
@@ -485,6 +505,7 @@ struct TsqrPanel {
     double* R; int ldr;
     const double* src;   // input panel (null: A itself), leading dimension lds
     int lds;
+    int e32;             // A and src stored in FP32
 };
 
 // Thin QR of m x nb panels (nb <= QC, m >= nb) by TSQR: leaf tiles of <= 128
@@ -493,6 +514,10 @@ struct TsqrPanel {
 // the down pass then forms each node's Q times its parent's coefficient
 // block (the root's is I) level by level, in place, and finally each leaf
 // tile's Q times its coefficient block, in place in the panel.
+// p advanced by e elements of its storage type (FP32 when f32)
+static double* adv_e(double* p, size_t e, bool f32) { return f32 ? f64p(f32p(p) + e) : p + e; }
+static const double* adv_e(const double* p, size_t e, bool f32) { return f32 ? f64p(f32p(p) + e) : p + e; }
+
 static void tsqr(Ctx& c, const std::vector<TsqrPanel>& ps) {
     if (ps.empty()) return;
     struct Tile { int panel; int row0; int rows; int node; };
@@ -528,11 +553,11 @@ static void tsqr(Ctx& c, const std::vector<TsqrPanel>& ps) {
         for (const auto& t : tiles) {
             const auto& P = ps[t.panel];
             const bool single = cur[t.panel].size() == 1;
-            double* a = P.A + t.row0;
-            const double* in = P.src ? P.src + t.row0 : a;
+            double* a = adv_e(P.A, t.row0, P.e32);
+            const double* in = P.src ? adv_e(P.src, t.row0, P.e32) : a;
             ds.push_back({in, P.src ? P.lds : P.lda, 1, t.rows, 0, a, P.lda,
                           single ? P.R : RT + (size_t)t.node * RS, single ? P.ldr : QC,
-                          ZT + (size_t)t.node * RS, P.nb, 0});
+                          ZT + (size_t)t.node * RS, P.nb, P.e32});
         }
         launch_tile_qr(c, ds);
     }
@@ -595,8 +620,8 @@ static void tsqr(Ctx& c, const std::vector<TsqrPanel>& ps) {
         std::vector<WyDesc> wd;
         for (const auto& t : tiles) {
             const auto& P = ps[t.panel];
-            double* a = P.A + t.row0;
-            wd.push_back({a, P.lda, t.rows, ZT + (size_t)t.node * RS, cptr[t.node], cld[t.node], P.nb, a, P.lda, 0});
+            double* a = adv_e(P.A, t.row0, P.e32);
+            wd.push_back({a, P.lda, t.rows, ZT + (size_t)t.node * RS, cptr[t.node], cld[t.node], P.nb, a, P.lda, P.e32});
         }
         wy_apply(c, wd);
     }
@@ -628,27 +653,38 @@ void qr_panels(Ctx& c, const std::vector<QrPanel>& ps) {
         for (const auto& p : ps) {
             if (p.r <= c0) continue;
             const int nb = std::min(QC, p.r - c0);
-            double* Aj = p.A + (size_t)c0 * p.lda;
+            const uint8_t e = p.e32;
+            double* Aj = adv_e(p.A, (size_t)c0 * p.lda, e);
             double* Rj = p.R + (size_t)c0 * p.ldr;
             // this column block's input: the source panel on first touch
-            const double* sj = p.src ? p.src + (size_t)c0 * p.lds : nullptr;
+            const double* sj = p.src ? adv_e(p.src, (size_t)c0 * p.lds, e) : nullptr;
             if (sj && c0 > 64) {   // beyond the out-of-place update's k range: copy first
-                pre.push_back({sj, p.lds, Aj, p.lda, p.m, nb, 1.0});
+                Copy2D cp{sj, p.lds, Aj, p.lda, p.m, nb, 1.0};
+                cp.es = cp.ed = e;
+                pre.push_back(cp);
                 sj = nullptr;
             }
             if (c0 > 0) {
                 double* t2 = T2 + toff;
                 toff += (size_t)c0 * nb;
-                p1.push_back({c0, nb, p.m, p.A, p.lda, sj ? sj : Aj, sj ? p.lds : p.lda, Rj, p.ldr, 1.0, 0.0, 0});
+                // projections Q^T X (FP64 out), updates X -= Q R (X in the panel's type)
+                GemmProb g0{c0, nb, p.m, p.A, p.lda, sj ? sj : Aj, sj ? p.lds : p.lda, Rj, p.ldr, 1.0, 0.0, 0};
+                g0.ea = g0.eb = e;
+                p1.push_back(g0);
                 GemmProb g1{p.m, nb, c0, p.A, p.lda, Rj, p.ldr, Aj, p.lda, -1.0, 1.0, 0};
                 g1.Cin = sj;
+                g1.ea = g1.ec = e;
                 u1.push_back(g1);
-                p2.push_back({c0, nb, p.m, p.A, p.lda, Aj, p.lda, t2, c0, 1.0, 0.0, 0});
-                u2.push_back({p.m, nb, c0, p.A, p.lda, t2, c0, Aj, p.lda, -1.0, 1.0, 0});
+                GemmProb g2{c0, nb, p.m, p.A, p.lda, Aj, p.lda, t2, c0, 1.0, 0.0, 0};
+                g2.ea = g2.eb = e;
+                p2.push_back(g2);
+                GemmProb g3{p.m, nb, c0, p.A, p.lda, t2, c0, Aj, p.lda, -1.0, 1.0, 0};
+                g3.ea = g3.ec = e;
+                u2.push_back(g3);
                 acc.push_back({t2, c0, Rj, p.ldr, c0, nb, 1.0, 1.0});
-                tp.push_back({Aj, p.m, nb, p.lda, Rj + c0, p.ldr, nullptr, 0});
+                tp.push_back({Aj, p.m, nb, p.lda, Rj + c0, p.ldr, nullptr, 0, e});
             } else {
-                tp.push_back({Aj, p.m, nb, p.lda, Rj + c0, p.ldr, sj, p.lds});
+                tp.push_back({Aj, p.m, nb, p.lda, Rj + c0, p.ldr, sj, p.lds, e});
             }
         }
         copy2d(c, pre);

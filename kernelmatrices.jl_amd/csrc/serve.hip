@@ -194,15 +194,32 @@ static void finish_batch(Ctx& c, std::vector<Req*>& rs) {
         }
         copy2d(c, cp);
         copy2d(c, cp2);
-        hodlr_factor_apply_batch(c, KMH_WINV, ok, T, k);   // z = W^{-1} y
+        // FP64 form: z = W^{-1} y, y' K^{-1} y = z' z.  Mixed form: x = K^{-1} y
+        // refined against the FP64 operator, y' K^{-1} y = y' x.
+        std::vector<Hodlr*> h64, h32;
+        std::vector<double*> t64, t32;
         std::vector<DotJob> dj;
-        for (size_t i = 0; i < ok.size(); ++i) dj.push_back({T[i], T[i], ok[i]->n, dots + i});
+        off = 0;
+        for (size_t i = 0; i < ok.size(); ++i) {
+            if (ok[i]->p32()) {
+                h32.push_back(ok[i]);
+                t32.push_back(T[i]);
+                dj.push_back({T[i], buf + off, ok[i]->n, dots + i});
+            } else {
+                h64.push_back(ok[i]);
+                t64.push_back(T[i]);
+                dj.push_back({T[i], T[i], ok[i]->n, dots + i});
+            }
+            off += 2 * ok[i]->n;
+        }
+        hodlr_factor_apply_batch(c, KMH_WINV, h64, t64, std::vector<int>(h64.size(), 1));
+        if (!h32.empty()) hodlr_solve_batch(c, h32, t32, std::vector<int>(h32.size(), 1));
         dots_batch(c, dj);
         std::vector<Hodlr*> gh;
         std::vector<double*> gT;
         std::vector<int> gk;
         for (size_t i = 0; i < ok.size(); ++i)
-            if (!okr[i]->dks.empty()) {
+            if (!okr[i]->dks.empty() && !ok[i]->p32()) {
                 gh.push_back(ok[i]);
                 gT.push_back(T[i]);
                 gk.push_back(1);

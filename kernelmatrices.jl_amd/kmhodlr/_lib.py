@@ -20,6 +20,8 @@ KMH_D0, KMH_D1, KMH_D2, KMH_D11, KMH_D12, KMH_D22 = 0, 1, 2, 11, 12, 22
 KMH_ACA, KMH_NYSTROM = 0, 1
 KMH_FIXED_LEVEL, KMH_LOG_LEVEL = 0, 1
 KMH_FP64 = 0
+KMH_MIXED = 1
+KMH_MIXED_Q = 2
 KMH_W, KMH_WT, KMH_WINV, KMH_WTINV = 0, 1, 2, 3
 KMH_PROF_GEMM, KMH_PROF_NN_WIDE, KMH_PROF_TN_TALL, KMH_PROF_NN_ROWS, KMH_PROF_TILE_QR = 0, 1, 2, 3, 4
 
@@ -95,6 +97,7 @@ SIGNATURES = {
     "kmh_hodlr_solve_dev": [_P, _P, _P, _I64],
     "kmh_hodlr_solve_dev_noalloc": [_P, _P, _P, _I64],
     "kmh_hodlr_logdet": [_P, _D],
+    "kmh_hodlr_refinement": [_P, C.POINTER(C.c_int32), _D],
     "kmh_stoch_gradient": [_P, C.POINTER(kmh_kernel), C.c_int32, _P, _P, _I64, _P],
     "kmh_stoch_gradient_dev": [_P, C.POINTER(kmh_kernel), C.c_int32, _P, _P, _I64, _P],
     "kmh_hodlr_build_batch": [_P, C.c_int32, C.POINTER(kmh_build_req), C.POINTER(_P), C.POINTER(C.c_int32)],

@@ -233,7 +233,7 @@ class KernelHODLR:
     sorted=false): built on the GPU (constructor.jl:6-58).  `plel` is
     accepted and ignored (the GPU runs every block in parallel)."""
 
-    def __init__(self, K, ep, maxrank, lvl, nystrom=False, plel=False, sorted=False, ctx=None):
+    def __init__(self, K, ep, maxrank, lvl, nystrom=False, plel=False, sorted=False, ctx=None, mixed=False):
         if not (K.x1 is K.x2 or np.array_equal(K.x1, K.x2)):
             raise ErrorException("This function builds symmetric matrices. For non-symmetric matrices, "
                                  "please use the recursive RKernelHODLR, which will only allow matvecs.")
@@ -246,7 +246,12 @@ class KernelHODLR:
         o.level = lvl.lv
         o.maxrank = int(maxrank)
         o.tol = float(ep)
-        o.precision = L.KMH_FP64
+        # mixed: False (FP64), True or "panels" (KMH_MIXED: FP32 panels,
+        # cascade panels and Q factors) or "q" (KMH_MIXED_Q: FP32 Q factors
+        # and panel QR); FP64 leaves / capacitance / logdet, solves refined
+        # to FP64 (no reference counterpart)
+        o.precision = {False: L.KMH_FP64, True: L.KMH_MIXED, "panels": L.KMH_MIXED,
+                       "q": L.KMH_MIXED_Q}[mixed]
         kd = K.kernel.desc(K.parms)
         p = C.c_void_p()
         pts = _f64(K.x1)
