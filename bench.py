@@ -44,18 +44,40 @@ TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profile
                             "tile_qr_nn_wide_served_n65536_traffic.md")
 
 
+KERNEL_SOURCES = ("la.hip", "qr.hip", "common.h", "la.h")
+
+
+def kernel_source_hash():
+    """sha256 of the sources of the timed kernels (the GEMM and tile-QR files)."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "kernelmatrices.jl_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (a line 'traffic_bytes_per_launch <kernel> <bytes>'), or None."""
+    summary (a line 'traffic_bytes_per_launch <kernel> <bytes>'), when the
+    summary's 'source_sha256' line matches the current kernel sources;
+    (None, reason) otherwise."""
+    stamp, val = None, None
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
                 parts = line.split()
+                if len(parts) == 2 and parts[0] == "source_sha256":
+                    stamp = parts[1]
                 if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
-                    return int(float(parts[2]))
+                    val = int(float(parts[2]))
     except OSError:
-        pass
-    return None
+        return None, "no committed rocprofv3 --pmc summary"
+    if val is None:
+        return None, "kernel not in the committed summary"
+    if stamp != kernel_source_hash():
+        return None, "stale: the kernel sources changed since the committed rocprofv3 --pmc run"
+    return val, "rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel (%s)" % os.path.basename(TRAFFIC_FILE)
 
 
 def make_inputs(n, seed=0):
@@ -91,47 +113,158 @@ def factor_flops(n, m, ranks, nblocks):
     return f
 
 
-def cpu_baseline(n, lvl, params):
-    """The oracle (numpy/scipy restatement of the reference) on the host,
-    one BLAS thread, one evaluation."""
+def cpu_baseline(n, lvl, params, timed=3):
+    """The oracle (numpy/scipy restatement of the reference's op sequence)
+    timed on the host at the bench's own workload: one negloglik eval
+    (build + symmetricfactorize! + logdet + y'K^-1 y) on 1 BLAS thread (the
+    reference's BLAS.set_num_threads(1), test/runtests.jl:9) and on all the
+    host threads the job may use; median of `timed` evals after one
+    warm-up each."""
+    import statistics
     from threadpoolctl import threadpool_limits
     from oracle import hodlr as H
     from oracle import kernels as kk
     x, y = make_inputs(n)
-    with threadpool_limits(1):
+
+    def one():
         t0 = time.perf_counter()
         K = H.KernelMatrix(x, x, kk.Kernel(kk.MATERN32, params, nugget=1e-6))
         HK = H.KernelHODLR(K, 0.0, 72, H.FixedLevel(lvl), nystrom=True)
         H.symmetricfactorize(HK)
         nll = H.negloglik(HK, y)
-        dt = time.perf_counter() - t0
-    return dt, nll
+        return time.perf_counter() - t0, nll
+
+    nthreads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+    res = {}
+    for th in (1, nthreads):
+        with threadpool_limits(th):
+            one()
+            ts = [one()[0] for _ in range(timed)]
+        res[th] = statistics.median(ts)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return res, nthreads, model
 
 
-GEMM_SHAPES = [  # (ta, tb, m, n, k): cascade update, projections, long-k split, TSQR tiles
-    (0, 0, 65536, 144, 72), (1, 0, 72, 144, 256), (1, 0, 72, 144, 32768), (0, 0, 256, 32, 32),
-    (1, 0, 32, 32, 256), (0, 0, 4096, 144, 72), (0, 0, 65536, 64, 72)]
-
-
-def gemm_shapes():
+def poisson_leg(args, lib, L, ctx, kernel_of, check):
+    """Open-loop serving of likelihood + gradient requests arriving as a
+    Poisson stream: n log-uniform in [4096, 262144] (FixedLevel so leaves
+    hold about 256 points), sigma ~ U[0.5, 2], rho ~ U[0.02, 0.3],
+    Matern-3/2, Nystrom rank 72, y ~ N(0,1), 16 SAA vectors, a 200 GB HBM
+    budget for the in-flight requests.  Inputs are generated and put in HBM
+    before the stream starts.  Reports served evals/s over the stream,
+    p50/p99 latency and queue time, beside the plain per-request FP64 path
+    over the same requests."""
     import numpy as np
+    import torch
     import kmhodlr as km
-    lib = km.load()
-    ctx = km.Context(0)
-    rng = np.random.default_rng(0)
-    for ta, tb, m, n, k in GEMM_SHAPES:
-        A = np.asfortranarray(rng.random((k, m) if ta else (m, k)))
-        B = np.asfortranarray(rng.random((n, k) if tb else (k, n)))
-        Cm = np.asfortranarray(rng.random((m, n)))
-        ref = (A.T if ta else A) @ (B.T if tb else B)
-        C0 = Cm.copy(order="F")
-        for rep in range(3):
-            rc = lib.kmh_test_gemm(ctx.ptr, ta, tb, m, n, k, A.ctypes.data, A.shape[0], B.ctypes.data, B.shape[0],
-                                   Cm.ctypes.data, m, 1.0, 1.0, 0)
-            if rc != 0:
-                raise RuntimeError(lib.last_error())
-        err = np.abs(Cm - (C0 + 3 * ref)).max() / np.abs(C0 + 3 * ref).max()
-        print("gemm shape ta=%d tb=%d m=%d n=%d k=%d: 3 launches, rel err %.1e" % (ta, tb, m, n, k, err), flush=True)
+    rng = np.random.default_rng(9)
+    count, nv, rate = args.poisson_requests, 16, args.poisson_rate
+    dev = torch.device("cuda:0")
+    reqs = []
+    for i in range(count):
+        nn = int(round(math.exp(rng.uniform(math.log(4096), math.log(262144)))))
+        x = rng.random((nn, 2))
+        x = np.ascontiguousarray(x[km.kd_order(x)])
+        y = rng.standard_normal(nn)
+        saa = rng.choice([-1.0, 1.0], size=(nv, nn))       # row j = vector j: column-major n x nv
+        sig, rho = rng.uniform(0.5, 2.0), rng.uniform(0.02, 0.3)
+        lvl = max(1, int(math.floor(math.log2(nn / 256))))
+        reqs.append({"n": nn, "x": torch.from_numpy(x).to(dev), "y": torch.from_numpy(y).to(dev),
+                     "saa": torch.from_numpy(saa).to(dev), "p": (float(sig * sig), float(rho)), "lvl": lvl})
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    torch.cuda.synchronize()
+
+    def make(q):
+        r = L.kmh_serve_req()
+        r.pts = q["x"].data_ptr()
+        r.n = q["n"]
+        r.d = 2
+        r.on_device = 1
+        r.kernel = kernel_of(q["p"])
+        r.opts.method = L.KMH_NYSTROM
+        r.opts.level_kind = L.KMH_FIXED_LEVEL
+        r.opts.level = q["lvl"]
+        r.opts.maxrank = 72
+        r.opts.precision = L.KMH_FP64
+        r.y = q["y"].data_ptr()
+        r.ndk = 2
+        r.saa = q["saa"].data_ptr()
+        r.nv = nv
+        r.dks[0] = kernel_of(q["p"], L.KMH_D1)
+        r.dks[1] = kernel_of(q["p"], L.KMH_D2)
+        return r
+
+    srv = C.c_void_p()
+    check(lib.kmh_server_create(ctx.ptr, args.inflight, 200 * (1 << 30), C.byref(srv)))
+    keep, live, res = [], {}, {}
+    t0 = time.perf_counter()
+    nxt = 0
+    while nxt < count or live:
+        now = time.perf_counter() - t0
+        while nxt < count and arrivals[nxt] <= now:
+            r = make(reqs[nxt])
+            keep.append(r)
+            rid = C.c_int64()
+            check(lib.kmh_server_submit(srv, C.byref(r), C.byref(rid)))
+            live[rid.value] = nxt
+            nxt += 1
+        if not live:
+            time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
+            continue
+        done = C.c_int32()
+        check(lib.kmh_server_step(srv, C.byref(done)))
+        if done.value:
+            for rid in list(live):
+                o = L.kmh_serve_result()
+                rc = lib.kmh_server_result(srv, rid, C.byref(o))
+                if rc == L.KMH_E_STATE:
+                    continue
+                check(rc)
+                if o.status != 0:
+                    raise RuntimeError("request failed: %s" % lib.last_error())
+                res[live.pop(rid)] = (o.nll, o.latency_us * 1e-3, o.queue_us * 1e-3)
+    el = time.perf_counter() - t0
+    lib.kmh_server_destroy(srv)
+    lat = [v[1] for v in res.values()]
+    qt = [v[2] for v in res.values()]
+    # the plain per-request FP64 path over the same requests
+    t1 = time.perf_counter()
+    plain_err = 0.0
+    for i, q in enumerate(reqs):
+        h = C.c_void_p()
+        kern = kernel_of(q["p"])
+        o = L.kmh_build_opts()
+        o.method, o.level_kind, o.level, o.maxrank, o.precision = L.KMH_NYSTROM, L.KMH_FIXED_LEVEL, q["lvl"], 72, 0
+        check(lib.kmh_hodlr_build_dev(ctx.ptr, q["x"].data_ptr(), q["n"], 2, C.byref(kern), C.byref(o), C.byref(h)))
+        check(lib.kmh_hodlr_factorize(h))
+        ld = C.c_double()
+        check(lib.kmh_hodlr_logdet(h, C.byref(ld)))
+        z = torch.empty_like(q["y"])
+        check(lib.kmh_hodlr_factor_apply_dev(h, L.KMH_WINV, q["y"].data_ptr(), z.data_ptr(), 1))
+        check(lib.kmh_ctx_synchronize(ctx.ptr))
+        nll = 0.5 * ld.value + 0.5 * float(torch.dot(z, z))
+        dks = (L.kmh_kernel * 2)(kernel_of(q["p"], L.KMH_D1), kernel_of(q["p"], L.KMH_D2))
+        g = np.zeros(2)
+        check(lib.kmh_stoch_gradient_dev(h, dks, 2, q["y"].data_ptr(), q["saa"].data_ptr(), nv, g.ctypes.data))
+        check(lib.kmh_hodlr_destroy(h))
+        plain_err = max(plain_err, abs(nll - res[i][0]) / abs(nll))
+    pel = time.perf_counter() - t1
+    ns = [q["n"] for q in reqs]
+    return {"requests": count, "arrival_rate_per_s": rate, "evals_per_s": count / el, "p50_ms": pct(lat, 0.5),
+            "p99_ms": pct(lat, 0.99), "queue_p50_ms": pct(qt, 0.5), "queue_p99_ms": pct(qt, 0.99),
+            "n_min": min(ns), "n_max": max(ns), "n_mean": float(np.mean(ns)), "saa_vectors": nv,
+            "hbm_budget_bytes": 200 * (1 << 30), "plain_evals_per_s": count / pel,
+            "max_rel_nll_err_served_vs_plain": plain_err,
+            "note": "open loop: a request is submitted at its arrival time; evals/s over the whole stream "
+                    "(arrival span included)"}
 
 
 def pct(v, q):
@@ -148,16 +281,12 @@ def main():
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--inflight", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--legs", default="plain,served,plain_grad,served_grad,roofline")
+    ap.add_argument("--legs", default="plain,served,plain_grad,served_grad,served_mixed,roofline,aca,poisson")
     ap.add_argument("--lanes", type=int, default=0, help="server lanes (0: the library's default)")
     ap.add_argument("--grad-vecs", type=int, default=32, help="SAA vectors of the gradient legs")
-    ap.add_argument("--gemm-shapes", action="store_true",
-                    help="time the grouped GEMM alone on the skinny shapes of the factorization "
-                         "(one problem per launch, 3 launches each; read kernel times with rocprofv3)")
+    ap.add_argument("--poisson-requests", type=int, default=48)
+    ap.add_argument("--poisson-rate", type=float, default=60.0, help="arrivals per second of the Poisson leg")
     args = ap.parse_args()
-    if args.gemm_shapes:
-        gemm_shapes()
-        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(independent GP requests shard across GPUs without collectives)", file=sys.stderr)
@@ -272,7 +401,7 @@ def main():
         out_["nn_wide_tflops"] = fl.value / (ms.value * 1e-3) / 1e12 if ms.value else 0.0
         return out_
 
-    plain_ld = {}
+    plain_ld, plain_nll = {}, {}
     if "plain" in legs:
         for i in range(args.warmup):
             plain_eval(params[i])
@@ -285,6 +414,7 @@ def main():
             nll, ld = plain_eval(params[i])
             lat.append(1e3 * (time.perf_counter() - s))
             plain_ld[i] = ld
+            plain_nll[i] = nll
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         out["plain"] = {"evals_per_s": args.steps / el, "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99)}
@@ -314,6 +444,7 @@ def main():
         reqs = []
 
         grad_mode = [False]
+        prec_mode = [L.KMH_FP64]
 
         def submit(i):
             r = L.kmh_serve_req()
@@ -324,6 +455,7 @@ def main():
             r.on_device = 1
             r.kernel = kernel_of(params[i])
             r.opts = opts
+            r.opts.precision = prec_mode[0]
             r.y = y_d.data_ptr()
             if grad_mode[0]:
                 r.ndk = 2
@@ -374,6 +506,23 @@ def main():
             if plain_ld:
                 errs = [abs(res[i][1] - plain_ld[i]) / abs(plain_ld[i]) for i in plain_ld]
                 out["served"]["max_rel_logdet_err_vs_plain"] = max(errs)
+        if "served_mixed" in legs:
+            # the same requests in the mixed form KMH_MIXED_Q (FP32 panel QR and
+            # Q factors, FP64 cascade, leaves, capacitance and logdet; y'K^-1 y
+            # from the solve refined against the FP64 operator)
+            prec_mode[0] = L.KMH_MIXED_Q
+            run(range(args.warmup), False)
+            torch.cuda.synchronize()
+            res, el, _ = run(range(args.warmup, total), False)
+            prec_mode[0] = L.KMH_FP64
+            lat = [v[2] for v in res.values()]
+            out["served_mixed"] = {"evals_per_s": args.steps / el, "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99),
+                                   "inflight": args.inflight, "precision": "KMH_MIXED_Q"}
+            if plain_ld:
+                errs = [abs(res[i][1] - plain_ld[i]) / abs(plain_ld[i]) for i in plain_ld]
+                nerr = [abs(res[i][0] - plain_nll[i]) / abs(plain_nll[i]) for i in plain_nll]
+                out["served_mixed"]["max_rel_logdet_err_vs_plain_fp64"] = max(errs)
+                out["served_mixed"]["max_rel_nll_err_vs_plain_fp64"] = max(nerr)
         if "served_grad" in legs:
             grad_mode[0] = True
             run(range(args.warmup), False)
@@ -403,15 +552,57 @@ def main():
             out["roofline_pass"].update(prof)
         lib.kmh_server_destroy(srv)
 
+    # ---- the same workload compressed by ACA at tol 1e-10 ---------------
+    if "aca" in legs:
+        aopts = L.kmh_build_opts()
+        aopts.method = L.KMH_ACA
+        aopts.level_kind = L.KMH_FIXED_LEVEL
+        aopts.level = args.level
+        aopts.maxrank = 0
+        aopts.tol = 1e-10
+        aopts.precision = L.KMH_FP64
+        kern = kernel_of((1.0, 0.1))          # sigma 1, rho 0.1
+        times = []
+        for rep_ in range(3):                 # one warm-up, two timed
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            h = C.c_void_p()
+            check(lib.kmh_hodlr_build_dev(ctx.ptr, pts_d.data_ptr(), n, 2, C.byref(kern), C.byref(aopts), C.byref(h)))
+            check(lib.kmh_ctx_synchronize(ctx.ptr))
+            t1 = time.perf_counter()
+            check(lib.kmh_hodlr_factorize(h))
+            ld = C.c_double()
+            check(lib.kmh_hodlr_logdet(h, C.byref(ld)))
+            check(lib.kmh_hodlr_solve_dev(h, y_d.data_ptr(), sol_d.data_ptr(), 1))
+            check(lib.kmh_ctx_synchronize(ctx.ptr))
+            t2 = time.perf_counter()
+            ii = L.kmh_info()
+            check(lib.kmh_hodlr_info(h, C.byref(ii)))
+            nb = 2 ** ii.levels - 1
+            rk = (C.c_int32 * nb)()
+            check(lib.kmh_hodlr_ranks(h, rk))
+            check(lib.kmh_hodlr_destroy(h))
+            if rep_:
+                times.append((t1 - t0, t2 - t1))
+        bt = min(t[0] for t in times)
+        ft = min(t[1] for t in times)
+        out["aca"] = {"evals_per_s": 1.0 / (bt + ft), "build_s": bt, "factorize_logdet_solve_s": ft,
+                      "tol": 1e-10, "max_rank": max(rk), "top_level_rank": rk[nb - 1], "logdet": ld.value,
+                      "note": "sigma 1, rho 0.1, best of two timed runs after one warm-up"}
+
+    # ---- open-loop Poisson serving (n log-uniform in [4096, 262144]) -----
+    if "poisson" in legs:
+        out["poisson"] = poisson_leg(args, lib, L, ctx, kernel_of, check)
+
     head = (out.get("served") or out.get("plain") or out.get("served_grad") or out.get("plain_grad")
-            or out.get("roofline_pass"))
+            or out.get("roofline_pass") or out.get("served_mixed") or out.get("aca") or out.get("poisson"))
 
     KERNELS = {
-        "tile_qr": ("k_tile_qr<24>",
+        "tile_qr": ("k_tile_qr<24, double>",
                     "Householder QR of <= 128 x 24 tiles with Q left in compact-WY form (TSQR leaves and "
                     "inner nodes), one wave per tile; bytes = tiles read, Householder vectors and R written; "
                     "bound by LDS traffic and dependent latency, so far below its HBM roofline"),
-        "nn_wide": ("k_gemm_nn_wide", "skinny FP64 GEMM on v_mfma_f64_16x16x4f64 (Woodbury updates, leaf-solve "
+        "nn_wide": ("k_gemm_nn_wide<0>", "skinny FP64 GEMM on v_mfma_f64_16x16x4f64 (Woodbury updates, leaf-solve "
                                       "updates, Nystrom product); bytes = A, B, C written, C read when beta != 0"),
     }
 
@@ -444,7 +635,7 @@ def main():
                     "algorithmic_bytes_per_launch": round(h[k + "_bytes_per_launch"]),
                     "us_per_launch": round(h[k + "_us_per_launch"], 2), "kernel_ms": round(h[k + "_ms"], 3),
                     "timed_launches": h[k + "_timed_launches"], "measured_in": where,
-                    "traffic": measured_traffic(name)}
+                    "traffic": measured_traffic(name)[0], "traffic_source": measured_traffic(name)[1]}
         keys.sort(key=lambda k: -h[k + "_ms"])
         r = one(keys[0])
         if len(keys) > 1:
@@ -466,8 +657,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(1e3 / head["evals_per_s"], 3),
-        "p50_ms": round(head["p50_ms"], 3),
-        "p99_ms": round(head["p99_ms"], 3),
+        "p50_ms": round(head["p50_ms"], 3) if "p50_ms" in head else None,
+        "p99_ms": round(head["p99_ms"], 3) if "p99_ms" in head else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -478,7 +669,7 @@ def main():
                                % ("served+batched FP64" if "served" in out else "plain FP64"),
                    "n": n, "leaf": m, "nystrom_rank": 72, "levels": L_, "inflight": args.inflight,
                    "server_lanes": args.lanes or (2 if args.inflight >= 2 else 1)},
-        "legs": {k: {kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in d.items()}
+        "legs": {k: {kk: (float("%.5g" % v) if isinstance(v, float) else v) for kk, v in d.items()}
                  for k, d in out.items()},
         "roofline": roofline(head),
     }
@@ -488,17 +679,16 @@ def main():
         line["speedup_served_vs_plain_with_gradient"] = round(
             out["served_grad"]["evals_per_s"] / out["plain_grad"]["evals_per_s"], 3)
     if not args.no_cpu:
-        # bounded CPU sample: one eval of the same workload shape at n=16384
-        # (FixedLevel(6), same 256-point leaves), scaled to n=65536 by the
-        # ratio of structured factorization flops
-        ns, ls = 16384, args.level - 2
-        dt, _ = cpu_baseline(ns, ls, params[0])
-        ratio = ff / factor_flops(ns, ns // 2 ** ls, [72] * ls, [2 ** (ls - l - 1) for l in range(ls)])
-        line["cpu_baseline"] = {"value": round(1.0 / (dt * ratio), 6), "unit": "evals/s", "cores": 1,
+        # the oracle at the same n = 65536 workload: 1 thread and all threads
+        res, nth, model = cpu_baseline(n, args.level, params[args.warmup])
+        line["cpu_baseline"] = {"value": round(1.0 / res[nth], 6), "unit": "evals/s", "cores": nth,
                                 "kind": "port",
-                                "sample": "oracle (numpy/scipy restatement, 1 BLAS thread) one eval at n=%d "
-                                          "FixedLevel(%d) took %.2f s; scaled by structured-flop ratio %.2f to n=%d"
-                                          % (ns, ls, dt, ratio, n)}
+                                "value_1_core": round(1.0 / res[1], 6),
+                                "sample": "oracle (numpy/scipy restatement of the reference's op sequence), one "
+                                          "negloglik eval at n=%d FixedLevel(%d) Nystrom 72 per run, median of 3 "
+                                          "after 1 warm-up: %.2f s on %d threads, %.2f s on 1 thread; host %s, %d "
+                                          "threads available" % (n, args.level, res[nth], nth, res[1], model,
+                                                                 len(os.sched_getaffinity(0)))}
     print(json.dumps(line))
 
 

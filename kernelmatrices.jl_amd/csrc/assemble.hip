@@ -78,14 +78,15 @@ __device__ __forceinline__ void bit_set(unsigned* m, int i) { m[i >> 5] |= 1u <<
 //   k_aca_col   v /= pivot (:56) and its squared norm per column chunk; the
 //               residual column at the column pivot (:58-61), its squared
 //               norm and the argmax of |u| over unused rows per row chunk
-//   k_aca_dots  u.U_t and v.V_t for the earlier terms t (:63-65), one wave
-//               per term
+//   k_aca_dots  u.U_t and v.V_t for the earlier terms t (:63-65), one
+//               workgroup per term (four waves over quarters of the vectors)
 //   k_aca_fin   ||Z||_F^2 update (:63-67), rank + 1, stop test, next row
 //               pivot (:43-46)
 // The element arithmetic, pivot tie rule and stop test are the reference's;
 // sums over chunks are added in chunk order, so results are deterministic.
 // ---------------------------------------------------------------------------
-constexpr int ACH = 1024;   // rows or columns per chunk (256 threads x 4)
+constexpr int ACH = 256;    // rows or columns per chunk (one per thread): a
+                            // 32768-row block spreads over 128 workgroups
 constexpr int ACA_COEF = 4096;   // LDS coefficients: the rank limit
 
 struct AcaArg {
@@ -401,30 +402,46 @@ __global__ __launch_bounds__(256) void k_aca_col(AcaState* __restrict__ sts, con
     }
 }
 
-// wave w of the grid: plan entry w / fb, earlier term t = w % fb
+// workgroup g of the grid: plan entry g / fb, earlier term t = g % fb; wave
+// w sums quarter w of the vectors (four interleaved partial sums per lane),
+// and the quarters are added in order
+__device__ __forceinline__ double quarter_dot(const double* __restrict__ x, const double* __restrict__ y, int n,
+                                              int w, int lane) {
+    const int q = (n + 3) >> 2, i0 = w * q, i1 = min(n, i0 + q);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int i = i0 + lane;
+    for (; i + 192 < i1; i += 256) {
+        a0 += x[i] * y[i];
+        a1 += x[i + 64] * y[i + 64];
+        a2 += x[i + 128] * y[i + 128];
+        a3 += x[i + 192] * y[i + 192];
+    }
+    for (; i < i1; i += 64) a0 += x[i] * y[i];
+    double a = (a0 + a1) + (a2 + a3);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+    return a;
+}
+
 __global__ __launch_bounds__(256) void k_aca_dots(AcaState* __restrict__ sts, const AcaPlan* __restrict__ pl,
                                                   int np, int fb) {
-    const int wid = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int p = wid / fb, t = wid - p * fb;
-    if (p >= np) return;
+    __shared__ double sh[2][4];
+    const int g = (int)blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int p = g / fb, t = g - p * fb;
+    if (p >= np) return;   // uniform over the workgroup
     AcaState& s = sts[pl[p].b];
-    if (s.done || t >= s.frnk) return;
+    if (s.done || t >= s.frnk) return;   // uniform over the workgroup
     const size_t ld = (size_t)s.ld;
-    const double* ut = s.U + (size_t)t * ld;
-    const double* vt = s.V + (size_t)t * ld;
-    const double* un = s.U + (size_t)s.frnk * ld;
-    const double* vn = s.V + (size_t)s.frnk * ld;
-    double a = 0.0, c = 0.0;
-    for (int i = lane; i < s.nro; i += 64) a += ut[i] * un[i];
-    for (int j = lane; j < s.ncl; j += 64) c += vt[j] * vn[j];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        a += __shfl_xor(a, off);
-        c += __shfl_xor(c, off);
-    }
+    const double a = quarter_dot(s.U + (size_t)t * ld, s.U + (size_t)s.frnk * ld, s.nro, w, lane);
+    const double c = quarter_dot(s.V + (size_t)t * ld, s.V + (size_t)s.frnk * ld, s.ncl, w, lane);
     if (lane == 0) {
-        s.du[t] = a;
-        s.dv[t] = c;
+        sh[0][w] = a;
+        sh[1][w] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s.du[t] = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+        s.dv[t] = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
     }
 }
 
@@ -568,8 +585,7 @@ void AcaRun::run() {
             KMH_LAUNCH(c, k_aca_row, dim3(cch), dim3(256), m.d_st, d_pl, np);
             KMH_LAUNCH(c, k_aca_piv, dim3(np), dim3(64), m.d_st, d_pl);
             KMH_LAUNCH(c, k_aca_col, dim3(cch + rch), dim3(256), m.d_st, d_pl, np, cch);
-            KMH_LAUNCH(c, k_aca_dots, dim3((unsigned)ceil_div((int64_t)np * fb, 4)), dim3(256), m.d_st, d_pl, np,
-                       fb);
+            KMH_LAUNCH(c, k_aca_dots, dim3((unsigned)((int64_t)np * fb)), dim3(256), m.d_st, d_pl, np, fb);
             KMH_LAUNCH(c, k_aca_fin, dim3(np), dim3(64), m.d_st, d_pl);
         }
     }

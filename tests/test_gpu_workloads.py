@@ -1,0 +1,264 @@
+"""The batch, gradient/Hessian and serving workloads at their own sizes,
+against the oracle's results for the same inputs (committed by
+tests/golden/make_cfg3.py, make_cfg4.py and make_cfg5.py; the oracle needs
+minutes to an hour there) and through size-independent properties."""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests.devbuf import DevArray
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _golden(name):
+    p = os.path.join(GOLD, name)
+    if not os.path.exists(p):
+        pytest.skip("fixture missing: " + name)
+    with open(p) as f:
+        return json.load(f)
+
+
+def _req(L, x, fam, params, nugget, method, level, rank, tol, precision):
+    r = L.kmh_build_req()
+    r.pts = x.ctypes.data
+    r.n = x.shape[0]
+    r.d = 2
+    r.on_device = 0
+    r.kernel.family = fam
+    r.kernel.params[0], r.kernel.params[1], r.kernel.nugget = params[0], params[1], nugget
+    r.opts.method = method
+    r.opts.maxrank = rank
+    r.opts.tol = tol
+    r.opts.level_kind = L.KMH_FIXED_LEVEL
+    r.opts.level = level
+    r.opts.precision = precision
+    return r
+
+
+def _batch(lib, L, ctx, reqs, allow_not_pd=False):
+    """Batched build + factorization; with allow_not_pd the per-problem
+    statuses are returned beside the handles (KMH_E_NOT_PD allowed)."""
+    arr = (L.kmh_build_req * len(reqs))(*reqs)
+    hs = (C.c_void_p * len(reqs))()
+    st = (C.c_int32 * len(reqs))()
+    assert lib.kmh_hodlr_build_batch(ctx.ptr, len(reqs), arr, hs, st) == 0, lib.last_error()
+    assert all(v == 0 for v in st), [lib.last_error(), list(st)]
+    rc = lib.kmh_hodlr_factorize_batch(ctx.ptr, len(reqs), hs, st)
+    if allow_not_pd:
+        assert all(v in (0, L.KMH_E_NOT_PD) for v in st), list(st)
+        return [C.c_void_p(h) for h in hs], list(st)
+    assert rc == 0 and all(v == 0 for v in st), list(st)
+    return [C.c_void_p(h) for h in hs]
+
+
+def _ranks(lib, h, level):
+    r = (C.c_int32 * (2 ** level - 1))()
+    assert lib.kmh_hodlr_ranks(h, r) == 0
+    out, q = [], 0
+    for j in range(1, level + 1):
+        m = 2 ** (level - j)
+        out.append(list(r[q:q + m]))
+        q += m
+    return out
+
+
+def _logdet(lib, h):
+    ld = C.c_double()
+    assert lib.kmh_hodlr_logdet(h, C.byref(ld)) == 0
+    return ld.value
+
+
+def test_batch64_aca_mixed_with_refinement(gpu_lib):
+    """64 problems of uneven n (log-uniform in [4096, 65536]), Matern-1/2,
+    ACA tol 1e-6, mixed form (FP32 ACA panels and Q factors, FP64 leaves and
+    logdet), 4 right-hand sides each: one batched build, one batched
+    factorization, one batched refined solve.  At tol 1e-6 the compression
+    error of the largest blocks exceeds the 1e-6 nugget, and seven of the 64
+    HODLR matrices are not positive definite: their factorization fails
+    with KMH_E_NOT_PD (the reference's PosDefException) alone, the same
+    seven in the FP64 form, and the other 57 proceed.  Every solve reaches
+    a residual <= 1e-9 against its FP64 HODLR operator; the three smallest
+    problems match the oracle (ranks equal; FP64 form's logdet to 1e-10 and
+    solve to 1e-8; the mixed form's logdet to 1e-7); two problems give
+    bitwise the same results alone as in the batch."""
+    import kmhodlr as km
+    from kmhodlr import _lib as L
+    from tests.golden.make_cfg3 import NRHS, NUGGET, TOL, problems
+    g = _golden("cfg3_batch64.json")
+    lib = km.load()
+    ctx = km.Context(0)
+    probs = problems(g["count"], g["seed"])
+    print("batch of %d problems, n in [%d, %d], total %d points" % (
+        len(probs), min(p[0].shape[0] for p in probs), max(p[0].shape[0] for p in probs),
+        sum(p[0].shape[0] for p in probs)), flush=True)
+    xs = [np.ascontiguousarray(p[0]) for p in probs]
+
+    def reqs(sel, prec):
+        return [_req(L, xs[i], L.KMH_MATERN12, (1.0, probs[i][2]), NUGGET, L.KMH_ACA, probs[i][3], 0, TOL, prec)
+                for i in sel]
+
+    def solve(hs, sel):
+        bs = [DevArray(probs[i][1]) for i in sel]
+        xd = [DevArray(np.zeros_like(probs[i][1])) for i in sel]
+        hp = (C.c_void_p * len(hs))(*[h.value for h in hs])
+        bp = (C.c_void_p * len(hs))(*[b.addr for b in bs])
+        xp = (C.c_void_p * len(hs))(*[x.addr for x in xd])
+        nr = (C.c_int64 * len(hs))(*([NRHS] * len(hs)))
+        assert lib.kmh_hodlr_solve_batch_dev(ctx.ptr, len(hs), hp, bp, xp, nr) == 0, lib.last_error()
+        assert lib.kmh_ctx_synchronize(ctx.ptr) == 0
+        return [x.get() for x in xd]
+
+    allsel = list(range(len(probs)))
+    hs, st = _batch(lib, L, ctx, reqs(allsel, L.KMH_MIXED), allow_not_pd=True)
+    h64all, st64 = _batch(lib, L, ctx, reqs(allsel, L.KMH_FP64), allow_not_pd=True)
+    for h in h64all:
+        assert lib.kmh_hodlr_destroy(h) == 0
+    failed = [i for i in allsel if st[i] != 0]
+    print("not positive definite at tol 1e-6: %s" % [(i, probs[i][0].shape[0]) for i in failed], flush=True)
+    assert failed == [i for i in allsel if st64[i] != 0]
+    assert len(failed) < len(probs) // 4
+    okset = [i for i in allsel if st[i] == 0]
+    sol = dict(zip(okset, solve([hs[i] for i in okset], okset)))
+    worst = 0.0
+    for i in okset:
+        h = hs[i]
+        b = probs[i][1]
+        kx = np.zeros_like(b)
+        assert lib.kmh_hodlr_matvec(h, sol[i].ctypes.data, kx.ctypes.data, NRHS) == 0
+        rel = (np.linalg.norm(b - kx, axis=0) / np.linalg.norm(b, axis=0)).max()
+        worst = max(worst, rel)
+        assert rel <= 1e-9, (i, rel)
+    print("worst refined residual over the %d factorized problems: %.2e" % (len(okset), worst))
+    # oracle: FP64 form and mixed form of the three smallest problems
+    small = sorted(int(k) for k in g["oracle"])
+    h64 = _batch(lib, L, ctx, reqs(small, L.KMH_FP64))
+    s64 = solve(h64, small)
+    for j, i in enumerate(small):
+        o = g["oracle"][str(i)]
+        level = probs[i][3]
+        assert _ranks(lib, h64[j], level) == o["ranks"]
+        assert _ranks(lib, hs[i], level) == o["ranks"]
+        ld64, ld32 = _logdet(lib, h64[j]), _logdet(lib, hs[i])
+        assert abs(ld64 - o["logdet"]) <= 1e-10 * abs(o["logdet"])
+        print("problem %d n=%d: mixed logdet rel err vs oracle %.2e" % (i, o["n"], abs(ld32 - o["logdet"]) / abs(o["logdet"])))
+        assert abs(ld32 - o["logdet"]) <= 1e-7 * abs(o["logdet"])
+        s = s64[j][:, 0]
+        assert abs(np.linalg.norm(s) - o["solve_norm"]) <= 1e-8 * o["solve_norm"]
+        assert abs(np.dot(s, probs[i][1][:, 0]) - o["solve_dot_b"]) <= 1e-8 * abs(o["solve_dot_b"])
+    for h in h64:
+        assert lib.kmh_hodlr_destroy(h) == 0
+    # alone == in the batch
+    for i in (small[0], okset[-1]):
+        (h1,) = _batch(lib, L, ctx, reqs([i], L.KMH_MIXED))
+        (s1,) = solve([h1], [i])
+        assert _logdet(lib, h1) == _logdet(lib, hs[i]) and np.array_equal(s1, sol[i])
+        assert lib.kmh_hodlr_destroy(h1) == 0
+    for h in hs:
+        assert lib.kmh_hodlr_destroy(h) == 0
+
+
+def test_gradient_fisher_hessian_n131072(gpu_lib):
+    """n = 131072 Matern-3/2 (t0 1, rho 0.08, nugget 1e-6), Nystrom rank
+    72, FixedLevel(9), 64 Rademacher vectors: logdet and negloglik to 1e-10,
+    the stochastic gradient to 1e-8, Fisher and Hessian (the reference's
+    second-derivative terms) to 1e-8 of their largest entry, all against the
+    oracle with the same vectors.  The mixed form's gradient is printed
+    beside the FP64 one and held to 1e-4 (its trace terms use the FP32
+    factor)."""
+    import kmhodlr as km
+    from tests.golden.make_cfg4 import inputs
+    g = _golden("cfg4_n131072.json")
+    lib = km.load()
+    x, y, saa = inputs(g["n"], g["nvec"])
+    K = km.KernelMatrix(x, x, tuple(g["params"]), km.matern32(g["nugget"]))
+    res = {}
+    for mixed in (False, "q"):
+        print("n=%d mixed=%s" % (g["n"], mixed), flush=True)
+        HK = km.KernelHODLR(K, 0.0, g["rank"], km.FixedLevel(g["level"]), nystrom=True, mixed=mixed)
+        km.symmetricfactorize(HK)
+        d1 = [K.kernel.d1, K.kernel.d2]
+        grad = km.stoch_gradient(K, HK, y, d1, saa)
+        if not mixed:
+            ld = HK.logdet()
+            assert abs(ld - g["logdet"]) <= 1e-10 * abs(g["logdet"])
+            nll = km.negloglik(HK, y)
+            assert abs(nll - g["negloglik"]) <= 1e-10 * abs(g["negloglik"])
+            F = km.stoch_fisher(K, HK, d1, saa)
+            Hs = km.stoch_hessian(K, HK, y, d1, [[None, K.kernel.d12], [K.kernel.d22]], saa)
+            Fo, Ho = np.asarray(g["fisher"]), np.asarray(g["hessian_reference_terms"])
+            assert np.abs(F - Fo).max() <= 1e-8 * np.abs(Fo).max()
+            assert np.abs(Hs - Ho).max() <= 1e-8 * np.abs(Ho).max()
+        res[mixed] = np.asarray(grad)
+        HK.close()
+    go = np.asarray(g["gradient"])
+    print("gradient fp64 %s oracle %s mixed %s" % (res[False], go, res["q"]))
+    assert np.allclose(res[False], go, rtol=1e-8, atol=0)
+    assert np.allclose(res["q"], go, rtol=1e-4, atol=0)
+
+
+def test_serving_request_n262144(gpu_lib):
+    """The largest serving request (n = 2^18, FixedLevel(10)) through the
+    server with its gradient (8 vectors) and a resident factor: nll and
+    logdet within 1e-10 of the oracle, the gradient within 1e-8, and a
+    follow-up solve on the resident factor with backward error
+    ||W W^T x - b|| / ||x|| <= 1e-12."""
+    import kmhodlr as km
+    from kmhodlr import _lib as L
+    from tests.golden.make_cfg5 import inputs
+    g = _golden("cfg5_n262144.json")
+    lib = km.load()
+    ctx = km.Context(0)
+    x, y, saa = inputs()
+    x = np.ascontiguousarray(x)
+    srv = C.c_void_p()
+    assert lib.kmh_server_create(ctx.ptr, 4, 200 * (1 << 30), C.byref(srv)) == 0
+    r = L.kmh_serve_req()
+    r.pts = x.ctypes.data
+    r.n = g["n"]
+    r.d = 2
+    r.kernel.family = L.KMH_MATERN32
+    r.kernel.params[0], r.kernel.params[1], r.kernel.nugget = g["params"][0], g["params"][1], g["nugget"]
+    r.opts.method = L.KMH_NYSTROM
+    r.opts.maxrank = g["rank"]
+    r.opts.level_kind = L.KMH_FIXED_LEVEL
+    r.opts.level = g["level"]
+    r.opts.precision = L.KMH_FP64
+    r.y = y.ctypes.data
+    r.ndk = 2
+    r.saa = saa.ctypes.data
+    r.nv = g["nvec"]
+    r.keep_factor = 1
+    for j, code in enumerate((L.KMH_D1, L.KMH_D2)):
+        r.dks[j].family = L.KMH_MATERN32
+        r.dks[j].deriv = code
+        r.dks[j].params[0], r.dks[j].params[1] = g["params"]
+    rid = C.c_int64()
+    assert lib.kmh_server_submit(srv, C.byref(r), C.byref(rid)) == 0, lib.last_error()
+    done = C.c_int32()
+    for _ in range(100):
+        assert lib.kmh_server_step(srv, C.byref(done)) == 0, lib.last_error()
+        if done.value:
+            break
+    o = L.kmh_serve_result()
+    assert lib.kmh_server_result(srv, rid.value, C.byref(o)) == 0 and o.status == 0, lib.last_error()
+    print("n=%d served: nll %.12e (oracle %.12e), latency %.1f ms" % (g["n"], o.nll, g["negloglik"],
+                                                                      o.latency_us * 1e-3))
+    assert abs(o.logdet - g["logdet"]) <= 1e-10 * abs(g["logdet"])
+    assert abs(o.nll - g["negloglik"]) <= 1e-10 * abs(g["negloglik"])
+    assert np.allclose([o.grad[0], o.grad[1]], g["gradient"], rtol=1e-8, atol=0)
+    h = C.c_void_p()
+    assert lib.kmh_server_factor(srv, rid.value, C.byref(h)) == 0
+    b = np.random.default_rng(8).standard_normal(g["n"])
+    s = np.zeros_like(b)
+    assert lib.kmh_hodlr_solve(h, b.ctypes.data, s.ctypes.data, 1) == 0
+    ws = np.zeros_like(b)
+    assert lib.kmh_hodlr_matvec(h, s.ctypes.data, ws.ctypes.data, 1) == 0
+    assert np.linalg.norm(ws - b) <= 1e-12 * np.linalg.norm(s)
+    assert lib.kmh_server_release(srv, rid.value) == 0
+    assert lib.kmh_server_destroy(srv) == 0
