@@ -281,7 +281,7 @@ def main():
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--inflight", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--legs", default="plain,served,plain_grad,served_grad,served_mixed,roofline,aca,poisson")
+    ap.add_argument("--legs", default="plain,served,plain_grad,served_grad,served_mixed,roofline,aca,followup,poisson")
     ap.add_argument("--lanes", type=int, default=0, help="server lanes (0: the library's default)")
     ap.add_argument("--grad-vecs", type=int, default=32, help="SAA vectors of the gradient legs")
     ap.add_argument("--poisson-requests", type=int, default=48)
@@ -590,12 +590,44 @@ def main():
                       "tol": 1e-10, "max_rank": max(rk), "top_level_rank": rk[nb - 1], "logdet": ld.value,
                       "note": "sigma 1, rho 0.1, best of two timed runs after one warm-up"}
 
+    # ---- follow-up solves on a resident factor: eager vs captured graphs --
+    if "followup" in legs:
+        h = C.c_void_p()
+        kern = kernel_of(params[0])
+        check(lib.kmh_hodlr_build_dev(ctx.ptr, pts_d.data_ptr(), n, 2, C.byref(kern), C.byref(opts), C.byref(h)))
+        check(lib.kmh_hodlr_factorize(h))
+        fo = {}
+        for mode, en in (("eager", 0), ("graph", 1)):
+            check(lib.kmh_hodlr_solve_graphs(h, en, None, None))
+            for _ in range(3):
+                check(lib.kmh_hodlr_solve_dev(h, y_d.data_ptr(), sol_d.data_ptr(), 1))
+            check(lib.kmh_ctx_synchronize(ctx.ptr))
+            lat = []
+            t0 = time.perf_counter()
+            for _ in range(50):
+                s_ = time.perf_counter()
+                check(lib.kmh_hodlr_solve_dev(h, y_d.data_ptr(), sol_d.data_ptr(), 1))
+                check(lib.kmh_ctx_synchronize(ctx.ptr))
+                lat.append(1e3 * (time.perf_counter() - s_))
+            fo[mode] = {"solves_per_s": 50 / (time.perf_counter() - t0), "p50_ms": pct(lat, 0.5),
+                        "p99_ms": pct(lat, 0.99)}
+        cap, rep_ = C.c_int64(), C.c_int64()
+        check(lib.kmh_hodlr_solve_graphs(h, -1, C.byref(cap), C.byref(rep_)))
+        check(lib.kmh_hodlr_destroy(h))
+        out["followup_solve"] = {"eager_solves_per_s": fo["eager"]["solves_per_s"],
+                                 "eager_p50_ms": fo["eager"]["p50_ms"],
+                                 "graph_solves_per_s": fo["graph"]["solves_per_s"],
+                                 "graph_p50_ms": fo["graph"]["p50_ms"], "graph_p99_ms": fo["graph"]["p99_ms"],
+                                 "graph_captures": cap.value, "graph_replays": rep_.value,
+                                 "note": "K^-1 y on a resident FP64 factor, synchronized per solve"}
+
     # ---- open-loop Poisson serving (n log-uniform in [4096, 262144]) -----
     if "poisson" in legs:
         out["poisson"] = poisson_leg(args, lib, L, ctx, kernel_of, check)
 
     head = (out.get("served") or out.get("plain") or out.get("served_grad") or out.get("plain_grad")
-            or out.get("roofline_pass") or out.get("served_mixed") or out.get("aca") or out.get("poisson"))
+            or out.get("roofline_pass") or out.get("served_mixed") or out.get("aca") or out.get("poisson")
+            or {"evals_per_s": None})
 
     KERNELS = {
         "tile_qr": ("k_tile_qr<24, double>",
@@ -651,12 +683,12 @@ def main():
     ff = factor_flops(n, m, [72] * L_, nblocks)
     line = {
         "metric": "GP loglik evals/s + p99 latency, n=2^16, served vs plain GPU at 1e-8 rel err",
-        "value": round(head["evals_per_s"], 4),
+        "value": round(head["evals_per_s"], 4) if head["evals_per_s"] else None,
         "unit": "evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 / head["evals_per_s"], 3),
+        "ms_per_step": round(1e3 / head["evals_per_s"], 3) if head["evals_per_s"] else None,
         "p50_ms": round(head["p50_ms"], 3) if "p50_ms" in head else None,
         "p99_ms": round(head["p99_ms"], 3) if "p99_ms" in head else None,
         "higher_is_better": True,

@@ -191,6 +191,14 @@ int kmh_hodlr_factor_apply_dev(kmh_hodlr* h, int32_t op, const double* x, double
 /* ---- HK \ b  (HODLR/src/baseoverloads.jl:181-192, 252-263) ---- */
 int kmh_hodlr_solve(kmh_hodlr* h, const double* b, double* x, int64_t nrhs);
 int kmh_hodlr_solve_dev(kmh_hodlr* h, const double* b, double* x, int64_t nrhs);
+/* kmh_hodlr_solve_dev on an FP64 handle replays a hipGraph of the solve for
+ * repeated (b, x, nrhs): the second solve with the same device buffers
+ * captures the launches (descriptors and temporaries in blocks the graph
+ * owns), later ones replay them; results are bitwise those of the uncaptured
+ * solve.  At most 8 graphs per handle, freed with it.  enable: 1 on
+ * (default), 0 off (frees the graphs), other values leave it; returns the
+ * captures and replays so far.  No reference counterpart.                   */
+int kmh_hodlr_solve_graphs(kmh_hodlr* h, int32_t enable, int64_t* captures, int64_t* replays);
 
 /* The same solve without any device allocation.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under

@@ -311,8 +311,10 @@ int kmh_hodlr_solve_dev(kmh_hodlr* h, const double* b, double* x, int64_t nrhs) 
         KMH_CHECK(H.factorized, KMH_E_NOT_FACTORIZED, "No solves without factorization.");
         KMH_CHECK(b && x && nrhs >= 1, KMH_E_BAD_ARG, "invalid vectors");
         kmh::Ctx& c = *H.ctx;
-        if (b != x) kmh::copy2d(c, {{b, (int)H.n, x, (int)H.n, (int)H.n, (int)nrhs, 1.0}});
-        kmh::hodlr_solve(H, x, (int)nrhs);
+        if (!kmh::solve_dev_graph(H, b, x, nrhs)) {
+            if (b != x) kmh::copy2d(c, {{b, (int)H.n, x, (int)H.n, (int)H.n, (int)nrhs, 1.0}});
+            kmh::hodlr_solve(H, x, (int)nrhs);
+        }
     })
 }
 

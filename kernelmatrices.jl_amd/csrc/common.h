@@ -142,6 +142,13 @@ struct Ctx {
         bool freed;
     };
     std::vector<FixedBlock> fixed_stack;
+    // Graph capture of a call's launches: while set, upload() copies the
+    // descriptors into cap_host and hands out addresses in the device block
+    // cap_dev (cap_size bytes), which the caller fills from cap_host once
+    // the capture ends; the arena is not touched.
+    char* cap_dev = nullptr;
+    size_t cap_size = 0;
+    std::vector<char>* cap_host = nullptr;
     size_t live_bytes() const {
         size_t b = 0;
         for (const auto& kv : live) b += kv.second.bytes;

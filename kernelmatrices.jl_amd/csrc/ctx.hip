@@ -53,6 +53,13 @@ void DevBuf::release() {
 // whole ring of uploads earlier, so the wait rarely blocks).
 void* Ctx::upload(const void* host, size_t bytes) {
     const size_t need = (bytes + 255) & ~size_t(255);
+    if (cap_dev) {
+        const size_t off = cap_host->size();
+        KMH_CHECK(off + need <= cap_size, KMH_E_BAD_ARG, "graph capture: descriptor block full");
+        cap_host->resize(off + need);
+        if (bytes) std::memcpy(cap_host->data() + off, host, bytes);
+        return cap_dev + off;
+    }
     const size_t seg = arena_size / ARENA_SEGS;
     if (need > seg) {
         KMH_CHECK(!fixed_base, KMH_E_BAD_ARG,

@@ -16,6 +16,18 @@ struct Block {
     int64_t a0, b1, c0, d1;
 };
 
+// A solve captured as a hipGraph (graph.hip): the launches of K^-1 b -> x
+// for one (b, x, nrhs), with their descriptors and temporaries in blocks of
+// the context's pool owned by the graph.
+struct SolveGraph {
+    const double* b = nullptr;
+    double* x = nullptr;
+    int64_t nrhs = 0;
+    hipGraphExec_t exec = nullptr;
+    void* desc = nullptr;
+    void* tmp = nullptr;
+};
+
 struct Hodlr {
     Ctx* ctx = nullptr;
     int64_t n = 0;
@@ -63,10 +75,20 @@ struct Hodlr {
     double* parts = nullptr;                     // log-diagonal partial sums
     int nparts = 0;
 
+    // captured solves (graph.hip)
+    bool graphs_on = true;
+    int64_t graph_captures = 0, graph_replays = 0;
+    std::vector<SolveGraph> graphs;
+    std::vector<SolveGraph> graph_seen;   // solved once eagerly, not captured yet
+    void release_graphs() noexcept;
+
     Hodlr() = default;
     Hodlr(const Hodlr&) = delete;
     Hodlr& operator=(const Hodlr&) = delete;
-    ~Hodlr() { release_transient(); }
+    ~Hodlr() {
+        release_graphs();
+        release_transient();
+    }
     void release_transient() noexcept {
         if (!ctx) return;
         for (double** p : {&AB, &parts}) {
@@ -122,6 +144,30 @@ struct Hodlr {
                ynys.bytes + X.bytes + G.bytes;
     }
 };
+
+// The temporaries of one solve of H with nrhs columns (an upper bound of the
+// scratch it holds at once), and a scope that puts the context's scratch on
+// a fixed block for one call (noalloc.hip).
+size_t solve_scratch_bound(const Hodlr& H, int64_t nrhs);
+struct FixedScope {
+    Ctx& c;
+    FixedScope(Ctx& cx, char* base, size_t size) : c(cx) {
+        c.fixed_base = base;
+        c.fixed_size = size;
+        c.fixed_stack.clear();
+    }
+    ~FixedScope() {
+        c.fixed_base = nullptr;
+        c.fixed_size = 0;
+        c.fixed_stack.clear();
+    }
+};
+
+// K^-1 b -> x on the device through a captured graph: replays the graph of
+// (b, x, nrhs) if there is one, captures it on the second solve with the same
+// buffers; false when the solve is not graph-eligible (mixed form, profiling,
+// graphs off) or this is the first solve with these buffers.
+bool solve_dev_graph(Hodlr& H, const double* b, double* x, int64_t nrhs);
 
 struct BuildReq {
     const double* pts;

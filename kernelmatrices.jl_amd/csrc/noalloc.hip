@@ -44,6 +44,8 @@ char* fixed_scratch_ptr(int device) {
 
 size_t round256(size_t b) { return std::max<size_t>(256, (b + 255) & ~size_t(255)); }
 
+}  // namespace
+
 // An upper bound of the stack of temporaries a solve with nrhs columns
 // holds at once: per tree level, the Woodbury products S and S2 (2 r nrhs
 // each per block) plus the k-slab partials of the level's products, counted
@@ -68,22 +70,6 @@ size_t solve_scratch_bound(const Hodlr& H, int64_t nrhs) {
     return peak;
 }
 
-// Puts the context's scratch on the fixed array for one call.
-struct FixedScope {
-    Ctx& c;
-    FixedScope(Ctx& cx, char* base, size_t size) : c(cx) {
-        c.fixed_base = base;
-        c.fixed_size = size;
-        c.fixed_stack.clear();
-    }
-    ~FixedScope() {
-        c.fixed_base = nullptr;
-        c.fixed_size = 0;
-        c.fixed_stack.clear();
-    }
-};
-
-}  // namespace
 }  // namespace kmh
 
 extern "C" int kmh_hodlr_solve_dev_noalloc(kmh_hodlr* h, const double* b, double* x, int64_t nrhs) {

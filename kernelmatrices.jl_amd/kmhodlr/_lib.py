@@ -96,6 +96,7 @@ SIGNATURES = {
     "kmh_hodlr_solve": [_P, _P, _P, _I64],
     "kmh_hodlr_solve_dev": [_P, _P, _P, _I64],
     "kmh_hodlr_solve_dev_noalloc": [_P, _P, _P, _I64],
+    "kmh_hodlr_solve_graphs": [_P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "kmh_hodlr_logdet": [_P, _D],
     "kmh_hodlr_refinement": [_P, C.POINTER(C.c_int32), _D],
     "kmh_stoch_gradient": [_P, C.POINTER(kmh_kernel), C.c_int32, _P, _P, _I64, _P],
