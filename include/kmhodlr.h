@@ -152,6 +152,18 @@ int kmh_ctx_profile_query(kmh_ctx* ctx, int32_t tag, double* amount, double* ms,
  * totals of end cover the timed launches only.  Not inside begin..end. */
 int kmh_ctx_profile_config(kmh_ctx* ctx, uint32_t tag_mask, int32_t stride);
 
+/* GEMM autotuner (no reference counterpart).  For each of count shapes
+ * (6 ints each: ta, tb, m, n, k, problems per launch) time every kernel
+ * class that can take the shape (generic MFMA tiles, tall TN, row-per-thread
+ * NN, wide NN), median of reps launches, and keep the fastest in the
+ * context's table keyed by (ta, tb, ceil(log2) of m, n, k); the table is
+ * written to path (text) when path is not null.  tuning_load replaces the
+ * context's table by a file's (null path: clear it; the built-in shape
+ * rules apply to every class the table does not name).                     */
+int kmh_gemm_autotune(kmh_ctx* ctx, int32_t count, const int32_t* shapes, int32_t reps, const char* path);
+int kmh_gemm_tuning_load(kmh_ctx* ctx, const char* path);
+int kmh_gemm_tuning_size(kmh_ctx* ctx, int32_t* entries);
+
 /* ---- HODLR.KernelHODLR(K, ep, maxrank, lvl; nystrom)  constructor.jl:6-58 ----
  * K = KernelMatrix(pts, pts, params, kernel)  src/structstypes.jl:17-23      */
 int kmh_hodlr_build(kmh_ctx* ctx, const double* pts, int64_t n, int32_t d,

@@ -149,6 +149,11 @@ struct Ctx {
     char* cap_dev = nullptr;
     size_t cap_size = 0;
     std::vector<char>* cap_host = nullptr;
+    // GEMM kernel-class table of the autotuner (la.hip): shape class ->
+    // kernel class; gemm_force routes every eligible problem to one class
+    // (the autotuner's own timing runs); 0 = the built-in rules
+    std::unordered_map<uint32_t, uint8_t> gemm_tune;
+    int gemm_force = 0;
     size_t live_bytes() const {
         size_t b = 0;
         for (const auto& kv : live) b += kv.second.bytes;

@@ -702,6 +702,47 @@ static void gemm_rows_nn(Ctx& c, const std::vector<GemmProb>& probs, int tc) {
     }
 }
 
+// Kernel classes of a GEMM problem.  The built-in rules pick by shape; a
+// tuning table (kmh_gemm_autotune) may name another eligible class per shape
+// class: the transposes and the ceil(log2) of m, n and k.  Either way the
+// class depends on the problem's own shape only.
+enum GemmClass { GC_RULE = 0, GC_GENERIC = 1, GC_TALL = 2, GC_ROWS = 3, GC_WIDE = 4 };
+
+static int ceil_log2(int64_t v) {
+    int l = 0;
+    while ((int64_t(1) << l) < v) ++l;
+    return l;
+}
+
+uint32_t gemm_class_key(bool ta, bool tb, int m, int n, int k) {
+    return (uint32_t(ta) << 31) | (uint32_t(tb) << 30) | (uint32_t(ceil_log2(m)) << 20) |
+           (uint32_t(ceil_log2(n)) << 10) | uint32_t(ceil_log2(k));
+}
+
+bool gemm_class_eligible(int cls, bool ta, bool tb, const GemmProb& p) {
+    const bool inplace = !p.Cin || p.Cin == p.C;
+    switch (cls) {
+        case GC_GENERIC: return inplace;
+        case GC_TALL: return inplace && tall_tn(ta, tb, p);
+        case GC_ROWS: return rows_nn(ta, tb, p);
+        case GC_WIDE: return inplace && !ta && !tb && !p.lower && p.n <= WN_MAX;
+        default: return false;
+    }
+}
+
+static int gemm_route(const Ctx& c, bool ta, bool tb, const GemmProb& p) {
+    int cls = c.gemm_force;
+    if (cls == GC_RULE && !c.gemm_tune.empty()) {
+        auto it = c.gemm_tune.find(gemm_class_key(ta, tb, p.m, p.n, p.k));
+        if (it != c.gemm_tune.end()) cls = it->second;
+    }
+    if (cls != GC_RULE && gemm_class_eligible(cls, ta, tb, p)) return cls;
+    if (tall_tn(ta, tb, p)) return GC_TALL;
+    if (rows_nn(ta, tb, p)) return GC_ROWS;
+    if (wide_nn(ta, tb, p)) return GC_WIDE;
+    return GC_GENERIC;
+}
+
 void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
     // one pass per storage-type code present (problem lists are independent)
     int seen = 0;
@@ -713,10 +754,12 @@ void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs) {
             if (p.m <= 0 || p.n <= 0 || type_code(p) != tc) continue;
             KMH_CHECK(!p.Cin || p.Cin == p.C || rows_nn(ta, tb, p), KMH_E_BAD_ARG,
                       "gemm: an out-of-place beta term needs n <= 32, k <= 2048 / (8, 16 or 32), no transposes");
-            if (tall_tn(ta, tb, p)) tall.push_back(p);
-            else if (rows_nn(ta, tb, p)) rows.push_back(p);
-            else if (wide_nn(ta, tb, p)) wide.push_back(p);
-            else gen.push_back(p);
+            switch (gemm_route(c, ta, tb, p)) {
+                case GC_TALL: tall.push_back(p); break;
+                case GC_ROWS: rows.push_back(p); break;
+                case GC_WIDE: wide.push_back(p); break;
+                default: gen.push_back(p); break;
+            }
         }
         if (!gen.empty()) gemm_generic(c, ta, tb, gen, tc);
         if (!tall.empty()) gemm_tall_tn(c, tall, tc);

@@ -352,6 +352,10 @@ int server_step(Server& S) {
         S.lanes[best].inflight.push_back(q);
         S.inflight.push_back(q);
     }
+    // the lanes route their products by the server context's GEMM table, so
+    // that a request's result does not depend on its lane
+    for (auto& ln : S.lanes)
+        if (ln.ctx != S.ctx && ln.ctx->gemm_tune != S.ctx->gemm_tune) ln.ctx->gemm_tune = S.ctx->gemm_tune;
     // advance the lanes: the first active one on this thread, the others on
     // their own threads (each flushes its own launch queue)
     std::vector<Lane*> act;

@@ -80,6 +80,9 @@ SIGNATURES = {
     "kmh_ctx_profile_end": [_P, _D, _D, C.POINTER(C.c_int64)],
     "kmh_ctx_profile_query": [_P, C.c_int32, _D, _D, C.POINTER(C.c_int64)],
     "kmh_ctx_profile_config": [_P, C.c_uint32, C.c_int32],
+    "kmh_gemm_autotune": [_P, C.c_int32, C.POINTER(C.c_int32), C.c_int32, C.c_char_p],
+    "kmh_gemm_tuning_load": [_P, C.c_char_p],
+    "kmh_gemm_tuning_size": [_P, C.POINTER(C.c_int32)],
     "kmh_hodlr_build": [_P, _P, _I64, C.c_int32, C.POINTER(kmh_kernel), C.POINTER(kmh_build_opts), C.POINTER(_P)],
     "kmh_hodlr_build_dev": [_P, _P, _I64, C.c_int32, C.POINTER(kmh_kernel), C.POINTER(kmh_build_opts), C.POINTER(_P)],
     "kmh_hodlr_destroy": [_P],

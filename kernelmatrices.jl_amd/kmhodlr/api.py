@@ -13,6 +13,7 @@ Every computation runs in libkmhodlr on the GPU; this module only marshals
 arguments.  Without the built library or a HIP device it raises.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -130,14 +131,21 @@ class KernelMatrix:
 # context
 # ---------------------------------------------------------------------------
 
-class Context:
-    """A device, a stream and an HBM budget for resident factors."""
+TUNING_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gemm_tuning_gfx950.txt")
 
-    def __init__(self, device=0, hbm_budget_bytes=0):
+
+class Context:
+    """A device, a stream and an HBM budget for resident factors.  The GEMM
+    kernel-class table written by kmhodlr.autotune (TUNING_FILE) is loaded
+    when present; tuning=None keeps the built-in shape rules."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0, tuning=TUNING_FILE):
         self.lib = L.load()
         p = C.c_void_p()
         _check(self.lib, self.lib.kmh_ctx_create(int(device), int(hbm_budget_bytes), C.byref(p)))
         self.ptr = p
+        if tuning and os.path.exists(tuning):
+            _check(self.lib, self.lib.kmh_gemm_tuning_load(self.ptr, tuning.encode()))
 
     def synchronize(self):
         _check(self.lib, self.lib.kmh_ctx_synchronize(self.ptr))
