@@ -275,11 +275,11 @@ def pct(v, q):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=32, help="timed requests per leg")
-    ap.add_argument("--warmup", type=int, default=4, help="untimed requests per leg")
+    ap.add_argument("--steps", type=int, default=48, help="timed requests per leg")
+    ap.add_argument("--warmup", type=int, default=8, help="untimed requests per leg")
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--level", type=int, default=8)
-    ap.add_argument("--inflight", type=int, default=16)
+    ap.add_argument("--inflight", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--legs", default="plain,served,plain_grad,served_grad,served_mixed,roofline,aca,followup,poisson")
     ap.add_argument("--lanes", type=int, default=0, help="server lanes (0: the library's default)")
