@@ -38,6 +38,25 @@ inline float* f32p(double* p) { return reinterpret_cast<float*>(p); }
 inline const float* f32p(const double* p) { return reinterpret_cast<const float*>(p); }
 void gemm(Ctx& c, bool ta, bool tb, const std::vector<GemmProb>& probs);
 
+// Global-memory view of a pointer.  Pointers a kernel reads from a
+// descriptor are generic, and generic (flat) loads and stores count on the
+// LDS counter as well, so every LDS wait after one also waits for it (a
+// prefetch stops overlapping the compute that reads LDS); global accesses
+// leave LDS waits alone.
+template <class T> using Glob = __attribute__((address_space(1))) T*;
+template <class T> __device__ __forceinline__ Glob<T> glob(T* p) { return (Glob<T>)p; }
+
+// v where keep holds, else +0.0, by a bit mask.  Kernels load a clamped,
+// always valid element and zero it with this instead of `keep ? load : 0`:
+// the compiler turns that select into a branch around the load with a wait
+// for it inside, which serialises a row of loads on the memory latency.
+// The empty asm keeps the mask opaque, so the AND is not folded back into
+// the select.
+__device__ __forceinline__ double keep_or_zero(double v, bool keep) {
+    const unsigned long long m = keep ? ~0ull : 0ull;
+    return __longlong_as_double(__double_as_longlong(v) & m);
+}
+
 // A square matrix (or its leading n x n part) with leading dimension ld.
 struct SqMat {
     double* A;

@@ -841,7 +841,8 @@ __global__ __launch_bounds__(256) void k_trsm_diag(const LdiagDesc* __restrict__
     __shared__ double Lt[PB * (PB + 1) / 2];   // L's lower triangle, identity beyond nb
     __shared__ double Xp[16][PB + 1];          // the solved rows of the current group, published to the others
     const LdiagDesc d = ds[map[blockIdx.x]];
-    EB* Bp = reinterpret_cast<EB*>(d.B);
+    const Glob<EB> Bp = glob(reinterpret_cast<EB*>(d.B));
+    const Glob<const double> Lg = glob(d.L);
     const int tid = threadIdx.x, c = tid & 63;
     const int g = __builtin_amdgcn_readfirstlane(tid >> 6);   // row group of this wave
     const int c0 = ((int)blockIdx.x - d.chunk0) * PB;
@@ -851,7 +852,7 @@ __global__ __launch_bounds__(256) void k_trsm_diag(const LdiagDesc* __restrict__
         const int i = e & 63, l = e >> 6;
         if (i < l) continue;
         double v;
-        if (i < nb && l < nb) v = d.L[i + (size_t)l * d.ldl];
+        if (i < nb && l < nb) v = Lg[i + (size_t)l * d.ldl];
         else v = (i == l) ? 1.0 : 0.0;
         Lt[tri_col(l) + i - l] = v;
     }
@@ -863,9 +864,11 @@ __global__ __launch_bounds__(256) void k_trsm_diag(const LdiagDesc* __restrict__
     for (int i = 0; i < 16; ++i) {
         const int row = 16 * g + i;
         const bool ok = c < nc && row < nb;
-        x[i] = ok ? (MODE == 2 ? (double)Bp[(size_t)(c0 + c) + (size_t)row * d.ldb]
-                               : (double)Bp[row + (size_t)(c0 + c) * d.ldb])
-                  : 0.0;
+        // clamped, always valid element, zeroed where out of range
+        const int cc = c < nc ? c : 0, rr = row < nb ? row : 0;
+        const double v = MODE == 2 ? (double)Bp[(size_t)(c0 + cc) + (size_t)rr * d.ldb]
+                                   : (double)Bp[rr + (size_t)(c0 + cc) * d.ldb];
+        x[i] = keep_or_zero(v, ok);
     }
     __syncthreads();
     const int ng = (nb + 15) >> 4;   // row groups holding rows of L

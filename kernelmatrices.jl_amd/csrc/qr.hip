@@ -168,28 +168,42 @@ __global__ __launch_bounds__(64, NC <= 16 ? 4 : 3) void k_tile_qr(const SqrDesc*
     const int r0 = lane, r1 = lane + 64;
     double a0[NC], a1[NC];
     {
-        // branch-free loads: out-of-range rows and columns read a valid
-        // element and are replaced by zero
+        // Branch-free loads: out-of-range rows and columns read a valid
+        // element of the tile, and a bit mask (not a select, which the
+        // compiler turns into a branch around a load and a wait for it per
+        // column) zeroes them.  Global address space: the loads count on
+        // the vector-memory counter only, so all 2 NC stay in flight.
+        typedef const __attribute__((address_space(1))) E* GSrc;
         const bool in0 = r0 < mt, in1 = r1 < mt;
-        const E* s0 = reinterpret_cast<const E*>(d.src) + (in0 ? (size_t)(r0 / d.nbs) * d.slot + (r0 % d.nbs) : 0);
-        const E* s1 = reinterpret_cast<const E*>(d.src) + (in1 ? (size_t)(r1 / d.nbs) * d.slot + (r1 % d.nbs) : 0);
+        const GSrc s0 = (GSrc)d.src + (in0 ? (size_t)(r0 / d.nbs) * d.slot + (r0 % d.nbs) : 0);
+        const GSrc s1 = (GSrc)d.src + (in1 ? (size_t)(r1 / d.nbs) * d.slot + (r1 % d.nbs) : 0);
 #pragma unroll
         for (int k = 0; k < NC; ++k) {
             const size_t off = (size_t)(k < nc ? k : 0) * d.ld;
-            const double x0 = (double)s0[off], x1 = (double)s1[off];
-            a0[k] = (in0 && k < nc) ? x0 : 0.0;
-            a1[k] = (in1 && k < nc) ? x1 : 0.0;
+            const unsigned long long m0 = (in0 && k < nc) ? ~0ull : 0ull;
+            const unsigned long long m1 = (in1 && k < nc) ? ~0ull : 0ull;
+            a0[k] = __longlong_as_double(__double_as_longlong((double)s0[off]) & m0);
+            a1[k] = __longlong_as_double(__double_as_longlong((double)s1[off]) & m1);
         }
     }
-    E* q0 = reinterpret_cast<E*>(d.qout) + (r0 < mt ? r0 : 0);
-    E* q1 = reinterpret_cast<E*>(d.qout) + (r1 < mt ? r1 : 0);
+    // global-space views of the outputs (generic stores would make every
+    // LDS wait of the steps wait for them too)
+    const Glob<E> q0 = glob(reinterpret_cast<E*>(d.qout)) + (r0 < mt ? r0 : 0);
+    const Glob<E> q1 = glob(reinterpret_cast<E*>(d.qout)) + (r1 < mt ? r1 : 0);
+    const Glob<double> rout = glob(d.rout), zout = glob(d.zout);
 
-    // factorization steps j0 <= j < j1 with window W
-    auto fwd = [&](auto wc, int j0, int j1) {
+    // the eight factorization steps j0 <= j < j0 + 8 of a phase with window
+    // W, unrolled, so the slot shifts are register renaming.  Every step
+    // runs (a short last phase's steps past nc see zero columns: tau = 0,
+    // the identity) and only the stores stop at nc, so the slots end where
+    // a full phase leaves them.
+    auto fwd = [&](auto wc, int j0) {
         constexpr int W = decltype(wc)::value;
-#pragma unroll 1
-        for (int j = j0; j < j1; ++j) {
-            const int vb = W - (j - j0);   // first vector slot of this phase
+#pragma unroll
+        for (int s8 = 0; s8 < 8; ++s8) {
+            const int j = j0 + s8;
+            const bool live = j < nc;   // uniform
+            const int vb = W - s8;   // first vector slot of this phase
             // S_k = sum over rows > j of a(row, 0) a(row, k)   (rows of a1 are >= 64 > j)
             const double h0 = r0 > j ? a0[0] : 0.0;
             if (r0 == j) {
@@ -223,18 +237,20 @@ __global__ __launch_bounds__(64, NC <= 16 ? 4 : 3) void k_tile_qr(const SqrDesc*
                 }
                 __builtin_amdgcn_wave_barrier();
             }
-            if (r0 < mt) q0[(size_t)j * d.ldq] = (E)v0;
-            if (r1 < mt) q1[(size_t)j * d.ldq] = (E)v1;
-            // row j of R: lane k holds column j + k (the pivot row's updated
-            // entry, the same arithmetic as the pivot lane's update above)
-            if (lane < nc - j) {
-                const double rv = lane == 0 ? beta : (tau != 0.0 ? __fma_rn(-tau, wl, rk) : rk);
-                d.rout[j + (size_t)(j + lane) * d.ldr] = rv;
+            if (live) {
+                if (r0 < mt) q0[(size_t)j * d.ldq] = (E)v0;
+                if (r1 < mt) q1[(size_t)j * d.ldq] = (E)v1;
+                // row j of R: lane k holds column j + k (the pivot row's updated
+                // entry, the same arithmetic as the pivot lane's update above)
+                if (lane < nc - j) {
+                    const double rv = lane == 0 ? beta : (tau != 0.0 ? __fma_rn(-tau, wl, rk) : rk);
+                    rout[j + (size_t)(j + lane) * d.ldr] = rv;
+                }
+                // column j of Z: tau on the diagonal, v_i^T v_j of this phase's vectors above it
+                const Glob<double> zc = zout + (size_t)j * QC;
+                if (lane == 0) zc[j] = tau;
+                else if (lane >= vb && lane < W) zc[j0 + lane - vb] = wl;
             }
-            // column j of Z: tau on the diagonal, v_i^T v_j of this phase's vectors above it
-            double* zc = d.zout + (size_t)j * QC;
-            if (lane == 0) zc[j] = tau;
-            else if (lane >= vb && lane < W) zc[j0 + lane - vb] = wl;
 #pragma unroll
             for (int k = 0; k < W - 1; ++k) {
                 a0[k] = a0[k + 1];
@@ -243,18 +259,6 @@ __global__ __launch_bounds__(64, NC <= 16 ? 4 : 3) void k_tile_qr(const SqrDesc*
             a0[W - 1] = v0;
             a1[W - 1] = v1;
         }
-    };
-    // rotate the window [0, W) left by one slot
-    auto rot1 = [&](auto wc) {
-        constexpr int W = decltype(wc)::value;
-        const double t0 = a0[0], t1 = a1[0];
-#pragma unroll
-        for (int k = 0; k < W - 1; ++k) {
-            a0[k] = a0[k + 1];
-            a1[k] = a1[k + 1];
-        }
-        a0[W - 1] = t0;
-        a1[W - 1] = t1;
     };
     // Z entries v_i^T v_j, i in phase PI < PJ the phase of j: 16 pairs
     // (i = 8 PI + Q0 + k / 8, j = 8 PJ + k % 8) per reduction round
@@ -268,21 +272,14 @@ __global__ __launch_bounds__(64, NC <= 16 ? 4 : 3) void k_tile_qr(const SqrDesc*
         }
         const double g = lds_colsum<16>(pr, pw, lane);
         const int i = 8 * PI + Q0 + (lane & 15) / 8, j = 8 * PJ + (lane & 7);
-        if (lane < 16 && j < nc) d.zout[i + (size_t)j * QC] = g;
+        if (lane < 16 && j < nc) zout[i + (size_t)j * QC] = g;
     };
-    fwd(ic<NC>{}, 0, min(nc, 8));
-    if constexpr (NC > 8) fwd(ic<NC - 8>{}, 8, min(nc, 16));
-    if constexpr (NC > 16) fwd(ic<NC - 16>{}, 16, min(nc, 24));
+    fwd(ic<NC>{}, 0);
+    if constexpr (NC > 8)
+        if (nc > 8) fwd(ic<NC - 8>{}, 8);
+    if constexpr (NC > 16)
+        if (nc > 16) fwd(ic<NC - 16>{}, 16);
     if constexpr (NC > 8) {
-        // a short last phase leaves its vectors at the top of its window:
-        // rotate them down to slots NC - 8(p + 1) + q
-        const int lastp = (nc - 1) / 8, nrot = 8 * (lastp + 1) - nc;
-#pragma unroll 1
-        for (int t = 0; t < nrot; ++t) {
-            if (lastp == 1) rot1(ic<NC - 8>{});
-            if constexpr (NC > 16)
-                if (lastp == 2) rot1(ic<NC - 16>{});
-        }
         if (nc > 8) {
             cross(ic<0>{}, ic<1>{}, ic<0>{});
             cross(ic<0>{}, ic<1>{}, ic<2>{});
@@ -304,7 +301,7 @@ __global__ __launch_bounds__(64, NC <= 16 ? 4 : 3) void k_tile_qr(const SqrDesc*
     __builtin_amdgcn_wave_barrier();
     for (int e = lane; e < nc * nc; e += 64) {   // zero strict lower triangle of R
         const int i = e % nc, cc = e / nc;
-        if (i > cc) d.rout[i + (size_t)cc * d.ldr] = 0.0;
+        if (i > cc) rout[i + (size_t)cc * d.ldr] = 0.0;
     }
 }
 
