@@ -478,19 +478,34 @@ static void woodbury_batch(Ctx& c, int mode, const std::vector<Rhs>& rs) {
             double* s = S + off;
             double* s2 = S2 + off;
             off += (size_t)2 * r * nc;
-            GemmProb t1{r, nc, h1, Q1, (int)n, tadv(x.T, b.c0, t32), (int)x.ldT, s, 2 * r, 1.0, 0.0, 0};
+            // G and X have zero first r rows (G = I - chol(C)^-1, X = chol(C) - I
+            // with chol(C) = [I, 0; Bm^T, L22]).  Untransposed, the first r
+            // rows of M * (G M^T t) vanish: only the second child's rows change
+            // and only the last r rows of G (X) are applied.  Transposed, the
+            // first r columns of G^T (X^T) vanish: the first child's projection
+            // Q_V^T t_V is not needed.
+            const double* Mk = Mb + H.coff_blk[lv][k];
             GemmProb t2{r, nc, h2, Q2, (int)n, tadv(x.T, b.a0, t32), (int)x.ldT, s + r, 2 * r, 1.0, 0.0, 0};
-            t1.ea = t2.ea = q32;
-            t1.eb = t2.eb = t32;
-            tn.push_back(t1);
+            t2.ea = q32;
+            t2.eb = t32;
             tn.push_back(t2);
-            op.push_back({2 * r, nc, 2 * r, Mb + H.coff_blk[lv][k], 2 * r, s, 2 * r, s2, 2 * r, 1.0, 0.0, 0});
-            GemmProb u1{h1, nc, r, Q1, (int)n, s2, 2 * r, tadv(x.T, b.c0, t32), (int)x.ldT, sg, 1.0, 0};
             GemmProb u2{h2, nc, r, Q2, (int)n, s2 + r, 2 * r, tadv(x.T, b.a0, t32), (int)x.ldT, sg, 1.0, 0};
-            u1.ea = u2.ea = q32;
-            u1.ec = u2.ec = t32;
-            up.push_back(u1);
+            u2.ea = q32;
+            u2.ec = t32;
             up.push_back(u2);
+            if (!tr) {
+                GemmProb t1{r, nc, h1, Q1, (int)n, tadv(x.T, b.c0, t32), (int)x.ldT, s, 2 * r, 1.0, 0.0, 0};
+                t1.ea = q32;
+                t1.eb = t32;
+                tn.push_back(t1);
+                op.push_back({r, nc, 2 * r, Mk + r, 2 * r, s, 2 * r, s2 + r, 2 * r, 1.0, 0.0, 0});
+            } else {
+                op.push_back({2 * r, nc, r, Mk + r, 2 * r, s + r, 2 * r, s2, 2 * r, 1.0, 0.0, 0});
+                GemmProb u1{h1, nc, r, Q1, (int)n, s2, 2 * r, tadv(x.T, b.c0, t32), (int)x.ldT, sg, 1.0, 0};
+                u1.ea = q32;
+                u1.ec = t32;
+                up.push_back(u1);
+            }
         }
     }
     gemm(c, true, false, tn);
@@ -551,12 +566,12 @@ static void woodbury_shared_basis(Ctx& c, const std::vector<Hodlr*>& hs, const s
             tn.push_back(t2);
             rc.push_back({R1, r, s + (size_t)r * l2, l2, r, r, 1.0});
             rc.push_back({R2, r, s + r, l2, r, r, 1.0});
-            op.push_back({l2, l2, l2, G + H.coff_blk[lv][k], l2, s, l2, s2, l2, 1.0, 0.0, 0});
-            GemmProb u1{h1, l2, r, Q1, (int)n, s2, l2, tadv(AB, b.c0, e), (int)n, -1.0, 1.0, 0};
+            // the first r rows of G are zero: only the second child's rows of
+            // the panels change (see woodbury_batch)
+            op.push_back({r, l2, l2, G + H.coff_blk[lv][k] + r, l2, s, l2, s2 + r, l2, 1.0, 0.0, 0});
             GemmProb u2{h2, l2, r, Q2, (int)n, s2 + r, l2, tadv(AB, b.a0, e), (int)n, -1.0, 1.0, 0};
-            u1.ea = u2.ea = eq;
-            u1.ec = u2.ec = e;
-            up.push_back(u1);
+            u2.ea = eq;
+            u2.ec = e;
             up.push_back(u2);
         }
     }
