@@ -593,7 +593,7 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
     std::vector<QrPanel> qp;
     std::vector<Copy2D> qcopy;
     std::vector<GemmProb> bm;
-    std::vector<DiagOp> ident_x, ident_g, plus1, minus1, eye_s, eye_g;
+    std::vector<DiagOp> plus1, minus1, eye_s, eye_g;
     std::vector<GemmProb> schur, g21;
     std::vector<Copy2D> lowleft;
     std::vector<SqMat> cm;
@@ -646,13 +646,13 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
             // S is factored:
             //   X = chol(C) - I    = [0, 0; Bm^T, L22 - I]
             //   G = I - chol(C)^-1 = [0, 0; L22^-1 Bm^T, I - L22^-1]
+            // The Woodbury products read only the last r rows of X and G, so
+            // the zero first r rows are neither written nor read.
             const int l2 = 2 * r;
             double* X21 = Xb + r;
             double* X22 = Xb + r + (size_t)r * l2;
             double* G21 = Gb + r;
             double* G22 = Gb + r + (size_t)r * l2;
-            ident_x.push_back({Xb, l2, l2, 0.0});            // X = 0, then the blocks below
-            ident_g.push_back({Gb, l2, l2, 0.0});
             lowleft.push_back({Bm, r, X21, l2, r, r, 1.0});   // X21 = Bm^T
             eye_s.push_back({X22, r, l2, 1.0});              // S = I ...
             schur.push_back({r, r, r, Bm, r, Bm, r, X22, l2, -1.0, 1.0, 0});   // ... - Bm^T Bm
@@ -669,8 +669,6 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
     copy2d(c, qcopy);
     qr_panels(c, qp);
     gemm(c, false, true, bm);                    // Bm = R1 R2^T
-    set_identity(c, ident_x);                    // X = 0
-    set_identity(c, ident_g);                    // G = 0
     copy2d(c, lowleft, true);                    // X21 = Bm^T
     set_identity(c, eye_s);
     gemm(c, true, false, schur);                 // S = I - Bm^T Bm
