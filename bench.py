@@ -44,7 +44,7 @@ TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profile
                             "tile_qr_nn_wide_served_n65536_traffic.md")
 
 
-KERNEL_SOURCES = ("la.hip", "qr.hip", "common.h", "la.h")
+KERNEL_SOURCES = ("la.hip", "qr.hip", "common.h", "la.h", "wavesum.h")
 
 
 def kernel_source_hash():

@@ -10,6 +10,7 @@
 #include <cstring>
 
 #include "assemble.h"
+#include "wavesum.h"
 
 namespace kmh {
 
@@ -70,24 +71,34 @@ __device__ __forceinline__ void bit_set(unsigned* m, int i) { m[i >> 5] |= 1u <<
 //
 // Every block of every level runs the reference's loop.  One loop iteration
 // of all running blocks is five grouped launches, so a large block's
-// residual rows and columns and its norm-update products spread over many
-// workgroups:
-//   k_aca_row   residual row at the row pivot (:48-50), per 1024-column
+// residual rows and columns spread over many workgroups:
+//   k_aca_row   residual row at the row pivot (:48-50), per 256-column
 //               chunk, with the chunk's argmax of |v| over unused columns
 //   k_aca_piv   column pivot (:52-55, restrictedmaxabs over the chunks)
 //   k_aca_col   v /= pivot (:56) and its squared norm per column chunk; the
 //               residual column at the column pivot (:58-61), its squared
 //               norm and the argmax of |u| over unused rows per row chunk
-//   k_aca_dots  u.U_t and v.V_t for the earlier terms t (:63-65), one
-//               workgroup per term (four waves over quarters of the vectors)
-//   k_aca_fin   ||Z||_F^2 update (:63-67), rank + 1, stop test, next row
+//   k_aca_dred  the norm-update products u.U_t and v.V_t (:63-65) of the
+//               previous term, summed from the per-wave partials that the
+//               row and column passes left
+//   k_aca_fin   ||Z||_F^2 update (:63-67), stop test, rank + 1, next row
 //               pivot (:43-46)
-// The element arithmetic, pivot tie rule and stop test are the reference's;
-// sums over chunks are added in chunk order, so results are deterministic.
+// The products of a term against the earlier ones are formed one step
+// late: the next step's residual passes stream the earlier terms' columns
+// anyway (the reference's loop reads U and V once for the residual and
+// once more for the products), so they multiply each element by the
+// previous term's element on the way.  That makes the next step
+// speculative: it runs before the stop test that would have allowed it,
+// and when the test (evaluated with the now complete ||Z||_F^2) stops the
+// loop, its row and column stay unused beyond the block's rank.  The
+// element arithmetic, pivot tie rule and stop test are the reference's; sums
+// over chunks and waves are added in a fixed order, so results are
+// deterministic.
 // ---------------------------------------------------------------------------
 constexpr int ACH = 256;    // rows or columns per chunk (one per thread): a
                             // 32768-row block spreads over 128 workgroups
 constexpr int ACA_COEF = 4096;   // LDS coefficients: the rank limit
+constexpr int ACA_B = 16;        // terms per batch of a residual pass
 
 struct AcaArg {
     double a, v;   // |value|, value
@@ -111,8 +122,11 @@ struct AcaState {
     unsigned *urow, *ucol;
     AcaArg *rpart, *cpart;   // per row / column chunk argmax
     double *rsum, *csum;     // per column / row chunk squared norms
-    double *du, *dv;         // norm-update products of the current step
-    // evolving state
+    double *du, *dv;         // norm-update products of the pending term
+    double *pdu, *pdv;       // their per-wave partials: [t][wave slot], rows of plu / plv
+    int plu, plv;            // 4 x row chunks, 4 x column chunks
+    // evolving state: frnk accepted terms; the last one's norms rn2, cn2 and
+    // products are pending, znorm2 holds ||Z||^2 before its contribution
     int frnk, done, st, ri, ci, pad1;
     double mv, rn2, cn2, znorm2;
 };
@@ -175,6 +189,31 @@ __device__ void pick(const AcaArg* part, int n, int len, double* val, int* idx) 
     else { *val = 0.0; *idx = ff; }
 }
 
+// pick() by the 64 lanes of one wave, every lane getting the result: the
+// largest entry under arg_better's total order (|value| descending, then
+// index ascending) and the least ff do not depend on the scan order, so this
+// equals the sequential scan.
+__device__ void pick_wave(const AcaArg* part, int n, int len, double* val, int* idx) {
+    const int lane = threadIdx.x & 63;
+    double ba = -1.0, bv = 0.0;
+    int bi = 0x7fffffff, ff = 0x7fffffff;
+    for (int q = lane; q < n; q += 64) {
+        const AcaArg p = part[q];
+        if (arg_better(p.a, p.i, ba, bi)) { ba = p.a; bv = p.v; bi = p.i; }
+        ff = min(ff, p.ff);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double oa = __shfl_xor(ba, off), ov = __shfl_xor(bv, off);
+        const int oi = __shfl_xor(bi, off), of = __shfl_xor(ff, off);
+        if (arg_better(oa, oi, ba, bi)) { ba = oa; bv = ov; bi = oi; }
+        ff = min(ff, of);
+    }
+    if (ff == 0x7fffffff) ff = len - 1;
+    if (ba > 0.0) { *val = bv; *idx = bi; }
+    else { *val = 0.0; *idx = ff; }
+}
+
 __device__ void aca_stop(AcaState& s, int st) {
     s.done = 1;
     s.st = st;
@@ -182,21 +221,19 @@ __device__ void aca_stop(AcaState& s, int st) {
     *s.rank = s.frnk;
 }
 
-// loop head (:43-46): stop test, capacity, next row pivot
-__device__ void aca_loop_head(AcaState& s) {
-    if (!(sqrt(s.rn2) * sqrt(s.cn2) > s.rtol * sqrt(s.znorm2) && s.frnk < s.maxsz)) {
+
+// before a (speculative) step: the loop's rank limit (:43) and the panel's
+// capacity for the step's column; false when the block stopped
+__device__ bool aca_can_step(AcaState& s) {
+    if (s.frnk >= s.maxsz) {
         aca_stop(s, 0);
-        return;
+        return false;
     }
     if (s.frnk >= s.cap) {   // more columns than the panel holds: the host widens it
         aca_stop(s, 2);
-        return;
+        return false;
     }
-    double v;
-    int ri;
-    pick(s.rpart, s.rch, s.nro, &v, &ri);
-    s.ri = ri;
-    bit_set(s.urow, ri);
+    return true;
 }
 
 __device__ __forceinline__ int find_plan(const AcaPlan* pl, int np, int x, bool rows) {
@@ -284,28 +321,102 @@ __global__ __launch_bounds__(256) void k_aca_init(AcaState* __restrict__ sts) {
         bit_set(s.ucol, mi);
         s.rn2 = rn2;
         s.cn2 = cn2;
-        s.znorm2 = rn2 * cn2;
+        s.znorm2 = 0.0;   // the first term's rn2 * cn2 is added by the first k_aca_fin
         s.frnk = 1;
         s.st = 0;
         s.done = 0;
-        s.rpart[0] = br;   // the next row pivot comes from U[:, 0]
-        for (int q = 1; q < s.rch; ++q) s.rpart[q] = arg_none();
-        aca_loop_head(s);
+        if (aca_can_step(s)) {   // the next row pivot comes from U[:, 0]
+            double v;
+            int ri;
+            pick(&br, 1, s.nro, &v, &ri);
+            s.ri = ri;
+            bit_set(s.urow, ri);
+        }
     }
 }
 
-// A block stopped at capacity continues with wider panels.
-__global__ void k_aca_resume(AcaState* __restrict__ sts, const AcaPlan* __restrict__ pl) {
-    if (threadIdx.x != 0) return;
+// One lane's residual chain and its share of the previous term's products:
+//   returns v - c_0 x_0 - c_1 x_1 - ... - c_{f-1} x_{f-1}, x_t = p[t ld],
+//   each product rounded and subtracted in term order (src/factorizations.jl
+//   :49-50, 59-60: tmprow .-= U[j][mi] * V[j] for j in 1:frnk);
+//   and for t < f - 1 stores the wave's sum of x_t x_{f-1} (the products of
+//   the last term against the earlier ones, :63-65) at part[t * pl].
+// All lanes of the wave take part (an inactive lane, act false, contributes
+// zero products; its p must still point into the panel).  coef[f .. f +
+// ACA_B) must be zero.  Loads go ACA_B
+// terms at a time, the next batch issued before the current one is used: a
+// lane's chain runs over up to thousands of column-strided loads, and a
+// block of 32768 columns has only 512 waves, so the loads in flight per
+// wave set the bandwidth.  The batch's products are summed over the wave by
+// column sums in LDS (pw: 64 x 17 doubles of this wave), fixed order.
+__device__ __forceinline__ double aca_pass(double v, const double* __restrict__ coef, const double* __restrict__ p,
+                                           size_t ld, int f, bool act, double* pw, int lane, double* part,
+                                           int pl) {
+    typedef const __attribute__((address_space(1))) double* GP;
+    const GP g = (GP)p;
+    const int k = f - 1;   // the previous term; products against t < k
+    const double last = (act && k > 0) ? g[(size_t)k * ld] : 0.0;
+    double xa[ACA_B], xb[ACA_B];
+    // terms t >= f load a valid element masked to +0 (a bit mask, not a
+    // select, which would become a branch around the load); with coef[t] = 0
+    // there, v - (+0 * 0) = v + (-0) = v exactly, so the chain needs no
+    // condition
+    // (the mask is applied where the value is used, so the next batch's
+    // loads stay in flight while the current one is consumed)
+    auto load = [&](double (&x)[ACA_B], int t0) {
+#pragma unroll
+        for (int u = 0; u < ACA_B; ++u) x[u] = g[(size_t)min(t0 + u, f - 1) * ld];
+    };
+    auto use = [&](double (&x)[ACA_B], int t0) {
+#pragma unroll
+        for (int u = 0; u < ACA_B; ++u) {
+            const unsigned long long mk = (t0 + u < f) ? ~0ull : 0ull;
+            x[u] = __longlong_as_double(__double_as_longlong(x[u]) & mk);
+            v = __dsub_rn(v, __dmul_rn(coef[t0 + u], x[u]));
+        }
+        if (t0 < k) {   // uniform
+            double pr[ACA_B];
+#pragma unroll
+            for (int u = 0; u < ACA_B; ++u) pr[u] = (t0 + u < k) ? x[u] * last : 0.0;
+            const double sm = lds_colsum<ACA_B>(pr, pw, lane);
+            if (lane < ACA_B && t0 + lane < k) part[(size_t)(t0 + lane) * pl] = sm;
+        }
+    };
+    if (f <= 0) return v;
+    // the next batch is loaded unconditionally (addresses clamped, values
+    // masked where used): a conditional load would make the wait before the
+    // current batch's first use a wait for every load in flight
+    load(xa, 0);
+    for (int t0 = 0; t0 < f; t0 += ACA_B) {
+        load(xb, t0 + ACA_B);
+        use(xa, t0);
+#pragma unroll
+        for (int u = 0; u < ACA_B; ++u) xa[u] = xb[u];
+    }
+    return v;
+}
+
+// A block stopped at capacity continues with wider panels: pick its next
+// row pivot (one wave per block).
+__global__ __launch_bounds__(64) void k_aca_resume(AcaState* __restrict__ sts, const AcaPlan* __restrict__ pl) {
     AcaState& s = sts[pl[blockIdx.x].b];
-    s.done = 0;
-    s.st = 0;
-    aca_loop_head(s);
+    const int ok = __builtin_amdgcn_readfirstlane(s.frnk < s.cap && s.frnk < s.maxsz ? 1 : 0);
+    if (!ok) return;
+    double v;
+    int ri;
+    pick_wave(s.rpart, s.rch, s.nro, &v, &ri);
+    if (threadIdx.x == 0) {
+        s.done = 0;
+        s.st = 0;
+        s.ri = ri;
+        bit_set(s.urow, ri);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_aca_row(AcaState* __restrict__ sts, const AcaPlan* __restrict__ pl,
                                                  int np) {
-    __shared__ double coef[ACA_COEF];
+    __shared__ double coef[ACA_COEF + ACA_B];
+    __shared__ double pws[4][64 * (ACA_B + 1)];
     __shared__ AcaArg sha[4];
     const int p = find_plan(pl, np, blockIdx.x, false);
     AcaState& s = sts[pl[p].b];
@@ -313,32 +424,39 @@ __global__ __launch_bounds__(256) void k_aca_row(AcaState* __restrict__ sts, con
     const int chunk = blockIdx.x - pl[p].cch0;
     const int f = s.frnk, ri = s.ri;
     const size_t ld = (size_t)s.ld;
-    for (int t = threadIdx.x; t < f; t += 256) coef[t] = s.U[ri + (size_t)t * ld];
+    for (int t = threadIdx.x; t < f + ACA_B; t += 256) coef[t] = t < f ? s.U[ri + (size_t)t * ld] : 0.0;
     __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const double xr = s.rx[ri], yr = s.ry[ri];
     double* vnew = s.V + (size_t)f * ld;
-    AcaArg m = arg_none();
     const int j0 = chunk * ACH, j1 = min(s.ncl, j0 + ACH);
-    for (int j = j0 + threadIdx.x; j < j1; j += 256) {
-        double v = kernel_eval(s.k, xr, yr, s.cx[j], s.cy[j]);
-        for (int t = 0; t < f; ++t) v = __dsub_rn(v, __dmul_rn(coef[t], s.V[j + (size_t)t * ld]));
+    const int j = j0 + (int)threadIdx.x;
+    const bool act = j < j1;
+    const int jc = act ? j : j0;
+    const double v = aca_pass(act ? kernel_eval(s.k, xr, yr, s.cx[jc], s.cy[jc]) : 0.0, coef, s.V + jc, ld, f,
+                              act, pws[w], lane, s.pdv + chunk * 4 + w, s.plv);
+    AcaArg m = arg_none();
+    if (act) {
         vnew[j] = v;
-        if (bit_get(s.ucol, j)) continue;
-        if (j < m.ff) m.ff = j;
-        const double a = fabs(v);
-        if (a > m.a) { m.a = a; m.v = v; m.i = j; }
+        if (!bit_get(s.ucol, j)) {
+            m.ff = j;
+            m.a = fabs(v);
+            m.v = v;
+            m.i = j;
+        }
     }
     const AcaArg r = block_argmax(m, sha);
     if (threadIdx.x == 0) s.cpart[chunk] = r;
 }
 
-__global__ void k_aca_piv(AcaState* __restrict__ sts, const AcaPlan* __restrict__ pl) {
-    if (threadIdx.x != 0) return;
+// one wave per block
+__global__ __launch_bounds__(64) void k_aca_piv(AcaState* __restrict__ sts, const AcaPlan* __restrict__ pl) {
     AcaState& s = sts[pl[blockIdx.x].b];
-    if (s.done) return;
+    if (s.done) return;   // uniform
     double mv;
     int ci;
-    pick(s.cpart, s.cch, s.ncl, &mv, &ci);
+    pick_wave(s.cpart, s.cch, s.ncl, &mv, &ci);
+    if (threadIdx.x != 0) return;
     if (mv == 0.0) {   // :55
         aca_stop(s, 0);
         return;
@@ -350,7 +468,8 @@ __global__ void k_aca_piv(AcaState* __restrict__ sts, const AcaPlan* __restrict_
 
 __global__ __launch_bounds__(256) void k_aca_col(AcaState* __restrict__ sts, const AcaPlan* __restrict__ pl,
                                                  int np, int ncch) {
-    __shared__ double coef[ACA_COEF];
+    __shared__ double coef[ACA_COEF + ACA_B];
+    __shared__ double pws[4][64 * (ACA_B + 1)];
     __shared__ AcaArg sha[4];
     __shared__ double shd[4];
     const bool colpart = (int)blockIdx.x < ncch;
@@ -377,22 +496,28 @@ __global__ __launch_bounds__(256) void k_aca_col(AcaState* __restrict__ sts, con
     }
     const int chunk = x - pl[p].rch0;
     const int ci = s.ci;
-    for (int t = threadIdx.x; t < f; t += 256) coef[t] = s.V[ci + (size_t)t * ld];
+    for (int t = threadIdx.x; t < f + ACA_B; t += 256) coef[t] = t < f ? s.V[ci + (size_t)t * ld] : 0.0;
     __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const double xc = s.cx[ci], yc = s.cy[ci];
     double* unew = s.U + (size_t)f * ld;
+    const int i0 = chunk * ACH, i1 = min(s.nro, i0 + ACH);
+    const int i = i0 + (int)threadIdx.x;
+    const bool act = i < i1;
+    const int ic = act ? i : i0;
+    const double v = aca_pass(act ? kernel_eval(s.k, s.rx[ic], s.ry[ic], xc, yc) : 0.0, coef, s.U + ic, ld, f,
+                              act, pws[w], lane, s.pdu + chunk * 4 + w, s.plu);
     double cs = 0.0;
     AcaArg m = arg_none();
-    const int i0 = chunk * ACH, i1 = min(s.nro, i0 + ACH);
-    for (int i = i0 + threadIdx.x; i < i1; i += 256) {
-        double v = kernel_eval(s.k, s.rx[i], s.ry[i], xc, yc);
-        for (int t = 0; t < f; ++t) v = __dsub_rn(v, __dmul_rn(coef[t], s.U[i + (size_t)t * ld]));
+    if (act) {
         unew[i] = v;
-        cs += v * v;
-        if (bit_get(s.urow, i)) continue;
-        if (i < m.ff) m.ff = i;
-        const double a = fabs(v);
-        if (a > m.a) { m.a = a; m.v = v; m.i = i; }
+        cs = v * v;
+        if (!bit_get(s.urow, i)) {
+            m.ff = i;
+            m.a = fabs(v);
+            m.v = v;
+            m.i = i;
+        }
     }
     const double t = block_sum_fixed(cs, shd);
     const AcaArg r = block_argmax(m, sha);
@@ -402,64 +527,101 @@ __global__ __launch_bounds__(256) void k_aca_col(AcaState* __restrict__ sts, con
     }
 }
 
-// workgroup g of the grid: plan entry g / fb, earlier term t = g % fb; wave
-// w sums quarter w of the vectors (four interleaved partial sums per lane),
-// and the quarters are added in order
-__device__ __forceinline__ double quarter_dot(const double* __restrict__ x, const double* __restrict__ y, int n,
-                                              int w, int lane) {
-    const int q = (n + 3) >> 2, i0 = w * q, i1 = min(n, i0 + q);
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int i = i0 + lane;
-    for (; i + 192 < i1; i += 256) {
-        a0 += x[i] * y[i];
-        a1 += x[i + 64] * y[i + 64];
-        a2 += x[i + 128] * y[i + 128];
-        a3 += x[i + 192] * y[i + 192];
-    }
-    for (; i < i1; i += 64) a0 += x[i] * y[i];
-    double a = (a0 + a1) + (a2 + a3);
+// The previous term's products u_k.U_t and v_k.V_t for t < k (k = frnk - 1)
+// from the per-wave partials of the residual passes: wave w of workgroup g
+// takes term t = 4 (g % fb4) + w of plan entry g / fb4; its lanes add the
+// slots lane, lane + 64, ... in order and the lanes meet by xor-shuffles.
+__global__ __launch_bounds__(256) void k_aca_dred(AcaState* __restrict__ sts, const AcaPlan* __restrict__ pl,
+                                                  int np, int fb4) {
+    const int g = (int)blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int p = g / fb4;
+    if (p >= np) return;
+    AcaState& s = sts[pl[p].b];
+    const int t = 4 * (g - p * fb4) + w;
+    if (s.done || t >= s.frnk - 1) return;   // uniform over the wave
+    const double* ru = s.pdu + (size_t)t * s.plu;
+    const double* rv = s.pdv + (size_t)t * s.plv;
+    double a = 0.0, c = 0.0;
+    for (int q = lane; q < s.plu; q += 64) a += ru[q];
+    for (int q = lane; q < s.plv; q += 64) c += rv[q];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
-    return a;
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        c += __shfl_xor(c, off);
+    }
+    if (lane == 0) {
+        s.du[t] = a;
+        s.dv[t] = c;
+    }
 }
 
-__global__ __launch_bounds__(256) void k_aca_dots(AcaState* __restrict__ sts, const AcaPlan* __restrict__ pl,
-                                                  int np, int fb) {
-    __shared__ double sh[2][4];
-    const int g = (int)blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int p = g / fb, t = g - p * fb;
-    if (p >= np) return;   // uniform over the workgroup
-    AcaState& s = sts[pl[p].b];
-    if (s.done || t >= s.frnk) return;   // uniform over the workgroup
-    const size_t ld = (size_t)s.ld;
-    const double a = quarter_dot(s.U + (size_t)t * ld, s.U + (size_t)s.frnk * ld, s.nro, w, lane);
-    const double c = quarter_dot(s.V + (size_t)t * ld, s.V + (size_t)s.frnk * ld, s.ncl, w, lane);
-    if (lane == 0) {
-        sh[0][w] = a;
-        sh[1][w] = c;
+// One 256-thread workgroup per block: ||Z||^2 with the previous term k
+// (:63-67), the stop test of the step just taken (:43), and if it stands,
+// rank + 1 and the next row pivot.  The chunk norms and the product terms
+// are staged in LDS by all threads and summed by one in the sequential
+// order (z starts from the stored ||Z||^2 and takes the terms t = 0, 1, ...
+// as :63-66 do); the next row pivot is one wave's pick.
+constexpr int FIN_STAGE = 1024;
+__global__ __launch_bounds__(256) void k_aca_fin(AcaState* __restrict__ sts, const AcaPlan* __restrict__ pl) {
+    __shared__ double sa[FIN_STAGE], sb[FIN_STAGE];
+    __shared__ int go;
+    AcaState& s = sts[pl[blockIdx.x].b];
+    if (s.done) return;   // uniform
+    const int tid = threadIdx.x;
+    const int k = s.frnk - 1;
+    double z = s.znorm2;
+    for (int b0 = 0; b0 < k; b0 += FIN_STAGE) {
+        const int m = min(FIN_STAGE, k - b0);
+        for (int q = tid; q < m; q += 256) {
+            sa[q] = s.du[b0 + q];
+            sb[q] = s.dv[b0 + q];
+        }
+        __syncthreads();
+        if (tid == 0)
+            for (int q = 0; q < m; ++q) z += 2.0 * fabs(sa[q]) * fabs(sb[q]);
+        __syncthreads();
+    }
+    if (tid == 0) {
+        z += s.rn2 * s.cn2;
+        // :43, the test the speculative step should have passed
+        go = (sqrt(s.rn2) * sqrt(s.cn2) > s.rtol * sqrt(z)) ? 1 : 0;
+        if (!go) aca_stop(s, 0);
+        else s.znorm2 = z;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        s.du[t] = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
-        s.dv[t] = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
-    }
-}
-
-__global__ void k_aca_fin(AcaState* __restrict__ sts, const AcaPlan* __restrict__ pl) {
-    if (threadIdx.x != 0) return;
-    AcaState& s = sts[pl[blockIdx.x].b];
-    if (s.done) return;
+    if (!go) return;
     double rn2 = 0.0, cn2 = 0.0;
-    for (int q = 0; q < s.cch; ++q) rn2 += s.rsum[q];
-    for (int q = 0; q < s.rch; ++q) cn2 += s.csum[q];
-    double z = s.znorm2;
-    for (int t = 0; t < s.frnk; ++t) z += 2.0 * fabs(s.du[t]) * fabs(s.dv[t]);
-    z += rn2 * cn2;
-    s.rn2 = rn2;
-    s.cn2 = cn2;
-    s.znorm2 = z;
-    s.frnk += 1;
-    aca_loop_head(s);
+    for (int b0 = 0; b0 < s.cch; b0 += FIN_STAGE) {
+        const int m = min(FIN_STAGE, s.cch - b0);
+        for (int q = tid; q < m; q += 256) sa[q] = s.rsum[b0 + q];
+        __syncthreads();
+        if (tid == 0)
+            for (int q = 0; q < m; ++q) rn2 += sa[q];
+        __syncthreads();
+    }
+    for (int b0 = 0; b0 < s.rch; b0 += FIN_STAGE) {
+        const int m = min(FIN_STAGE, s.rch - b0);
+        for (int q = tid; q < m; q += 256) sa[q] = s.csum[b0 + q];
+        __syncthreads();
+        if (tid == 0)
+            for (int q = 0; q < m; ++q) cn2 += sa[q];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        s.rn2 = rn2;
+        s.cn2 = cn2;
+        s.frnk = k + 2;
+        go = aca_can_step(s) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!go || tid >= 64) return;
+    double v;
+    int ri;
+    pick_wave(s.rpart, s.rch, s.nro, &v, &ri);
+    if (tid == 0) {
+        s.ri = ri;
+        bit_set(s.urow, ri);
+    }
 }
 
 struct AcaRun::Impl {
@@ -469,6 +631,7 @@ struct AcaRun::Impl {
     AcaState* d_st = nullptr;
     std::vector<AcaState> h;
     std::vector<int> widen;   // blocks to continue
+    std::vector<char*> extra;  // product partials of widened blocks
     explicit Impl(Ctx& cc) : c(cc) {}
 };
 
@@ -476,7 +639,7 @@ AcaRun::AcaRun(Ctx& c, const std::vector<AcaDesc>& ds) : m_(new Impl(c)) {
     Impl& m = *m_;
     m.nb = (int)ds.size();
     if (m.nb == 0) return;
-    size_t bits = 0, rparts = 0, cparts = 0, ts = 0;
+    size_t bits = 0, rparts = 0, cparts = 0, ts = 0, pp = 0;
     m.h.resize(m.nb);
     for (int b = 0; b < m.nb; ++b) {
         const auto& d = ds[b];
@@ -490,16 +653,20 @@ AcaRun::AcaRun(Ctx& c, const std::vector<AcaDesc>& ds) : m_(new Impl(c)) {
         s.nro = d.nro; s.ncl = d.ncl; s.ld = d.ld; s.maxsz = d.maxsz; s.cap = d.cap;
         s.rch = (int)ceil_div(d.nro, ACH);
         s.cch = (int)ceil_div(d.ncl, ACH);
+        s.plu = 4 * s.rch;
+        s.plv = 4 * s.cch;
         s.rank = d.rank; s.status = d.status;
         s.rtol = d.rtol; s.k = d.k;
         bits += (size_t)(d.nro + 31) / 32 + (size_t)(d.ncl + 31) / 32;
         rparts += s.rch;
         cparts += s.cch;
         ts += std::min(d.maxsz, ACA_COEF);
+        pp += (size_t)d.cap * (s.plu + s.plv);   // products of terms t < frnk - 1 < cap
     }
     auto al = [](size_t n) { return (n + 255) & ~size_t(255); };
     const size_t bytes = al(sizeof(AcaState) * m.nb) + al(bits * 4) + al((rparts + cparts) * sizeof(AcaArg)) +
-                         al((rparts + cparts) * sizeof(double)) + al(2 * ts * sizeof(double));
+                         al((rparts + cparts) * sizeof(double)) + al(2 * ts * sizeof(double)) +
+                         al(pp * sizeof(double));
     m.mem = static_cast<char*>(c.scratch(bytes));
     char* w = m.mem;
     auto take = [&](size_t n) { char* p = w; w += al(n); return p; };
@@ -511,7 +678,10 @@ AcaRun::AcaRun(Ctx& c, const std::vector<AcaDesc>& ds) : m_(new Impl(c)) {
     double* csum = rsum + cparts;
     double* du = reinterpret_cast<double*>(take(2 * ts * sizeof(double)));
     double* dv = du + ts;
+    double* pd = reinterpret_cast<double*>(take(pp * sizeof(double)));
     for (auto& s : m.h) {
+        s.pdu = pd; pd += (size_t)s.cap * s.plu;
+        s.pdv = pd; pd += (size_t)s.cap * s.plv;
         s.urow = bitp; bitp += (s.nro + 31) / 32;
         s.ucol = bitp; bitp += (s.ncl + 31) / 32;
         s.rpart = rp; rp += s.rch;
@@ -526,6 +696,7 @@ AcaRun::AcaRun(Ctx& c, const std::vector<AcaDesc>& ds) : m_(new Impl(c)) {
 }
 
 AcaRun::~AcaRun() {
+    for (char* p : m_->extra) m_->c.scratch_free(p);
     if (m_->mem) m_->c.scratch_free(m_->mem);
 }
 
@@ -534,6 +705,12 @@ void AcaRun::widen(int b, double* U, double* V, int cap) {
     KMH_CHECK(b >= 0 && b < m.nb && cap <= ACA_COEF, KMH_E_BAD_ARG,
               "ACA rank exceeds the supported maximum of 4096");
     AcaState& s = m.h[b];
+    if (cap > s.cap) {   // the product partials grow with the capacity
+        char* pd = static_cast<char*>(m.c.scratch((size_t)cap * (s.plu + s.plv) * sizeof(double)));
+        m.extra.push_back(pd);
+        s.pdu = reinterpret_cast<double*>(pd);
+        s.pdv = s.pdu + (size_t)cap * s.plu;
+    }
     s.U = U;
     s.V = V;
     s.cap = cap;
@@ -557,6 +734,8 @@ void AcaRun::run() {
             cur[b].U = m.h[b].U;
             cur[b].V = m.h[b].V;
             cur[b].cap = m.h[b].cap;
+            cur[b].pdu = m.h[b].pdu;
+            cur[b].pdv = m.h[b].pdv;
             pl.push_back({b, 0, 0, 0});
         }
         m.widen.clear();
@@ -585,8 +764,9 @@ void AcaRun::run() {
             KMH_LAUNCH(c, k_aca_row, dim3(cch), dim3(256), m.d_st, d_pl, np);
             KMH_LAUNCH(c, k_aca_piv, dim3(np), dim3(64), m.d_st, d_pl);
             KMH_LAUNCH(c, k_aca_col, dim3(cch + rch), dim3(256), m.d_st, d_pl, np, cch);
-            KMH_LAUNCH(c, k_aca_dots, dim3((unsigned)((int64_t)np * fb)), dim3(256), m.d_st, d_pl, np, fb);
-            KMH_LAUNCH(c, k_aca_fin, dim3(np), dim3(64), m.d_st, d_pl);
+            const int fb4 = (fb + 3) / 4;
+            KMH_LAUNCH(c, k_aca_dred, dim3((unsigned)((int64_t)np * fb4)), dim3(256), m.d_st, d_pl, np, fb4);
+            KMH_LAUNCH(c, k_aca_fin, dim3(np), dim3(256), m.d_st, d_pl);
         }
     }
 }
