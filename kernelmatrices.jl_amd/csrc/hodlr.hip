@@ -521,18 +521,18 @@ static void woodbury_batch(Ctx& c, int mode, const std::vector<Rhs>& rs) {
 // W_lv^{-1} applied to the shared Nystrom panels [A B] (n x 2r).  With
 // M = blkdiag(Q_V, Q_U), V = B rows of the first child and U = A rows of the
 // second child, M^T [A B] has the blocks Q_V^T B_c = R_V and Q_U^T A_a = R_U
-// already: only Q_V^T A_c and Q_U^T B_a need a product.  Rb holds R_V, R_U
-// (r x r each, then an r x r slot) per block, in block order.
+// already: only Q_V^T A_c and Q_U^T B_a need a product.  sbs[i] holds, per
+// block of handle i in block order, a 2r x 2r slot (ld 2r) into which the
+// panel QR wrote R_V (top right) and R_U (bottom left), followed by r^2
+// doubles of the factor stage (5 r^2 per block).
 static void woodbury_shared_basis(Ctx& c, const std::vector<Hodlr*>& hs, const std::vector<int>& lvs,
-                                  const std::vector<double*>& rbs) {
+                                  const std::vector<double*>& sbs) {
     size_t ssz = 0;
     for (size_t i = 0; i < hs.size(); ++i)
         for (int r : hs[i]->ranks[lvs[i]]) ssz += (size_t)4 * r * r;
     if (ssz == 0) return;
-    double* S = static_cast<double*>(c.scratch(2 * ssz * sizeof(double)));
-    double* S2 = S + ssz;
+    double* S2 = static_cast<double*>(c.scratch(ssz * sizeof(double)));
     std::vector<GemmProb> tn, op, up;
-    std::vector<Copy2D> rc;
     size_t off = 0;
     for (size_t i = 0; i < hs.size(); ++i) {
         Hodlr& H = *hs[i];
@@ -542,7 +542,7 @@ static void woodbury_shared_basis(Ctx& c, const std::vector<Hodlr*>& hs, const s
         const double* G = H.G.as<double>();
         const bool e = H.ab32(), eq = H.p32();
         double* AB = H.AB;
-        const double* rb = rbs[i];
+        double* s = sbs[i];
         for (size_t k = 0; k < H.blocks[lv].size(); ++k) {
             const auto& b = H.blocks[lv][k];
             const int r = H.ranks[lv][k];
@@ -550,11 +550,7 @@ static void woodbury_shared_basis(Ctx& c, const std::vector<Hodlr*>& hs, const s
             const int h1 = (int)(b.d1 - b.c0), h2 = (int)(b.b1 - b.a0);
             const double* Q1 = H.Pat(P + b.c0);
             const double* Q2 = H.Pat(P + b.a0);
-            const double* R1 = rb;
-            const double* R2 = rb + (size_t)r * r;
-            rb += (size_t)3 * r * r;
-            double* s = S + off;     // 2r x 2r, ld 2r: [Q_V^T A_c, R_V; R_U, Q_U^T B_a]
-            double* s2 = S2 + off;
+            double* s2 = S2 + off;   // s = [Q_V^T A_c, R_V; R_U, Q_U^T B_a], 2r x 2r, ld 2r
             off += (size_t)4 * r * r;
             const int l2 = 2 * r;
             GemmProb t1{r, r, h1, Q1, (int)n, tadv(AB, b.c0, e), (int)n, s, l2, 1.0, 0.0, 0};
@@ -564,8 +560,6 @@ static void woodbury_shared_basis(Ctx& c, const std::vector<Hodlr*>& hs, const s
             t1.eb = t2.eb = e;
             tn.push_back(t1);
             tn.push_back(t2);
-            rc.push_back({R1, r, s + (size_t)r * l2, l2, r, r, 1.0});
-            rc.push_back({R2, r, s + r, l2, r, r, 1.0});
             // the first r rows of G are zero: only the second child's rows of
             // the panels change (see woodbury_batch)
             op.push_back({r, l2, l2, G + H.coff_blk[lv][k] + r, l2, s, l2, s2 + r, l2, 1.0, 0.0, 0});
@@ -573,23 +567,26 @@ static void woodbury_shared_basis(Ctx& c, const std::vector<Hodlr*>& hs, const s
             u2.ea = eq;
             u2.ec = e;
             up.push_back(u2);
+            s += (size_t)5 * r * r;
         }
     }
     gemm(c, true, false, tn);
-    copy2d(c, rc);
     gemm(c, false, false, op);
     gemm(c, false, false, up);
-    c.scratch_free(S);
+    c.scratch_free(S2);
 }
 
 // One level per handle (levels may differ across handles).
 void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vector<int>& lvs) {
     // Nystrom: this level's panels are row ranges of the cascade panels; the
     // panel QR reads them there and writes Q into the level's slab of P
+    // per block 5 r^2 doubles: a 2r x 2r slot for the R factors (laid out as
+    // the shared-basis coefficient matrix when a Woodbury update of Nystrom
+    // panels follows, else R1 and R2 at its start, ld r), then Bm
     size_t rsz = 0;
     for (size_t ai = 0; ai < act.size(); ++ai)
-        for (int r : act[ai]->ranks[lvs[ai]]) rsz += (size_t)3 * r * r;
-    double* Rb = static_cast<double*>(c.scratch(rsz * sizeof(double)));
+        for (int r : act[ai]->ranks[lvs[ai]]) rsz += (size_t)5 * r * r;
+    double* Rb = static_cast<double*>(c.scratch(std::max<size_t>(1, rsz) * sizeof(double)));
     std::vector<QrPanel> qp;
     std::vector<Copy2D> qcopy;
     std::vector<GemmProb> bm;
@@ -600,6 +597,7 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
     std::vector<double*> cm_out;
     std::vector<TrsmProb> ginv;
     std::vector<Axpby2D> neg;
+    std::vector<CapDesc> cap;
     size_t off = 0;
     for (size_t ai = 0; ai < act.size(); ++ai) {
         Hodlr& H = *act[ai];
@@ -611,16 +609,19 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
         double* G = H.G.as<double>();
         int pb = H.nleaves();
         for (int l = 0; l < lv; ++l) pb += (int)H.blocks[l].size();
+        const bool shared = H.AB && lv + 1 < H.L;   // woodbury_shared_basis follows
         for (size_t k = 0; k < H.blocks[lv].size(); ++k) {
             const auto& b = H.blocks[lv][k];
             const int r = H.ranks[lv][k];
-            double* R1 = Rb + off;
-            double* R2 = R1 + (size_t)r * r;
-            double* Bm = R2 + (size_t)r * r;
-            off += (size_t)3 * r * r;
+            double* slot = Rb + off;
+            const int ldr = shared ? 2 * r : r;
+            double* R1 = shared ? slot + (size_t)r * 2 * r : slot;   // R_V: top right of the coefficient slot
+            double* R2 = shared ? slot + r : slot + (size_t)r * r;   // R_U: bottom left
+            double* Bm = slot + (size_t)4 * r * r;
+            off += (size_t)5 * r * r;
             // lrsymfact(V, U): V (first child) first (symfact.jl:31, HODLR/src/utils.jl:29-35)
-            QrPanel qv{H.Pat(P + b.c0), (int)(b.d1 - b.c0), r, (int)n, R1, r};
-            QrPanel qu{H.Pat(P + b.a0), (int)(b.b1 - b.a0), r, (int)n, R2, r};
+            QrPanel qv{H.Pat(P + b.c0), (int)(b.d1 - b.c0), r, (int)n, R1, ldr};
+            QrPanel qu{H.Pat(P + b.a0), (int)(b.b1 - b.a0), r, (int)n, R2, ldr};
             qv.e32 = qu.e32 = e;
             if (H.AB && e == eab) {   // V: B half of the first child's rows; U: A half of the second child's
                 qv.src = tadv(H.AB, (size_t)n * r + b.c0, e);
@@ -637,9 +638,15 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
             }
             qp.push_back(qv);
             qp.push_back(qu);
-            bm.push_back({r, r, r, R1, r, R2, r, Bm, r, 1.0, 0.0, 0});
             double* Xb = X + H.coff_blk[lv][k];
             double* Gb = G + H.coff_blk[lv][k];
+            const int l2 = 2 * r;
+            if (r <= CAP_MAX) {
+                // the whole capacitance computation in one workgroup (la.h)
+                cap.push_back({R1, R2, ldr, r, Xb, Gb, l2, H.parts + pb + (int)k, H.info.as<int>()});
+                continue;
+            }
+            bm.push_back({r, r, r, R1, ldr, R2, ldr, Bm, r, 1.0, 0.0, 0});
             // C = [I, Bm; Bm^T, I] (2r x 2r, Bm = R1 R2^T) has the Cholesky
             // factor [I, 0; Bm^T, L22] with L22 = chol(S), S = I - Bm^T Bm (the
             // first r columns of a blocked Cholesky of C are exact), so only
@@ -648,7 +655,6 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
             //   G = I - chol(C)^-1 = [0, 0; L22^-1 Bm^T, I - L22^-1]
             // The Woodbury products read only the last r rows of X and G, so
             // the zero first r rows are neither written nor read.
-            const int l2 = 2 * r;
             double* X21 = Xb + r;
             double* X22 = Xb + r + (size_t)r * l2;
             double* G21 = Gb + r;
@@ -668,6 +674,7 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
     }
     copy2d(c, qcopy);
     qr_panels(c, qp);
+    capacitance(c, cap);
     gemm(c, false, true, bm);                    // Bm = R1 R2^T
     copy2d(c, lowleft, true);                    // X21 = Bm^T
     set_identity(c, eye_s);
@@ -691,7 +698,7 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
         Hodlr& H = *h;
         const int lv = lvs[ai];
         double* r0 = Rb + off;
-        for (int r : H.ranks[lv]) off += (size_t)3 * r * r;
+        for (int r : H.ranks[lv]) off += (size_t)5 * r * r;
         if (lv + 1 >= H.L) continue;
         if (H.AB) {
             nys.push_back(h);

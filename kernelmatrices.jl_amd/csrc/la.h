@@ -93,6 +93,25 @@ struct QrPanel {
 };
 void qr_panels(Ctx& c, const std::vector<QrPanel>& panels);
 
+// The capacitance factors of one block of lrsymfact (HODLR/src/utils.jl:29-35)
+// for r <= CAP_MAX, in one workgroup per block: with Bm = R1 R2^T and
+// C = [I, Bm; Bm^T, I] = chol(C) chol(C)^T, chol(C) = [I, 0; Bm^T, L22] and
+// L22 = chol(I - Bm^T Bm), it writes rows r..2r-1 of the 2r x 2r blocks
+//   X = chol(C) - I    = [Bm^T, L22 - I]
+//   G = I - chol(C)^-1 = [L22^-1 Bm^T, I - L22^-1]
+// (their first r rows are zero and are neither written nor read), the sum of
+// log diag(L22) to *part, and sets *info to 1 when I - Bm^T Bm is not
+// positive definite.
+constexpr int CAP_MAX = 80;
+struct CapDesc {
+    const double* R1; const double* R2; int ldr;   // R_V and R_U, r x r upper (zero below)
+    int r;
+    double* X; double* G; int ld;                  // ld >= 2r
+    double* part;
+    int* info;
+};
+void capacitance(Ctx& c, const std::vector<CapDesc>& ds);
+
 // Batched 2-D copy / transpose / elementwise helpers.
 struct Copy2D {
     const double* src; int lds;

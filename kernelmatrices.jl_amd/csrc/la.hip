@@ -992,6 +992,296 @@ void potrf(Ctx& c, const std::vector<SqMat>& mats) {
     KMH_LAUNCH(c, k_zero_upper, dim3(16, (unsigned)mats.size()), dim3(256), dm);
 }
 
+// ---------------------------------------------------------------------------
+// Capacitance factors of lrsymfact (see la.h), one 256-thread workgroup per
+// block, r <= CAP_MAX, everything in LDS in 16 x 16 blocks (the matrices are
+// padded to nt = ceil(r / 16) blocks with the identity, which the Cholesky
+// factor and its inverse keep):
+//   1. Bm = R1 R2^T on v_mfma_f64_16x16x4f64 from the upper triangles
+//      staged in LDS (output tile (bi, bj) starts at k = 16 max(bi, bj));
+//      Bm to LDS and Bm^T straight to X's lower left block;
+//   2. S = I - Bm^T Bm, the lower tiles on MFMA from LDS, into a packed
+//      lower triangle in LDS;
+//   3. blocked right-looking Cholesky: per block column p, wave 0 factors
+//      the diagonal block in registers and replaces it by its inverse after
+//      writing it out as part of X22 = L22 - I; the blocks below it
+//      become S_ip L_pp^-T and the trailing blocks lose L_ip L_jp^T (MFMA);
+//   4. blocked inverse, block columns from the last: with the diagonal
+//      blocks already inverted, U_k = L_kj Linv_jj replaces L_kj, then
+//      Linv_ij = -sum_{j<k<=i} Linv_ik U_k (MFMA);
+//   5. G21 = L22^-1 Bm^T as its transpose Bm L22^-T on MFMA (a store writes a
+//      16-row column segment), G22 = I - L22^-1, the log-diagonal sum.
+constexpr int CAP_LB = CAP_MAX + 1;   // row stride of Bm in LDS
+
+// element (i, j), i >= j, of the packed lower triangle (CAP_MAX rows, by columns)
+__device__ __forceinline__ int cap_ix(int i, int j) { return j * CAP_MAX - ((j * (j - 1)) >> 1) + i - j; }
+
+__global__ __launch_bounds__(256, 2) void k_capacitance(const CapDesc* __restrict__ ds) {
+    __shared__ double Bs[CAP_MAX * CAP_LB];             // Bm(i, j) at i * CAP_LB + j
+    __shared__ double Ls[CAP_MAX * (CAP_MAX + 1) / 2];  // S -> L22 (diagonal blocks: their inverses) -> L22^-1
+    const CapDesc d = ds[blockIdx.x];
+    const int r = d.r, t = threadIdx.x;
+    const int wave = t >> 6, lane = t & 63, li = lane & 15, lk = lane >> 4;
+    if (r <= 0) {
+        if (t == 0) *d.part = 0.0;
+        return;
+    }
+    const int nt = (r + 15) >> 4, R = 16 * nt;
+    const Glob<const double> R1 = glob(d.R1), R2 = glob(d.R2);
+    const Glob<double> X = glob(d.X), G = glob(d.G);
+    const d4 zero4 = {0.0, 0.0, 0.0, 0.0};
+    // 1. Bm = R1 R2^T: R1 and R2 staged in LDS first (upper triangles
+    //    packed by columns, R1 in Ls and R2 in Bs; all loads of a thread in
+    //    flight together), the tiles in registers until every wave has read
+    //    R2, then Bm to Bs and Bm^T to X's lower left block
+    {
+        constexpr int NL = (CAP_MAX * CAP_MAX + 255) / 256;
+        double v1[NL], v2[NL];
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+            const int e = t + 256 * q, j = e / CAP_MAX, i = e - j * CAP_MAX;
+            const bool in = i <= j && j < r;
+            const size_t off = in ? (size_t)i + (size_t)j * d.ldr : 0;
+            v1[q] = keep_or_zero(R1[off], in);
+            v2[q] = keep_or_zero(R2[off], in);
+        }
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+            const int e = t + 256 * q, j = e / CAP_MAX, i = e - j * CAP_MAX;
+            if (i <= j && j < CAP_MAX) {
+                Ls[((j * (j + 1)) >> 1) + i] = v1[q];
+                Bs[((j * (j + 1)) >> 1) + i] = v2[q];
+            }
+        }
+    }
+    __syncthreads();
+    auto rup = [](const double* P, int i, int j) { return i <= j ? P[((j * (j + 1)) >> 1) + i] : 0.0; };
+    constexpr int BMT = ((CAP_MAX / 16) * (CAP_MAX / 16) + 3) / 4;   // tiles per wave
+    d4 bmt[BMT];
+#pragma unroll
+    for (int q8 = 0; q8 < BMT; ++q8) {
+        const int tile = wave + 4 * q8;
+        d4 acc = zero4;
+        if (tile < nt * nt) {
+            const int bi = tile / nt, bj = tile - bi * nt;
+            for (int k0 = 16 * max(bi, bj); k0 < r; k0 += 4) {
+                const int k = k0 + lk;   // < CAP_MAX; R1(ra, k) and R2(rb, k) are zero beyond r
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(rup(Ls, 16 * bi + li, k), rup(Bs, 16 * bj + li, k), acc, 0, 0, 0);
+            }
+        }
+        bmt[q8] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q8 = 0; q8 < BMT; ++q8) {
+        const int tile = wave + 4 * q8;
+        if (tile < nt * nt) {
+            const int bi = tile / nt, bj = tile - bi * nt, rb = 16 * bj + li;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int m = 16 * bi + mfma_d_row(lk, q);   // D(m, rb) = Bm(m, rb)
+                if (m < r && rb < r) {
+                    Bs[m * CAP_LB + rb] = bmt[q8][q];
+                    X[(size_t)(r + rb) + (size_t)m * d.ld] = bmt[q8][q];   // X21 = Bm^T
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // 2. S = I - Bm^T Bm (lower tiles), identity in the padding
+    for (int tile = wave; tile < nt * (nt + 1) / 2; tile += 4) {
+        int bi = 0;
+        while ((bi + 1) * (bi + 2) / 2 <= tile) ++bi;
+        const int bj = tile - bi * (bi + 1) / 2;
+        const int ci = 16 * bi + li, cj = 16 * bj + li;   // < CAP_MAX
+        d4 acc = zero4;
+        for (int k0 = 0; k0 < r; k0 += 4) {
+            const int k = k0 + lk;   // < CAP_MAX
+            const double a = keep_or_zero(Bs[k * CAP_LB + ci], k < r && ci < r);   // Bm^T(ci, k)
+            const double b = keep_or_zero(Bs[k * CAP_LB + cj], k < r && cj < r);   // Bm(k, cj)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int m = 16 * bi + mfma_d_row(lk, q);
+            if (m >= cj) Ls[cap_ix(m, cj)] = (m == cj ? 1.0 : 0.0) - ((m < r && cj < r) ? acc[q] : 0.0);
+        }
+    }
+    __syncthreads();
+    // 3. blocked Cholesky
+    double lsum = 0.0;   // wave 0, lane i < 16: log L22(16p + i, 16p + i) summed over p
+    for (int p = 0; p < nt; ++p) {
+        const int o = 16 * p;
+        if (wave == 0) {
+            // the diagonal block in registers, lane i < 16 holding row i
+            // (dpotf2 arithmetic, values of other rows by lane broadcasts)
+            const int ri = li;   // lanes >= 16 repeat lanes 0..15 and store nothing
+            double a[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) a[k] = k <= ri ? Ls[cap_ix(o + ri, o + k)] : 0.0;
+            bool bad = false;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const double ajj = lane_bcast(a[j], j);
+                bad |= (o + j < r) && !(ajj > 0.0);
+                const double ljj = sqrt(ajj);
+                a[j] = ri == j ? ljj : (ri > j ? a[j] / ljj : 0.0);
+#pragma unroll
+                for (int k = j + 1; k < 16; ++k) a[k] = __fma_rn(-a[j], lane_bcast(a[j], k), a[k]);
+            }
+            if (bad && lane == 0) atomicExch(d.info, 1);
+            if (lane < 16 && o + ri < r) lsum += log(a[ri < 16 ? ri : 0]);
+            // X22 = L22 - I on the diagonal block (zero above the diagonal)
+            if (lane < 16 && o + ri < r) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k)
+                    if (o + k < r) X[(size_t)(r + o + ri) + (size_t)(r + o + k) * d.ld] = k < ri ? a[k] : (k == ri ? a[k] - 1.0 : 0.0);
+            }
+            // its inverse, lane c < 16 solving column c: y_i = (delta_ic -
+            // sum_{k<i} L(i,k) y_k) / L(i,i)
+            double y[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                double sacc = i == ri ? 1.0 : 0.0;
+#pragma unroll
+                for (int k = 0; k < i; ++k) sacc = __fma_rn(-lane_bcast(a[k], i), y[k], sacc);
+                y[i] = sacc / lane_bcast(a[i], i);
+            }
+            if (lane < 16) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i)
+                    if (i >= ri) Ls[cap_ix(o + i, o + ri)] = y[i];
+            }
+        }
+        __syncthreads();
+        // the panel below: L_ip = S_ip L_pp^-T (one block per wave)
+        for (int bi = p + 1 + wave; bi < nt; bi += 4) {
+            d4 acc = zero4;
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int k = 4 * s4 + lk;
+                const double a = Ls[cap_ix(16 * bi + li, o + k)];                      // S_ip(li, k)
+                const double b = li >= k ? Ls[cap_ix(o + li, o + k)] : 0.0;            // Linv_pp(li, k)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Ls[cap_ix(16 * bi + mfma_d_row(lk, q), o + li)] = acc[q];
+        }
+        __syncthreads();
+        // trailing blocks (bi >= bj > p): S_ij -= L_ip L_jp^T
+        const int nrem = nt - 1 - p;
+        for (int tile = wave; tile < nrem * (nrem + 1) / 2; tile += 4) {
+            int ti = 0;
+            while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
+            const int bi = p + 1 + ti, bj = p + 1 + tile - ti * (ti + 1) / 2;
+            d4 acc;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int m = 16 * bi + mfma_d_row(lk, q), n = 16 * bj + li;
+                acc[q] = m >= n ? Ls[cap_ix(m, n)] : 0.0;
+            }
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int k = o + 4 * s4 + lk;
+                const double a = -Ls[cap_ix(16 * bi + li, k)];
+                const double b = Ls[cap_ix(16 * bj + li, k)];
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int m = 16 * bi + mfma_d_row(lk, q), n = 16 * bj + li;
+                if (m >= n) Ls[cap_ix(m, n)] = acc[q];
+            }
+        }
+        __syncthreads();
+    }
+    // X22 below the diagonal blocks (L22) and above them (zero)
+    for (int e = t; e < r * r; e += 256) {
+        const int c = e / r, i = e - c * r;
+        if ((i >> 4) != (c >> 4)) X[(size_t)(r + i) + (size_t)(r + c) * d.ld] = i > c ? Ls[cap_ix(i, c)] : 0.0;
+    }
+    // 4. blocked inverse, block columns from the last
+    for (int bj = nt - 2; bj >= 0; --bj) {
+        const int o = 16 * bj;
+        d4 acc = zero4;
+        const int bk = bj + 1 + wave;   // U_k = L_kj Linv_jj
+        if (bk < nt) {
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int k = 4 * s4 + lk;
+                const double a = Ls[cap_ix(16 * bk + li, o + k)];               // L_kj(li, k)
+                const double b = k >= li ? Ls[cap_ix(o + k, o + li)] : 0.0;     // Linv_jj(k, li)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+        if (bk < nt) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Ls[cap_ix(16 * bk + mfma_d_row(lk, q), o + li)] = acc[q];
+        }
+        __syncthreads();
+        const int bi = bj + 1 + wave;   // Linv_ij = -sum_{j<k<=i} Linv_ik U_k
+        acc = zero4;
+        if (bi < nt) {
+            for (int bk2 = bj + 1; bk2 <= bi; ++bk2) {
+#pragma unroll
+                for (int s4 = 0; s4 < 4; ++s4) {
+                    const int k = 4 * s4 + lk;
+                    const int ii = 16 * bi + li, kk = 16 * bk2 + k;
+                    const double a = ii >= kk ? -Ls[cap_ix(ii, kk)] : 0.0;          // -Linv_ik(li, k)
+                    const double b = Ls[cap_ix(kk, o + li)];                        // U_k(k, li)
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+        if (bi < nt) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Ls[cap_ix(16 * bi + mfma_d_row(lk, q), o + li)] = acc[q];
+        }
+        __syncthreads();
+    }
+    // 5. G21 = L22^-1 Bm^T as Bm L22^-T; G22 = I - L22^-1
+    auto linv = [&](int i, int k) { return i >= k ? Ls[cap_ix(i, k)] : 0.0; };
+    for (int tile = wave; tile < nt * nt; tile += 4) {
+        const int bi = tile / nt, bj = tile - bi * nt;
+        const int mi = 16 * bi + li, nj = 16 * bj + li;   // < CAP_MAX
+        d4 acc = zero4;
+        const int kend = min(r, 16 * (bj + 1));
+        for (int k0 = 0; k0 < kend; k0 += 4) {
+            const int k = k0 + lk;   // < CAP_MAX
+            const bool ok = k < r;
+            const double a = keep_or_zero(Bs[mi * CAP_LB + k], ok && mi < r);   // Bm(mi, k)
+            const double b = keep_or_zero(linv(nj, k), ok && nj < r);          // L22^-T(k, nj)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int m = 16 * bi + mfma_d_row(lk, q);   // D(m, nj) = G21(nj, m)
+            if (m < r && nj < r) G[(size_t)(r + nj) + (size_t)m * d.ld] = acc[q];
+        }
+    }
+    for (int e = t; e < r * r; e += 256) {
+        const int c = e / r, i = e - c * r;
+        G[(size_t)(r + i) + (size_t)(r + c) * d.ld] = (i == c ? 1.0 : 0.0) - linv(i, c);
+    }
+    if (wave == 0) {
+        double tot = 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) tot += lane_bcast(lsum, i);
+        if (lane == 0) *d.part = tot;
+    }
+}
+
+void capacitance(Ctx& c, const std::vector<CapDesc>& ds) {
+    if (ds.empty()) return;
+    for (const auto& d : ds)
+        KMH_CHECK(d.r <= CAP_MAX && d.ld >= 2 * d.r && d.ldr >= d.r, KMH_E_BAD_ARG, "capacitance: block beyond its limits");
+    CapDesc* dd = c.upload_vec(ds);
+    KMH_LAUNCH(c, k_capacitance, dim3((unsigned)ds.size()), dim3(256), dd);
+}
+
 void trsm_left(Ctx& c, bool trans, const std::vector<TrsmProb>& probs) {
     // FP32 right-hand sides (left lower solves only) in launches of their own
     bool any32 = false, any64 = false;
