@@ -18,10 +18,11 @@ namespace kmh {
 // ---------------------------------------------------------------------------
 // Householder QR of small tiles (TSQR building block)
 // ---------------------------------------------------------------------------
-constexpr int QT = 128;   // max rows of a tile: one wave, two rows per lane
+constexpr int QT = 128;   // tiles of at most QT rows: one wave, two rows per lane
+constexpr int QTL = 256;  // max rows of a tile: one wave, four rows per lane
 constexpr int QC = 24;    // max columns of a tile (wider panels: block Gram-Schmidt;
                           // the Nystrom rank 72 splits into three blocks)
-constexpr int QF = 4;     // TSQR fan-in: QF stacked QC x QC factors fill one tile
+constexpr int QF = 8;     // TSQR fan-in: QF stacked QC x QC factors fill one tile
 
 // A tile is nstack blocks of nbs rows stacked vertically; block i starts at
 // src + i * slot (column-major, leading dimension ld).  A leaf tile is one
@@ -42,28 +43,33 @@ __device__ __forceinline__ double qr_bcast(double v, int lane) {
     return __hiloint2double(hi, lo);
 }
 
-// One round of column sums over the wave's 128 rows of p(row, k) =
-// c(row) a(row, k) for the H columns cb..cb+H-1: lane l ends with the sum of
-// column cb + l % H.  Lane l holds rows l and l + 64 (c0, a0 and c1, a1).
-template <int NC, int H, int CB>
-__device__ __forceinline__ double colsum_round(double c0, const double (&a0)[NC], double c1, const double (&a1)[NC],
-                                               double* pw, int lane) {
+// One round of column sums over the wave's rows of p(row, k) = c(row)
+// a(row, k) for the H columns cb..cb+H-1: lane l ends with the sum of
+// column cb + l % H.  Lane l holds rows l + 64 i, i < RPL (c[i], a[i]).
+template <int RPL, int NC, int H, int CB>
+__device__ __forceinline__ double colsum_round(const double (&c)[RPL], const double (&a)[RPL][NC], double* pw,
+                                               int lane) {
 #pragma unroll
-    for (int k = 0; k < H; ++k) pw[lane * (H + 1) + k] = c0 * a0[CB + k] + c1 * a1[CB + k];
+    for (int k = 0; k < H; ++k) {
+        double v = c[0] * a[0][CB + k];
+#pragma unroll
+        for (int i = 1; i < RPL; ++i) v = __fma_rn(c[i], a[i][CB + k], v);
+        pw[lane * (H + 1) + k] = v;
+    }
     return lds_colsum_read<H>(pw, lane);
 }
 
 // Column sums of the first W columns (W a multiple of 8): lane l < W ends
 // with S_l (rounds of 16 columns, then one of 8 if W is odd in eighths).
-template <int NC, int W, int CB = 0>
-__device__ __forceinline__ double wave_colsum(double c0, const double (&a0)[NC], double c1, const double (&a1)[NC],
-                                              double* pw, int lane, double res = 0.0) {
+template <int RPL, int NC, int W, int CB = 0>
+__device__ __forceinline__ double wave_colsum(const double (&c)[RPL], const double (&a)[RPL][NC], double* pw, int lane,
+                                              double res = 0.0) {
     if constexpr (CB + 16 <= W) {
-        const double s = colsum_round<NC, 16, CB>(c0, a0, c1, a1, pw, lane);
+        const double s = colsum_round<RPL, NC, 16, CB>(c, a, pw, lane);
         if (lane >= CB && lane < CB + 16) res = s;
-        return wave_colsum<NC, W, CB + 16>(c0, a0, c1, a1, pw, lane, res);
+        return wave_colsum<RPL, NC, W, CB + 16>(c, a, pw, lane, res);
     } else if constexpr (CB + 8 <= W) {
-        const double s = colsum_round<NC, 8, CB>(c0, a0, c1, a1, pw, lane);
+        const double s = colsum_round<RPL, NC, 8, CB>(c, a, pw, lane);
         if (lane >= CB && lane < CB + 8) res = s;
         return res;
     } else {
@@ -73,9 +79,9 @@ __device__ __forceinline__ double wave_colsum(double c0, const double (&a0)[NC],
 
 template <int N> using ic = std::integral_constant<int, N>;
 
-// Householder QR (LAPACK dgeqr2 arithmetic) of a tile of <= 128 rows and
-// nc <= NC columns by one wave: lane l holds rows l and l + 64 in the
-// register slots a0[NC], a1[NC].  The outputs are R, the Householder vectors
+// Householder QR (LAPACK dgeqr2 arithmetic) of a tile of <= 64 RPL rows and
+// nc <= NC columns by one wave: lane l holds rows l + 64 i (i < RPL) in the
+// register slots a[i][NC].  The outputs are R, the Householder vectors
 // V (explicit: zero above row j, 1 at row j, below it scal times the pivot
 // column) and the compact-WY coupling Z = diag(tau) + striu(V^T V) (ld QC),
 // from which k_wy_apply forms Q C: with T the triangular factor of
@@ -97,43 +103,56 @@ template <int N> using ic = std::integral_constant<int, N>;
 // from the same wave-wide reduction over pair products of the slots.
 // Pivot-row values come from lane j through LDS (rowj), column sums from the
 // wave-local reduction, and w reaches every lane through LDS (wsh).  Lanes
-// k < nc - j write row j of R straight to the R output.  The 24-column tile
-// runs three waves per SIMD: its step loop spills at the 128 registers of
-// four, which costs more than the fourth wave gains.
-template <int NC, class E>
-__global__ __launch_bounds__(64, NC <= 16 ? 4 : 3) void k_tile_qr(const SqrDesc* __restrict__ ds) {
+// k < nc - j write row j of R straight to the R output.  Pivot rows are
+// rows < 24 < 64, so only slot row 0 can lie at or above the pivot.
+//   RPL = 2 (tiles <= 128 rows): the 24-column tile runs three waves per
+//   SIMD; its step loop spills at the 128 registers of four.
+//   RPL = 4 (tiles <= 256 rows): two waves per SIMD; half the LDS
+//   reduction work per row, and a 256-row panel is one tile.
+template <int NC, class E, int RPL>
+__global__ __launch_bounds__(64, RPL == 4 ? 2 : (NC <= 16 ? 4 : 3)) void k_tile_qr(const SqrDesc* __restrict__ ds) {
     static_assert(NC % 8 == 0 && NC <= QC, "tile width");
+    static_assert(RPL == 2 || RPL == 4, "rows per lane");
     __shared__ double pw[64 * 17];
     __shared__ double rowj[NC];
     __shared__ double wsh[NC];   // w_k of the current step, read back as broadcasts
     const SqrDesc d = ds[blockIdx.x];
     const int nc = d.nc, mt = d.nstack * d.nbs;
     const int lane = threadIdx.x;
-    const int r0 = lane, r1 = lane + 64;
-    double a0[NC], a1[NC];
+    double a[RPL][NC];
     {
         // Branch-free loads: out-of-range rows and columns read a valid
         // element of the tile, and a bit mask (not a select, which the
         // compiler turns into a branch around a load and a wait for it per
         // column) zeroes them.  Global address space: the loads count on
-        // the vector-memory counter only, so all 2 NC stay in flight.
+        // the vector-memory counter only, so all of them stay in flight.
         typedef const __attribute__((address_space(1))) E* GSrc;
-        const bool in0 = r0 < mt, in1 = r1 < mt;
-        const GSrc s0 = (GSrc)d.src + (in0 ? (size_t)(r0 / d.nbs) * d.slot + (r0 % d.nbs) : 0);
-        const GSrc s1 = (GSrc)d.src + (in1 ? (size_t)(r1 / d.nbs) * d.slot + (r1 % d.nbs) : 0);
+        GSrc sp[RPL];
+        bool in[RPL];
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) {
+            const int ri = lane + 64 * i;
+            in[i] = ri < mt;
+            sp[i] = (GSrc)d.src + (in[i] ? (size_t)(ri / d.nbs) * d.slot + (ri % d.nbs) : 0);
+        }
 #pragma unroll
         for (int k = 0; k < NC; ++k) {
             const size_t off = (size_t)(k < nc ? k : 0) * d.ld;
-            const unsigned long long m0 = (in0 && k < nc) ? ~0ull : 0ull;
-            const unsigned long long m1 = (in1 && k < nc) ? ~0ull : 0ull;
-            a0[k] = __longlong_as_double(__double_as_longlong((double)s0[off]) & m0);
-            a1[k] = __longlong_as_double(__double_as_longlong((double)s1[off]) & m1);
+#pragma unroll
+            for (int i = 0; i < RPL; ++i) {
+                const unsigned long long m = (in[i] && k < nc) ? ~0ull : 0ull;
+                a[i][k] = __longlong_as_double(__double_as_longlong((double)sp[i][off]) & m);
+            }
         }
     }
     // global-space views of the outputs (generic stores would make every
     // LDS wait of the steps wait for them too)
-    const Glob<E> q0 = glob(reinterpret_cast<E*>(d.qout)) + (r0 < mt ? r0 : 0);
-    const Glob<E> q1 = glob(reinterpret_cast<E*>(d.qout)) + (r1 < mt ? r1 : 0);
+    Glob<E> qo[RPL];
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) {
+        const int ri = lane + 64 * i;
+        qo[i] = glob(reinterpret_cast<E*>(d.qout)) + (ri < mt ? ri : 0);
+    }
     const Glob<double> rout = glob(d.rout), zout = glob(d.zout);
 
     // the eight factorization steps j0 <= j < j0 + 8 of a phase with window
@@ -148,13 +167,16 @@ __global__ __launch_bounds__(64, NC <= 16 ? 4 : 3) void k_tile_qr(const SqrDesc*
             const int j = j0 + s8;
             const bool live = j < nc;   // uniform
             const int vb = W - s8;   // first vector slot of this phase
-            // S_k = sum over rows > j of a(row, 0) a(row, k)   (rows of a1 are >= 64 > j)
-            const double h0 = r0 > j ? a0[0] : 0.0;
-            if (r0 == j) {
+            // S_k = sum over rows > j of a(row, 0) a(row, k)
+            double h[RPL];
+            h[0] = lane > j ? a[0][0] : 0.0;
 #pragma unroll
-                for (int k = 0; k < W; ++k) rowj[k] = a0[k];
+            for (int i = 1; i < RPL; ++i) h[i] = a[i][0];
+            if (lane == j) {
+#pragma unroll
+                for (int k = 0; k < W; ++k) rowj[k] = a[0][k];
             }
-            const double sk = wave_colsum<NC, W>(h0, a0, a1[0], a1, pw, lane);   // its LDS traffic orders rowj too
+            const double sk = wave_colsum<RPL, NC, W>(h, a, pw, lane);   // its LDS traffic orders rowj too
             const double rk = rowj[lane < W ? lane : 0];
             const double alpha = qr_bcast(rk, 0);
             const double x2 = qr_bcast(sk, 0);
@@ -166,24 +188,29 @@ __global__ __launch_bounds__(64, NC <= 16 ? 4 : 3) void k_tile_qr(const SqrDesc*
                 tau = (beta - alpha) / beta;
                 scal = 1.0 / (alpha - beta);
             }
-            const double v0 = (r0 > j) ? scal * a0[0] : (r0 == j ? 1.0 : 0.0);
-            const double v1 = scal * a1[0];
+            double v[RPL];
+            v[0] = (lane > j) ? scal * a[0][0] : (lane == j ? 1.0 : 0.0);
+#pragma unroll
+            for (int i = 1; i < RPL; ++i) v[i] = scal * a[i][0];
             const double wl = rk + scal * sk;   // v^T a_k, or v_i^T v_j in a vector slot
             if (tau != 0.0) {
                 if (lane < W) wsh[lane] = lane < vb ? wl : 0.0;
                 __builtin_amdgcn_wave_barrier();
-                const double t0 = tau * v0, t1 = tau * v1;
+                double tv[RPL];
+#pragma unroll
+                for (int i = 0; i < RPL; ++i) tv[i] = tau * v[i];
 #pragma unroll
                 for (int k = 1; k < W; ++k) {
                     const double wk = wsh[k];
-                    a0[k] = __fma_rn(-t0, wk, a0[k]);
-                    a1[k] = __fma_rn(-t1, wk, a1[k]);
+#pragma unroll
+                    for (int i = 0; i < RPL; ++i) a[i][k] = __fma_rn(-tv[i], wk, a[i][k]);
                 }
                 __builtin_amdgcn_wave_barrier();
             }
             if (live) {
-                if (r0 < mt) q0[(size_t)j * d.ldq] = (E)v0;
-                if (r1 < mt) q1[(size_t)j * d.ldq] = (E)v1;
+#pragma unroll
+                for (int i = 0; i < RPL; ++i)
+                    if (lane + 64 * i < mt) qo[i][(size_t)j * d.ldq] = (E)v[i];
                 // row j of R: lane k holds column j + k (the pivot row's updated
                 // entry, the same arithmetic as the pivot lane's update above)
                 if (lane < nc - j) {
@@ -197,11 +224,11 @@ __global__ __launch_bounds__(64, NC <= 16 ? 4 : 3) void k_tile_qr(const SqrDesc*
             }
 #pragma unroll
             for (int k = 0; k < W - 1; ++k) {
-                a0[k] = a0[k + 1];
-                a1[k] = a1[k + 1];
+#pragma unroll
+                for (int i = 0; i < RPL; ++i) a[i][k] = a[i][k + 1];
             }
-            a0[W - 1] = v0;
-            a1[W - 1] = v1;
+#pragma unroll
+            for (int i = 0; i < RPL; ++i) a[i][W - 1] = v[i];
         }
     };
     // Z entries v_i^T v_j, i in phase PI < PJ the phase of j: 16 pairs
@@ -212,7 +239,10 @@ __global__ __launch_bounds__(64, NC <= 16 ? 4 : 3) void k_tile_qr(const SqrDesc*
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const int si = NC - 8 * (PI + 1) + Q0 + k / 8, sj = NC - 8 * (PJ + 1) + k % 8;
-            pr[k] = a0[si] * a0[sj] + a1[si] * a1[sj];
+            double v = a[0][si] * a[0][sj];
+#pragma unroll
+            for (int i = 1; i < RPL; ++i) v = __fma_rn(a[i][si], a[i][sj], v);
+            pr[k] = v;
         }
         const double g = lds_colsum<16>(pr, pw, lane);
         const int i = 8 * PI + Q0 + (lane & 15) / 8, j = 8 * PJ + (lane & 7);
@@ -249,16 +279,16 @@ __global__ __launch_bounds__(64, NC <= 16 ? 4 : 3) void k_tile_qr(const SqrDesc*
     }
 }
 
-// One launch per column class (8, 16, QC = 24) and storage type of the
-// descriptors.
-template <class E>
+// One launch per column class (8, 16, QC = 24), row class (tiles of at
+// most QT rows: two rows per lane; more: four) and storage type.
+template <class E, int RPL>
 static void launch_tile_qr_t(Ctx& c, std::vector<SqrDesc> (&cls)[3]) {
     for (int k = 0; k < 3; ++k) {
         if (cls[k].empty()) continue;
         SqrDesc* dd = c.upload_vec(cls[k]);
         const dim3 g((unsigned)cls[k].size());
-        if (k == 0) KMH_LAUNCH(c, (k_tile_qr<8, E>), g, dim3(64), dd);
-        else if (k == 1) KMH_LAUNCH(c, (k_tile_qr<16, E>), g, dim3(64), dd);
+        if (k == 0) KMH_LAUNCH(c, (k_tile_qr<8, E, RPL>), g, dim3(64), dd);
+        else if (k == 1) KMH_LAUNCH(c, (k_tile_qr<16, E, RPL>), g, dim3(64), dd);
         else {
             // algorithmic bytes: the tile read, V and R written
             double by = 0.0;
@@ -267,20 +297,22 @@ static void launch_tile_qr_t(Ctx& c, std::vector<SqrDesc> (&cls)[3]) {
                     const double mt = (double)d.nstack * d.nbs;
                     by += 2.0 * sizeof(E) * mt * d.nc + 8.0 * (double)d.nc * d.nc;
                 }
-            KMH_LAUNCH_TIMED(c, KMH_PROF_TILE_QR, by, -1.0, (k_tile_qr<QC, E>), g, dim3(64), dd);
+            KMH_LAUNCH_TIMED(c, KMH_PROF_TILE_QR, by, -1.0, (k_tile_qr<QC, E, RPL>), g, dim3(64), dd);
         }
     }
 }
 
 static void launch_tile_qr(Ctx& c, const std::vector<SqrDesc>& ds) {
-    std::vector<SqrDesc> cls[2][3];
+    std::vector<SqrDesc> cls[2][2][3];   // [FP32][more than QT rows][NC = 8, 16, 24]
     for (const auto& d : ds) {
-        KMH_CHECK(d.nc <= QC && d.nstack * d.nbs <= QT && d.nstack * d.nbs >= d.nc, KMH_E_BAD_ARG,
-                  "tile QR beyond its size limits");
-        cls[d.e32 ? 1 : 0][d.nc <= 8 ? 0 : d.nc <= 16 ? 1 : 2].push_back(d);   // NC = 8, 16, 24
+        const int mt = d.nstack * d.nbs;
+        KMH_CHECK(d.nc <= QC && mt <= QTL && mt >= d.nc, KMH_E_BAD_ARG, "tile QR beyond its size limits");
+        cls[d.e32 ? 1 : 0][mt > QT ? 1 : 0][d.nc <= 8 ? 0 : d.nc <= 16 ? 1 : 2].push_back(d);
     }
-    launch_tile_qr_t<double>(c, cls[0]);
-    launch_tile_qr_t<float>(c, cls[1]);
+    launch_tile_qr_t<double, 2>(c, cls[0][0]);
+    launch_tile_qr_t<double, 4>(c, cls[0][1]);
+    launch_tile_qr_t<float, 2>(c, cls[1][0]);
+    launch_tile_qr_t<float, 4>(c, cls[1][1]);
 }
 
 // out = Q C = [C; 0] - V Y with Y = T V1^T C (V1 the top nb rows of V) for
@@ -305,7 +337,7 @@ typedef double qd4 __attribute__((ext_vector_type(4)));
 // accumulator starting from Cpad^T), so that a store instruction writes
 // 16-row column segments of out.  A wave reads a row block of V before it
 // writes that block, and V1 is in LDS before any block is written, so out
-// may alias V.  (Up to 128 rows: eight 16-row blocks, four per wave.)
+// may alias V.  (Up to QTL = 256 rows: sixteen 16-row blocks, eight per wave.)
 template <class E>
 __global__ __launch_bounds__(128, 4) void k_wy_apply(const WyDesc* __restrict__ ds) {
     constexpr int S = QC + 1;
@@ -317,7 +349,7 @@ __global__ __launch_bounds__(128, 4) void k_wy_apply(const WyDesc* __restrict__ 
     const WyDesc d = ds[blockIdx.x];
     const int nb = d.nb, t = threadIdx.x;
     const int wave = t >> 6, lane = t & 63, li = lane & 15, lk = lane >> 4;
-    // wave w takes row blocks w, w + 2, w + 4, w + 6 in two passes of two;
+    // wave w takes row blocks w + 4 p and w + 4 p + 2 in pass p;
     // the first pass's V^T fragments V(row, 4 ks + lk) load before the
     // prologue so that their latency overlaps it
     double vf[2][KS];
@@ -412,7 +444,7 @@ __global__ __launch_bounds__(128, 4) void k_wy_apply(const WyDesc* __restrict__ 
             yf[cb][ks] = (k < nb && col < nb) ? -Ys[k * QC + col] : 0.0;
         }
 #pragma unroll 1
-    for (int pass = 0; pass < 2; ++pass) {
+    for (int pass = 0; pass < QTL / 64; ++pass) {
         if ((wave + 4 * pass) * 16 >= d.mt) break;
         if (pass > 0) load_rows(pass);
         qd4 acc[2][2];
@@ -456,7 +488,7 @@ static void wy_apply(Ctx& c, const std::vector<WyDesc>& ds) {
     if (ds.empty()) return;
     std::vector<WyDesc> by[2];
     for (const auto& d : ds) {
-        KMH_CHECK(d.nb <= QC && d.mt <= QT && d.mt >= d.nb, KMH_E_BAD_ARG, "WY apply beyond its size limits");
+        KMH_CHECK(d.nb <= QC && d.mt <= QTL && d.mt >= d.nb, KMH_E_BAD_ARG, "WY apply beyond its size limits");
         by[d.e32 ? 1 : 0].push_back(d);
     }
     if (!by[0].empty()) {
@@ -495,7 +527,7 @@ static void tsqr(Ctx& c, const std::vector<TsqrPanel>& ps) {
     int nnodes = 0;
     for (size_t p = 0; p < ps.size(); ++p) {
         KMH_CHECK(ps[p].nb <= QC && ps[p].m >= ps[p].nb, KMH_E_BAD_ARG, "tsqr: invalid panel shape");
-        const int nt = std::max<int>(1, (int)ceil_div(ps[p].m, QT));
+        const int nt = std::max<int>(1, (int)ceil_div(ps[p].m, QTL));
         const int base = ps[p].m / nt, rem = ps[p].m % nt;
         int r0 = 0;
         for (int t = 0; t < nt; ++t) {

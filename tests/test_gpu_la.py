@@ -232,8 +232,9 @@ def test_profile_hooks_time_sampled_launches(gpu_lib):
 def test_profile_hooks_time_tile_qr(gpu_lib):
     """KMH_PROF_TILE_QR: the 24-column tile QR launches of a panel QR are
     timed with their algorithmic bytes (tiles read, Householder vectors and R
-    written) and stay out of the GEMM totals.  A 300 x 24 panel: three leaf
-    tiles (100 rows each) and one inner node stacking their R factors."""
+    written) and stay out of the GEMM totals.  A 300 x 24 panel: two leaf
+    tiles (150 rows each, four rows per lane) and one inner node stacking
+    their R factors."""
     import kmhodlr
     from kmhodlr import _lib as L
     ctx = kmhodlr.Context(0)
@@ -251,8 +252,8 @@ def test_profile_hooks_time_tile_qr(gpu_lib):
     assert lib.kmh_ctx_profile_query(ctx.ptr, L.KMH_PROF_TILE_QR, C.byref(b), C.byref(ms), C.byref(n)) == 0
     assert gl.value == 0 and gfl.value == 0.0           # the mask times the tile QR only
     assert n.value == 2 and ms.value > 0                # the leaf launch and the inner level
-    leaves = 3 * 8.0 * (2 * 100 * r + r * r)
-    inner = 8.0 * (2 * 3 * r * r + r * r)
+    leaves = 2 * 8.0 * (2 * 150 * r + r * r)
+    inner = 8.0 * (2 * 2 * r * r + r * r)
     assert b.value == leaves + inner
     assert np.abs(A @ R - A0).max() <= 1e-13 * np.abs(A0).max()
 
