@@ -54,6 +54,7 @@ struct Hodlr {
     std::vector<int64_t> nyind;
     DevBuf lm;                                   // landmark points, x[0..r), y[0..r)
     DevBuf kpp;                                  // chol(K(P,P)) lower, r x r
+    DevBuf kinv;                                 // K(P,P)^{-1}, r x r
     DevBuf kxp;                                  // K(x, P), n x r, ld n
     DevBuf ynys;                                 // K(x, P) K(P,P)^{-1}, n x r, ld n
 
@@ -140,8 +141,8 @@ struct Hodlr {
     }
     double* leaf_ptr(int i) const { return leaves.as<double>() + leaf_mat_off[i]; }
     size_t device_bytes() const {
-        return pts.bytes + info.bytes + leaves.bytes + P.bytes + UV.bytes + lm.bytes + kpp.bytes + kxp.bytes +
-               ynys.bytes + X.bytes + G.bytes;
+        return pts.bytes + info.bytes + leaves.bytes + P.bytes + UV.bytes + lm.bytes + kpp.bytes + kinv.bytes +
+               kxp.bytes + ynys.bytes + X.bytes + G.bytes;
     }
 };
 

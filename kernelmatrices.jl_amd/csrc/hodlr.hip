@@ -368,12 +368,11 @@ std::vector<Hodlr*> hodlr_build_batch(Ctx& c, const std::vector<BuildReq>& reqs,
         std::vector<DiagOp> eye;
         std::vector<TrsmProb> tr;
         std::vector<GemmProb> yg;
-        std::vector<double*> kis;
         for (int i : nys) {
             Hodlr& H = *hs[i];
             const int r = H.nys_r;
-            double* ki = static_cast<double*>(c.scratch((size_t)r * r * sizeof(double)));
-            kis.push_back(ki);
+            H.kinv.alloc(&c, (size_t)r * r * sizeof(double));   // kept: the factorization's leaf stage uses it
+            double* ki = H.kinv.as<double>();
             eye.push_back({ki, r, r, 1.0});
             tr.push_back({H.kpp.as<double>(), r, r, ki, r, r});
             yg.push_back({(int)H.n, r, r, H.kxp.as<double>(), (int)H.n, ki, r, H.ynys.as<double>(), (int)H.n, 1.0, 0.0, 0});
@@ -382,7 +381,6 @@ std::vector<Hodlr*> hodlr_build_batch(Ctx& c, const std::vector<BuildReq>& reqs,
         trsm_left(c, false, tr);
         trsm_left(c, true, tr);
         gemm(c, false, false, yg);
-        for (double* p : kis) c.scratch_free(p);
     }
     if (!aca.empty()) build_aca_batch(c, hs, st, aca);
     c.sync();
@@ -721,6 +719,7 @@ void factor_begin_batch(Ctx& c, const std::vector<Hodlr*>& hs) {
     std::vector<double*> lm_out;
     std::vector<Copy2D> abinit;
     std::vector<Rhs> lt;
+    std::vector<GemmProb> bhalf;
     for (Hodlr* h : hs) {
         Hodlr& H = *h;
         H.release_transient();
@@ -754,13 +753,18 @@ void factor_begin_batch(Ctx& c, const std::vector<Hodlr*>& hs) {
             const int r = H.nys_r;
             const bool e = H.ab32();
             H.P.alloc(&c, (size_t)n * H.Rtot * (H.p32() ? sizeof(float) : sizeof(double)));
+            // L^{-1} B = (L^{-1} A) K(P,P)^{-1} (B = A K(P,P)^{-1} row by row,
+            // and L^{-1} acts on rows): the leaf solves take the A half only,
+            // one n x r x r product gives the B half
             H.AB = static_cast<double*>(c.scratch((size_t)n * 2 * r * (e ? sizeof(float) : sizeof(double))));
             Copy2D a{H.kxp.as<double>(), (int)n, H.AB, (int)n, (int)n, r, 1.0};
-            Copy2D b{H.ynys.as<double>(), (int)n, tadv(H.AB, (size_t)n * r, e), (int)n, (int)n, r, 1.0};
-            a.ed = b.ed = e;
+            a.ed = e;
             abinit.push_back(a);
-            abinit.push_back(b);
-            lt.push_back({h, H.AB, n, 2 * r, 0, e});
+            lt.push_back({h, H.AB, n, r, 0, e});
+            GemmProb bh{(int)n, r, r, H.AB, (int)n, H.kinv.as<double>(), r, tadv(H.AB, (size_t)n * r, e), (int)n,
+                        1.0, 0.0, 0};
+            bh.ea = bh.ec = e;
+            bhalf.push_back(bh);
         } else {
             if (H.p32()) {
                 // the mixed form keeps U/V beside the Q slabs for the refinement's matvec
@@ -778,6 +782,7 @@ void factor_begin_batch(Ctx& c, const std::vector<Hodlr*>& hs) {
     log_diag_sums(c, lm, lm_out);
     copy2d(c, abinit);
     leaf_trsm_batch(c, false, lt);
+    gemm(c, false, false, bhalf);
 }
 
 // logdet accumulation, failure flags, release of the transient state.

@@ -120,7 +120,7 @@ uint64_t request_footprint(int64_t n, const kmh_build_opts& o, int nv) {
         xg = 8 * (uint64_t)(nleaf - 1) * 2 * (4 * r * r);
     }
     uint64_t b = 16 * (uint64_t)n + 8 * (uint64_t)n * m + slabs + xg;
-    if (o.method == KMH_NYSTROM) b += 8 * (uint64_t)n * r * 4 + 16 * r * r;
+    if (o.method == KMH_NYSTROM) b += 8 * (uint64_t)n * r * 4 + 24 * r * r;
     b += 8 * (uint64_t)n * (2 + nv) * 2;                          // right-hand sides
     if (nv > 0) b += 8 * (uint64_t)n * (m + r + 2);               // derivative leaves and panel
     return b;
