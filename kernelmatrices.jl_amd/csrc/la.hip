@@ -831,6 +831,7 @@ struct LdiagDesc {
     double* B;
     int ncols, ldb;
     int chunk0, pad;
+    double* Z = nullptr;   // MODE 2: the nb x ncols block zeroed beside the solve (ld ldb)
 };
 
 // lower triangle of a 64 x 64 block packed by columns: column l holds rows l..63
@@ -848,13 +849,32 @@ __global__ __launch_bounds__(256) void k_trsm_diag(const LdiagDesc* __restrict__
     const int c0 = ((int)blockIdx.x - d.chunk0) * PB;
     const int nc = min(PB, d.ncols - c0);
     const int nb = __builtin_amdgcn_readfirstlane(d.nb);
-    for (int e = tid; e < PB * PB; e += 256) {
-        const int i = e & 63, l = e >> 6;
-        if (i < l) continue;
-        double v;
-        if (i < nb && l < nb) v = Lg[i + (size_t)l * d.ldl];
-        else v = (i == l) ? 1.0 : 0.0;
-        Lt[tri_col(l) + i - l] = v;
+    {
+        // the triangle's loads all in flight (clamped addresses, masked
+        // values), then the LDS stores
+        constexpr int NE = PB * PB / 256;
+        double lv[NE];
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+            const int e = tid + 256 * q, i = e & 63, l = e >> 6;
+            const bool in = i >= l && i < nb && l < nb;
+            lv[q] = keep_or_zero(Lg[in ? i + (size_t)l * d.ldl : 0], in);
+        }
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+            const int e = tid + 256 * q, i = e & 63, l = e >> 6;
+            if (i >= l) Lt[tri_col(l) + i - l] = (i < nb && l < nb) ? lv[q] : (i == l ? 1.0 : 0.0);
+        }
+    }
+    if (MODE == 2 && d.Z) {
+        // the upper triangle's block mirroring this chunk (potrf zeroes
+        // the strict upper triangle this way)
+        const Glob<double> Zg = glob(d.Z);
+#pragma unroll
+        for (int q = 0; q < PB * PB / 256; ++q) {
+            const int e = tid + 256 * q, i = e & 63, cc = e >> 6;
+            if (i < nb && cc < nc) Zg[i + (size_t)(c0 + cc) * d.ldb] = 0.0;
+        }
     }
     // L(i, l) for i >= l (all lanes read the same element)
     auto Lv = [&](int i, int l) { return Lt[tri_col(l) + i - l]; };
@@ -931,15 +951,6 @@ __global__ __launch_bounds__(256) void k_trsm_diag(const LdiagDesc* __restrict__
     }
 }
 
-__global__ __launch_bounds__(256) void k_zero_upper(const SqMat* __restrict__ ms) {
-    const SqMat m = ms[blockIdx.y];
-    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < (long long)m.n * m.n;
-         e += (long long)gridDim.x * 256) {
-        const int i = (int)(e % m.n), j = (int)(e / m.n);
-        if (i < j) m.A[i + (size_t)j * m.ld] = 0.0;
-    }
-}
-
 static void launch_trsm_diag(Ctx& c, int mode, const std::vector<LdiagDesc>& ds, int chunks, bool b32 = false) {
     if (ds.empty()) return;
     std::vector<int> map;
@@ -975,7 +986,7 @@ void potrf(Ctx& c, const std::vector<SqMat>& mats) {
             diag.push_back({Akk, nb, m.ld, m.info});
             const int rest = m.n - k0 - nb;
             if (rest > 0) {
-                pan.push_back({Akk, nb, m.ld, Akk + nb, rest, m.ld, chunks, 0});
+                pan.push_back({Akk, nb, m.ld, Akk + nb, rest, m.ld, chunks, 0, Akk + (size_t)nb * m.ld});
                 chunks += (int)ceil_div(rest, PB);
                 upd.push_back({rest, rest, nb, Akk + nb, m.ld, Akk + nb, m.ld,
                                Akk + nb + (size_t)nb * m.ld, m.ld, -1.0, 1.0, 1});
@@ -988,8 +999,8 @@ void potrf(Ctx& c, const std::vector<SqMat>& mats) {
             gemm(c, false, true, upd);
         }
     }
-    SqMat* dm = c.upload_vec(mats);
-    KMH_LAUNCH(c, k_zero_upper, dim3(16, (unsigned)mats.size()), dim3(256), dm);
+    // the strict upper triangle is zero: the diagonal blocks' by
+    // k_potrf_diag, the rest by the panel solves (LdiagDesc::Z)
 }
 
 // ---------------------------------------------------------------------------
