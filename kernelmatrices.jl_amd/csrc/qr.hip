@@ -34,6 +34,11 @@ struct SqrDesc {
     double* zout;     // compact-WY coupling Z, ld QC
     int nc;
     int e32;          // src and qout stored in FP32 (R and Z stay FP64)
+    // optional projection removed on load (single-block leaf tiles):
+    // tile <- tile - P T, P the tile's rows of kp prior columns (storage as
+    // src, ld ldp), T (kp x nc, FP64, ld kp)
+    const double* pq = nullptr; int ldp = 0; int kp = 0;
+    const double* pt = nullptr;
 };
 
 // value of v held by lane `lane` (uniform lane index)
@@ -142,6 +147,39 @@ __global__ __launch_bounds__(64, RPL == 4 ? 2 : (NC <= 16 ? 4 : 3)) void k_tile_
             for (int i = 0; i < RPL; ++i) {
                 const unsigned long long m = (in[i] && k < nc) ? ~0ull : 0ull;
                 a[i][k] = __longlong_as_double(__double_as_longlong((double)sp[i][off]) & m);
+            }
+        }
+    }
+    if (d.kp > 0) {
+        // the block Gram-Schmidt reorthogonalisation update X -= P T, one
+        // prior column at a time; T's entries are wave-uniform and read
+        // through the constant address space, i.e. by scalar loads into
+        // SGPRs (T was written by an earlier launch and is not written
+        // during this one)
+        typedef const __attribute__((address_space(1))) E* GSrc;
+        typedef const __attribute__((address_space(4))) double* CSrc;
+        const GSrc pl = (GSrc)d.pq + lane;   // row lane + 64 i at pl[64 i]
+        const CSrc tc = (CSrc)d.pt;
+        auto load_p = [&](int l, double (&q)[RPL]) {   // P's rows of prior column l (clamped to a valid one)
+            const size_t lc = (size_t)(l < d.kp ? l : 0) * d.ldp;
+#pragma unroll
+            for (int i = 0; i < RPL; ++i) {
+                const bool ok = lane + 64 * i < mt;
+                q[i] = keep_or_zero((double)pl[(ok ? 64 * i : 0) + lc], ok);
+            }
+        };
+        double qn[RPL];
+        load_p(0, qn);
+        for (int l = 0; l < d.kp; ++l) {
+            double q[RPL];
+#pragma unroll
+            for (int i = 0; i < RPL; ++i) q[i] = qn[i];
+            load_p(l + 1, qn);   // the next column's loads overlap this column's products
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                const double tk = k < nc ? tc[l + (size_t)k * d.kp] : 0.0;
+#pragma unroll
+                for (int i = 0; i < RPL; ++i) a[i][k] = __fma_rn(-q[i], tk, a[i][k]);
             }
         }
     }
@@ -290,12 +328,14 @@ static void launch_tile_qr_t(Ctx& c, std::vector<SqrDesc> (&cls)[3]) {
         if (k == 0) KMH_LAUNCH(c, (k_tile_qr<8, E, RPL>), g, dim3(64), dd);
         else if (k == 1) KMH_LAUNCH(c, (k_tile_qr<16, E, RPL>), g, dim3(64), dd);
         else {
-            // algorithmic bytes: the tile read, V and R written
+            // algorithmic bytes: the tile read, V and R written, and the
+            // prior columns and coefficients of a fused projection read
             double by = 0.0;
             if (c.prof)
                 for (const auto& d : cls[k]) {
                     const double mt = (double)d.nstack * d.nbs;
                     by += 2.0 * sizeof(E) * mt * d.nc + 8.0 * (double)d.nc * d.nc;
+                    by += sizeof(E) * mt * d.kp + 8.0 * (double)d.kp * d.nc;
                 }
             KMH_LAUNCH_TIMED(c, KMH_PROF_TILE_QR, by, -1.0, (k_tile_qr<QC, E, RPL>), g, dim3(64), dd);
         }
@@ -507,6 +547,10 @@ struct TsqrPanel {
     const double* src;   // input panel (null: A itself), leading dimension lds
     int lds;
     int e32;             // A and src stored in FP32
+    // optional projection removed from the input on load: input - P T with
+    // P (m x kp, same storage, ld ldp) and T (kp x nb, FP64, ld kp)
+    const double* pq = nullptr; int ldp = 0; int kp = 0;
+    const double* pt = nullptr;
 };
 
 // Thin QR of m x nb panels (nb <= QC, m >= nb) by TSQR: leaf tiles of <= 128
@@ -556,9 +600,16 @@ static void tsqr(Ctx& c, const std::vector<TsqrPanel>& ps) {
             const bool single = cur[t.panel].size() == 1;
             double* a = adv_e(P.A, t.row0, P.e32);
             const double* in = P.src ? adv_e(P.src, t.row0, P.e32) : a;
-            ds.push_back({in, P.src ? P.lds : P.lda, 1, t.rows, 0, a, P.lda,
-                          single ? P.R : RT + (size_t)t.node * RS, single ? P.ldr : QC,
-                          ZT + (size_t)t.node * RS, P.nb, P.e32});
+            SqrDesc sd{in, P.src ? P.lds : P.lda, 1, t.rows, 0, a, P.lda,
+                       single ? P.R : RT + (size_t)t.node * RS, single ? P.ldr : QC,
+                       ZT + (size_t)t.node * RS, P.nb, P.e32};
+            if (P.kp > 0) {
+                sd.pq = adv_e(P.pq, t.row0, P.e32);
+                sd.ldp = P.ldp;
+                sd.kp = P.kp;
+                sd.pt = P.pt;
+            }
+            ds.push_back(sd);
         }
         launch_tile_qr(c, ds);
     }
@@ -643,7 +694,7 @@ void qr_panels(Ctx& c, const std::vector<QrPanel>& ps) {
     set_identity(c, z);
     for (int c0 = 0; c0 < rmax; c0 += QC) {
         std::vector<TsqrPanel> tp;
-        std::vector<GemmProb> p1, u1, p2, u2;
+        std::vector<GemmProb> p1, u1, p2;
         std::vector<Axpby2D> acc;
         size_t tsz = 0;
         for (const auto& p : ps)
@@ -679,11 +730,15 @@ void qr_panels(Ctx& c, const std::vector<QrPanel>& ps) {
                 GemmProb g2{c0, nb, p.m, p.A, p.lda, Aj, p.lda, t2, c0, 1.0, 0.0, 0};
                 g2.ea = g2.eb = e;
                 p2.push_back(g2);
-                GemmProb g3{p.m, nb, c0, p.A, p.lda, t2, c0, Aj, p.lda, -1.0, 1.0, 0};
-                g3.ea = g3.ec = e;
-                u2.push_back(g3);
                 acc.push_back({t2, c0, Rj, p.ldr, c0, nb, 1.0, 1.0});
-                tp.push_back({Aj, p.m, nb, p.lda, Rj + c0, p.ldr, nullptr, 0, e});
+                // the second pass's update X -= Q T2 happens as the leaf
+                // tiles load X (k_tile_qr)
+                TsqrPanel tpj{Aj, p.m, nb, p.lda, Rj + c0, p.ldr, nullptr, 0, e};
+                tpj.pq = p.A;
+                tpj.ldp = p.lda;
+                tpj.kp = c0;
+                tpj.pt = t2;
+                tp.push_back(tpj);
             } else {
                 tp.push_back({Aj, p.m, nb, p.lda, Rj + c0, p.ldr, sj, p.lds, e});
             }
@@ -693,7 +748,6 @@ void qr_panels(Ctx& c, const std::vector<QrPanel>& ps) {
             gemm(c, true, false, p1);
             gemm(c, false, false, u1);
             gemm(c, true, false, p2);
-            gemm(c, false, false, u2);
             axpby2d(c, acc);
         }
         tsqr(c, tp);
