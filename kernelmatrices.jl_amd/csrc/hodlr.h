@@ -69,10 +69,27 @@ struct Hodlr {
     int ir_iters = 0;
     double ir_relres = 0.0;
 
+    // W^{-1} y carried through the factorization (FP64 Nystrom handles with
+    // levels): with carry_dev (device) or carry_host (host) set before
+    // factor_begin_batch, y rides in the cascade panels as one more column
+    // beside the K(x,P) half and one beside the other half (the level
+    // products that need Q^T y read it there), the leaf solve and every
+    // W_l^{-1} act on it, and winvy holds W^{-1} y after the last level
+    const double* carry_dev = nullptr;
+    const double* carry_host = nullptr;
+    bool carry = false;                          // this factorization carries y
+    bool has_winvy = false;
+    DevBuf winvy;
+    // cascade panel columns: [y | K(x,P) half | other half | y] with a
+    // carry, else the two halves
+    int ab_a() const { return carry ? 1 : 0; }
+    int ab_b() const { return ab_a() + nys_r; }
+    int ab_w() const { return 2 * nys_r + (carry ? 2 : 0); }
+
     // transient state of a factorization in progress (scratch blocks of
     // ctx's pool, released by factor_end_batch, by a new factorization or
     // with the handle)
-    double* AB = nullptr;                        // Nystrom cascade panels (n x 2r)
+    double* AB = nullptr;                        // Nystrom cascade panels (n x ab_w())
     double* parts = nullptr;                     // log-diagonal partial sums
     int nparts = 0;
 
@@ -142,7 +159,7 @@ struct Hodlr {
     double* leaf_ptr(int i) const { return leaves.as<double>() + leaf_mat_off[i]; }
     size_t device_bytes() const {
         return pts.bytes + info.bytes + leaves.bytes + P.bytes + UV.bytes + lm.bytes + kpp.bytes + kinv.bytes +
-               kxp.bytes + ynys.bytes + X.bytes + G.bytes;
+               kxp.bytes + ynys.bytes + X.bytes + G.bytes + winvy.bytes;
     }
 };
 

@@ -516,21 +516,30 @@ static void woodbury_batch(Ctx& c, int mode, const std::vector<Rhs>& rs) {
 // symmetricfactorize!  (symfact.jl:6-50), across handles
 // ---------------------------------------------------------------------------
 
-// W_lv^{-1} applied to the shared Nystrom panels [A B] (n x 2r).  With
-// M = blkdiag(Q_V, Q_U), V = B rows of the first child and U = A rows of the
-// second child, M^T [A B] has the blocks Q_V^T B_c = R_V and Q_U^T A_a = R_U
-// already: only Q_V^T A_c and Q_U^T B_a need a product.  sbs[i] holds, per
-// block of handle i in block order, a 2r x 2r slot (ld 2r) into which the
-// panel QR wrote R_V (top right) and R_U (bottom left), followed by r^2
-// doubles of the factor stage (5 r^2 per block).
+// Per-block slot of the factor stage (doubles): a 2r x w coefficient
+// matrix, w the cascade panel width (2r without one), then Bm (r x r).
+static size_t level_slot(const Hodlr& H, int r) {
+    const size_t w = H.AB ? (size_t)H.ab_w() : (size_t)2 * r;
+    return (size_t)2 * r * w + (size_t)r * r;
+}
+
+// W_lv^{-1} applied to the shared Nystrom panels [A B] (n x 2r; with a
+// carried right-hand side [y A B y], n x (2r + 2)).  With M = blkdiag(Q_V,
+// Q_U), V = B rows of the first child and U = A rows of the second child,
+// M^T [A B] has the blocks Q_V^T B_c = R_V and Q_U^T A_a = R_U already: only
+// Q_V^T A_c and Q_U^T B_a need a product (and Q_V^T y_c, Q_U^T y_a, which sit
+// beside them in the panels).  sbs[i] holds, per block of handle i in block
+// order, a 2r x w slot (ld 2r, w the panel width) into which the panel QR
+// wrote R_V (top, B columns) and R_U (bottom, A columns).
 static void woodbury_shared_basis(Ctx& c, const std::vector<Hodlr*>& hs, const std::vector<int>& lvs,
                                   const std::vector<double*>& sbs) {
     size_t ssz = 0;
     for (size_t i = 0; i < hs.size(); ++i)
-        for (int r : hs[i]->ranks[lvs[i]]) ssz += (size_t)4 * r * r;
+        for (int r : hs[i]->ranks[lvs[i]]) ssz += (size_t)2 * r * hs[i]->ab_w();
     if (ssz == 0) return;
     double* S2 = static_cast<double*>(c.scratch(ssz * sizeof(double)));
     std::vector<GemmProb> tn, op, up;
+    std::vector<Copy2D> ycp;
     size_t off = 0;
     for (size_t i = 0; i < hs.size(); ++i) {
         Hodlr& H = *hs[i];
@@ -541,6 +550,7 @@ static void woodbury_shared_basis(Ctx& c, const std::vector<Hodlr*>& hs, const s
         const bool e = H.ab32(), eq = H.p32();
         double* AB = H.AB;
         double* s = sbs[i];
+        const int w = H.ab_w(), ca = H.ab_a(), cb = H.ab_b();
         for (size_t k = 0; k < H.blocks[lv].size(); ++k) {
             const auto& b = H.blocks[lv][k];
             const int r = H.ranks[lv][k];
@@ -548,42 +558,90 @@ static void woodbury_shared_basis(Ctx& c, const std::vector<Hodlr*>& hs, const s
             const int h1 = (int)(b.d1 - b.c0), h2 = (int)(b.b1 - b.a0);
             const double* Q1 = H.Pat(P + b.c0);
             const double* Q2 = H.Pat(P + b.a0);
-            double* s2 = S2 + off;   // s = [Q_V^T A_c, R_V; R_U, Q_U^T B_a], 2r x 2r, ld 2r
-            off += (size_t)4 * r * r;
+            double* s2 = S2 + off;   // s = [Q_V^T (y) A_c, R_V; R_U, Q_U^T B_a (y)], 2r x w, ld 2r
+            off += (size_t)2 * r * w;
             const int l2 = 2 * r;
-            GemmProb t1{r, r, h1, Q1, (int)n, tadv(AB, b.c0, e), (int)n, s, l2, 1.0, 0.0, 0};
-            GemmProb t2{r, r, h2, Q2, (int)n, tadv(AB, (size_t)n * r + b.a0, e), (int)n, s + r + (size_t)r * l2, l2,
-                        1.0, 0.0, 0};
+            // Q_V^T of the first child's [y] A columns, Q_U^T of the second child's B [y] columns
+            GemmProb t1{r, r + ca, h1, Q1, (int)n, tadv(AB, b.c0, e), (int)n, s, l2, 1.0, 0.0, 0};
+            GemmProb t2{r, w - cb, h2, Q2, (int)n, tadv(AB, (size_t)n * cb + b.a0, e), (int)n,
+                        s + r + (size_t)cb * l2, l2, 1.0, 0.0, 0};
             t1.ea = t2.ea = eq;
             t1.eb = t2.eb = e;
             tn.push_back(t1);
             tn.push_back(t2);
+            if (H.carry) {
+                // both y columns of s hold M^T y
+                ycp.push_back({s + r + (size_t)(w - 1) * l2, l2, s + r, l2, r, 1, 1.0});
+                ycp.push_back({s, l2, s + (size_t)(w - 1) * l2, l2, r, 1, 1.0});
+            }
             // the first r rows of G are zero: only the second child's rows of
             // the panels change (see woodbury_batch)
-            op.push_back({r, l2, l2, G + H.coff_blk[lv][k] + r, l2, s, l2, s2 + r, l2, 1.0, 0.0, 0});
-            GemmProb u2{h2, l2, r, Q2, (int)n, s2 + r, l2, tadv(AB, b.a0, e), (int)n, -1.0, 1.0, 0};
+            op.push_back({r, w, l2, G + H.coff_blk[lv][k] + r, l2, s, l2, s2 + r, l2, 1.0, 0.0, 0});
+            GemmProb u2{h2, w, r, Q2, (int)n, s2 + r, l2, tadv(AB, b.a0, e), (int)n, -1.0, 1.0, 0};
             u2.ea = eq;
             u2.ec = e;
             up.push_back(u2);
-            s += (size_t)5 * r * r;
+            s += level_slot(H, r);
         }
+    }
+    gemm(c, true, false, tn);
+    copy2d(c, ycp);
+    gemm(c, false, false, op);
+    gemm(c, false, false, up);
+    c.scratch_free(S2);
+}
+
+// W_top^{-1} on the carried y (column 0 of the cascade panels) at the last
+// level, which updates no panels: y_a -= Q_U G_low [Q_V^T y_c; Q_U^T y_a],
+// then y (now W^{-1} y) to winvy.
+static void carry_last_level(Ctx& c, const std::vector<Hodlr*>& hs, const std::vector<int>& lvs) {
+    size_t vsz = 0;
+    for (size_t i = 0; i < hs.size(); ++i)
+        for (int r : hs[i]->ranks[lvs[i]]) vsz += (size_t)3 * r;
+    if (vsz == 0) return;
+    double* V = static_cast<double*>(c.scratch(vsz * sizeof(double)));
+    std::vector<GemmProb> tn, op, up;
+    std::vector<Copy2D> out;
+    size_t off = 0;
+    for (size_t i = 0; i < hs.size(); ++i) {
+        Hodlr& H = *hs[i];
+        const int lv = lvs[i];
+        const int64_t n = H.n;
+        const size_t P = (size_t)H.coff[lv] * n;
+        const double* G = H.G.as<double>();
+        double* y = H.AB;
+        for (size_t k = 0; k < H.blocks[lv].size(); ++k) {
+            const auto& b = H.blocks[lv][k];
+            const int r = H.ranks[lv][k];
+            const int h1 = (int)(b.d1 - b.c0), h2 = (int)(b.b1 - b.a0), l2 = 2 * r;
+            double* v = V + off;   // M^T y (2r), then G_low M^T y (r)
+            off += (size_t)3 * r;
+            tn.push_back({r, 1, h1, H.Pat(P + b.c0), (int)n, y + b.c0, (int)n, v, l2, 1.0, 0.0, 0});
+            tn.push_back({r, 1, h2, H.Pat(P + b.a0), (int)n, y + b.a0, (int)n, v + r, l2, 1.0, 0.0, 0});
+            op.push_back({r, 1, l2, G + H.coff_blk[lv][k] + r, l2, v, l2, v + l2, r, 1.0, 0.0, 0});
+            up.push_back({h2, 1, r, H.Pat(P + b.a0), (int)n, v + l2, r, y + b.a0, (int)n, -1.0, 1.0, 0});
+        }
+        H.winvy.alloc(&c, (size_t)n * sizeof(double));
+        out.push_back({y, (int)n, H.winvy.as<double>(), (int)n, (int)n, 1, 1.0});
+        H.has_winvy = true;
     }
     gemm(c, true, false, tn);
     gemm(c, false, false, op);
     gemm(c, false, false, up);
-    c.scratch_free(S2);
+    copy2d(c, out);
+    c.scratch_free(V);
 }
 
 // One level per handle (levels may differ across handles).
 void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vector<int>& lvs) {
     // Nystrom: this level's panels are row ranges of the cascade panels; the
     // panel QR reads them there and writes Q into the level's slab of P
-    // per block 5 r^2 doubles: a 2r x 2r slot for the R factors (laid out as
-    // the shared-basis coefficient matrix when a Woodbury update of Nystrom
+    // per block a slot (level_slot) for the R factors (laid out as the
+    // shared-basis coefficient matrix when a Woodbury update of Nystrom
     // panels follows, else R1 and R2 at its start, ld r), then Bm
     size_t rsz = 0;
     for (size_t ai = 0; ai < act.size(); ++ai)
-        for (int r : act[ai]->ranks[lvs[ai]]) rsz += (size_t)5 * r * r;
+        for (int r : act[ai]->ranks[lvs[ai]]) rsz += level_slot(*act[ai], r);
     double* Rb = static_cast<double*>(c.scratch(std::max<size_t>(1, rsz) * sizeof(double)));
     std::vector<QrPanel> qp;
     std::vector<Copy2D> qcopy;
@@ -613,23 +671,24 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
             const int r = H.ranks[lv][k];
             double* slot = Rb + off;
             const int ldr = shared ? 2 * r : r;
-            double* R1 = shared ? slot + (size_t)r * 2 * r : slot;   // R_V: top right of the coefficient slot
-            double* R2 = shared ? slot + r : slot + (size_t)r * r;   // R_U: bottom left
-            double* Bm = slot + (size_t)4 * r * r;
-            off += (size_t)5 * r * r;
+            const int ca = H.AB ? H.ab_a() : 0, cb = H.AB ? H.ab_b() : r;   // panel columns of the A and B halves
+            double* R1 = shared ? slot + (size_t)cb * 2 * r : slot;      // R_V: top rows, B columns of the slot
+            double* R2 = shared ? slot + r + (size_t)ca * 2 * r : slot + (size_t)r * r;   // R_U: bottom rows, A columns
+            double* Bm = slot + level_slot(H, r) - (size_t)r * r;
+            off += level_slot(H, r);
             // lrsymfact(V, U): V (first child) first (symfact.jl:31, HODLR/src/utils.jl:29-35)
             QrPanel qv{H.Pat(P + b.c0), (int)(b.d1 - b.c0), r, (int)n, R1, ldr};
             QrPanel qu{H.Pat(P + b.a0), (int)(b.b1 - b.a0), r, (int)n, R2, ldr};
             qv.e32 = qu.e32 = e;
             if (H.AB && e == eab) {   // V: B half of the first child's rows; U: A half of the second child's
-                qv.src = tadv(H.AB, (size_t)n * r + b.c0, e);
+                qv.src = tadv(H.AB, (size_t)n * cb + b.c0, e);
                 qv.lds = (int)n;
-                qu.src = tadv(H.AB, b.a0, e);
+                qu.src = tadv(H.AB, (size_t)n * ca + b.a0, e);
                 qu.lds = (int)n;
             } else if (H.AB) {
                 // KMH_MIXED_Q: the FP64 cascade panels rounded into the FP32 slab, QR in place
-                Copy2D cv{H.AB + (size_t)n * r + b.c0, (int)n, qv.A, (int)n, qv.m, r, 1.0};
-                Copy2D cu{H.AB + b.a0, (int)n, qu.A, (int)n, qu.m, r, 1.0};
+                Copy2D cv{H.AB + (size_t)n * cb + b.c0, (int)n, qv.A, (int)n, qv.m, r, 1.0};
+                Copy2D cu{H.AB + (size_t)n * ca + b.a0, (int)n, qu.A, (int)n, qu.m, r, 1.0};
                 cv.ed = cu.ed = 1;
                 qcopy.push_back(cv);
                 qcopy.push_back(cu);
@@ -690,14 +749,22 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
     std::vector<Hodlr*> nys;
     std::vector<int> nys_lv;
     std::vector<double*> nys_r;
+    std::vector<Hodlr*> last;
+    std::vector<int> last_lv;
     off = 0;
     for (size_t ai = 0; ai < act.size(); ++ai) {
         Hodlr* h = act[ai];
         Hodlr& H = *h;
         const int lv = lvs[ai];
         double* r0 = Rb + off;
-        for (int r : H.ranks[lv]) off += (size_t)5 * r * r;
-        if (lv + 1 >= H.L) continue;
+        for (int r : H.ranks[lv]) off += level_slot(H, r);
+        if (lv + 1 >= H.L) {
+            if (H.carry) {
+                last.push_back(h);
+                last_lv.push_back(lv);
+            }
+            continue;
+        }
         if (H.AB) {
             nys.push_back(h);
             nys_lv.push_back(lv);
@@ -708,6 +775,7 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
     }
     woodbury_batch(c, W_INV, wr);
     woodbury_shared_basis(c, nys, nys_lv, nys_r);
+    carry_last_level(c, last, last_lv);
     c.scratch_free(Rb);
 }
 
@@ -720,9 +788,12 @@ void factor_begin_batch(Ctx& c, const std::vector<Hodlr*>& hs) {
     std::vector<Copy2D> abinit;
     std::vector<Rhs> lt;
     std::vector<GemmProb> bhalf;
+    std::vector<Copy2D> ycopy;
     for (Hodlr* h : hs) {
         Hodlr& H = *h;
         H.release_transient();
+        H.carry = false;
+        H.has_winvy = false;
         H.factor_started = true;
         const int64_t n = H.n;
         const int L = H.L, nl = H.nleaves();
@@ -756,13 +827,20 @@ void factor_begin_batch(Ctx& c, const std::vector<Hodlr*>& hs) {
             // L^{-1} B = (L^{-1} A) K(P,P)^{-1} (B = A K(P,P)^{-1} row by row,
             // and L^{-1} acts on rows): the leaf solves take the A half only,
             // one n x r x r product gives the B half
-            H.AB = static_cast<double*>(c.scratch((size_t)n * 2 * r * (e ? sizeof(float) : sizeof(double))));
-            Copy2D a{H.kxp.as<double>(), (int)n, H.AB, (int)n, (int)n, r, 1.0};
+            H.carry = (H.carry_dev || H.carry_host) && !H.p32();
+            const int w = H.ab_w(), ca = H.ab_a(), cb = H.ab_b();
+            H.AB = static_cast<double*>(c.scratch((size_t)n * w * (e ? sizeof(float) : sizeof(double))));
+            Copy2D a{H.kxp.as<double>(), (int)n, tadv(H.AB, (size_t)n * ca, e), (int)n, (int)n, r, 1.0};
             a.ed = e;
             abinit.push_back(a);
-            lt.push_back({h, H.AB, n, r, 0, e});
-            GemmProb bh{(int)n, r, r, H.AB, (int)n, H.kinv.as<double>(), r, tadv(H.AB, (size_t)n * r, e), (int)n,
-                        1.0, 0.0, 0};
+            if (H.carry) {
+                if (H.carry_dev) abinit.push_back({H.carry_dev, (int)n, H.AB, (int)n, (int)n, 1, 1.0});
+                else KMH_HIP(hipMemcpyAsync(H.AB, H.carry_host, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c.q()));
+                ycopy.push_back({H.AB, (int)n, H.AB + (size_t)n * (w - 1), (int)n, (int)n, 1, 1.0});
+            }
+            lt.push_back({h, H.AB, n, r + ca, 0, e});   // [y] and the K(x,P) half
+            GemmProb bh{(int)n, r, r, tadv(H.AB, (size_t)n * ca, e), (int)n, H.kinv.as<double>(), r,
+                        tadv(H.AB, (size_t)n * cb, e), (int)n, 1.0, 0.0, 0};
             bh.ea = bh.ec = e;
             bhalf.push_back(bh);
         } else {
@@ -783,6 +861,7 @@ void factor_begin_batch(Ctx& c, const std::vector<Hodlr*>& hs) {
     copy2d(c, abinit);
     leaf_trsm_batch(c, false, lt);
     gemm(c, false, false, bhalf);
+    copy2d(c, ycopy);   // L^{-1} y into the last panel column too
 }
 
 // logdet accumulation, failure flags, release of the transient state.

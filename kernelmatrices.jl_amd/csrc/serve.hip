@@ -120,7 +120,7 @@ uint64_t request_footprint(int64_t n, const kmh_build_opts& o, int nv) {
         xg = 8 * (uint64_t)(nleaf - 1) * 2 * (4 * r * r);
     }
     uint64_t b = 16 * (uint64_t)n + 8 * (uint64_t)n * m + slabs + xg;
-    if (o.method == KMH_NYSTROM) b += 8 * (uint64_t)n * r * 4 + 24 * r * r;
+    if (o.method == KMH_NYSTROM) b += 8 * (uint64_t)n * (r * 4 + 3) + 24 * r * r;   // + carried y columns, W^-1 y
     b += 8 * (uint64_t)n * (2 + nv) * 2;                          // right-hand sides
     if (nv > 0) b += 8 * (uint64_t)n * (m + r + 2);               // derivative leaves and panel
     return b;
@@ -138,7 +138,7 @@ static uint64_t built_footprint(const Hodlr& H, int nv) {
         }
     const uint64_t n = (uint64_t)H.n;
     uint64_t b = H.device_bytes() + xg + 8 * n * (2 + nv) * 2;
-    if (H.nystrom()) b += 8 * n * (uint64_t)H.Rtot + 8 * n * 2 * rmax;   // level slabs + cascade panels
+    if (H.nystrom()) b += 8 * n * (uint64_t)H.Rtot + 8 * n * (2 * rmax + 3);   // level slabs, cascade panels, W^-1 y
     if (nv > 0) {
         int64_t m = 0;
         for (int i = 0; i < H.nleaves(); ++i) m = std::max<int64_t>(m, H.leaf_size(i));
@@ -196,22 +196,30 @@ static void finish_batch(Ctx& c, std::vector<Req*>& rs) {
         copy2d(c, cp2);
         // FP64 form: z = W^{-1} y, y' K^{-1} y = z' z.  Mixed form: x = K^{-1} y
         // refined against the FP64 operator, y' K^{-1} y = y' x.
+        // (W^{-1} y of a factorization that carried y is already there)
         std::vector<Hodlr*> h64, h32;
         std::vector<double*> t64, t32;
         std::vector<DotJob> dj;
+        std::vector<Copy2D> wcp;
         off = 0;
         for (size_t i = 0; i < ok.size(); ++i) {
+            const int64_t n = ok[i]->n;
             if (ok[i]->p32()) {
                 h32.push_back(ok[i]);
                 t32.push_back(T[i]);
-                dj.push_back({T[i], buf + off, ok[i]->n, dots + i});
+                dj.push_back({T[i], buf + off, n, dots + i});
             } else {
-                h64.push_back(ok[i]);
-                t64.push_back(T[i]);
-                dj.push_back({T[i], T[i], ok[i]->n, dots + i});
+                if (ok[i]->has_winvy) {
+                    wcp.push_back({ok[i]->winvy.as<double>(), (int)n, T[i], (int)n, (int)n, 1, 1.0});
+                } else {
+                    h64.push_back(ok[i]);
+                    t64.push_back(T[i]);
+                }
+                dj.push_back({T[i], T[i], n, dots + i});
             }
-            off += 2 * ok[i]->n;
+            off += 2 * n;
         }
+        copy2d(c, wcp);
         hodlr_factor_apply_batch(c, KMH_WINV, h64, t64, std::vector<int>(h64.size(), 1));
         if (!h32.empty()) hodlr_solve_batch(c, h32, t32, std::vector<int>(h32.size(), 1));
         dots_batch(c, dj);
@@ -296,7 +304,12 @@ static void lane_step(Lane& ln, const std::function<void(std::vector<Req*>&)>& a
     }
     if (!leaf.empty()) {
         std::vector<Hodlr*> hs;
-        for (Req* q : leaf) hs.push_back(q->H.get());
+        for (Req* q : leaf) {
+            // y rides through the factorization (W^{-1} y for the quadratic form)
+            q->H->carry_dev = q->dy;
+            q->H->carry_host = q->dy ? nullptr : q->y.data();
+            hs.push_back(q->H.get());
+        }
         factor_begin_batch(c, hs);
         for (Req* q : leaf) {
             q->lv = 0;
