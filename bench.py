@@ -70,7 +70,7 @@ def measured_traffic(kernel):
                 if len(parts) == 2 and parts[0] == "source_sha256":
                     stamp = parts[1]
                 if parts and parts[0] == "traffic_bytes_per_launch":
-                    # the kernel name may hold spaces ("k_tile_qr<24, double>")
+                    # the kernel name may hold spaces ("k_tile_qr<24, double, *>")
                     name, _, num = line.strip()[len("traffic_bytes_per_launch"):].strip().rpartition(" ")
                     if name.strip() == kernel:
                         val = int(float(num))
@@ -653,10 +653,12 @@ def main():
             or {"evals_per_s": None})
 
     KERNELS = {
-        "tile_qr": ("k_tile_qr<24, double>",
-                    "Householder QR of <= 128 x 24 tiles with Q left in compact-WY form (TSQR leaves and "
-                    "inner nodes), one wave per tile; bytes = tiles read, Householder vectors and R written; "
-                    "bound by LDS traffic and dependent latency, so far below its HBM roofline"),
+        "tile_qr": ("k_tile_qr<24, double, *>",
+                    "Householder QR of <= 256 x 24 tiles (four rows per lane; tiles of <= 128 rows two) with Q "
+                    "left in compact-WY form (TSQR leaves and inner nodes), one wave per tile; the leaf tiles of "
+                    "a panel's second and third column blocks remove the reorthogonalisation projection Q T2 as "
+                    "they load; bytes = tiles and prior columns read, Householder vectors and R written; bound "
+                    "by LDS traffic and dependent latency, so below its HBM roofline"),
         "nn_wide": ("k_gemm_nn_wide<0>", "skinny FP64 GEMM on v_mfma_f64_16x16x4f64 (Woodbury updates, leaf-solve "
                                       "updates, Nystrom product); bytes = A, B, C written, C read when beta != 0"),
     }

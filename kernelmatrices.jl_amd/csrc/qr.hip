@@ -152,14 +152,12 @@ __global__ __launch_bounds__(64, RPL == 4 ? 2 : (NC <= 16 ? 4 : 3)) void k_tile_
     }
     if (d.kp > 0) {
         // the block Gram-Schmidt reorthogonalisation update X -= P T, one
-        // prior column at a time; T's entries are wave-uniform and read
-        // through the constant address space, i.e. by scalar loads into
-        // SGPRs (T was written by an earlier launch and is not written
-        // during this one)
+        // prior column l at a time: P's rows of column l and lane k's T(l, k)
+        // loaded one column ahead, T(l, k) reaching every lane by a lane
+        // broadcast (two SGPRs live at a time)
         typedef const __attribute__((address_space(1))) E* GSrc;
-        typedef const __attribute__((address_space(4))) double* CSrc;
         const GSrc pl = (GSrc)d.pq + lane;   // row lane + 64 i at pl[64 i]
-        const CSrc tc = (CSrc)d.pt;
+        const Glob<const double> tg = glob(d.pt);
         auto load_p = [&](int l, double (&q)[RPL]) {   // P's rows of prior column l (clamped to a valid one)
             const size_t lc = (size_t)(l < d.kp ? l : 0) * d.ldp;
 #pragma unroll
@@ -168,16 +166,23 @@ __global__ __launch_bounds__(64, RPL == 4 ? 2 : (NC <= 16 ? 4 : 3)) void k_tile_
                 q[i] = keep_or_zero((double)pl[(ok ? 64 * i : 0) + lc], ok);
             }
         };
+        auto load_t = [&](int l) {   // T(l, lane), zero for lanes >= nc
+            const bool ok = lane < nc && l < d.kp;
+            return keep_or_zero(tg[ok ? (size_t)l + (size_t)lane * d.kp : 0], ok);
+        };
         double qn[RPL];
         load_p(0, qn);
+        double tn = load_t(0);
         for (int l = 0; l < d.kp; ++l) {
             double q[RPL];
 #pragma unroll
             for (int i = 0; i < RPL; ++i) q[i] = qn[i];
+            const double tl = tn;
             load_p(l + 1, qn);   // the next column's loads overlap this column's products
+            tn = load_t(l + 1);
 #pragma unroll
             for (int k = 0; k < NC; ++k) {
-                const double tk = k < nc ? tc[l + (size_t)k * d.kp] : 0.0;
+                const double tk = qr_bcast(tl, k);
 #pragma unroll
                 for (int i = 0; i < RPL; ++i) a[i][k] = __fma_rn(-q[i], tk, a[i][k]);
             }
