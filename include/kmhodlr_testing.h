@@ -26,6 +26,13 @@ int kmh_test_trsm(kmh_ctx* ctx, int32_t trans, int32_t n, const double* Lm, int3
                   int32_t ncols, double* B, int32_t ldb);
 /* thin QR of an m x r panel: A <- Q, R (r x r) */
 int kmh_test_qr(kmh_ctx* ctx, int32_t m, int32_t r, double* A, int32_t lda, double* R, int32_t ldr);
+/* capacitance factors of one lrsymfact block (r <= 80): from R1, R2 (r x r
+ * upper, ld ldr) the last r rows of X = chol(C) - I and G = I - chol(C)^-1
+ * (2r x 2r, ld ld >= 2r; the first r rows are left as given), *logsum =
+ * sum log diag(chol(I - Bm^T Bm)), Bm = R1 R2^T; returns KMH_E_NOT_PD when
+ * I - Bm^T Bm is not positive definite                                   */
+int kmh_test_capacitance(kmh_ctx* ctx, int32_t r, const double* R1, const double* R2, int32_t ldr, double* X,
+                         double* G, int32_t ld, double* logsum);
 /* descriptor-arena ring: `count` uploads of `words` 64-bit words, each read
  * back by a kernel that checks its contents; *bad = uploads whose kernel saw
  * other data (0 when the ring's fences hold) */

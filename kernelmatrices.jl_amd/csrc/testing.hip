@@ -49,6 +49,31 @@ int kmh_test_gemm(kmh_ctx* ctx, int32_t ta, int32_t tb, int32_t m, int32_t n, in
     })
 }
 
+int kmh_test_capacitance(kmh_ctx* ctx, int32_t r, const double* R1, const double* R2, int32_t ldr, double* X,
+                         double* G, int32_t ld, double* logsum) {
+    TRY_({
+        kmh::Ctx& c = ctx->c;
+        c.set_device();
+        KMH_CHECK(r >= 0 && r <= kmh::CAP_MAX && ldr >= r && ld >= 2 * r && logsum, KMH_E_BAD_ARG,
+                  "kmh_test_capacitance: invalid shape");
+        KMH_HIP(hipMemsetAsync(c.d_info, 0, sizeof(int), c.q()));
+        const size_t nr = (size_t)ldr * r, nx = (size_t)ld * 2 * r;
+        Dev a(c, R1, nr), b(c, R2, nr), x(c, X, nx), g(c, G, nx), part(c, nullptr, 1);
+        kmh::capacitance(c, {{a.p, b.p, ldr, r, x.p, g.p, ld, part.p, c.d_info}});
+        x.to_host(X, nx);
+        g.to_host(G, nx);
+        part.to_host(logsum, 1);
+        int info = 0;
+        KMH_HIP(hipMemcpyAsync(&info, c.d_info, sizeof(int), hipMemcpyDeviceToHost, c.q()));
+        c.sync();
+        if (info) {
+            KMH_HIP(hipMemsetAsync(c.d_info, 0, sizeof(int), c.q()));
+            c.sync();
+            throw kmh::Error(KMH_E_NOT_PD, "PosDefException: capacitance block is not positive definite");
+        }
+    })
+}
+
 int kmh_test_potrf(kmh_ctx* ctx, int32_t n, double* A, int32_t lda) {
     TRY_({
         kmh::Ctx& c = ctx->c;

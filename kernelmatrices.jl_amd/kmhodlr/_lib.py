@@ -138,6 +138,7 @@ TEST_SIGNATURES = {
     "kmh_test_potrf": [_P, C.c_int32, _P, C.c_int32],
     "kmh_test_trsm": [_P, C.c_int32, C.c_int32, _P, C.c_int32, C.c_int32, _P, C.c_int32],
     "kmh_test_qr": [_P, C.c_int32, C.c_int32, _P, C.c_int32, _P, C.c_int32],
+    "kmh_test_capacitance": [_P, C.c_int32, _P, _P, C.c_int32, _P, _P, C.c_int32, _D],
     "kmh_test_arena": [_P, C.c_int32, C.c_int32, C.POINTER(C.c_int64)],
     "kmh_test_profile_spans": [_P, C.c_int32, C.POINTER(C.c_int32), _D, _D, _D, C.POINTER(C.c_int32),
                                C.POINTER(C.c_int32)],

@@ -92,6 +92,60 @@ def test_potrf_not_pd(ctx):
     assert ctx.lib.kmh_test_potrf(ctx.ptr, 3, B.ctypes.data, 3) == 0
 
 
+def _capacitance(ctx, R1, R2, ld=None):
+    r = R1.shape[0]
+    ld = ld or 2 * r
+    X = np.asfortranarray(np.full((ld, 2 * r), 7.0))
+    G = np.asfortranarray(np.full((ld, 2 * r), 7.0))
+    ls = C.c_double()
+    R1 = np.asfortranarray(R1)   # held: a temporary's address would be freed before the call
+    R2 = np.asfortranarray(R2)
+    rc = ctx.lib.kmh_test_capacitance(ctx.ptr, r, R1.ctypes.data, R2.ctypes.data, r, X.ctypes.data,
+                                      G.ctypes.data, ld, C.byref(ls))
+    return rc, X, G, ls.value
+
+
+@pytest.mark.parametrize("r", [1, 5, 16, 17, 40, 72, 80])
+def test_capacitance_matches_numpy(ctx, r):
+    """The fused capacitance kernel against the dense closed forms of
+    lrsymfact (HODLR/src/utils.jl:29-35): with Bm = R1 R2^T and
+    L22 = chol(I - Bm^T Bm), the last r rows of X = chol(C) - I are
+    [Bm^T, L22 - I] and of G = I - chol(C)^-1 are [L22^-1 Bm^T, I - L22^-1];
+    the first r rows are not written; the log-sum is sum log diag L22.
+    Sizes cover one block, a partial block and the padded 16 x 16 tiling."""
+    print("capacitance", r, flush=True)
+    rng = np.random.default_rng(100 + r)
+    R1 = np.triu(rng.standard_normal((r, r)))
+    R2 = np.triu(rng.standard_normal((r, r)))
+    Bm = R1 @ R2.T
+    s = np.linalg.norm(Bm, 2)
+    R1 *= 0.97 / s                      # singular values of Bm up to 0.97: S well defined, condition ~ 17
+    Bm = R1 @ R2.T
+    rc, X, G, ls = _capacitance(ctx, R1, R2, ld=2 * r + 3)
+    assert rc == 0, ctx.lib.last_error()
+    L = np.linalg.cholesky(np.eye(r) - Bm.T @ Bm)
+    Li = np.linalg.inv(L)
+    Xr = np.hstack([Bm.T, L - np.eye(r)])
+    Gr = np.hstack([Li @ Bm.T, np.eye(r) - Li])
+    scale = max(1.0, np.abs(Li).max())
+    assert np.abs(X[r:2 * r] - Xr).max() <= 1e-13 * scale
+    assert np.abs(G[r:2 * r] - Gr).max() <= 1e-12 * scale
+    assert np.all(X[:r] == 7.0) and np.all(G[:r] == 7.0) and np.all(X[2 * r:] == 7.0)
+    assert abs(ls - np.log(np.diag(L)).sum()) <= 1e-12 * max(1.0, abs(ls))
+
+
+def test_capacitance_not_pd(ctx):
+    """|Bm| > 1 makes I - Bm^T Bm indefinite: the kernel's flag turns into
+    KMH_E_NOT_PD (the reference's PosDefException), and the next block factors."""
+    r = 20
+    R1 = 2.0 * np.eye(r)
+    R2 = np.eye(r)
+    rc, _, _, _ = _capacitance(ctx, R1, R2)
+    assert rc == 2 and "PosDef" in ctx.lib.last_error()
+    rc, _, _, ls = _capacitance(ctx, 0.5 * np.eye(r), np.eye(r))
+    assert rc == 0 and abs(ls - r * 0.5 * np.log(0.75)) <= 1e-13 * r
+
+
 @pytest.mark.parametrize("trans", [0, 1])
 @pytest.mark.parametrize("n,ncols", [(3, 2), (64, 300), (257, 70)])
 def test_trsm(ctx, trans, n, ncols):

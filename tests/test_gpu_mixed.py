@@ -28,7 +28,8 @@ def _build(lib, L, ctx, x, fam, params, nugget, method, level, rank, tol, precis
     opts.precision = precision
     h = C.c_void_p()
     n = x.shape[0]
-    assert lib.kmh_hodlr_build(ctx.ptr, np.ascontiguousarray(x).ctypes.data, n, 2, C.byref(kern), C.byref(opts),
+    x = np.ascontiguousarray(x)   # held across the call
+    assert lib.kmh_hodlr_build(ctx.ptr, x.ctypes.data, n, 2, C.byref(kern), C.byref(opts),
                                C.byref(h)) == 0, lib.last_error()
     return h
 

@@ -199,7 +199,8 @@ def test_context_destroyed_before_its_handles(gpu_lib):
     opts.level_kind = L.KMH_FIXED_LEVEL
     opts.level = 2
     h = C.c_void_p()
-    assert lib.kmh_hodlr_build(ctx, np.ascontiguousarray(x).ctypes.data, n, 2, C.byref(kern), C.byref(opts),
+    x = np.ascontiguousarray(x)   # held across the call
+    assert lib.kmh_hodlr_build(ctx, x.ctypes.data, n, 2, C.byref(kern), C.byref(opts),
                                C.byref(h)) == 0, lib.last_error()
     srv = C.c_void_p()
     assert lib.kmh_server_create(ctx, 4, 0, C.byref(srv)) == 0
