@@ -44,6 +44,9 @@ struct Hodlr {
     DevBuf pts;                                  // x[0..n), y[0..n)
     DevBuf info;                                 // device failure flag (Cholesky)
     DevBuf leaves;                               // leaf i at leaf_mat_off[i], ld = size
+    bool leaves_ready = false;                   // leaves hold K's leaves (assembled on first
+                                                 // use, ensure_leaves; a factorization of
+                                                 // leaves of <= 256 rows evaluates them itself)
     std::vector<int64_t> leaf_mat_off;
     DevBuf P;                                    // n x Rtot, ld n: ACA U/V, then Q per level
                                                  // (FP32 elements in the mixed form)

@@ -24,6 +24,7 @@
 #include "assemble.h"
 #include "hodlr.h"
 #include "la.h"
+#include "leaf.h"
 
 namespace kmh {
 
@@ -335,12 +336,9 @@ std::vector<Hodlr*> hodlr_build_batch(Ctx& c, const std::vector<BuildReq>& reqs,
         const double* px = H.pts.as<double>();
         const double* py = px + H.n;
         const KDev kd = to_kdev(H.kern);
-        // leaves (constructor.jl:39)
-        for (int l = 0; l < H.nleaves(); ++l) {
-            const int64_t s = H.leaf_off[l];
-            const int m = (int)H.leaf_size(l);
-            ev.push_back({px + s, py + s, px + s, py + s, m, m, H.leaf_ptr(l), m, 0, 0, kd});
-        }
+        // the leaves (constructor.jl:39) are assembled when first needed
+        // (ensure_leaves); the factorization evaluates leaves of at most
+        // LEAF_FUSED_MAX rows itself
         if (H.L > 0 && H.nystrom()) {
             nys.push_back((int)i);
             const int r = H.nys_r;
@@ -396,6 +394,33 @@ Hodlr* hodlr_build(Ctx& c, const double* pts, bool dev, int64_t n, int d, const 
     auto hs = hodlr_build_batch(c, {{pts, dev, n, d, k, o}}, st);
     if (st[0].code != KMH_OK) throw Error(st[0].code, st[0].msg);
     return hs[0];
+}
+
+// K's leaves (constructor.jl:39) in the handle's leaf storage, for the
+// handles whose leaves are not assembled yet.
+static void ensure_leaves(Ctx& c, const std::vector<Hodlr*>& hs) {
+    std::vector<EvalDesc> ev;
+    for (Hodlr* h : hs) {
+        Hodlr& H = *h;
+        if (H.leaves_ready) continue;
+        const double* px = H.pts.as<double>();
+        const double* py = px + H.n;
+        const KDev kd = to_kdev(H.kern);
+        for (int l = 0; l < H.nleaves(); ++l) {
+            const int64_t s = H.leaf_off[l];
+            const int m = (int)H.leaf_size(l);
+            ev.push_back({px + s, py + s, px + s, py + s, m, m, H.leaf_ptr(l), m, 0, 0, kd});
+        }
+        H.leaves_ready = true;
+    }
+    eval_blocks(c, ev);
+}
+
+// Leaves that the fused leaf kernels take (every leaf at most LEAF_FUSED_MAX rows).
+static bool fused_leaves(const Hodlr& H) {
+    for (int i = 0; i < H.nleaves(); ++i)
+        if (H.leaf_size(i) > LEAF_FUSED_MAX) return false;
+    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -783,26 +808,79 @@ void factor_level_batch(Ctx& c, const std::vector<Hodlr*>& act, const std::vecto
 void factor_begin_batch(Ctx& c, const std::vector<Hodlr*>& hs) {
     c.set_device();
     reset_info(c, hs);
+    // Leaves of at most LEAF_FUSED_MAX rows: evaluated, factored and their
+    // diagonal-block inverses kept by one fused kernel, then the panels'
+    // leaf solves by the second (leaf.h).  Larger leaves are assembled and
+    // go through potrf and the blocked triangular solve.
+    std::vector<Hodlr*> assemble;
+    std::vector<char> fz(hs.size());
+    size_t linv_sz = 0;
+    for (size_t hi = 0; hi < hs.size(); ++hi) {
+        Hodlr& H = *hs[hi];
+        fz[hi] = fused_leaves(H);
+        if (!fz[hi]) assemble.push_back(hs[hi]);
+        else
+            for (int i = 0; i < H.nleaves(); ++i) linv_sz += leaf_linv_size((int)H.leaf_size(i));
+    }
+    ensure_leaves(c, assemble);
+    struct Hold {
+        Ctx& c;
+        void* p;
+        ~Hold() { if (p) c.scratch_free(p); }
+    } linv{c, linv_sz ? c.scratch(linv_sz * sizeof(double)) : nullptr};
+    size_t loff = 0;
     std::vector<SqMat> lm;
     std::vector<double*> lm_out;
+    std::vector<LeafChol> lc;
+    std::vector<LeafSolve> ls;
     std::vector<Copy2D> abinit;
     std::vector<Rhs> lt;
     std::vector<GemmProb> bhalf;
     std::vector<Copy2D> ycopy;
-    for (Hodlr* h : hs) {
+    for (size_t hi = 0; hi < hs.size(); ++hi) {
+        Hodlr* h = hs[hi];
         Hodlr& H = *h;
         H.release_transient();
         H.carry = false;
         H.has_winvy = false;
         H.factor_started = true;
+        H.leaves_ready = false;   // the leaf storage receives L
         const int64_t n = H.n;
         const int L = H.L, nl = H.nleaves();
         H.nparts = nl + H.nblocks();
         H.parts = static_cast<double*>(c.scratch((size_t)(H.nparts + 1) * sizeof(double)));
-        for (int i = 0; i < nl; ++i) {
-            lm.push_back({H.leaf_ptr(i), (int)H.leaf_size(i), (int)H.leaf_size(i), H.info.as<int>()});
-            lm_out.push_back(H.parts + i);
+        double* hlinv = static_cast<double*>(linv.p) + loff;   // this handle's diagonal-block inverses
+        if (fz[hi]) {
+            const double* px = H.pts.as<double>();
+            const double* py = px + n;
+            const KDev kd = to_kdev(H.kern);
+            size_t o = 0;
+            for (int i = 0; i < nl; ++i) {
+                const int m = (int)H.leaf_size(i);
+                const int64_t s = H.leaf_off[i];
+                lc.push_back({px + s, py + s, m, H.leaf_ptr(i), hlinv + o, H.parts + i, H.info.as<int>(), kd});
+                o += leaf_linv_size(m);
+            }
+            loff += o;
+        } else {
+            for (int i = 0; i < nl; ++i) {
+                lm.push_back({H.leaf_ptr(i), (int)H.leaf_size(i), (int)H.leaf_size(i), H.info.as<int>()});
+                lm_out.push_back(H.parts + i);
+            }
         }
+        // the panels' leaf solves: fused for FP64 panels beside fused leaves
+        auto leaf_solves = [&](double* T, int ncols, bool e32) {
+            if (!fz[hi] || e32) {
+                lt.push_back({h, T, n, ncols, 0, e32});
+                return;
+            }
+            size_t o = 0;
+            for (int i = 0; i < nl; ++i) {
+                const int m = (int)H.leaf_size(i);
+                ls.push_back({H.leaf_ptr(i), hlinv + o, m, T + H.leaf_off[i], (int)n, ncols});
+                o += leaf_linv_size(m);
+            }
+        };
         // per-block factor storage
         H.coff_blk.assign(L, {});
         size_t xtot = 0;
@@ -838,7 +916,7 @@ void factor_begin_batch(Ctx& c, const std::vector<Hodlr*>& hs) {
                 else KMH_HIP(hipMemcpyAsync(H.AB, H.carry_host, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c.q()));
                 ycopy.push_back({H.AB, (int)n, H.AB + (size_t)n * (w - 1), (int)n, (int)n, 1, 1.0});
             }
-            lt.push_back({h, H.AB, n, r + ca, 0, e});   // [y] and the K(x,P) half
+            leaf_solves(H.AB, r + ca, e);   // [y] and the K(x,P) half
             GemmProb bh{(int)n, r, r, tadv(H.AB, (size_t)n * ca, e), (int)n, H.kinv.as<double>(), r,
                         tadv(H.AB, (size_t)n * cb, e), (int)n, 1.0, 0.0, 0};
             bh.ea = bh.ec = e;
@@ -852,14 +930,16 @@ void factor_begin_batch(Ctx& c, const std::vector<Hodlr*>& hs) {
                 abinit.push_back(u);
             }
             // leaf inverse on every level's U and V (symfact.jl:21)
-            lt.push_back({h, H.P.as<double>(), n, (int)H.Rtot, 0, H.p32()});
+            leaf_solves(H.P.as<double>(), (int)H.Rtot, H.p32());
         }
     }
     // leaf Cholesky (symfact.jl:17)
+    leaf_chol(c, lc);
     potrf(c, lm);
     log_diag_sums(c, lm, lm_out);
     copy2d(c, abinit);
-    leaf_trsm_batch(c, false, lt);
+    leaf_trsm_batch(c, false, lt);   // (symfact.jl:21)
+    leaf_solve(c, ls);
     gemm(c, false, false, bhalf);
     copy2d(c, ycopy);   // L^{-1} y into the last panel column too
 }
@@ -1133,6 +1213,7 @@ static void operator_matvec(Hodlr& H, const double* x, double* y, int nrhs) {
         leaf_mul(H, true, y, nrhs);
         leaf_mul(H, false, y, nrhs);
     } else {
+        ensure_leaves(c, {&H});
         std::vector<GemmProb> g;
         for (int i = 0; i < H.nleaves(); ++i) {
             const int m = (int)H.leaf_size(i);
@@ -1197,6 +1278,7 @@ void hodlr_full_host(Hodlr& H, double* out) {
     KMH_CHECK(!H.factorized, KMH_E_STATE, "full(HK) needs the unfactorized U and V.");
     Ctx& c = *H.ctx;
     const int64_t n = H.n;
+    ensure_leaves(c, {&H});
     std::vector<double> lv(H.leaves.bytes / sizeof(double));
     KMH_HIP(hipMemcpyAsync(lv.data(), H.leaves.p, H.leaves.bytes, hipMemcpyDeviceToHost, c.q()));
     c.sync();

@@ -122,6 +122,7 @@ uint64_t request_footprint(int64_t n, const kmh_build_opts& o, int nv) {
     uint64_t b = 16 * (uint64_t)n + 8 * (uint64_t)n * m + slabs + xg;
     if (o.method == KMH_NYSTROM) b += 8 * (uint64_t)n * (r * 4 + 3) + 24 * r * r;   // + carried y columns, W^-1 y
     b += 8 * (uint64_t)n * (2 + nv) * 2;                          // right-hand sides
+    b += 8 * (uint64_t)n * 16;                                    // the leaf stage's diagonal-block inverses
     if (nv > 0) b += 8 * (uint64_t)n * (m + r + 2);               // derivative leaves and panel
     return b;
 }
@@ -137,7 +138,7 @@ static uint64_t built_footprint(const Hodlr& H, int nv) {
             rmax = std::max<uint64_t>(rmax, (uint64_t)r);
         }
     const uint64_t n = (uint64_t)H.n;
-    uint64_t b = H.device_bytes() + xg + 8 * n * (2 + nv) * 2;
+    uint64_t b = H.device_bytes() + xg + 8 * n * (2 + nv) * 2 + 8 * n * 16;   // + leaf-stage block inverses
     if (H.nystrom()) b += 8 * n * (uint64_t)H.Rtot + 8 * n * (2 * rmax + 3);   // level slabs, cascade panels, W^-1 y
     if (nv > 0) {
         int64_t m = 0;

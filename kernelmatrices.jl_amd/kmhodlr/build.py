@@ -16,7 +16,7 @@ INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(HERE, "libkmhodlr.so")
 
-SOURCES = ["ctx.hip", "la.hip", "qr.hip", "assemble.hip", "hodlr.hip", "gradient.hip", "hessian.hip", "capi.hip", "serve.hip", "ordering.hip", "rhodlr.hip", "testing.hip", "noalloc.hip", "graph.hip", "tune.hip"]
+SOURCES = ["ctx.hip", "la.hip", "qr.hip", "assemble.hip", "hodlr.hip", "gradient.hip", "hessian.hip", "capi.hip", "serve.hip", "ordering.hip", "rhodlr.hip", "testing.hip", "noalloc.hip", "graph.hip", "tune.hip", "leaf.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

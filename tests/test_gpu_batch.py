@@ -40,7 +40,10 @@ def test_alone_equals_in_the_mix(gpu_lib):
     specs = [(3000, L.KMH_MATERN32, (1.0, 0.1), L.KMH_NYSTROM, 3, 48, 0.0),
              (1500, L.KMH_MATERN12, (0.7, 0.2), L.KMH_ACA, 2, 0, 1e-8),
              (5000, L.KMH_MATERN32, (2.0, 0.05), L.KMH_NYSTROM, 4, 72, 0.0),
-             (800, L.KMH_MATERN12, (1.0, 0.3), L.KMH_ACA, 1, 0, 1e-10)]
+             (800, L.KMH_MATERN12, (1.0, 0.3), L.KMH_ACA, 1, 0, 1e-10),
+             # leaves of 250 and 150 rows (fused leaf kernels) beside the larger ones
+             (2000, L.KMH_MATERN32, (1.0, 0.1), L.KMH_NYSTROM, 3, 40, 0.0),
+             (1200, L.KMH_MATERN12, (1.0, 0.2), L.KMH_ACA, 3, 0, 1e-9)]
     xs = [np.ascontiguousarray(kd_sort(rng.random((n, 2)))) for (n, *_) in specs]
     ys = [rng.standard_normal(n) for (n, *_) in specs]
     reqs = [_req(L, x, f, p, 1e-6, m, lv, mr, tol) for x, (n, f, p, m, lv, mr, tol) in zip(xs, specs)]
@@ -67,7 +70,7 @@ def test_alone_equals_in_the_mix(gpu_lib):
     alone = {}
     for i in range(len(specs)):
         alone.update(run([i]))
-    mixed = run([2, 0, 3, 1])
+    mixed = run([4, 2, 0, 5, 3, 1])
     for i in range(len(specs)):
         print("problem", i, "logdet", alone[i][0])
         assert alone[i][2] == mixed[i][2]

@@ -48,6 +48,13 @@ CASES = {
                             ep=2.2e-16, maxrank=72, lvl=("log", 8), nys=True, w_fixed=False),
     "aca_maxrank": dict(n=2048, pts="unif", fam=kk.MATERN32, params=(1.0, 0.1), nugget=1e-1,
                         ep=1e-12, maxrank=24, lvl=("fixed", 3), nys=False),
+    # leaves of 750 rows: beyond the fused leaf kernels (leaves assembled on
+    # first use, blocked Cholesky and triangular solves)
+    "matern32_nys_bigleaf": dict(n=3000, pts="unif", fam=kk.MATERN32, params=(1.0, 0.1), nugget=1e-6,
+                                 ep=0.0, maxrank=40, lvl=("fixed", 2), nys=True, w_fixed=False),
+    # one leaf, no off-diagonal levels
+    "rational_single_leaf": dict(n=200, pts="normal", fam=kk.RATIONAL2, params=(1.0, 1.0), nugget=1.0,
+                                 ep=1e-12, maxrank=0, lvl=("fixed", 0), nys=False),
 }
 
 

@@ -169,7 +169,7 @@ def test_serve_admission_budget(gpu_lib):
     # the server counts it (n=2048, 8 leaves of 256, r=40, 3 levels, nv=8)
     n, m, r, L3, nv = 2048, 256, 40, 3, 8
     fp = (16 * n + 8 * n * m + 8 * n * r * L3 + 8 * 7 * 2 * 4 * r * r + 8 * n * (r * 4 + 3) + 24 * r * r
-          + 8 * n * (2 + nv) * 2)
+          + 8 * n * (2 + nv) * 2 + 8 * n * 16)
     res, peak = _serve(lib, L, ctx, probs, 8, budget=2 * fp + 1)
     assert peak <= 2
     assert all(r[1] == 0 for r in res.values())
