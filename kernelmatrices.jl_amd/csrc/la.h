@@ -93,6 +93,22 @@ struct QrPanel {
 };
 void qr_panels(Ctx& c, const std::vector<QrPanel>& panels);
 
+// One block Gram-Schmidt pass of the panel QR fused with the projection of
+// the next: X = S - Q T1 (written to X; S may be X itself) and T2 = Q^T X,
+// for Q m x kq (kq <= CGS_KQ_MAX), S and X m x nb (nb <= CGS_NB_MAX),
+// T1 kq x nb.  FP64 only.
+constexpr int CGS_KQ_MAX = 48;
+constexpr int CGS_NB_MAX = 32;
+struct CgsProb {
+    const double* Q; int ldq; int m; int kq;
+    const double* S; int lds;
+    const double* T1; int ldt1;
+    double* X; int ldx;
+    int nb;
+    double* T2; int ldt2;
+};
+void cgs_pass(Ctx& c, const std::vector<CgsProb>& ps);
+
 // The capacitance factors of one block of lrsymfact (HODLR/src/utils.jl:29-35)
 // for r <= CAP_MAX, in one workgroup per block: with Bm = R1 R2^T and
 // C = [I, Bm; Bm^T, I] = chol(C) chol(C)^T, chol(C) = [I, 0; Bm^T, L22] and

@@ -643,6 +643,162 @@ static void gemm_tall_tn(Ctx& c, const std::vector<GemmProb>& probs, int tc) {
     if (W) c.scratch_free(W);
 }
 
+// ---------------------------------------------------------------------------
+// Fused Gram-Schmidt pass (la.h): one workgroup per slab of TT_KS rows, in
+// chunks of CG_KC rows staged in LDS.  Per chunk, each wave forms one 16 x 16
+// tile of X = S - Q T1 on MFMA (T1 held in LDS for the whole slab), writes it
+// out and back to LDS, and the waves accumulate the slab's Q^T X on MFMA; the
+// slabs' partial products are summed by the split-k reduction.  Q and S are
+// read once and X written once, where the unfused pass reads Q and X again
+// for the projection.
+constexpr int CG_KC = 32;
+
+struct CgsDesc {
+    const double* Q; const double* S; const double* T1;
+    double* X; double* W;
+    int ldq, lds, ldt1, ldx;
+    int rows, kq, nb, pad;
+};
+
+template <int KQ>
+__global__ __launch_bounds__(256, 2) void k_cgs_pass(const CgsDesc* __restrict__ ds) {
+    constexpr int NB = CGS_NB_MAX;
+    constexpr int NQ = KQ / 16;                          // 16-row blocks of Q^T X
+    constexpr int NT = (NQ * (NB / 16) + 3) / 4;         // its tiles per wave
+    constexpr int QPT = (CG_KC * KQ) / 256;              // staged Q values per thread
+    constexpr int SPT = (CG_KC * NB) / 256;              // staged S values per thread
+    __shared__ double Qs[CG_KC][KQ + 1];
+    __shared__ double Xs[CG_KC][NB + 1];
+    __shared__ double Ts[KQ][NB + 1];
+    const CgsDesc d = ds[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, li = lane & 15, lg = lane >> 4;
+    const Glob<const double> Qg = glob(d.Q), Sg = glob(d.S);
+    const Glob<double> Xg = glob(d.X);
+    for (int e = tid; e < KQ * NB; e += 256) {
+        const int i = e % KQ, j = e / KQ;
+        Ts[i][j] = (i < d.kq && j < d.nb) ? d.T1[i + (size_t)j * d.ldt1] : 0.0;
+    }
+    // staging: thread tid moves row tid % CG_KC of columns tid / CG_KC + 8q
+    const int sk = tid % CG_KC, sc = tid / CG_KC;
+    double rq[QPT], rs[SPT];
+    auto fetch = [&](int r0) {
+        const int row = r0 + sk;
+        const bool rok = row < d.rows;
+#pragma unroll
+        for (int q = 0; q < QPT; ++q) {
+            const int col = sc + 8 * q;
+            const bool ok = rok && col < d.kq;
+            rq[q] = keep_or_zero(Qg[ok ? row + (size_t)col * d.ldq : 0], ok);
+        }
+#pragma unroll
+        for (int q = 0; q < SPT; ++q) {
+            const int col = sc + 8 * q;
+            const bool ok = rok && col < d.nb;
+            rs[q] = keep_or_zero(Sg[ok ? row + (size_t)col * d.lds : 0], ok);
+        }
+    };
+    d4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
+    fetch(0);
+    const int xrb = wave & 1, xcb = wave >> 1;   // this wave's tile of X in a chunk
+    for (int r0 = 0; r0 < d.rows; r0 += CG_KC) {
+#pragma unroll
+        for (int q = 0; q < QPT; ++q) Qs[sk][sc + 8 * q] = rq[q];
+#pragma unroll
+        for (int q = 0; q < SPT; ++q) Xs[sk][sc + 8 * q] = rs[q];
+        __syncthreads();
+        if (r0 + CG_KC < d.rows) fetch(r0 + CG_KC);
+        // X = S - Q T1, tile (xrb, xcb)
+        d4 xa = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s4 = 0; s4 < KQ / 4; ++s4) {
+            const int k = 4 * s4 + lg;
+            xa = __builtin_amdgcn_mfma_f64_16x16x4f64(Qs[16 * xrb + li][k], Ts[k][16 * xcb + li], xa, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * xrb + mfma_d_row(lg, r), col = 16 * xcb + li;
+            const double x = Xs[row][col] - xa[r];
+            Xs[row][col] = x;
+            if (r0 + row < d.rows && col < d.nb) Xg[r0 + row + (size_t)col * d.ldx] = x;
+        }
+        __syncthreads();
+        // Q^T X of the chunk: block (mb, nb) of the output, A = Q^T from Qs
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int b = wave + 4 * t;
+            if (b < NQ * (NB / 16)) {
+                const int mb = b / (NB / 16), nb = b - mb * (NB / 16);
+#pragma unroll
+                for (int s4 = 0; s4 < CG_KC / 4; ++s4) {
+                    const int kk = 4 * s4 + lg;
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(Qs[kk][16 * mb + li], Xs[kk][16 * nb + li], acc[t], 0, 0,
+                                                                  0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // the slab's partial Q^T X (kq x nb, ld kq)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int b = wave + 4 * t;
+        if (b < NQ * (NB / 16)) {
+            const int mb = b / (NB / 16), nb = b - mb * (NB / 16);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * mb + mfma_d_row(lg, r), j = 16 * nb + li;
+                if (i < d.kq && j < d.nb) d.W[i + (size_t)j * d.kq] = acc[t][r];
+            }
+        }
+    }
+}
+
+void cgs_pass(Ctx& c, const std::vector<CgsProb>& ps) {
+    if (ps.empty()) return;
+    size_t wsz = 0;
+    for (const auto& p : ps) {
+        KMH_CHECK(p.kq > 0 && p.kq <= CGS_KQ_MAX && p.nb > 0 && p.nb <= CGS_NB_MAX, KMH_E_BAD_ARG,
+                  "cgs_pass: block beyond its limits");
+        wsz += (size_t)ceil_div(p.m, TT_KS) * p.kq * p.nb;
+    }
+    double* W = static_cast<double*>(c.scratch(wsz * sizeof(double)));
+    std::vector<CgsDesc> d32, d48;
+    std::vector<RedDesc> rs;
+    std::vector<int> rmap;
+    size_t woff = 0;
+    long long rblk = 0;
+    double by32 = 0.0, by48 = 0.0;   // algorithmic bytes: Q and S read, X written
+    for (const auto& p : ps) {
+        const int s = (int)ceil_div(p.m, TT_KS);
+        for (int q = 0; q < s; ++q) {
+            const int r0 = q * TT_KS;
+            CgsDesc dd{p.Q + r0, p.S + r0, p.T1, p.X + r0, W + woff + (size_t)q * p.kq * p.nb,
+                       p.ldq, p.lds, p.ldt1, p.ldx, std::min(TT_KS, p.m - r0), p.kq, p.nb, 0};
+            (p.kq <= 32 ? d32 : d48).push_back(dd);
+        }
+        const long long nb = ceil_div((long long)p.kq * p.nb, 256);
+        map_append(rmap, (int)rs.size(), nb);
+        rs.push_back({p.T2, W + woff, p.ldt2, p.kq, p.nb, s, 1.0, 0.0, rblk});
+        woff += (size_t)s * p.kq * p.nb;
+        rblk += nb;
+        (p.kq <= 32 ? by32 : by48) += 8.0 * p.m * (p.kq + 2.0 * p.nb);
+    }
+    if (!d32.empty()) {
+        CgsDesc* dd = c.upload_vec(d32);
+        KMH_LAUNCH_TIMED(c, KMH_PROF_TN_TALL, by32, 0.0, k_cgs_pass<32>, dim3((unsigned)d32.size()), dim3(256), dd);
+    }
+    if (!d48.empty()) {
+        CgsDesc* dd = c.upload_vec(d48);
+        KMH_LAUNCH_TIMED(c, KMH_PROF_TN_TALL, by48, 0.0, k_cgs_pass<48>, dim3((unsigned)d48.size()), dim3(256), dd);
+    }
+    RedDesc* dr = c.upload_vec(rs);
+    const int* drm = c.upload_vec(rmap);
+    KMH_LAUNCH_TIMED(c, KMH_PROF_GEMM, 0.0, 0.0, k_splitk_reduce<double>, dim3((unsigned)rblk), dim3(256), dr, drm);
+    c.scratch_free(W);
+}
+
 static void gemm_wide_nn(Ctx& c, const std::vector<GemmProb>& probs, int tc) {
     std::vector<GemmDesc> ds;
     std::vector<int> map;

@@ -700,6 +700,7 @@ void qr_panels(Ctx& c, const std::vector<QrPanel>& ps) {
     for (int c0 = 0; c0 < rmax; c0 += QC) {
         std::vector<TsqrPanel> tp;
         std::vector<GemmProb> p1, u1, p2;
+        std::vector<CgsProb> cg;
         std::vector<Axpby2D> acc;
         size_t tsz = 0;
         for (const auto& p : ps)
@@ -715,7 +716,9 @@ void qr_panels(Ctx& c, const std::vector<QrPanel>& ps) {
             double* Rj = p.R + (size_t)c0 * p.ldr;
             // this column block's input: the source panel on first touch
             const double* sj = p.src ? adv_e(p.src, (size_t)c0 * p.lds, e) : nullptr;
-            if (sj && c0 > 64) {   // beyond the out-of-place update's k range: copy first
+            // FP64 panels: the update X = S - Q T1 and the projection Q^T X in one pass
+            const bool fused = !e && c0 <= CGS_KQ_MAX && nb <= CGS_NB_MAX;
+            if (sj && c0 > 64 && !fused) {   // beyond the out-of-place update's k range: copy first
                 Copy2D cp{sj, p.lds, Aj, p.lda, p.m, nb, 1.0};
                 cp.es = cp.ed = e;
                 pre.push_back(cp);
@@ -728,13 +731,17 @@ void qr_panels(Ctx& c, const std::vector<QrPanel>& ps) {
                 GemmProb g0{c0, nb, p.m, p.A, p.lda, sj ? sj : Aj, sj ? p.lds : p.lda, Rj, p.ldr, 1.0, 0.0, 0};
                 g0.ea = g0.eb = e;
                 p1.push_back(g0);
-                GemmProb g1{p.m, nb, c0, p.A, p.lda, Rj, p.ldr, Aj, p.lda, -1.0, 1.0, 0};
-                g1.Cin = sj;
-                g1.ea = g1.ec = e;
-                u1.push_back(g1);
-                GemmProb g2{c0, nb, p.m, p.A, p.lda, Aj, p.lda, t2, c0, 1.0, 0.0, 0};
-                g2.ea = g2.eb = e;
-                p2.push_back(g2);
+                if (fused) {
+                    cg.push_back({p.A, p.lda, p.m, c0, sj ? sj : Aj, sj ? p.lds : p.lda, Rj, p.ldr, Aj, p.lda, nb, t2, c0});
+                } else {
+                    GemmProb g1{p.m, nb, c0, p.A, p.lda, Rj, p.ldr, Aj, p.lda, -1.0, 1.0, 0};
+                    g1.Cin = sj;
+                    g1.ea = g1.ec = e;
+                    u1.push_back(g1);
+                    GemmProb g2{c0, nb, p.m, p.A, p.lda, Aj, p.lda, t2, c0, 1.0, 0.0, 0};
+                    g2.ea = g2.eb = e;
+                    p2.push_back(g2);
+                }
                 acc.push_back({t2, c0, Rj, p.ldr, c0, nb, 1.0, 1.0});
                 // the second pass's update X -= Q T2 happens as the leaf
                 // tiles load X (k_tile_qr)
@@ -751,6 +758,7 @@ void qr_panels(Ctx& c, const std::vector<QrPanel>& ps) {
         copy2d(c, pre);
         if (!p1.empty()) {
             gemm(c, true, false, p1);
+            cgs_pass(c, cg);
             gemm(c, false, false, u1);
             gemm(c, true, false, p2);
             axpby2d(c, acc);
