@@ -33,9 +33,10 @@ namespace {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int LF_NT = LEAF_FUSED_MAX / 16;               // tiles per dimension
-constexpr int LF_TILES = LF_NT * (LF_NT + 1) / 2;         // lower tiles
+constexpr int LF_TILES = LF_NT * (LF_NT - 1) / 2;         // tiles below the diagonal
 constexpr int LF_WAVES = 8;
 constexpr int LF_SLOTS = (LF_TILES + LF_WAVES - 1) / LF_WAVES;
+constexpr int LF_RS = LF_SLOTS - 2;   // slots held in registers; the last two in LDS
 constexpr int LS_CW = 80;                                 // columns per solve workgroup
 constexpr int LS_CB = LS_CW / 16;
 
@@ -50,12 +51,12 @@ __device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// tile id (row-major over the lower triangle) -> block row i, column j
+// tile id (row-major over the tiles below the diagonal) -> block row i > column j
 __device__ __forceinline__ void tile_ij(int id, int& i, int& j) {
-    int r = 0;
-    while ((r + 1) * (r + 2) / 2 <= id) ++r;
+    int r = 1;
+    while (r * (r + 1) / 2 <= id) ++r;
     i = r;
-    j = id - r * (r + 1) / 2;
+    j = id - r * (r - 1) / 2;
 }
 
 // The owned tiles of K (identity beyond m) for one covariance family: with
@@ -64,10 +65,21 @@ __device__ __forceinline__ void tile_ij(int id, int& i, int& j) {
 template <int FAM>
 __device__ __forceinline__ void eval_tiles(const KDev& k0, d4 (&R)[LF_SLOTS], const int (&ti)[LF_SLOTS],
                                            const int (&tj)[LF_SLOTS], const double (*pts)[LEAF_FUSED_MAX], int m,
-                                           int li, int lg) {
+                                           int li, int lg, int wave, int nt, double (*dtile)[256], double* lt) {
     KDev k = k0;
     k.family = FAM;
     k.deriv = KMH_D0;
+    const int lane = li + 16 * lg;
+    for (int p = wave; p < nt; p += LF_WAVES) {   // the wave's diagonal tiles, to LDS
+        const int row = 16 * p + li;
+        const double xr = pts[0][row], yr = pts[1][row];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int col = 16 * p + 4 * q + lg;
+            dtile[p][q * 64 + lane] = (row < m && col < m) ? kernel_eval(k, xr, yr, pts[0][col], pts[1][col])
+                                                           : (row == col ? 1.0 : 0.0);
+        }
+    }
 #pragma unroll
     for (int t = 0; t < LF_SLOTS; ++t) {
         if (ti[t] < 0) continue;
@@ -76,8 +88,10 @@ __device__ __forceinline__ void eval_tiles(const KDev& k0, d4 (&R)[LF_SLOTS], co
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int col = 16 * tj[t] + 4 * q + lg;
-            R[t][q] = (row < m && col < m) ? kernel_eval(k, xr, yr, pts[0][col], pts[1][col])
-                                           : (row == col ? 1.0 : 0.0);
+            const double v = (row < m && col < m) ? kernel_eval(k, xr, yr, pts[0][col], pts[1][col])
+                                                  : (row == col ? 1.0 : 0.0);
+            if (t < LF_RS) R[t][q] = v;
+            else lt[(t - LF_RS) * 256 + q * 64 + lane] = v;
         }
         __builtin_amdgcn_sched_barrier(0);   // one tile's evaluations at a time
     }
@@ -89,7 +103,8 @@ __global__ __launch_bounds__(512) void k_leaf_chol(const LeafChol* __restrict__ 
     __shared__ double pan[LF_NT][256];   // L_ip of the current block column, tile layout
     __shared__ double dinv[256];         // its inverse, tile layout
     __shared__ double trs[16 * 17];      // a transposition of the diagonal owner
-    __shared__ double park[LF_SLOTS * 256];   // the diagonal tile owner's tiles, tile layout
+    __shared__ double dtile[LF_NT][256];      // the diagonal tiles, tile layout
+    __shared__ double ltile[LF_WAVES][(LF_SLOTS - LF_RS) * 256];   // each wave's last tile slots, tile layout
     __shared__ double pivs[LEAF_FUSED_MAX];   // diag(L), for the log sum at the end
     __shared__ double lsum[64 * LF_WAVES];
     const LeafChol d = ds[blockIdx.x];
@@ -102,7 +117,7 @@ __global__ __launch_bounds__(512) void k_leaf_chol(const LeafChol* __restrict__ 
         pts[1][e] = e < m ? d.py[e] : 0.0;
     }
     __syncthreads();
-    const int ntile = nt * (nt + 1) / 2;
+    const int ntile = nt * (nt - 1) / 2;
     double* __restrict__ L = d.L;
     int ti[LF_SLOTS], tj[LF_SLOTS];
     d4 R[LF_SLOTS];
@@ -114,21 +129,20 @@ __global__ __launch_bounds__(512) void k_leaf_chol(const LeafChol* __restrict__ 
         R[t] = d4{0.0, 0.0, 0.0, 0.0};
         if (id < ntile) {
             tile_ij(id, ti[t], tj[t]);
-            if (ti[t] > tj[t]) {
-                const int ur = 16 * tj[t] + li;
+            const int ur = 16 * tj[t] + li;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int uc = 16 * ti[t] + 4 * q + lg;
-                    if (ur < m && uc < m) L[ur + (size_t)uc * m] = 0.0;
-                }
+            for (int q = 0; q < 4; ++q) {
+                const int uc = 16 * ti[t] + 4 * q + lg;
+                if (ur < m && uc < m) L[ur + (size_t)uc * m] = 0.0;
             }
         }
     }
+    double* const lt = ltile[wave];
     switch (k.family) {
-        case KMH_RATIONAL2: eval_tiles<KMH_RATIONAL2>(k, R, ti, tj, pts, m, li, lg); break;
-        case KMH_CAUCHY: eval_tiles<KMH_CAUCHY>(k, R, ti, tj, pts, m, li, lg); break;
-        case KMH_MATERN12: eval_tiles<KMH_MATERN12>(k, R, ti, tj, pts, m, li, lg); break;
-        default: eval_tiles<KMH_MATERN32>(k, R, ti, tj, pts, m, li, lg); break;
+        case KMH_RATIONAL2: eval_tiles<KMH_RATIONAL2>(k, R, ti, tj, pts, m, li, lg, wave, nt, dtile, lt); break;
+        case KMH_CAUCHY: eval_tiles<KMH_CAUCHY>(k, R, ti, tj, pts, m, li, lg, wave, nt, dtile, lt); break;
+        case KMH_MATERN12: eval_tiles<KMH_MATERN12>(k, R, ti, tj, pts, m, li, lg, wave, nt, dtile, lt); break;
+        default: eval_tiles<KMH_MATERN32>(k, R, ti, tj, pts, m, li, lg, wave, nt, dtile, lt); break;
     }
     bool bad = false;
     for (int p = 0; p < nt; ++p) {
@@ -139,19 +153,10 @@ __global__ __launch_bounds__(512) void k_leaf_chol(const LeafChol* __restrict__ 
 #pragma unroll
         for (int t = 0; t < LF_SLOTS; ++t) __asm__ volatile("" : "+s"(ti[t]), "+s"(tj[t]));
         // 1. the diagonal tile: factor, write out, invert
-        const int did = p * (p + 3) / 2;
-        if (wave == (did % LF_WAVES)) {
-            // the owner's tiles wait in LDS meanwhile (the empty asm keeps
-            // the compiler from carrying them in registers across it)
-#pragma unroll
-            for (int t = 0; t < LF_SLOTS; ++t)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) park[t * 256 + q * 64 + lane] = R[t][q];
-            __asm__ volatile("" ::: "memory");
-            const double* dt = park + (did / LF_WAVES) * 256;
+        if (wave == p % LF_WAVES) {
             d4 st;   // the diagonal tile, tile layout
 #pragma unroll
-            for (int q = 0; q < 4; ++q) st[q] = dt[q * 64 + lane];
+            for (int q = 0; q < 4; ++q) st[q] = dtile[p][q * 64 + lane];
             d4 tmb, tmbt;   // blockdiag(M_0..M_3) and its transpose, tile layout
             // four 4-column steps; step jb: the 4 x 4 diagonal block by
             // broadcasts, its Cholesky factor and inverse M_jb in uniform
@@ -275,21 +280,23 @@ __global__ __launch_bounds__(512) void k_leaf_chol(const LeafChol* __restrict__ 
                 dinv[q * 64 + lane] = linv[q];
                 gl[q * 64 + lane] = linv[q];
             }
-            __asm__ volatile("" ::: "memory");
-#pragma unroll
-            for (int t = 0; t < LF_SLOTS; ++t)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) R[t][q] = park[t * 256 + q * 64 + lane];
         }
         __syncthreads();
         // 2. the tiles below it: L_ip = S_ip L_pp^-T, as (L_ip)^T = L_pp^-1 S_ip^T
 #pragma unroll
         for (int t = 0; t < LF_SLOTS; ++t) {
             if (tj[t] == p && ti[t] > p) {
+                d4 cur;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) cur[q] = t < LF_RS ? R[t][q] : lt[(t - LF_RS) * 256 + q * 64 + lane];
                 d4 acc = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int q = 0; q < 4; ++q) acc = mfma(dinv[q * 64 + lane], R[t][q], acc);
-                R[t] = acc;
+                for (int q = 0; q < 4; ++q) acc = mfma(dinv[q * 64 + lane], cur[q], acc);
+                if (t < LF_RS) R[t] = acc;
+                else {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) lt[(t - LF_RS) * 256 + q * 64 + lane] = acc[q];
+                }
                 const int row = 16 * ti[t] + li;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
@@ -300,15 +307,31 @@ __global__ __launch_bounds__(512) void k_leaf_chol(const LeafChol* __restrict__ 
             }
         }
         __syncthreads();
-        // 3. trailing tiles (i >= j > p): S_ij -= L_ip L_jp^T, as S_ij^T -= L_jp L_ip^T
+        // 3. trailing tiles (i >= j > p): S_ij -= L_ip L_jp^T, as S_ij^T -= L_jp L_ip^T;
+        //    the diagonal ones in LDS, each by the wave that factors it
+        for (int j = p + 1 + (wave + LF_WAVES - (p + 1) % LF_WAVES) % LF_WAVES; j < nt; j += LF_WAVES) {
+            d4 acc;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] = dtile[j][q * 64 + lane];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc = mfma(-pan[j][q * 64 + lane], pan[j][q * 64 + lane], acc);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) dtile[j][q * 64 + lane] = acc[q];
+        }
 #pragma unroll
         for (int t = 0; t < LF_SLOTS; ++t) {
             if (tj[t] > p) {
-                d4 acc = R[t];
+                d4 acc;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q] = t < LF_RS ? R[t][q] : lt[(t - LF_RS) * 256 + q * 64 + lane];
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
                     acc = mfma(-pan[tj[t]][q * 64 + lane], pan[ti[t]][q * 64 + lane], acc);
-                R[t] = acc;
+                if (t < LF_RS) R[t] = acc;
+                else {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) lt[(t - LF_RS) * 256 + q * 64 + lane] = acc[q];
+                }
             }
         }
     }
