@@ -460,6 +460,9 @@ def main():
         check(lib.kmh_server_create(ctx.ptr, args.inflight, 0, C.byref(srv)))
         if args.lanes:
             check(lib.kmh_server_set_lanes(srv, args.lanes))
+        # the server's device pools sized for the requests in flight (server
+        # set-up, before any leg: serving then makes no device allocation)
+        check(lib.kmh_server_reserve(srv, n, C.byref(opts), args.grad_vecs, args.inflight))
         reqs = []
 
         grad_mode = [False]
@@ -560,6 +563,7 @@ def main():
             # one served batch on one lane, every launch timed
             grad_mode[0] = False
             check(lib.kmh_server_set_lanes(srv, 1))
+            check(lib.kmh_server_reserve(srv, n, C.byref(opts), 0, args.inflight))
             check(lib.kmh_ctx_profile_config(ctx.ptr, PROF_MASK, 1))
             k = min(args.inflight, total)
             run(range(k), False)

@@ -115,6 +115,12 @@ int kmh_ctx_create(int device, uint64_t hbm_budget_bytes, kmh_ctx** out);
 int kmh_ctx_destroy(kmh_ctx* ctx);
 int kmh_ctx_synchronize(kmh_ctx* ctx);
 int kmh_ctx_bytes_in_use(kmh_ctx* ctx, uint64_t* out);
+/* Grow the context's device pool so that it holds a free block of at least
+ * `bytes` (one device allocation, nothing when such a block exists); the
+ * memory stays with the context and serves later handles and transients.
+ * No reference counterpart: for callers that size the pool before a
+ * latency-sensitive phase, so that it makes no device allocation.          */
+int kmh_ctx_reserve(kmh_ctx* ctx, uint64_t bytes);
 /* Measurement hooks (no counterpart in the reference): between begin and end
  * every FP64 GEMM kernel launch and every 24-column tile QR launch is
  * bracketed by HIP event markers on the context's stream, so a span is one
@@ -302,6 +308,12 @@ int kmh_server_create(kmh_ctx* ctx, int32_t max_inflight, uint64_t hbm_budget_by
  * running (no reference counterpart)                                        */
 int kmh_server_set_lanes(kmh_server* srv, int32_t lanes);
 int kmh_server_destroy(kmh_server* srv);
+/* Size every lane's pool for `requests` requests of n points built with
+ * `opts` and carrying nv SAA vectors, dealt evenly over the lanes, by the
+ * server's own per-request footprint estimate (kmh_ctx_reserve per lane), so
+ * that serving them makes no device allocation.  Call after
+ * kmh_server_set_lanes.  No reference counterpart.                          */
+int kmh_server_reserve(kmh_server* srv, int64_t n, const kmh_build_opts* opts, int32_t nv, int32_t requests);
 int kmh_server_submit(kmh_server* srv, const kmh_serve_req* req, int64_t* id);
 /* A failure inside a step (device allocation, budget, HIP error) fails every
  * request of the lane it happened on: their results carry the status and

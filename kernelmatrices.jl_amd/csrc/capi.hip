@@ -4,6 +4,7 @@
 #include "deriv.h"
 #include "hodlr.h"
 #include "la.h"
+#include "serve.h"
 
 using kmh::Error;
 
@@ -103,6 +104,13 @@ int kmh_ctx_bytes_in_use(kmh_ctx* ctx, uint64_t* out) {
     KMH_TRY({
         KMH_CHECK(ctx && out, KMH_E_BAD_ARG, "null argument");
         *out = ctx->c.in_use;
+    })
+}
+
+int kmh_ctx_reserve(kmh_ctx* ctx, uint64_t bytes) {
+    KMH_TRY({
+        KMH_CHECK(ctx, KMH_E_BAD_ARG, "null context");
+        kmh::ctx_reserve(ctx->c, (size_t)bytes);
     })
 }
 

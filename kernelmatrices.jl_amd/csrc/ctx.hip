@@ -2,6 +2,7 @@
 #include <algorithm>
 
 #include "common.h"
+#include "serve.h"
 
 namespace kmh {
 
@@ -208,6 +209,22 @@ void* Ctx::scratch(size_t bytes) {
     if (avail > bytes) free_blocks.emplace(p + bytes, std::make_pair(avail - bytes, chunk));
     live.emplace(p, LiveBlock{bytes, chunk, alloc_seq++});
     return p;
+}
+
+void ctx_reserve(Ctx& c, size_t bytes) {
+    bytes = (bytes + 255) & ~size_t(255);
+    for (const auto& fb : c.free_blocks)
+        if (fb.second.first >= bytes) return;
+    void* q = nullptr;
+    c.set_device();
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(KMH_E_OOM, "reserving " + std::to_string(bytes) + " bytes of device memory failed");
+    }
+    c.chunks.push_back({static_cast<char*>(q), bytes});
+    c.pool_bytes += bytes;
+    c.free_blocks.emplace(static_cast<char*>(q), std::make_pair(bytes, (int)c.chunks.size() - 1));
 }
 
 void Ctx::scratch_free(void* vp) {

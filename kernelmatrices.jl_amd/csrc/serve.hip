@@ -552,6 +552,17 @@ int kmh_server_set_lanes(kmh_server* srv, int32_t lanes) {
     })
 }
 
+int kmh_server_reserve(kmh_server* srv, int64_t n, const kmh_build_opts* opts, int32_t nv, int32_t requests) {
+    SRV_TRY({
+        KMH_CHECK(srv && opts && n > 0 && nv >= 0 && requests >= 0, KMH_E_BAD_ARG, "invalid reserve arguments");
+        const int nl = (int)srv->s.lanes.size();
+        const uint64_t per = kmh::request_footprint(n, *opts, nv);
+        const uint64_t each = per * (uint64_t)((requests + nl - 1) / nl);
+        if (each == 0) return KMH_OK;
+        for (auto& ln : srv->s.lanes) kmh::ctx_reserve(*ln.ctx, (size_t)each);
+    })
+}
+
 int kmh_server_destroy(kmh_server* srv) {
     SRV_TRY({
         if (srv) {

@@ -76,6 +76,7 @@ SIGNATURES = {
     "kmh_ctx_destroy": [_P],
     "kmh_ctx_synchronize": [_P],
     "kmh_ctx_bytes_in_use": [_P, C.POINTER(C.c_uint64)],
+    "kmh_ctx_reserve": [_P, C.c_uint64],
     "kmh_ctx_profile_begin": [_P],
     "kmh_ctx_profile_end": [_P, _D, _D, C.POINTER(C.c_int64)],
     "kmh_ctx_profile_query": [_P, C.c_int32, _D, _D, C.POINTER(C.c_int64)],
@@ -113,6 +114,7 @@ SIGNATURES = {
                           _P],
     "kmh_server_create": [_P, C.c_int32, C.c_uint64, C.POINTER(_P)],
     "kmh_server_set_lanes": [_P, C.c_int32],
+    "kmh_server_reserve": [_P, C.c_int64, _P, C.c_int32, C.c_int32],
     "kmh_server_destroy": [_P],
     "kmh_server_submit": [_P, C.POINTER(kmh_serve_req), C.POINTER(C.c_int64)],
     "kmh_server_step": [_P, C.POINTER(C.c_int32)],

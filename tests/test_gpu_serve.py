@@ -49,11 +49,14 @@ def _problems(n_list, seed=0, nv=8):
     return out
 
 
-def _serve(lib, L, ctx, probs, max_inflight, budget=0, order=None, lanes=None):
+def _serve(lib, L, ctx, probs, max_inflight, budget=0, order=None, lanes=None, reserve=None):
     srv = C.c_void_p()
     assert lib.kmh_server_create(ctx.ptr, max_inflight, budget, C.byref(srv)) == 0
     if lanes is not None:
         assert lib.kmh_server_set_lanes(srv, lanes) == 0, lib.last_error()
+    if reserve is not None:   # (n, opts, nv, requests)
+        n, opts, nv, count = reserve
+        assert lib.kmh_server_reserve(srv, n, C.byref(opts), nv, count) == 0, lib.last_error()
     ids = {}
     keep = []
     order = order or list(range(len(probs)))
@@ -489,3 +492,32 @@ def test_serve_aca_request_under_budget(gpu_lib):
     HK = km.KernelHODLR(K, 1e-10, 0, km.FixedLevel(level), nystrom=False)
     km.symmetricfactorize(HK)
     assert abs(out.nll - km.negloglik(HK, y)) <= 1e-10 * abs(out.nll)
+
+
+def test_reserved_pools_serve_the_same(gpu_lib):
+    """kmh_server_reserve sizes every lane's pool up front (and kmh_ctx_reserve
+    a context's): results are bitwise those of a server that grows its pools
+    while serving; bad arguments are refused."""
+    import kmhodlr as km
+    from kmhodlr import _lib as L
+    lib = km.load()
+    ctx = km.Context(0)
+    probs = _problems([2048, 3000, 2048, 4096], seed=11)
+    base, _ = _serve(lib, L, ctx, probs, 4, lanes=2)
+    opts = L.kmh_build_opts()
+    opts.method = L.KMH_NYSTROM
+    opts.level_kind = L.KMH_FIXED_LEVEL
+    opts.level = 4
+    opts.maxrank = 40
+    res, _ = _serve(lib, L, ctx, probs, 4, lanes=2, reserve=(4096, opts, 8, 4))
+    for i in range(len(probs)):
+        assert res[i][:5] == base[i][:5], i
+    assert lib.kmh_ctx_reserve(ctx.ptr, 64 << 20) == 0, lib.last_error()
+    assert lib.kmh_ctx_reserve(ctx.ptr, 0) == 0
+    assert lib.kmh_ctx_reserve(None, 1) == L.KMH_E_BAD_ARG
+    srv = C.c_void_p()
+    assert lib.kmh_server_create(ctx.ptr, 2, 0, C.byref(srv)) == 0
+    assert lib.kmh_server_reserve(srv, 0, C.byref(opts), 0, 1) == L.KMH_E_BAD_ARG
+    assert lib.kmh_server_reserve(srv, 1024, None, 0, 1) == L.KMH_E_BAD_ARG
+    assert lib.kmh_server_reserve(srv, 1024, C.byref(opts), 0, 0) == 0
+    assert lib.kmh_server_destroy(srv) == 0
