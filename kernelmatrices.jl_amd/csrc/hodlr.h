@@ -42,7 +42,11 @@ struct Hodlr {
     kmh_build_opts opts{};
 
     DevBuf pts;                                  // x[0..n), y[0..n)
-    DevBuf info;                                 // device failure flag (Cholesky)
+    DevBuf info;                                 // device failure flags: [0] Cholesky of the
+                                                 // factorization, [1] landmark Cholesky of a
+                                                 // build that deferred its check
+    bool landmark_check = false;                 // info[1] not read yet (read with the
+                                                 // factorization's flags, factor_end_batch)
     DevBuf leaves;                               // leaf i at leaf_mat_off[i], ld = size
     bool leaves_ready = false;                   // leaves hold K's leaves (assembled on first
                                                  // use, ensure_leaves; a factorization of
@@ -197,6 +201,12 @@ struct BuildReq {
     int d;
     kmh_kernel kern;
     kmh_build_opts opts;
+    // Leave the landmark Cholesky's failure flag to the factorization's
+    // final check instead of synchronizing on it here (the server's build
+    // stage, whose requests always go on to be factorized): with every
+    // request's points on the device the build then returns without waiting
+    // for the device.
+    bool defer_checks = false;
 };
 
 // Per-request outcome of a batched stage.

@@ -71,12 +71,27 @@ void check_kernel(const kmh_kernel& k, bool allow_deriv) {
 
 // Read the per-handle Cholesky flags (one copy, one sync) and mark failures;
 // extra doubles (e.g. per-handle sums) can ride along in the same copy.
+static const char* const LANDMARK_FAIL = "Cholesky of the Nystrom landmark matrix K(P,P) failed.";
+
+// Handles with a deferred landmark check (landmark_check) have their
+// info[1] read too; a set info[1] reports the landmark Cholesky's failure.
 static void collect_info(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<Status>& st, const char* what,
                          const std::vector<const double*>& extra = {}, double* extra_out = nullptr) {
     std::vector<const int*> fp;
     std::vector<size_t> at;
+    std::vector<char> lm;
     for (size_t i = 0; i < hs.size(); ++i)
-        if (hs[i]) { fp.push_back(hs[i]->info.as<int>()); at.push_back(i); }
+        if (hs[i]) {
+            fp.push_back(hs[i]->info.as<int>());
+            at.push_back(i);
+            lm.push_back(0);
+            if (hs[i]->landmark_check) {
+                fp.push_back(hs[i]->info.as<int>() + 1);
+                at.push_back(i);
+                lm.push_back(1);
+                hs[i]->landmark_check = false;
+            }
+        }
     const size_t nd = extra.size(), ni = fp.size();
     if (ni + nd == 0) return;
     char* buf = static_cast<char*>(c.scratch(nd * sizeof(double) + ni * sizeof(int)));
@@ -91,7 +106,11 @@ static void collect_info(Ctx& c, const std::vector<Hodlr*>& hs, std::vector<Stat
         int flag;
         std::memcpy(&flag, h.data() + nd * sizeof(double) + q * sizeof(int), sizeof(int));
         const size_t i = at[q];
-        if (flag && st[i].code == KMH_OK) {
+        if (flag && lm[q]) {
+            // the landmark matrix failed first; what followed is void
+            st[i].code = KMH_E_NOT_PD;
+            st[i].msg = std::string("PosDefException: matrix is not positive definite; ") + LANDMARK_FAIL;
+        } else if (flag && st[i].code == KMH_OK) {
             st[i].code = KMH_E_NOT_PD;
             st[i].msg = std::string("PosDefException: matrix is not positive definite; ") + what;
         }
@@ -281,6 +300,8 @@ std::vector<Hodlr*> hodlr_build_batch(Ctx& c, const std::vector<BuildReq>& reqs,
     std::vector<std::vector<double>> soa(R);
     std::vector<Copy2D> dev_pts;
     std::vector<int*> info0;
+    bool defer = R > 0;
+    for (const auto& q : reqs) defer = defer && q.defer_checks && q.on_device;
     for (size_t i = 0; i < R; ++i) {
         try {
             const BuildReq& q = reqs[i];
@@ -317,6 +338,7 @@ std::vector<Hodlr*> hodlr_build_batch(Ctx& c, const std::vector<BuildReq>& reqs,
             }
             hs[i] = own[i].get();
             info0.push_back(H.info.as<int>());   // only handles that survive validation
+            if (defer) info0.push_back(H.info.as<int>() + 1);
         } catch (const Error& e) {
             st[i].code = e.code;
             st[i].msg = e.what();
@@ -350,7 +372,8 @@ std::vector<Hodlr*> hodlr_build_batch(Ctx& c, const std::vector<BuildReq>& reqs,
             }
             ev.push_back({lx, ly, lx, ly, r, r, H.kpp.as<double>(), r, 0, 0, kd});
             ev.push_back({px, py, lx, ly, (int)H.n, r, H.kxp.as<double>(), (int)H.n, 0, 0, kd});
-            kpp.push_back({H.kpp.as<double>(), r, r, H.info.as<int>()});
+            kpp.push_back({H.kpp.as<double>(), r, r, H.info.as<int>() + (defer ? 1 : 0)});
+            H.landmark_check = defer;
         } else if (H.L > 0) {
             aca.push_back((int)i);
         }
@@ -359,7 +382,7 @@ std::vector<Hodlr*> hodlr_build_batch(Ctx& c, const std::vector<BuildReq>& reqs,
     eval_blocks(c, ev);
     if (!nys.empty()) {
         potrf(c, kpp);
-        collect_info(c, hs, st, "Cholesky of the Nystrom landmark matrix K(P,P) failed.");
+        if (!defer) collect_info(c, hs, st, LANDMARK_FAIL);
         // Y = K(x,P) K(P,P)^{-1} (constructor.jl / factorizations.jl:92):
         // K(P,P)^{-1} = L^{-T} L^{-1} from two triangular solves on the r x r
         // identity, then one n x r x r product.
@@ -381,7 +404,7 @@ std::vector<Hodlr*> hodlr_build_batch(Ctx& c, const std::vector<BuildReq>& reqs,
         gemm(c, false, false, yg);
     }
     if (!aca.empty()) build_aca_batch(c, hs, st, aca);
-    c.sync();
+    if (!defer) c.sync();
     std::vector<Hodlr*> out(R, nullptr);
     for (size_t i = 0; i < R; ++i)
         if (hs[i] && st[i].code == KMH_OK) out[i] = own[i].release();

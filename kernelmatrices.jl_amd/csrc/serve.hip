@@ -319,8 +319,10 @@ static void lane_step(Lane& ln, const std::function<void(std::vector<Req*>&)>& a
     }
     if (!build.empty()) {
         std::vector<BuildReq> br;
-        for (Req* q : build)
+        for (Req* q : build) {
             br.push_back({q->dpts ? q->dpts : q->pts.data(), q->dpts != nullptr, q->n, 2, q->kern, q->opts});
+            br.back().defer_checks = true;   // the leaf stage follows on the same stream
+        }
         std::vector<Status> st;
         auto hs = hodlr_build_batch(c, br, st);
         for (size_t i = 0; i < build.size(); ++i) {

@@ -250,7 +250,10 @@ def test_server_destroyed_with_requests_in_flight(gpu_lib):
 def test_serve_failed_request_leaves_the_others(gpu_lib):
     """A request whose kernel matrix is not positive definite (nugget -5)
     fails with KMH_E_NOT_PD; the requests beside it, in its lane and in the
-    other lane, finish with the same results as alone."""
+    other lane, finish with the same results as alone.  The server's build
+    leaves the landmark Cholesky's flag to the factorization's final check:
+    the failure still names the landmark matrix K(P,P), whose diagonal is
+    sigma^2 - 5 < 0."""
     import kmhodlr as km
     from kmhodlr import _lib as L
     lib = km.load()
@@ -281,6 +284,7 @@ def test_serve_failed_request_leaves_the_others(gpu_lib):
         rc = lib.kmh_server_result(srv, rid, C.byref(out))
         if i == len(probs):
             assert rc == 0 and out.status == L.KMH_E_NOT_PD
+            assert "landmark matrix" in lib.last_error(), lib.last_error()
         else:
             assert rc == 0 and out.status == 0, lib.last_error()
             assert out.nll == alone[i][2] and out.logdet == alone[i][3]
