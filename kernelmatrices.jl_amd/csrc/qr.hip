@@ -153,8 +153,9 @@ __global__ __launch_bounds__(64, RPL == 4 ? 2 : (NC <= 16 ? 4 : 3)) void k_tile_
     if (d.kp > 0) {
         // the block Gram-Schmidt reorthogonalisation update X -= P T, one
         // prior column l at a time: P's rows of column l and lane k's T(l, k)
-        // loaded one column ahead, T(l, k) reaching every lane by a lane
-        // broadcast (two SGPRs live at a time)
+        // loaded PD columns ahead (a ring of PD register slots, so that the
+        // loads' latency is covered by PD columns of products), T(l, k)
+        // reaching every lane by a lane broadcast (two SGPRs live at a time)
         typedef const __attribute__((address_space(1))) E* GSrc;
         const GSrc pl = (GSrc)d.pq + lane;   // row lane + 64 i at pl[64 i]
         const Glob<const double> tg = glob(d.pt);
@@ -170,21 +171,31 @@ __global__ __launch_bounds__(64, RPL == 4 ? 2 : (NC <= 16 ? 4 : 3)) void k_tile_
             const bool ok = lane < nc && l < d.kp;
             return keep_or_zero(tg[ok ? (size_t)l + (size_t)lane * d.kp : 0], ok);
         };
-        double qn[RPL];
-        load_p(0, qn);
-        double tn = load_t(0);
-        for (int l = 0; l < d.kp; ++l) {
-            double q[RPL];
+        constexpr int PD = 2;
+        double qb[PD][RPL], tb[PD];
 #pragma unroll
-            for (int i = 0; i < RPL; ++i) q[i] = qn[i];
-            const double tl = tn;
-            load_p(l + 1, qn);   // the next column's loads overlap this column's products
-            tn = load_t(l + 1);
+        for (int s = 0; s < PD; ++s) {
+            load_p(s, qb[s]);
+            tb[s] = load_t(s);
+        }
+        for (int l0 = 0; l0 < d.kp; l0 += PD) {
 #pragma unroll
-            for (int k = 0; k < NC; ++k) {
-                const double tk = qr_bcast(tl, k);
+            for (int s = 0; s < PD; ++s) {
+                const int l = l0 + s;
+                double q[RPL];
 #pragma unroll
-                for (int i = 0; i < RPL; ++i) a[i][k] = __fma_rn(-q[i], tk, a[i][k]);
+                for (int i = 0; i < RPL; ++i) q[i] = qb[s][i];
+                const double tl = tb[s];
+                load_p(l + PD, qb[s]);   // slot s refilled with column l + PD
+                tb[s] = load_t(l + PD);
+                if (l < d.kp) {
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) {
+                        const double tk = qr_bcast(tl, k);
+#pragma unroll
+                        for (int i = 0; i < RPL; ++i) a[i][k] = __fma_rn(-q[i], tk, a[i][k]);
+                    }
+                }
             }
         }
     }
