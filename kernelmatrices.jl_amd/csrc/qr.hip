@@ -405,9 +405,11 @@ __global__ __launch_bounds__(128, 4) void k_wy_apply(const WyDesc* __restrict__ 
     const WyDesc d = ds[blockIdx.x];
     const int nb = d.nb, t = threadIdx.x;
     const int wave = t >> 6, lane = t & 63, li = lane & 15, lk = lane >> 4;
-    // wave w takes row blocks w + 4 p and w + 4 p + 2 in pass p;
-    // the first pass's V^T fragments V(row, 4 ks + lk) load before the
-    // prologue so that their latency overlaps it
+    // wave w takes row blocks w + 4 p and w + 4 p + 2 in pass p, whose V^T
+    // fragments V(row, 4 ks + lk) load at the start of the pass (the first
+    // pass's right after the substitution: held through it, they made the
+    // substitution's 24 unknowns spill at the 128 registers of four waves
+    // per SIMD, and the kernel ran 10 % slower)
     double vf[2][KS];
     auto load_rows = [&](int pass) {
 #pragma unroll
@@ -423,7 +425,6 @@ __global__ __launch_bounds__(128, 4) void k_wy_apply(const WyDesc* __restrict__ 
             }
         }
     };
-    load_rows(0);
     {
         // V1, C and Z, zero outside nb x nb: all loads in flight before the LDS writes
         constexpr int NE = (QC * QC + 127) / 128;
@@ -491,6 +492,7 @@ __global__ __launch_bounds__(128, 4) void k_wy_apply(const WyDesc* __restrict__ 
             if (i < nb) Ys[i * QC + t] = x[i];
     }
     __syncthreads();
+    load_rows(0);
     double yf[2][KS];   // -Y(4 ks + lk, 16 cb + li)
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb)
